@@ -1,0 +1,177 @@
+#!/usr/bin/env python
+"""Benchmark of the render core: Mrays/s of the 1080p Mandelbulb frame.
+
+Workload (one step = one frame): Mandelbulb power 8, 1920x1080, N 250, camera
+at (0,-3.5,0) looking at the origin, fov 1, one hard-shadow ray per hit toward
+a seeded unit light direction, AO quality 8 (412 directions), volumetric fog
+density 0.5.  The frame renders into device-resident planes through the C ABI
+(mbr_render_device); Mrays/s = primary rays / wall time / 1e6.
+
+Printed as one JSON line: the FP32+FP64-fixup render is `value`; the plain
+FP64 GPU render, the pixel difference between the two, the roofline of the
+dominant kernel and the CPU baseline (the oracle on the host cores, a bounded
+sample of the same frame) ride along.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "Mrays/s at 1080p Mandelbulb: FP32+FP64-fixup vs plain FP64, ≤0.1% px diff"
+
+# Published MI355X vector peaks (TFLOP/s).  The FP32 figure assumes packed
+# FP32 issue, which this build leaves disabled; the unpacked FP32 rate equals
+# the FP64 one.
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+
+
+def planes_on_gpu(torch, w, h):
+    dev = torch.device("cuda:0")
+    return {
+        "rgb": torch.empty((3, h, w), dtype=torch.float32, device=dev),
+        "z": torch.empty((h, w), dtype=torch.float32, device=dev),
+        "colour": torch.empty((3, h, w), dtype=torch.uint8, device=dev),
+        "alpha": torch.empty((h, w), dtype=torch.int16, device=dev),
+        "opacity": torch.empty((h, w), dtype=torch.int16, device=dev),
+    }
+
+
+def to_frame(torch, planes):
+    from mandelbulber_amd.render import Frame
+    h, w = planes["z"].shape
+    f = Frame(w, h)
+    f.rgb[:] = planes["rgb"].cpu().numpy()
+    f.z[:] = planes["z"].cpu().numpy()
+    return f
+
+
+def timed(torch, r, planes, precision, steps, warmup):
+    for _ in range(warmup):
+        r.render_device(planes, precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        stats.append(r.render_device(planes, precision=precision))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, stats
+
+
+def cpu_baseline(sc, seconds, threads):
+    """The oracle (-O3 -ffast-math, the reference's release flags) on a band
+    of rows through the middle of the frame, sized to about `seconds`."""
+    from oracle import oracle
+    h = sc.image_height
+    rows = 2
+    while True:
+        r0 = h // 2 - rows // 2
+        t0 = time.perf_counter()
+        _, st = oracle.render(sc, rows=(r0, r0 + rows), threads=threads, fast=True)
+        dt = time.perf_counter() - t0
+        if dt >= seconds / 4 or rows >= h:
+            break
+        rows = min(h, max(rows * 2, int(rows * seconds / 4 / max(dt, 1e-3))))
+    # final timed sample
+    if dt < seconds / 2 and rows < h:
+        rows = min(h, int(rows * seconds / max(dt, 1e-3)))
+        r0 = h // 2 - rows // 2
+        t0 = time.perf_counter()
+        _, st = oracle.render(sc, rows=(r0, r0 + rows), threads=threads, fast=True)
+        dt = time.perf_counter() - t0
+    rays = rows * sc.image_width
+    return {
+        "value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "sample": f"rows {r0}..{r0 + rows} of the {sc.image_width}x{h} frame ({rays} rays, "
+                  f"{dt:.1f} s), oracle/liboracle_fast.so (-O3 -ffast-math), {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this render core runs on one GPU; multi-GPU runs are not part of this build "
+              "(frames and tiles would shard across GPUs with one framebuffer gather).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    from mandelbulber_amd import Renderer, _abi, scene
+    from mandelbulber_amd.flops import algorithmic_flops
+    from metrics import compare
+
+    sc = scene.mandelbulb_full(args.width, args.height, seed=args.seed)
+    r = Renderer(0)
+    r.set_scene(sc)
+    planes = planes_on_gpu(torch, args.width, args.height)
+    rays = args.width * args.height
+
+    dt32, st32 = timed(torch, r, planes, _abi.PREC_FP32_FIXUP, args.steps, args.warmup)
+    f32 = to_frame(torch, planes)
+    dt64, st64 = timed(torch, r, planes, _abi.PREC_FP64, args.fp64_steps, 0)
+    f64 = to_frame(torch, planes)
+    diff = compare(f32, f64)
+
+    value = rays * args.steps / dt32 / 1e6
+    value64 = rays * args.fp64_steps / dt64 / 1e6
+
+    # roofline of the dominant kernel (k_shade): algorithmic FLOPs per
+    # launch / average launch duration from the library's HIP events on the
+    # launch stream
+    shade_ms = sum(s.ms_shade for s in st32) / max(1, sum(s.shade_launches for s in st32))
+    shade_flops = sum(algorithmic_flops(sc, s, "shade") for s in st32) / max(1, sum(s.shade_launches for s in st32))
+    achieved = shade_flops / (shade_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None, "kernel": "k_shade<float>",
+            "flops_per_launch": shade_flops, "ms_per_launch": shade_ms}
+    shade64_ms = sum(s.ms_shade for s in st64) / max(1, sum(s.shade_launches for s in st64))
+    shade64_flops = sum(algorithmic_flops(sc, s, "shade") for s in st64) / max(1, sum(s.shade_launches for s in st64))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(sc, args.cpu_seconds, threads)
+
+    s0 = st32[-1]
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt32 / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"Mandelbulb power 8, {args.width}x{args.height}, N 250, hard shadow, AO quality 8 "
+                        f"({s0.shade_de_calls // max(1, s0.hits)} DE calls per hit), volumetric fog 0.5, "
+                        f"light seed {args.seed}",
+            "precision": "FP32 DE iteration and shading; FP64 ray origins, scan and final bisection",
+            "rays": rays, "hits": s0.hits,
+        },
+        "fp64": {"value": value64, "unit": "Mrays/s", "ms_per_step": dt64 / args.fp64_steps * 1e3,
+                 "shade_ms_per_launch": shade64_ms,
+                 "shade_achieved_tflops": shade64_flops / (shade64_ms * 1e-3) / 1e12},
+        "speedup_vs_fp64": value / value64,
+        "pixel_diff_vs_fp64": {k: diff[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
+        "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "finish": s0.ms_finish},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    r.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,324 @@
+/*
+ * mbr.h — C ABI of the MI355X render core for Mandelbulber 1.21 scenes.
+ *
+ * This boundary replaces the reference's OpenCL render seam, class CclSupport
+ * (src/cl_support.hpp:59-156, dummy fallback :158-170), which Render() calls
+ * at src/Render3D.cpp:719 and :1160-1251, and the CPU MainThread pool
+ * (src/Render3D.cpp:203-706) that Render() falls back to.  Parameters cross
+ * it as one FP64 POD (mbr_scene) that carries the fields of sParamRender
+ * (src/fractparams.h:18-146) and sFractal (src/fractal.h:251-293) used by the
+ * render core.  Output is the cImage buffer set (src/cimage.hpp:82-89,
+ * :157-162) as caller-owned planes.
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Every entry point
+ * returns an int status (MBR_OK = 0, negative = error); mbr_last_error()
+ * gives the message.  A context is not thread-safe; it owns one HIP stream
+ * unless the caller supplies one per call.
+ */
+#ifndef MBR_H_
+#define MBR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBR_ABI_VERSION 1
+
+/* ---- status codes (reference analogue: checkErr -> bool + CPU fallback,
+ *      src/cl_support.cpp:109-135, src/Render3D.cpp:719) ---- */
+enum mbr_status {
+  MBR_OK = 0,
+  MBR_ERR_INVALID_ARG = -1,
+  MBR_ERR_NO_DEVICE = -2,
+  MBR_ERR_HIP = -3,
+  MBR_ERR_UNSUPPORTED = -4, /* scene uses a feature outside the render core */
+  MBR_ERR_NO_SCENE = -5,
+  MBR_ERR_CANCELLED = -6,
+  MBR_ERR_IO = -7,
+  MBR_ERR_OUT_OF_MEMORY = -8
+};
+
+/* ---- formula enum: values of enumFractalFormula (src/fractal.h:25-65) ---- */
+enum mbr_formula {
+  MBR_FORMULA_NONE = 0,
+  MBR_FORMULA_TRIG_OPTIM = 2,    /* Mandelbulb, "trigonometric" in the GUI */
+  MBR_FORMULA_MENGER_SPONGE = 7,
+  MBR_FORMULA_TGLAD = 8          /* Mandelbox */
+};
+
+/* enumPerspectiveType (src/common_math.h) */
+enum mbr_perspective {
+  MBR_PERSP_THREE_POINT = 0,
+  MBR_PERSP_FISH_EYE = 1,
+  MBR_PERSP_EQUIRECTANGULAR = 2
+};
+
+/* enumObjectType (src/fractal.h:82-91) */
+enum mbr_object {
+  MBR_OBJ_FRACTAL = 0, MBR_OBJ_PLANE = 1, MBR_OBJ_WATER = 2, MBR_OBJ_SPHERE = 3,
+  MBR_OBJ_SPHERE_INV = 4, MBR_OBJ_BOX = 5, MBR_OBJ_BOX_INV = 6
+};
+
+/* March step jitter.  The reference multiplies every step by
+ * 1 - Random(1000)/10000 with glibc rand() shared by all threads
+ * (src/Render3D.cpp:129-134, src/common_math.cpp:25-32).  A GPU cannot
+ * replay that sequential stream, so the render core draws the same integer
+ * in [0,1000] from a counter-based hash of (seed, frame, pixel, step). */
+enum mbr_jitter {
+  MBR_JITTER_NONE = 0,
+  MBR_JITTER_HASH = 1
+};
+
+/* Arithmetic of the render: plain FP64 everywhere (the reference's CPU
+ * arithmetic), or FP32 distance-estimator iteration and shading with FP64
+ * ray origins / scan and an FP64 final bisection. */
+enum mbr_precision {
+  MBR_PREC_FP64 = 0,
+  MBR_PREC_FP32_FIXUP = 1
+};
+
+typedef struct mbr_rgb { int R, G, B; } mbr_rgb; /* sRGB (src/cimage.hpp:39-44) */
+
+/*
+ * Scene POD.  Field names follow sParamRender / sFractal; each comment names
+ * the .fract key that sets it (LoadOneSetting, src/settings.cpp:700-1183).
+ * Angles are radians (the loader converts the degree keys, settings.cpp:787).
+ */
+typedef struct mbr_scene {
+  int abi_version;                 /* = MBR_ABI_VERSION */
+
+  /* image */
+  int image_width, image_height;   /* image_width, image_height */
+  int tiles, tile_index;           /* tiles, tile_no */
+
+  /* camera */
+  double vp[3];                    /* view_point_x/y/z */
+  double alpha, beta, gamma;       /* angle_alfa/beta/gamma (radians) */
+  double zoom;                     /* zoom */
+  double persp;                    /* perspective (fov factor) */
+  int perspective_type;            /* fish_eye: enum mbr_perspective */
+  int fish_eye_cut;                /* fish_eye_180cut */
+  double view_distance_min;        /* view_distance_min */
+  double view_distance_max;        /* view_distance_max */
+
+  /* ray marching */
+  double quality;                  /* quality (DE threshold factor) */
+  double DE_factor;                /* DE_factor */
+  double smoothness;               /* smoothness */
+
+  /* fractal */
+  int formula;                     /* formula: enum mbr_formula */
+  int N;                           /* N */
+  int minN;                        /* minN */
+  double power;                    /* power */
+  double bailout;                  /* escape radius of trig formulas; the
+                                      reference hard-codes 1e2
+                                      (src/fractal.cpp:1153) */
+  double constant_factor;          /* fractal_constant_factor */
+  int julia_mode;                  /* julia_mode */
+  double julia[3];                 /* julia_a/b/c */
+  int interior_mode;               /* interior_mode */
+  int iter_thresh;                 /* iteration_threshold_mode */
+  int constant_DE_threshold;       /* constant_DE_threshold */
+  int limits_enabled;              /* limits_enabled */
+  double limit_min[3];             /* x_min, y_min, z_min */
+  double limit_max[3];             /* x_max, y_max, z_max */
+  int tglad_folding_mode;          /* tglad_folding_mode (pre-fold) */
+  double folding_limit;            /* folding_limit */
+  double folding_value;            /* folding_value */
+  int spherical_folding_mode;      /* spherical_folding_mode (pre-fold) */
+  double spherical_folding_fixed;  /* spherical_folding_fixed */
+  double spherical_folding_min;    /* spherical_folding_min */
+  int ifs_folding_mode;            /* IFS_folding_mode (not supported: must be 0) */
+
+  /* Mandelbox (sFractalMandelbox, src/fractal.h:172-179) */
+  double mb_scale;                 /* mandelbox_scale */
+  double mb_folding_limit;         /* mandelbox_folding_limit */
+  double mb_folding_value;         /* mandelbox_folding_value */
+  double mb_min_radius;            /* mandelbox_folding_min_radius */
+  double mb_fixed_radius;          /* mandelbox_folding_fixed_radius */
+  double mb_offset[3];             /* mandelbox_offset_X/Y/Z */
+  double mb_color_R, mb_color_X, mb_color_Y, mb_color_Z;  /* mandelbox_color_* */
+  double mb_color_Sp1, mb_color_Sp2;
+  int mb_rotations_enabled;        /* mandelbox_rotation_enabled */
+  double mb_rotation_main[3];      /* mandelbox_rotation_main_alfa/beta/gamma */
+  double mb_rotation[2][3][3];     /* mandelbox_rotation_{X,Y,Z}{1,2}_{alfa,beta,gamma} */
+
+  /* primitives (sFractalPrimitives[D], src/fractal.h:181-215) */
+  int primitive_only_plane;        /* primitive_only_plane */
+  int plane_enabled;   double plane_centre[3], plane_normal[3];
+  int box_enabled;     double box_centre[3], box_size[3];
+  int inv_box_enabled; double inv_box_centre[3], inv_box_size[3];
+  int sphere_enabled;  double sphere_centre[3], sphere_radius;
+  int inv_sphere_enabled; double inv_sphere_centre[3], inv_sphere_radius;
+  int water_enabled;   double water_height, water_amplitude, water_length,
+                              water_rotation, water_anim_speed;
+  int water_iterations;
+  int frame_no;                    /* frame_no (water animation phase) */
+  mbr_rgb plane_colour, box_colour, inv_box_colour, sphere_colour,
+          inv_sphere_colour, water_colour;
+
+  /* lighting and shading (sImageAdjustments, src/cimage.hpp:46-62) */
+  int shadow;                      /* shadows_enabled */
+  int global_illumination;         /* ambient_occlusion_enabled */
+  int fast_global_illumination;    /* fast_ambient_occlusion_mode */
+  int ao_quality;                  /* ambient_occlusion_quality */
+  int slow_shading;                /* slow_shading */
+  int penetrating_lights;          /* penetrating_lights */
+  double main_light_alpha;         /* main_light_alfa (radians) */
+  double main_light_beta;          /* main_light_beta (radians) */
+  double shadow_cone_angle;        /* shadows_cone_angle (degrees) */
+  double main_light_intensity;     /* main_light_intensity */
+  double direct_light;             /* shadows_intensity */
+  double shading;                  /* shading */
+  double specular;                 /* specular */
+  double global_ilum;              /* ambient_occlusion */
+  double fast_ao_tune;             /* ambient_occlusion_fast_tune */
+  double reflect;                  /* reflect (env mapping: must be 0) */
+  double glow_intensity;           /* glow_intensity */
+  mbr_rgb main_light_colour;       /* main_light_colour_R/G/B */
+  mbr_rgb glow_color1, glow_color2;/* glow_color_1_*, glow_color_2_* */
+  int coloring_enabled;            /* fractal_color */
+  double coloring_speed;           /* coloring_speed */
+  double palette_offset;           /* coloring_palette_offset */
+  mbr_rgb palette[256];            /* palette */
+
+  /* background (BackgroundShader, src/shaders.cpp:114-183) */
+  int textured_background;         /* textured_background (must be 0) */
+  mbr_rgb background_color1, background_color2, background_color3;
+
+  /* fog (VolumetricShader, src/shaders.cpp:185-478) */
+  int post_fog_enabled;            /* post_fog_enabled */
+  double fog_visibility;           /* post_fog_visibility */
+  mbr_rgb post_fog_color;          /* post_fog_color_R/G/B */
+  double fog_density;              /* volumetric_fog_density */
+  double fog_colour1_distance;     /* volumetric_fog_colour_1_distance */
+  double fog_colour2_distance;     /* volumetric_fog_colour_2_distance */
+  double fog_distance_factor;      /* volumetric_fog_distance_factor */
+  mbr_rgb fog_colour1, fog_colour2, fog_colour3; /* fog_colour_{1,2,3}_* */
+
+  /* features outside the render core: a scene that enables any of them is
+   * rejected with MBR_ERR_UNSUPPORTED */
+  int aux_light_number;            /* aux_light_number */
+  int aux_light_pre_enabled[4];    /* aux_light_predefined_N_enabled */
+  int volumetric_light_enabled[5]; /* volumetric_light_enabled_N */
+  int iter_fog_enabled;            /* iteration_fog_enable */
+  int fake_lights_enabled;         /* fake_lights_enabled */
+  int raytraced_reflections;       /* raytraced_reflections */
+  int reflections_max;             /* reflections_max */
+
+  /* settings of the render core that have no .fract key */
+  int jitter_mode;                 /* enum mbr_jitter */
+  uint32_t jitter_seed;
+} mbr_scene;
+
+/* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
+ * substitutes a white 100x100 texture when loading fails
+ * (src/texture.cpp:58-74); pass NULL for that fallback. */
+typedef struct mbr_texture {
+  int width, height;
+  const uint8_t *rgb;               /* width*height*3 bytes, row-major */
+} mbr_texture;
+
+/* Framebuffer planes (cImage layout as structure-of-arrays).  Any pointer
+ * may be NULL to skip that plane.  Index = x + y*width. */
+typedef struct mbr_framebuffer {
+  int width, height;
+  float *rgb[3];       /* imageFloat R, G, B (sRGBfloat) */
+  float *z;            /* zBuffer: depth of ray 0, 1e20 on miss */
+  uint8_t *colour[3];  /* colourBuffer R, G, B (sRGB8) */
+  uint16_t *alpha;     /* alphaBuffer */
+  uint16_t *opacity;   /* opacityBuffer */
+} mbr_framebuffer;
+
+typedef struct mbr_render_opts {
+  int precision;                   /* enum mbr_precision */
+  void *stream;                    /* hipStream_t; NULL = the context's stream */
+  volatile const int *cancel;      /* cooperative stop flag (programClosed,
+                                      src/interface.h:924); may be NULL */
+  int row_begin, row_end;          /* render rows [row_begin,row_end); 0,0 = all */
+} mbr_render_opts;
+
+/* Counters of one render (the reference prints iters/s, average N, average
+ * DE steps and missed-DE %, src/Render3D.cpp:638-652). */
+typedef struct mbr_stats {
+  double ms_total;                 /* GPU time of the render, HIP events */
+  double ms_march, ms_shade, ms_finish;
+  uint64_t rays, hits;
+  uint64_t march_de_calls;         /* DE calls of primary march + bisection */
+  uint64_t shade_de_calls;         /* DE calls of normals, shadow, AO, colour */
+  uint64_t iterations;             /* formula iterations over all DE calls */
+  uint64_t march_iterations;       /* ... of which in the march kernel */
+  uint64_t shade_iterations;       /* ... of which in the shade kernel */
+  uint64_t march_launches, shade_launches, finish_launches;
+  uint64_t missed_de;              /* hits with dist < 0.1*thresh */
+  uint64_t step_overflow_rays;     /* rays re-marched for the step record */
+  uint64_t fp64_redo_rays;         /* FP32-path rays re-marched in FP64 because a
+                                      hit or bisection decision fell inside the
+                                      FP32 error bound */
+} mbr_stats;
+
+typedef struct mbr_device_opts {
+  int device;                      /* HIP device ordinal */
+} mbr_device_opts;
+
+typedef struct mbr_ctx mbr_ctx;
+
+/* Library version (MBR_ABI_VERSION). */
+int mbr_version(void);
+
+/* Fill *s with the values of the reference defaults file (share/defaults),
+ * which LoadSettings reads before every scene (src/settings.cpp:611-626). */
+int mbr_scene_defaults(mbr_scene *s);
+
+/* LoadSettings (src/settings.cpp:611-698): defaults, then the .fract file,
+ * then "key=value" overrides (-o, src/settings.cpp:659-676), then
+ * LoadSettingsPost (random palette when the file has none, :1185-1195).
+ * Unknown keys are reported to stderr, as the reference does. */
+int mbr_load_settings(const char *path, const char *const *overrides,
+                      int n_overrides, mbr_scene *s);
+
+/* Parse one "key value" pair into *s (LoadOneSetting, :700). */
+int mbr_set_setting(mbr_scene *s, const char *key, const char *value);
+
+/* Replaces CclSupport::InitDevice/InitFractal (cl_support.cpp:137, :231). */
+int mbr_init(const mbr_device_opts *opts, mbr_ctx **out);
+void mbr_destroy(mbr_ctx *ctx);
+const char *mbr_last_error(const mbr_ctx *ctx);
+
+/* Replaces Params2Cl + CclSupport::SetParams (interface.cpp:3634,
+ * cl_support.cpp:484).  Copies the scene; derives the rotation matrix, ray
+ * start, light vector, resolution and AO direction table as MainThread does
+ * (src/Render3D.cpp:236-333) and the formula constants as
+ * RecalculateIFSParams does (src/callbacks.cpp:1538-1555). */
+int mbr_set_scene(mbr_ctx *ctx, const mbr_scene *scene,
+                  const mbr_texture *lightmap);
+
+/* Replaces CclSupport::Render (cl_support.cpp:527-742) and the CPU
+ * MainThread pool: renders the scene into caller-owned HOST planes. */
+int mbr_render(mbr_ctx *ctx, const mbr_render_opts *opts,
+               mbr_framebuffer *fb, mbr_stats *stats);
+
+/* Same, with the planes in DEVICE memory (no host copies); the render is
+ * enqueued on the stream and complete when the call returns. */
+int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
+                      mbr_framebuffer *fb, mbr_stats *stats);
+
+
+/* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
+ * abi_version, vp, formula, mb_rotation, palette, fog_colour3,
+ * reflections_max, jitter_seed (in that order). */
+size_t mbr_scene_layout(size_t offsets[8]);
+
+/* The jitter draw of MBR_JITTER_HASH: an integer in [0,1000] standing in for
+ * Random(1000) at march step `step` of pixel `pixel` in frame `frame`. */
+uint32_t mbr_jitter_draw(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBR_H_ */

@@ -1,0 +1,685 @@
+// mbr_api.cpp — host side of the C ABI (include/mbr.h).
+//
+// mbr_set_scene derives everything MainThread computes once per frame
+// (src/Render3D.cpp:236-333) and everything ReadInterface derives
+// (src/interface.cpp:688-730, src/callbacks.cpp:1538-1555) into the device
+// scene block, after the parameter clamps of CheckPrameters
+// (src/interface.cpp:3587-3602).  mbr_render[_device] runs the frame as
+// bands of at most kBandRays rays: march -> (overflow re-march) -> shade
+// over compacted hits -> finish.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mbr.h"
+#include "mbr_core.hpp"
+#include "mbr_kernels.hpp"
+#include "mbr_scene_dev.hpp"
+
+using namespace mbr;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kBandRays = 1 << 21;   // rays per band (one 1080p frame)
+constexpr int kSlab = 96;            // step records per ray kept in the slab
+
+// ---------------------------------------------------- rotation matrices
+// CRotationMatrix (src/algebra.cpp:80-218): row-major, post-multiplied
+// rotations, each skipped when its angle is 0; `identity` mirrors the
+// reference's `zero` flag (RotateVector returns its argument unchanged).
+struct Rot {
+  double m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  bool identity = true;
+  void mul(const double *b) {
+    double c[9];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) c[i * 3 + j] = m[i * 3] * b[j] + m[i * 3 + 1] * b[3 + j] + m[i * 3 + 2] * b[6 + j];
+    std::memcpy(m, c, sizeof c);
+    identity = false;
+  }
+  void rx(double a) {
+    if (a == 0.0) return;
+    double s = std::sin(a), c = std::cos(a), t[9] = {1, 0, 0, 0, c, -s, 0, s, c};
+    mul(t);
+  }
+  void ry(double a) {
+    if (a == 0.0) return;
+    double s = std::sin(a), c = std::cos(a), t[9] = {c, 0, s, 0, 1, 0, -s, 0, c};
+    mul(t);
+  }
+  void rz(double a) {
+    if (a == 0.0) return;
+    double s = std::sin(a), c = std::cos(a), t[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+    mul(t);
+  }
+  void apply(const double v[3], double o[3]) const {
+    if (identity) { o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; return; }
+    for (int i = 0; i < 3; i++) o[i] = m[i * 3] * v[0] + m[i * 3 + 1] * v[1] + m[i * 3 + 2] * v[2];
+  }
+};
+
+// SetRotation(angles[3]) (src/algebra.cpp:145-151)
+Rot set_rotation3(const double a[3]) {
+  Rot r;
+  r.rz(a[2]);
+  r.ry(a[1]);
+  r.rx(a[0]);
+  return r;
+}
+
+template <class T>
+void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
+  std::memset(&f, 0, sizeof f);
+  f.formula = s.formula;
+  f.N = s.N;
+  f.minN = s.minN;
+  f.julia_mode = s.julia_mode;
+  f.tglad_fold = s.tglad_folding_mode;
+  f.sph_fold = s.spherical_folding_mode;
+  f.mb_rot_on = s.mb_rotations_enabled;
+  f.interior = s.interior_mode;
+  f.iter_thresh = s.iter_thresh;
+  f.limits = s.limits_enabled;
+  f.only_plane = s.primitive_only_plane;
+  f.prim_plane = s.plane_enabled;
+  f.prim_box = s.box_enabled;
+  f.prim_ibox = s.inv_box_enabled;
+  f.prim_sphere = s.sphere_enabled;
+  f.prim_isphere = s.inv_sphere_enabled;
+  f.prim_water = s.water_enabled;
+  f.water_iterations = s.water_iterations;
+  f.frame_no = s.frame_no;
+  f.power = (T)s.power;
+  f.bailout = (T)s.bailout;
+  f.constant_factor = (T)s.constant_factor;
+  for (int i = 0; i < 3; i++) f.julia[i] = (T)s.julia[i];
+  f.fold_limit = (T)s.folding_limit;
+  f.fold_value = (T)s.folding_value;
+  const double sf2 = s.spherical_folding_fixed * s.spherical_folding_fixed;
+  const double sm2 = s.spherical_folding_min * s.spherical_folding_min;
+  f.sph_fR2 = (T)sf2;
+  f.sph_mR2 = (T)sm2;
+  f.sph_factor1 = (T)(sf2 / sm2);
+  f.mb_scale = (T)s.mb_scale;
+  f.mb_abs_scale = (T)std::fabs(s.mb_scale);
+  f.mb_limit = (T)s.mb_folding_limit;
+  f.mb_value = (T)s.mb_folding_value;
+  const double fR2 = s.mb_fixed_radius * s.mb_fixed_radius;
+  const double mR2 = s.mb_min_radius * s.mb_min_radius;
+  f.mb_fR2 = (T)fR2;
+  f.mb_mR2 = (T)mR2;
+  f.mb_factor1 = (T)(fR2 / mR2);
+  for (int i = 0; i < 3; i++) f.mb_offset[i] = (T)s.mb_offset[i];
+  f.mb_col_R = (T)s.mb_color_R;
+  f.mb_col_X = (T)s.mb_color_X;
+  f.mb_col_Y = (T)s.mb_color_Y;
+  f.mb_col_Z = (T)s.mb_color_Z;
+  f.mb_col_Sp1 = (T)s.mb_color_Sp1;
+  f.mb_col_Sp2 = (T)s.mb_color_Sp2;
+  // RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555)
+  Rot main = set_rotation3(s.mb_rotation_main);
+  f.mb_main_rot = !main.identity;
+  for (int i = 0; i < 9; i++) f.mb_main[i] = (T)main.m[i];
+  for (int fold = 0; fold < 2; fold++)
+    for (int axis = 0; axis < 3; axis++) {
+      Rot r = set_rotation3(s.mb_rotation[fold][axis]);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          f.mb_rot[fold][axis][i * 3 + j] = (T)r.m[i * 3 + j];
+          f.mb_rotinv[fold][axis][i * 3 + j] = (T)r.m[j * 3 + i];
+        }
+    }
+  for (int i = 0; i < 3; i++) {
+    f.limit_min[i] = (T)s.limit_min[i];
+    f.limit_max[i] = (T)s.limit_max[i];
+  }
+  // PrimitivePlane normalises its normal on every call (src/primitives.cpp:17-18)
+  {
+    const double *n = s.plane_normal;
+    double inv = 1.0 / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int i = 0; i < 3; i++) {
+      f.plane_n[i] = (T)(n[i] * inv);
+      f.plane_c[i] = (T)s.plane_centre[i];
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    f.box_c1[i] = (T)(s.box_centre[i] - 0.5 * s.box_size[i]);
+    f.box_c2[i] = (T)(s.box_centre[i] + 0.5 * s.box_size[i]);
+    f.ibox_c1[i] = (T)(s.inv_box_centre[i] - 0.5 * s.inv_box_size[i]);
+    f.ibox_c2[i] = (T)(s.inv_box_centre[i] + 0.5 * s.inv_box_size[i]);
+    f.sph_c[i] = (T)s.sphere_centre[i];
+    f.isph_c[i] = (T)s.inv_sphere_centre[i];
+  }
+  f.sph_r = (T)s.sphere_radius;
+  f.isph_r = (T)s.inv_sphere_radius;
+  f.water_height = (T)s.water_height;
+  f.water_amplitude = (T)s.water_amplitude;
+  f.water_length = (T)s.water_length;
+  Rot wr;
+  wr.rz(s.water_rotation / 180 * kPi);
+  f.water_rotated = !wr.identity;
+  for (int i = 0; i < 9; i++) f.water_rot[i] = (T)wr.m[i];
+  f.water_phase = (T)(s.water_anim_speed * s.frame_no);
+}
+
+template <class T>
+void fill_shade(ShadeK<T> &k, const mbr_scene &s, const double light[3], double resolution) {
+  std::memset(&k, 0, sizeof k);
+  k.quality = (T)s.quality;
+  k.DE_factor = (T)s.DE_factor;
+  k.smoothness = (T)s.smoothness;
+  k.resolution = (T)resolution;
+  k.persp = (T)s.persp;
+  k.view_distance_max = (T)s.view_distance_max;
+  for (int i = 0; i < 3; i++) k.light[i] = (T)light[i];
+  k.main_light[0] = (T)(s.main_light_intensity * s.main_light_colour.R / 65536.0);
+  k.main_light[1] = (T)(s.main_light_intensity * s.main_light_colour.G / 65536.0);
+  k.main_light[2] = (T)(s.main_light_intensity * s.main_light_colour.B / 65536.0);
+  k.direct_light = (T)s.direct_light;
+  k.shading = (T)s.shading;
+  k.specular = (T)s.specular;
+  k.global_ilum = (T)s.global_ilum;
+  k.fast_ao_tune = (T)s.fast_ao_tune;
+  k.soft_range = (T)std::tan(s.shadow_cone_angle / 180.0 * kPi);
+  k.coloring_speed = (T)s.coloring_speed;
+  k.palette_offset = (T)s.palette_offset;
+  k.fog_visibility = (T)std::pow(10.0, s.fog_visibility / 10 - 16.0);
+  k.fog_density = (T)s.fog_density;
+  k.fog_c1_dist = (T)s.fog_colour1_distance;
+  k.fog_c2_dist = (T)s.fog_colour2_distance;
+  k.fog_reduce = (T)s.fog_distance_factor;
+  k.glow_intensity = (T)s.glow_intensity;
+}
+
+void rgb3(int *o, const mbr_rgb &c) { o[0] = c.R; o[1] = c.G; o[2] = c.B; }
+
+// AO direction table (src/Render3D.cpp:271-313); NULL lightmap = the white
+// 100x100 fallback texture (src/texture.cpp:58-74).
+int ao_table(int AO_quality, const mbr_texture *lm, std::vector<double> &dirs, std::vector<int> &rgb) {
+  dirs.clear();
+  rgb.clear();
+  const int W = lm ? lm->width : 100, H = lm ? lm->height : 100;
+  int counter = 0;
+  for (double b = -0.49 * kPi; b < 0.49 * kPi; b += 1.0 / AO_quality) {
+    for (double a = 0.0; a < 2.0 * kPi; a += ((2.0 / AO_quality) / std::cos(b))) {
+      int X = (int)((a + b) / (2.0 * kPi) * W + W * 8.5) % W;
+      int Y = (int)(b / (kPi)*H + H * 8.5) % H;
+      int R = 255, G = 255, B = 255;
+      if (lm) {
+        const uint8_t *px = lm->rgb + 3 * ((size_t)X + (size_t)Y * W);
+        R = px[0]; G = px[1]; B = px[2];
+      }
+      if (R > 10 || G > 10 || B > 10) {
+        dirs.push_back(std::cos(a + b) * std::cos(b));
+        dirs.push_back(std::sin(a + b) * std::cos(b));
+        dirs.push_back(std::sin(b));
+        dirs.push_back(0.0);
+        rgb.push_back(R); rgb.push_back(G); rgb.push_back(B); rgb.push_back(0);
+        counter++;
+      }
+      if (counter >= 10000) break;
+    }
+    if (counter >= 10000) break;
+  }
+  if (counter == 0) {
+    counter = 1;
+    dirs.assign(4, 0.0);
+    rgb.assign(4, 0);
+  }
+  return counter;
+}
+
+}  // namespace
+
+struct mbr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool has_scene = false;
+  mbr_scene scene;
+  DevScene host_sc;
+  DevScene *d_sc = nullptr;
+  float *d_ao_f = nullptr;
+  double *d_ao_d = nullptr;
+  int *d_ao_rgb = nullptr;
+  int ao_count = 0;
+  BandBufs band{};
+  int band_prec = -1;
+  unsigned long long *d_stats = nullptr;
+  hipEvent_t ev[8] = {};
+  // scratch planes of mbr_render (host-buffer entry point)
+  void *d_planes = nullptr;
+  size_t d_planes_bytes = 0;
+
+  int fail(int rc, const std::string &msg) {
+    err = msg;
+    return rc;
+  }
+  int hip_fail(hipError_t e, const char *what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? MBR_ERR_OUT_OF_MEMORY : MBR_ERR_HIP;
+  }
+};
+
+#define HIPCHECK(ctx, expr)                                 \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return (ctx)->hip_fail(e_, #expr); \
+  } while (0)
+
+namespace {
+
+void free_band(mbr_ctx *c) {
+  BandBufs &b = c->band;
+  void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
+                  b.rec_thr, b.ovf_off, b.hit_list, b.ovf_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
+                  b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  b = BandBufs{};
+  c->band_prec = -1;
+}
+
+int alloc_band(mbr_ctx *c, int precision) {
+  if (c->band_prec == precision) return MBR_OK;
+  free_band(c);
+  BandBufs &b = c->band;
+  b.cap = kBandRays;
+  b.slab = kSlab;
+  b.rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
+  const size_t n = (size_t)b.cap;
+  HIPCHECK(c, hipMalloc((void **)&b.found, n));
+  HIPCHECK(c, hipMalloc((void **)&b.object, n));
+  HIPCHECK(c, hipMalloc((void **)&b.count, n * sizeof(int)));
+  HIPCHECK(c, hipMalloc((void **)&b.depth, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.thr, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.last, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.px, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.py, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.pz, n * sizeof(double)));
+  const size_t rec = n * (size_t)b.slab * b.rec_bytes;
+  HIPCHECK(c, hipMalloc(&b.rec_step, rec));
+  HIPCHECK(c, hipMalloc(&b.rec_dist, rec));
+  HIPCHECK(c, hipMalloc(&b.rec_thr, rec));
+  HIPCHECK(c, hipMalloc((void **)&b.ovf_off, n * sizeof(long long)));
+  HIPCHECK(c, hipMalloc((void **)&b.hit_list, n * sizeof(int)));
+  HIPCHECK(c, hipMalloc((void **)&b.ovf_list, n * sizeof(int)));
+  HIPCHECK(c, hipMalloc((void **)&b.counters, 4 * sizeof(unsigned long long)));
+  HIPCHECK(c, hipMalloc(&b.sh_r, n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc(&b.sh_g, n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc(&b.sh_b, n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc((void **)&b.col_r, n));
+  HIPCHECK(c, hipMalloc((void **)&b.col_g, n));
+  HIPCHECK(c, hipMalloc((void **)&b.col_b, n));
+  b.ovf_capacity = 0;
+  c->band_prec = precision;
+  return MBR_OK;
+}
+
+int check_supported(mbr_ctx *c, const mbr_scene &s) {
+  if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
+  if (s.formula != MBR_FORMULA_TRIG_OPTIM && s.formula != MBR_FORMULA_MENGER_SPONGE && s.formula != MBR_FORMULA_TGLAD)
+    return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
+                                            " is not in the render core (supported: 2 trig_optim, 7 menger_sponge, "
+                                            "8 tglad)");
+  if (s.ifs_folding_mode) return c->fail(MBR_ERR_UNSUPPORTED, "IFS folding mode is not in the render core");
+  if (s.reflect > 0.0 && !s.raytraced_reflections)
+    return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
+  if (s.raytraced_reflections && s.reflections_max > 0)
+    return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections are not in the render core");
+  if (s.aux_light_number > 0) return c->fail(MBR_ERR_UNSUPPORTED, "random aux lights are not in the render core");
+  for (int i = 0; i < 4; i++)
+    if (s.aux_light_pre_enabled[i]) return c->fail(MBR_ERR_UNSUPPORTED, "predefined aux lights are not in the render core");
+  for (int i = 0; i < 5; i++)
+    if (s.volumetric_light_enabled[i])
+      return c->fail(MBR_ERR_UNSUPPORTED, "volumetric lights are not in the render core");
+  if (s.iter_fog_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "iteration fog is not in the render core");
+  if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
+  if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
+  if (s.image_width <= 0 || s.image_height <= 0 || s.N <= 0 || s.N > 100000)
+    return c->fail(MBR_ERR_INVALID_ARG, "bad image size or N");
+  return MBR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbr_version(void) { return MBR_ABI_VERSION; }
+
+size_t mbr_scene_layout(size_t offsets[8]) {
+  if (offsets) {
+    offsets[0] = offsetof(mbr_scene, abi_version);
+    offsets[1] = offsetof(mbr_scene, vp);
+    offsets[2] = offsetof(mbr_scene, formula);
+    offsets[3] = offsetof(mbr_scene, mb_rotation);
+    offsets[4] = offsetof(mbr_scene, palette);
+    offsets[5] = offsetof(mbr_scene, fog_colour3);
+    offsets[6] = offsetof(mbr_scene, reflections_max);
+    offsets[7] = offsetof(mbr_scene, jitter_seed);
+  }
+  return sizeof(mbr_scene);
+}
+
+uint32_t mbr_jitter_draw(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t step) {
+  return mbr::jitter_draw(seed, frame, pixel, step);
+}
+
+int mbr_init(const mbr_device_opts *opts, mbr_ctx **out) {
+  if (!out) return MBR_ERR_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MBR_ERR_NO_DEVICE;
+  mbr_ctx *c = new mbr_ctx();
+  c->device = opts ? opts->device : 0;
+  if (c->device < 0 || c->device >= ndev) {
+    delete c;
+    return MBR_ERR_NO_DEVICE;
+  }
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void **)&c->d_sc, sizeof(DevScene)) != hipSuccess ||
+      hipMalloc((void **)&c->d_stats, ST_COUNT * sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return MBR_ERR_HIP;
+  }
+  for (hipEvent_t &e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return MBR_OK;
+}
+
+void mbr_destroy(mbr_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_band(c);
+  void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *mbr_last_error(const mbr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap) {
+  if (!c || !scene_in) return MBR_ERR_INVALID_ARG;
+  mbr_scene s = *scene_in;
+  int rc = check_supported(c, s);
+  if (rc != MBR_OK) return rc;
+  // CheckPrameters (src/interface.cpp:3587-3602)
+  if (s.zoom <= 1e-15) s.zoom = 1e-15;
+  if (s.DE_factor <= 0.0001) s.DE_factor = 0.0001;
+  if (s.smoothness <= 0.00001) s.smoothness = 0.00001;
+  if (s.persp <= 0.001) s.persp = 0.001;
+  if (s.ao_quality >= 30) s.ao_quality = 30;
+  if (s.ao_quality <= 1) s.ao_quality = 1;
+  if (s.tiles < 1) s.tiles = 1;
+  if (s.tiles > 100) s.tiles = 100;
+  if (s.fog_density < 0.0) s.fog_density = 0.0;
+  if (lightmap && (!lightmap->rgb || lightmap->width <= 0 || lightmap->height <= 0))
+    return c->fail(MBR_ERR_INVALID_ARG, "bad lightmap texture");
+
+  DevScene &d = c->host_sc;
+  std::memset(&d, 0, sizeof d);
+  d.width = s.image_width;
+  d.height = s.image_height;
+  d.tiles = s.tiles;
+  d.tile_index = s.tile_index;
+  d.persp_type = s.perspective_type;
+  d.fish_eye_cut = s.fish_eye_cut;
+  d.frame_no = s.frame_no;
+  d.jitter_mode = s.jitter_mode;
+  d.jitter_seed = s.jitter_seed;
+  d.pixel_base = (uint32_t)((long long)s.tile_index * s.image_width * s.image_height);
+  d.aspect = (double)s.image_width / s.image_height;
+  d.fov = s.persp;
+  d.tile_x = (double)(s.tile_index % s.tiles);
+  d.tile_y = (double)(s.tile_index / s.tiles);
+  // camera (src/Render3D.cpp:236-256, :325-333)
+  Rot R;
+  R.rz(s.alpha);
+  R.rx(s.beta);
+  R.ry(s.gamma);
+  std::memcpy(d.R, R.m, sizeof d.R);
+  d.R_identity = R.identity;
+  const double baseY0[3] = {0.0, 1.0, 0.0};
+  double baseY[3];
+  R.apply(baseY0, baseY);
+  const double lv[3] = {std::cos(s.main_light_alpha - 0.5 * kPi) * std::cos(-s.main_light_beta),
+                        std::sin(s.main_light_alpha - 0.5 * kPi) * std::cos(-s.main_light_beta),
+                        std::sin(-s.main_light_beta)};
+  double light[3];
+  R.apply(lv, light);
+  if (s.perspective_type == MBR_PERSP_FISH_EYE || s.perspective_type == MBR_PERSP_EQUIRECTANGULAR) {
+    for (int i = 0; i < 3; i++) d.start[i] = s.vp[i];
+  } else {
+    const double f = 1.0 / s.persp * s.zoom;
+    for (int i = 0; i < 3; i++) d.start[i] = s.vp[i] - baseY[i] * f;
+  }
+  d.view_distance_min = s.view_distance_min;
+  d.view_distance_max = s.view_distance_max;
+  const double resolution = 1.0 / s.image_width / s.tiles;  // src/interface.cpp:695
+
+  d.constant_thresh = s.constant_DE_threshold;
+  d.interior = s.interior_mode;
+  d.shadow = s.shadow;
+  d.penetrating = s.penetrating_lights;
+  d.ao = s.global_illumination;
+  d.fast_ao = s.fast_global_illumination;
+  d.ao_quality = s.ao_quality;
+  d.slow_shading = s.slow_shading;
+  d.coloring = s.coloring_enabled;
+  d.bsoft = (!s.limits_enabled && !s.iter_thresh) && std::tan(s.shadow_cone_angle / 180.0 * kPi) > 0.0;
+  d.post_fog = s.post_fog_enabled;
+  d.N = s.N;
+  d.minN = s.minN;
+  rgb3(d.main_light_colour, s.main_light_colour);
+  rgb3(d.glow1, s.glow_color1);
+  rgb3(d.glow2, s.glow_color2);
+  rgb3(d.bg1, s.background_color1);
+  rgb3(d.bg2, s.background_color2);
+  rgb3(d.bg3, s.background_color3);
+  rgb3(d.post_fog_colour, s.post_fog_color);
+  rgb3(d.fog1, s.fog_colour1);
+  rgb3(d.fog2, s.fog_colour2);
+  rgb3(d.fog3, s.fog_colour3);
+  rgb3(d.plane_col, s.plane_colour);
+  rgb3(d.box_col, s.box_colour);
+  rgb3(d.ibox_col, s.inv_box_colour);
+  rgb3(d.sphere_col, s.sphere_colour);
+  rgb3(d.isphere_col, s.inv_sphere_colour);
+  rgb3(d.water_col, s.water_colour);
+  for (int i = 0; i < 256; i++) rgb3(d.palette[i], s.palette[i]);
+  fill_fractal(d.fd, s);
+  fill_fractal(d.ff, s);
+  fill_shade(d.sd, s, light, resolution);
+  fill_shade(d.sf, s, light, resolution);
+
+  std::vector<double> dirs;
+  std::vector<int> rgb;
+  const int count = ao_table(s.ao_quality, lightmap, dirs, rgb);
+  d.ao_count = count;
+
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (count > c->ao_count) {
+    if (c->d_ao_f) (void)hipFree(c->d_ao_f);
+    if (c->d_ao_d) (void)hipFree(c->d_ao_d);
+    if (c->d_ao_rgb) (void)hipFree(c->d_ao_rgb);
+    c->d_ao_f = nullptr; c->d_ao_d = nullptr; c->d_ao_rgb = nullptr;
+    HIPCHECK(c, hipMalloc((void **)&c->d_ao_f, (size_t)count * 4 * sizeof(float)));
+    HIPCHECK(c, hipMalloc((void **)&c->d_ao_d, (size_t)count * 4 * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&c->d_ao_rgb, (size_t)count * 4 * sizeof(int)));
+  }
+  c->ao_count = std::max(c->ao_count, count);
+  std::vector<float> dirs_f(dirs.begin(), dirs.end());
+  HIPCHECK(c, hipMemcpyAsync(c->d_ao_f, dirs_f.data(), dirs_f.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_ao_d, dirs.data(), dirs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_ao_rgb, rgb.data(), rgb.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_sc, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->scene = s;
+  c->has_scene = true;
+  return MBR_OK;
+}
+
+int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr_stats *stats) {
+  if (!c || !fb) return MBR_ERR_INVALID_ARG;
+  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
+  const mbr_scene &s = c->scene;
+  if (fb->width != s.image_width || fb->height != s.image_height)
+    return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
+  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP)
+    return c->fail(MBR_ERR_INVALID_ARG, "unknown precision");
+  int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
+  if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
+  if (row0 < 0) row0 = 0;
+  if (row0 >= row1) return c->fail(MBR_ERR_INVALID_ARG, "empty row range");
+  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = alloc_band(c, precision);
+  if (rc != MBR_OK) return rc;
+  BandBufs &b = c->band;
+
+  FrameOut out;
+  for (int i = 0; i < 3; i++) {
+    out.rgb[i] = fb->rgb[i];
+    out.colour[i] = fb->colour[i];
+  }
+  out.z = fb->z;
+  out.alpha = fb->alpha;
+  out.opacity = fb->opacity;
+  AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
+
+  HIPCHECK(c, hipMemsetAsync(c->d_stats, 0, ST_COUNT * sizeof(unsigned long long), st));
+  double ms_march = 0, ms_shade = 0, ms_finish = 0;
+  unsigned long long overflow_rays = 0;
+  uint64_t n_march = 0, n_shade = 0, n_finish = 0;
+  const long long first = (long long)row0 * s.image_width;
+  const long long last = (long long)row1 * s.image_width;
+  for (long long band_first = first; band_first < last; band_first += b.cap) {
+    if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
+    const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
+    HIPCHECK(c, hipMemsetAsync(b.counters, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHECK(c, hipEventRecord(c->ev[0], st));
+    HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, nullptr, 0, c->d_stats, st));
+    n_march++;
+    unsigned long long counters[4];
+    HIPCHECK(c, hipMemcpyAsync(counters, b.counters, sizeof counters, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    const int n_hits = (int)counters[0];
+    const int n_ovf = (int)counters[1];
+    const long long ovf_total = (long long)counters[2];
+    if (n_ovf > 0) {
+      // step records longer than the slab: re-march those rays (same
+      // kernel, same arithmetic) into the overflow runs k_march reserved
+      if (ovf_total > b.ovf_capacity) {
+        if (b.ovf_step) (void)hipFree(b.ovf_step);
+        if (b.ovf_dist) (void)hipFree(b.ovf_dist);
+        if (b.ovf_thr) (void)hipFree(b.ovf_thr);
+        b.ovf_step = b.ovf_dist = b.ovf_thr = nullptr;
+        b.ovf_capacity = 0;
+        const size_t bytes = (size_t)ovf_total * b.rec_bytes;
+        HIPCHECK(c, hipMalloc(&b.ovf_step, bytes));
+        HIPCHECK(c, hipMalloc(&b.ovf_dist, bytes));
+        HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
+        b.ovf_capacity = ovf_total;
+      }
+      HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, b.ovf_list, n_ovf, c->d_stats, st));
+      n_march++;
+      overflow_rays += n_ovf;
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[1], st));
+    HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, n_hits, ao, c->d_stats, st));
+    if (n_hits > 0) n_shade++;
+    HIPCHECK(c, hipEventRecord(c->ev[2], st));
+    HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));
+    n_finish++;
+    HIPCHECK(c, hipEventRecord(c->ev[3], st));
+    HIPCHECK(c, hipEventSynchronize(c->ev[3]));
+    float t0 = 0, t1 = 0, t2 = 0;
+    HIPCHECK(c, hipEventElapsedTime(&t0, c->ev[0], c->ev[1]));
+    HIPCHECK(c, hipEventElapsedTime(&t1, c->ev[1], c->ev[2]));
+    HIPCHECK(c, hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+    ms_march += t0;
+    ms_shade += t1;
+    ms_finish += t2;
+  }
+  if (stats) {
+    unsigned long long v[ST_COUNT];
+    HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_march = ms_march;
+    stats->ms_shade = ms_shade;
+    stats->ms_finish = ms_finish;
+    stats->ms_total = ms_march + ms_shade + ms_finish;
+    stats->rays = (uint64_t)(last - first);
+    stats->hits = v[ST_HITS];
+    stats->march_de_calls = v[ST_MARCH_DE];
+    stats->shade_de_calls = v[ST_SHADE_DE];
+    stats->iterations = v[ST_MARCH_ITERS] + v[ST_SHADE_ITERS];
+    stats->march_iterations = v[ST_MARCH_ITERS];
+    stats->shade_iterations = v[ST_SHADE_ITERS];
+    stats->march_launches = n_march;
+    stats->shade_launches = n_shade;
+    stats->finish_launches = n_finish;
+    stats->missed_de = v[ST_MISSED];
+    stats->step_overflow_rays = overflow_rays;
+    stats->fp64_redo_rays = v[ST_REDO];
+  }
+  return MBR_OK;
+}
+
+int mbr_render(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr_stats *stats) {
+  if (!c || !fb) return MBR_ERR_INVALID_ARG;
+  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
+  const size_t n = (size_t)fb->width * fb->height;
+  // planes: rgb 3 x f32, z f32, colour 3 x u8, alpha u16, opacity u16
+  const size_t bytes = n * (4 * 4 + 3 + 2 * 2) + 64 * 9;
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (bytes > c->d_planes_bytes) {
+    if (c->d_planes) (void)hipFree(c->d_planes);
+    c->d_planes = nullptr;
+    HIPCHECK(c, hipMalloc(&c->d_planes, bytes));
+    c->d_planes_bytes = bytes;
+  }
+  char *base = (char *)c->d_planes;
+  size_t off = 0;
+  auto take = [&](size_t sz) { void *p = base + off; off += (sz + 63) / 64 * 64; return p; };
+  mbr_framebuffer dfb;
+  dfb.width = fb->width;
+  dfb.height = fb->height;
+  for (int i = 0; i < 3; i++) dfb.rgb[i] = fb->rgb[i] ? (float *)take(n * 4) : nullptr;
+  dfb.z = fb->z ? (float *)take(n * 4) : nullptr;
+  for (int i = 0; i < 3; i++) dfb.colour[i] = fb->colour[i] ? (uint8_t *)take(n) : nullptr;
+  dfb.alpha = fb->alpha ? (uint16_t *)take(n * 2) : nullptr;
+  dfb.opacity = fb->opacity ? (uint16_t *)take(n * 2) : nullptr;
+  int rc = mbr_render_device(c, opts, &dfb, stats);
+  if (rc != MBR_OK) return rc;
+  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
+  if (row1 <= 0 || row1 > fb->height) row1 = fb->height;
+  if (row0 < 0) row0 = 0;
+  const size_t o0 = (size_t)row0 * fb->width, cnt = (size_t)(row1 - row0) * fb->width;
+  for (int i = 0; i < 3; i++) {
+    if (fb->rgb[i]) HIPCHECK(c, hipMemcpyAsync(fb->rgb[i] + o0, dfb.rgb[i] + o0, cnt * 4, hipMemcpyDeviceToHost, st));
+    if (fb->colour[i])
+      HIPCHECK(c, hipMemcpyAsync(fb->colour[i] + o0, dfb.colour[i] + o0, cnt, hipMemcpyDeviceToHost, st));
+  }
+  if (fb->z) HIPCHECK(c, hipMemcpyAsync(fb->z + o0, dfb.z + o0, cnt * 4, hipMemcpyDeviceToHost, st));
+  if (fb->alpha) HIPCHECK(c, hipMemcpyAsync(fb->alpha + o0, dfb.alpha + o0, cnt * 2, hipMemcpyDeviceToHost, st));
+  if (fb->opacity) HIPCHECK(c, hipMemcpyAsync(fb->opacity + o0, dfb.opacity + o0, cnt * 2, hipMemcpyDeviceToHost, st));
+  HIPCHECK(c, hipStreamSynchronize(st));
+  return MBR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// mbr_kernels.hpp — launch interface between the host API and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mbr_scene_dev.hpp"
+
+namespace mbr {
+
+// Counters accumulated on the device (one atomic add per workgroup).
+enum StatIdx {
+  ST_MARCH_DE = 0, ST_SHADE_DE, ST_MARCH_ITERS, ST_SHADE_ITERS, ST_HITS, ST_MISSED, ST_REDO,
+  ST_COUNT = 8
+};
+
+// Per-band device buffers.  A band is a run of consecutive pixels of the
+// image (row-major), at most `cap` rays.  Step records are step-major
+// structure-of-arrays: record i of ray r lives at [i * cap + r], so the
+// march writes of one wave are coalesced.
+struct BandBufs {
+  int cap;          // rays per band
+  int slab;         // step records kept per ray in the slab
+  int rec_bytes;    // 4 (FP32 records) or 8 (FP64 records)
+  // per ray
+  uint8_t *found, *object;
+  int *count;       // step record length (buffCount)
+  double *depth, *thr, *last, *px, *py, *pz;
+  void *rec_step, *rec_dist, *rec_thr;   // slab planes [slab][cap]
+  long long *ovf_off;                    // per ray: offset into overflow planes or -1
+  // compaction lists
+  int *hit_list, *ovf_list;
+  unsigned long long *counters;          // [0] hits, [1] overflow rays, [2] overflow records
+  // overflow records (sized on demand)
+  void *ovf_step, *ovf_dist, *ovf_thr;
+  long long ovf_capacity;
+  // shade output per ray
+  void *sh_r, *sh_g, *sh_b;
+  uint8_t *col_r, *col_g, *col_b;
+};
+
+struct FrameOut {
+  float *rgb[3];
+  float *z;
+  uint8_t *colour[3];
+  uint16_t *alpha, *opacity;
+};
+
+struct AODev {
+  const float *dir_f;
+  const double *dir_d;
+  const int *rgb;
+  int count;
+};
+
+// precision: 0 = FP64, 1 = FP32 march/shade with FP64 fixup
+hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
+                        const int *list, int n_list, unsigned long long *stats, hipStream_t s);
+hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits,
+                        AODev ao, unsigned long long *stats, hipStream_t s);
+hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
+                         FrameOut out, hipStream_t s);
+
+}  // namespace mbr

@@ -1,0 +1,99 @@
+// mbr_scene_dev.hpp — the device-resident scene block.
+//
+// The host derives it once per mbr_set_scene from the FP64 mbr_scene POD:
+// formula constants, rotation matrices, camera, light vector and shading
+// constants, each in FP64 and FP32.  Kernels read it through a
+// `const DevScene* __restrict__` whose every access is wave-uniform, so the
+// compiler fetches it with scalar loads into SGPRs: the formula constants
+// never occupy vector registers or LDS.
+#pragma once
+
+#include <stdint.h>
+
+namespace mbr {
+
+// Formula and CalculateDistance constants (src/fractal.h:102-293).
+template <class T>
+struct FractalK {
+  int formula, N, minN;
+  int julia_mode, tglad_fold, sph_fold, mb_rot_on, mb_main_rot;
+  int interior, iter_thresh, limits, only_plane;
+  int prim_plane, prim_box, prim_ibox, prim_sphere, prim_isphere, prim_water;
+  int water_iterations, frame_no, water_rotated, pad0;
+  T power, bailout, constant_factor;
+  T julia[3];
+  T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;
+  T mb_scale, mb_abs_scale, mb_limit, mb_value, mb_mR2, mb_fR2, mb_factor1;
+  T mb_offset[3];
+  T mb_col_R, mb_col_X, mb_col_Y, mb_col_Z, mb_col_Sp1, mb_col_Sp2;
+  T mb_main[9];
+  T mb_rot[2][3][9];
+  T mb_rotinv[2][3][9];
+  T limit_min[3], limit_max[3];
+  T plane_c[3], plane_n[3];
+  T box_c1[3], box_c2[3], ibox_c1[3], ibox_c2[3];
+  T sph_c[3], sph_r, isph_c[3], isph_r;
+  T water_height, water_amplitude, water_length, water_rot[9], water_phase;
+};
+
+// Shading constants (sParamRender fields used by src/shaders.cpp).
+template <class T>
+struct ShadeK {
+  T quality, DE_factor, smoothness, resolution, persp, view_distance_max;
+  T light[3];           // shadow vector, MainThread src/Render3D.cpp:252-256
+  T main_light[3];      // mainLightIntensity * colour / 65536 (ObjectShader :28-30)
+  T direct_light, shading, specular, global_ilum, fast_ao_tune;
+  T soft_range;         // tan(shadowConeAngle)
+  T coloring_speed, palette_offset;
+  T fog_visibility;     // pow(10, post_fog_visibility/10 - 16)
+  T fog_density, fog_c1_dist, fog_c2_dist, fog_reduce;
+  T glow_intensity;
+};
+
+struct DevScene {
+  // image / camera (FP64: ray origins and scan stay FP64 in every path)
+  int width, height, tiles, tile_index;
+  int persp_type, fish_eye_cut, frame_no, jitter_mode;
+  uint32_t jitter_seed, pixel_base;  // pixel index offset of this tile
+  double aspect, fov, tile_x, tile_y;
+  double R[9];
+  int R_identity, pad1;
+  double start[3];
+  double view_distance_min, view_distance_max;
+
+  // switches
+  int constant_thresh, interior, shadow, penetrating;
+  int ao, fast_ao, ao_quality, ao_count;
+  int slow_shading, coloring, bsoft, post_fog;
+  int N, minN;
+
+  // integer colours (sRGB, 16-bit channel values)
+  int main_light_colour[3];
+  int glow1[3], glow2[3];
+  int bg1[3], bg2[3], bg3[3];
+  int post_fog_colour[3];
+  int fog1[3], fog2[3], fog3[3];
+  int plane_col[3], box_col[3], ibox_col[3], sphere_col[3], isphere_col[3], water_col[3];
+  int palette[256][3];
+
+  FractalK<double> fd;
+  FractalK<float> ff;
+  ShadeK<double> sd;
+  ShadeK<float> sf;
+
+  template <class T>
+  __host__ __device__ const FractalK<T> &frac() const;
+  template <class T>
+  __host__ __device__ const ShadeK<T> &shade() const;
+};
+
+template <>
+__host__ __device__ inline const FractalK<double> &DevScene::frac<double>() const { return fd; }
+template <>
+__host__ __device__ inline const FractalK<float> &DevScene::frac<float>() const { return ff; }
+template <>
+__host__ __device__ inline const ShadeK<double> &DevScene::shade<double>() const { return sd; }
+template <>
+__host__ __device__ inline const ShadeK<float> &DevScene::shade<float>() const { return sf; }
+
+}  // namespace mbr

@@ -1,0 +1,370 @@
+// mbr_shade.hpp — shaders of the render core, templated on arithmetic type.
+//
+//   normals<T>            CalculateNormals       src/shaders.cpp:675-757
+//   main_shadow<T>        MainShadow             src/shaders.cpp:480-570
+//   fast_ao2<T>           FastAmbientOcclusion2  src/shaders.cpp:585-605
+//   ambient_occlusion<T>  AmbientOcclusion       src/shaders.cpp:607-673
+//   surface_colour<T>     SurfaceColour          src/shaders.cpp:813-926
+//   index_to_colour       IndexToColour          src/shaders.cpp:1273-1305
+//   object_shader<T>      ObjectShader           src/shaders.cpp:21-112
+//   background<T>         BackgroundShader       src/shaders.cpp:114-183
+//   volumetric<T>         VolumetricShader       src/shaders.cpp:185-478
+//
+// Features outside the render core (env mapping, aux/fake/volumetric lights,
+// iteration fog) are rejected at mbr_set_scene, so their terms are zero here.
+#pragma once
+
+#include "mbr_fractal.hpp"
+
+namespace mbr {
+
+// Upper bound on the DE steps of one shadow or AO ray.  The reference loops
+// have no bound (src/shaders.cpp:508, :625); a ray that reaches this many
+// steps without terminating is ended as unoccluded, so every lane finishes.
+constexpr int kMaxShadeSteps = 20000;
+
+struct LaneCount {
+  uint32_t de_calls;
+  uint32_t iterations;
+  MBR_HD void add(const DEInfo &i) {
+    de_calls += 1;
+    iterations += (uint32_t)i.work;
+  }
+};
+
+template <class T>
+struct ShaderIn {
+  V3<T> point, view, normal, light;
+  T dist_thresh, lastDist, delta, depth;
+  T detail;  // calcParam->doubles.detailSize as the reference mutates it
+  int object;
+};
+
+template <class T>
+struct RGBT {
+  T R, G, B;
+};
+
+template <class T>
+MBR_HD T de(const DevScene &sc, V3<T> p, T detail, int N, int minN, bool normalMode, DEInfo &info,
+            LaneCount &cnt) {
+  T d = calc_distance<T>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
+  cnt.add(info);
+  return d;
+}
+
+template <class T>
+MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  V3<T> n = v3<T>(0, 0, 0);
+  DEInfo info;
+  const int N5 = sc.N * 5;
+  if (!sc.slow_shading) {
+    T delta = in.delta * k.smoothness;
+    if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
+    T s1 = de<T>(sc, in.point, in.detail, N5, 0, true, info, cnt);
+    T s2 = de<T>(sc, in.point + v3<T>(delta, 0, 0), in.detail, N5, 0, true, info, cnt);
+    T s3 = de<T>(sc, in.point + v3<T>(0, delta, 0), in.detail, N5, 0, true, info, cnt);
+    T s4 = de<T>(sc, in.point + v3<T>(0, 0, delta), in.detail, N5, 0, true, info, cnt);
+    n = v3<T>(s2 - s1, s3 - s1, s4 - s1);
+  } else {
+    T delta = in.delta * k.smoothness;
+    if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
+    // the 11^3 stencil with the reference's accumulated loop variables
+    for (double px = -1.0; px <= 1.0; px += 0.2)
+      for (double py = -1.0; py <= 1.0; py += 0.2)
+        for (double pz = -1.0; pz <= 1.0; pz += 0.2) {
+          V3<T> p2 = v3<T>((T)px, (T)py, (T)pz);
+          T d = de<T>(sc, in.point + p2 * delta, in.detail, N5, sc.minN, true, info, cnt);
+          n = n + p2 * d;
+        }
+  }
+  if (n.x == 0 && n.y == 0 && n.z == 0) n.x = 1;
+  else n = normalize(n);
+  return n;
+}
+
+template <class T>
+MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  T factor = in.delta / k.resolution;
+  if (!sc.penetrating) factor = k.view_distance_max;
+  T dist = in.dist_thresh;
+  in.detail = in.dist_thresh;
+  const T DE_factor = k.DE_factor;
+  T start = in.dist_thresh;
+  if (sc.interior) start = in.dist_thresh * DE_factor;
+  T shadowTemp = 1;
+  T maxSoft = 0;
+  DEInfo info;
+  int guard = 0;
+  for (T i = start; i < factor && guard < kMaxShadeSteps; i += dist * DE_factor, guard++) {
+    V3<T> point2 = in.point + in.light * i;
+    const float dist_thresh = (float)in.dist_thresh;
+    in.detail = (T)dist_thresh;
+    dist = de<T>(sc, point2, in.detail, sc.N, sc.minN, false, info, cnt);
+    if (sc.bsoft) {
+      T angle = (dist - (T)dist_thresh) / i;
+      if (angle < 0) angle = 0;
+      if (dist < (T)dist_thresh) angle = 0;
+      T softShadow = (T(1) - angle / k.soft_range);
+      if (sc.penetrating) softShadow *= (factor - i) / factor;
+      if (softShadow < 0) softShadow = 0;
+      if (softShadow > maxSoft) maxSoft = softShadow;
+    }
+    if (dist < (T)dist_thresh || shadowTemp < T(0)) {
+      shadowTemp -= (factor - i) / factor;
+      if (!sc.penetrating) shadowTemp = 0;
+      if (shadowTemp < T(0)) shadowTemp = 0;
+      break;
+    }
+  }
+  return sc.bsoft ? (T(1) - maxSoft) : shadowTemp;
+}
+
+template <class T>
+MBR_HD T fast_ao2(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  T delta = in.dist_thresh;
+  T aoTemp = 0;
+  const int q = sc.ao_quality;
+  DEInfo info;
+  T inv2i = T(1);
+  for (int i = 1; i < q * q; i++) {
+    inv2i *= T(0.5);  // 1/pow(2,i), exact
+    T scan = T(i * i) * delta;
+    T dist = de<T>(sc, in.point + in.normal * scan, in.detail, sc.N, sc.minN, false, info, cnt);
+    aoTemp += inv2i * (scan - k.fast_ao_tune * dist) / in.dist_thresh;
+  }
+  T ao = T(1) - T(0.2) * aoTemp;
+  if (ao < 0) ao = 0;
+  return ao;
+}
+
+template <class T>
+struct AOTable {
+  const T *dir;      // count x 4 (x, y, z, unused)
+  const int *rgb;    // count x 4 (R, G, B, unused)
+  int count;
+};
+
+template <class T>
+MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  RGBT<T> ao = {0, 0, 0};
+  const T start_dist = in.delta;
+  const T end_dist = in.delta / k.resolution;
+  const float dist_thresh = (float)in.dist_thresh;
+  DEInfo info;
+  for (int v = 0; v < tab.count; v++) {
+    const V3<T> dir = v3<T>(tab.dir[4 * v + 0], tab.dir[4 * v + 1], tab.dir[4 * v + 2]);
+    T dist = in.lastDist;
+    T shadowTemp = 1;
+    int guard = 0;
+    for (T r = start_dist; r < end_dist && guard < kMaxShadeSteps; r += dist * T(2), guard++) {
+      dist = de<T>(sc, in.point + dir * r, in.detail, sc.N, sc.minN, false, info, cnt);
+      if (dist < (T)dist_thresh || info.max_iter || shadowTemp < T(0)) {
+        shadowTemp -= (end_dist - r) / end_dist;
+        if (shadowTemp < T(0)) shadowTemp = 0;
+        break;
+      }
+    }
+    ao.R += shadowTemp * T(tab.rgb[4 * v + 0]);
+    ao.G += shadowTemp * T(tab.rgb[4 * v + 1]);
+    ao.B += shadowTemp * T(tab.rgb[4 * v + 2]);
+  }
+  const T den = T(tab.count) * T(256);
+  ao.R /= den;
+  ao.G /= den;
+  ao.B /= den;
+  return ao;
+}
+
+MBR_HD void index_to_colour(int index, const int (*palette)[3], int out[3]) {
+  if (index < 0) {
+    out[0] = palette[255][0]; out[1] = palette[255][1]; out[2] = palette[255][2];
+    return;
+  }
+  out[0] = out[1] = out[2] = 255;
+  index = index % (255 * 256);
+  int kol = index / 256;
+  if (kol < 255) {
+    int delta = index % 256;
+    for (int c = 0; c < 3; c++) {
+      double c1 = palette[kol][c], c2 = palette[kol + 1][c];
+      double ck = (c2 - c1) / 256.0;
+      out[c] = (int)(c1 + ck * delta);
+    }
+  }
+}
+
+template <class T>
+MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCount &cnt) {
+  RGBT<T> out;
+  const int *pc;
+  switch (in.object) {
+    case OBJ_FRACTAL: {
+      int L;
+      const int N10 = sc.N * 10;
+      T cval = compute<T, MODE_COLOURING>(sc.frac<T>(), in.point, N10, &L);
+      cnt.de_calls += 1;
+      cnt.iterations += (uint32_t)((L < N10) ? L + 1 : N10);
+      int nrCol = x86_d2i(floor((double)cval));
+      nrCol = x86_abs(nrCol) % (248 * 256);
+      int colour[3] = {256, 256, 256};
+      if (sc.coloring) {
+        const ShadeK<T> &k = sc.shade<T>();
+        int color_number;
+        if (nrCol >= 248 * 256) color_number = nrCol;
+        else color_number = x86_d2i((double)(T(nrCol) * k.coloring_speed + T(256) * k.palette_offset)) % 65536;
+        index_to_colour(color_number, sc.palette, colour);
+      }
+      out.R = T(colour[0]) / T(256); out.G = T(colour[1]) / T(256); out.B = T(colour[2]) / T(256);
+      return out;
+    }
+    case OBJ_WATER: pc = sc.water_col; break;
+    case OBJ_BOX: pc = sc.box_col; break;
+    case OBJ_BOX_INV: pc = sc.ibox_col; break;
+    case OBJ_SPHERE: pc = sc.sphere_col; break;
+    case OBJ_SPHERE_INV: pc = sc.isphere_col; break;
+    default: pc = sc.plane_col; break;
+  }
+  out.R = T(pc[0]) / T(65536); out.G = T(pc[1]) / T(65536); out.B = T(pc[2]) / T(65536);
+  return out;
+}
+
+// ObjectShader: returns object + specular terms; *colour_out is the surface
+// colour (colourBuffer source).
+template <class T>
+MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, RGBT<T> &colour_out,
+                             LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  in.normal = normals<T>(sc, in, cnt);
+  T sh = dot(in.normal, in.light);
+  if (sh < 0) sh = 0;
+  const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
+  T shadow = 1;
+  if (sc.shadow) shadow = main_shadow<T>(sc, in, cnt);
+  V3<T> half = normalize(in.light - in.view);
+  T shade2 = dot(in.normal, half);
+  if (shade2 < T(0)) shade2 = 0;
+  shade2 = Math<T>::pow(shade2, T(30));
+  if (shade2 > T(15)) shade2 = 15;
+  const T specular = shade2 * k.specular;
+  RGBT<T> colour = surface_colour<T>(sc, in, cnt);
+  colour_out = colour;
+  RGBT<T> ambient = {0, 0, 0};
+  if (sc.ao) {
+    if (sc.fast_ao) {
+      T a = fast_ao2<T>(sc, in, cnt);
+      ambient = RGBT<T>{a, a, a};
+    } else {
+      ambient = ambient_occlusion<T>(sc, in, tab, cnt);
+    }
+  }
+  RGBT<T> out;
+  out.R = (ambient.R * k.global_ilum + k.main_light[0] * shade * shadow) * colour.R + k.main_light[0] * specular * shadow;
+  out.G = (ambient.G * k.global_ilum + k.main_light[1] * shade * shadow) * colour.G + k.main_light[1] * specular * shadow;
+  out.B = (ambient.B * k.global_ilum + k.main_light[2] * shade * shadow) * colour.B + k.main_light[2] * specular * shadow;
+  return out;
+}
+
+template <class T>
+MBR_HD RGBT<T> background(const DevScene &sc, V3<T> view) {
+  T grad = -view.z + T(1);  // viewVector . (0,0,-1) + 1
+  const int *c_lo, *c_hi;
+  if (grad < T(1)) {
+    c_lo = sc.bg3; c_hi = sc.bg2;
+  } else {
+    grad = grad - T(1);
+    c_lo = sc.bg2; c_hi = sc.bg1;
+  }
+  const T Ngrad = T(1) - grad;
+  RGBT<T> out;
+  // sRGB16 storage of the gradient before scaling
+  out.R = T(x86_d2u16((double)(T(c_lo[0]) * Ngrad + T(c_hi[0]) * grad))) / T(65536);
+  out.G = T(x86_d2u16((double)(T(c_lo[1]) * Ngrad + T(c_hi[1]) * grad))) / T(65536);
+  out.B = T(x86_d2u16((double)(T(c_lo[2]) * Ngrad + T(c_hi[2]) * grad))) / T(65536);
+  return out;
+}
+
+// Step record of one ray, read back to front.  rec(i) yields
+// {step, distance, distThresh} of march step i.
+template <class T>
+struct VolOut {
+  RGBT<T> rgb;
+  T alpha;
+  T opacity;
+};
+
+template <class T, class RecFn>
+MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> px, T alpha, RecFn rec) {
+  const ShadeK<T> &k = sc.shade<T>();
+  VolOut<T> o;
+  o.rgb = px;
+  o.alpha = alpha;
+  o.opacity = 0;
+  float totalOpacity = 0.0f;
+  const T glow = T(stepCount) * k.glow_intensity / T(512) * k.DE_factor;
+  T glowN = T(1) - glow;
+  if (glowN < T(0)) glowN = 0;
+  const T glowR = (T(sc.glow1[0]) * glowN + T(sc.glow2[0]) * glow) / T(65536);
+  const T glowG = (T(sc.glow1[1]) * glowN + T(sc.glow2[1]) * glow) / T(65536);
+  const T glowB = (T(sc.glow1[2]) * glowN + T(sc.glow2[2]) * glow) / T(65536);
+  T totalStep = 0;
+  for (int index = stepCount - 1; index > 0; index--) {
+    T step, distance, dist_thresh;
+    rec(index, step, distance, dist_thresh);
+    totalStep += step;
+    if (totalStep < dist_thresh * k.quality) continue;
+    step = totalStep;
+    totalStep = 0;
+
+    const T glowOpacity = glow * step / depth;
+    o.rgb.R = glowOpacity * glowR + (T(1) - glowOpacity) * o.rgb.R;
+    o.rgb.G = glowOpacity * glowG + (T(1) - glowOpacity) * o.rgb.G;
+    o.rgb.B = glowOpacity * glowB + (T(1) - glowOpacity) * o.rgb.B;
+    o.alpha += glowOpacity;
+
+    if (sc.post_fog) {
+      T fd = step / k.fog_visibility;
+      if (fd > T(1)) fd = 1;
+      o.rgb.R = fd * T(sc.post_fog_colour[0]) / T(65536) + (T(1) - fd) * o.rgb.R;
+      o.rgb.G = fd * T(sc.post_fog_colour[1]) / T(65536) + (T(1) - fd) * o.rgb.G;
+      o.rgb.B = fd * T(sc.post_fog_colour[2]) / T(65536) + (T(1) - fd) * o.rgb.B;
+      totalOpacity = (float)(fd + (T(1) - fd) * T(totalOpacity));
+      o.alpha = fd + (T(1) - fd) * o.alpha;
+    }
+
+    if (k.fog_density > T(0)) {
+      const T fogReduce = k.fog_reduce;
+      const T densityTemp = (step * fogReduce) / (distance * distance + fogReduce * fogReduce);
+      T kk = distance / k.fog_c1_dist;
+      if (kk > T(1)) kk = 1;
+      T kn = T(1) - kk;
+      T fR = T(sc.fog1[0]) * kn + T(sc.fog2[0]) * kk;
+      T fG = T(sc.fog1[1]) * kn + T(sc.fog2[1]) * kk;
+      T fB = T(sc.fog1[2]) * kn + T(sc.fog2[2]) * kk;
+      T k2 = distance / k.fog_c2_dist * kk;
+      if (k2 > T(1)) k2 = 1;
+      kn = T(1) - k2;
+      fR = fR * kn + T(sc.fog3[0]) * k2;
+      fG = fG * kn + T(sc.fog3[1]) * k2;
+      fB = fB * kn + T(sc.fog3[2]) * k2;
+      T fogDensity = T(0.3) * k.fog_density * densityTemp / (T(1) + k.fog_density * densityTemp);
+      if (fogDensity > T(1)) fogDensity = 1;
+      o.rgb.R = fogDensity * fR / T(65536) + (T(1) - fogDensity) * o.rgb.R;
+      o.rgb.G = fogDensity * fG / T(65536) + (T(1) - fogDensity) * o.rgb.G;
+      o.rgb.B = fogDensity * fB / T(65536) + (T(1) - fogDensity) * o.rgb.B;
+      totalOpacity = (float)(fogDensity + (T(1) - fogDensity) * T(totalOpacity));
+      o.alpha = fogDensity + (T(1) - fogDensity) * o.alpha;
+    }
+
+    if (totalOpacity > 1.0f) totalOpacity = 1.0f;
+    if (o.alpha > T(1)) o.alpha = 1;
+    o.opacity = T(totalOpacity);
+  }
+  return o;
+}
+
+}  // namespace mbr

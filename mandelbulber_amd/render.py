@@ -1,0 +1,101 @@
+"""Renderer: the binding of mbr_init / mbr_set_scene / mbr_render[_device]."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._lib import check, lib
+
+
+class Frame:
+    """Host planes in cImage layout (src/cimage.hpp:157-162), one array per plane."""
+
+    def __init__(self, w, h):
+        self.w, self.h = w, h
+        self.rgb = np.zeros((3, h, w), np.float32)
+        self.z = np.zeros((h, w), np.float32)
+        self.colour = np.zeros((3, h, w), np.uint8)
+        self.alpha = np.zeros((h, w), np.uint16)
+        self.opacity = np.zeros((h, w), np.uint16)
+
+    def fb(self):
+        f = _abi.Framebuffer()
+        f.width, f.height = self.w, self.h
+        for c in range(3):
+            f.rgb[c] = self.rgb[c].ctypes.data
+            f.colour[c] = self.colour[c].ctypes.data
+        f.z = self.z.ctypes.data
+        f.alpha = self.alpha.ctypes.data
+        f.opacity = self.opacity.ctypes.data
+        return f
+
+
+def device_framebuffer(planes, w, h):
+    """mbr_framebuffer over device tensors: dict with keys rgb (3,h,w) f32,
+    z (h,w) f32, colour (3,h,w) u8, alpha/opacity (h,w) int16 storage."""
+    f = _abi.Framebuffer()
+    f.width, f.height = w, h
+    for c in range(3):
+        f.rgb[c] = planes["rgb"][c].data_ptr() if "rgb" in planes else None
+        f.colour[c] = planes["colour"][c].data_ptr() if "colour" in planes else None
+    f.z = planes["z"].data_ptr() if "z" in planes else None
+    f.alpha = planes["alpha"].data_ptr() if "alpha" in planes else None
+    f.opacity = planes["opacity"].data_ptr() if "opacity" in planes else None
+    return f
+
+
+class Renderer:
+    """One render context (mbr_ctx) on one HIP device."""
+
+    def __init__(self, device=0):
+        self._ctx = C.c_void_p()
+        opts = _abi.DeviceOpts(device)
+        check(lib().mbr_init(C.byref(opts), C.byref(self._ctx)))
+        self.scene = None
+
+    def close(self):
+        if self._ctx:
+            lib().mbr_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scene(self, scene, lightmap=None):
+        tex = None
+        if lightmap is not None:
+            self._lm = np.ascontiguousarray(lightmap, dtype=np.uint8)
+            tex = _abi.Texture(self._lm.shape[1], self._lm.shape[0],
+                               self._lm.ctypes.data_as(C.POINTER(C.c_uint8)))
+        check(lib().mbr_set_scene(self._ctx, C.byref(scene), C.byref(tex) if tex is not None else None),
+              self._ctx)
+        self.scene = scene
+
+    def _opts(self, precision, rows, stream):
+        o = _abi.RenderOpts()
+        o.precision = precision
+        o.stream = stream
+        o.row_begin, o.row_end = rows if rows is not None else (0, 0)
+        return o
+
+    def render(self, precision=_abi.PREC_FP32_FIXUP, rows=None):
+        """Render into host planes; returns (Frame, Stats)."""
+        s = self.scene
+        frame = Frame(s.image_width, s.image_height)
+        fb = frame.fb()
+        st = _abi.Stats()
+        check(lib().mbr_render(self._ctx, C.byref(self._opts(precision, rows, None)), C.byref(fb), C.byref(st)),
+              self._ctx)
+        return frame, st
+
+    def render_device(self, planes, precision=_abi.PREC_FP32_FIXUP, rows=None, stream=None):
+        """Render into device planes (see device_framebuffer); returns Stats."""
+        s = self.scene
+        fb = device_framebuffer(planes, s.image_width, s.image_height)
+        st = _abi.Stats()
+        check(lib().mbr_render_device(self._ctx, C.byref(self._opts(precision, rows, stream)), C.byref(fb),
+                                      C.byref(st)), self._ctx)
+        return st

@@ -1,0 +1,1137 @@
+/*
+ * mbr_oracle.c — CPU FP64 restatement of the Mandelbulber 1.21 render core.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP render core and the
+ * timed CPU baseline.  It is never linked or called by the product.
+ *
+ * It restates, in plain C, the per-pixel path of the reference CPU renderer:
+ *   Compute<normal|colouring>   src/fractal.cpp:40-1241 (formulas 2, 7, 8)
+ *   CalculateDistance           src/fractal.cpp:1245-1448
+ *   primitives                  src/primitives.cpp:15-119
+ *   rotation matrices           src/algebra.cpp:94-171
+ *   RayMarching                 src/Render3D.cpp:79-199
+ *   MainThread per-pixel body   src/Render3D.cpp:236-333, :387-590
+ *   shaders                     src/shaders.cpp:21-925, :1253-1305
+ * in the same order of floating-point operations, with the reference's FP32
+ * truncations (float dist_thresh, float totalOpacity, float framebuffer).
+ *
+ * Pinning (see tests/test_oracle_*.py): closed forms (far-field Mandelbulb
+ * DE, Menger axis points, Mandelbox affine regime, exact primitive SDFs),
+ * invariants (mirror and (p-1)-fold symmetry of the bulb, octahedral
+ * symmetry of box and sponge), textbook reductions (power-2 bulb on z=0 is
+ * the Mandelbrot set; integer-power bulb equals the complex-power form;
+ * Menger membership by base-3 digits), brute force on tiny frames, and the
+ * compiled reference (oracle/_ref: algebra.cpp, common_math.cpp,
+ * primitives.cpp) for camera, projection and primitive conventions.
+ */
+#include "mbr_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ vec3 */
+typedef struct { double x, y, z; } v3;
+
+static inline v3 v3m(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 v3add(v3 a, v3 b) { return v3m(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 v3sub(v3 a, v3 b) { return v3m(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 v3mul(v3 a, double s) { return v3m(a.x * s, a.y * s, a.z * s); }
+static inline double v3len(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline double v3dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* CVector3::Normalize (src/algebra.hpp:104-111): multiply by 1/length */
+static inline v3 v3norm(v3 a) { double n = 1.0 / v3len(a); return v3m(a.x * n, a.y * n, a.z * n); }
+static inline v3 v3from(const double *p) { return v3m(p[0], p[1], p[2]); }
+
+/* ------------------------------------------------------- rotation matrix */
+/* CRotationMatrix (src/algebra.cpp:80-218): row-major m[0..8] = m11..m33,
+ * rotations post-multiplied, each skipped when its angle is 0, and
+ * RotateVector returns its argument unchanged while no rotation was applied. */
+typedef struct { double m[9]; int zero; } rotm;
+
+static void rot_null(rotm *r) {
+  memset(r->m, 0, sizeof r->m);
+  r->m[0] = r->m[4] = r->m[8] = 1.0;
+  r->zero = 1;
+}
+
+static void mat_mul(double *a, const double *b) { /* a = a*b */
+  double c[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      c[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] + a[i * 3 + 2] * b[2 * 3 + j];
+  memcpy(a, c, sizeof c);
+}
+
+static void rot_x(rotm *r, double angle) {
+  if (angle == 0.0) return;
+  double s = sin(angle), c = cos(angle), t[9] = {0};
+  t[0] = 1.0; t[4] = c; t[8] = c; t[5] = -s; t[7] = s;
+  mat_mul(r->m, t); r->zero = 0;
+}
+static void rot_y(rotm *r, double angle) {
+  if (angle == 0.0) return;
+  double s = sin(angle), c = cos(angle), t[9] = {0};
+  t[4] = 1.0; t[8] = c; t[0] = c; t[6] = -s; t[2] = s;
+  mat_mul(r->m, t); r->zero = 0;
+}
+static void rot_z(rotm *r, double angle) {
+  if (angle == 0.0) return;
+  double s = sin(angle), c = cos(angle), t[9] = {0};
+  t[8] = 1.0; t[0] = c; t[4] = c; t[1] = -s; t[3] = s;
+  mat_mul(r->m, t); r->zero = 0;
+}
+static v3 rot_apply(const rotm *r, v3 v) {
+  if (r->zero) return v;
+  const double *m = r->m;
+  return v3m(m[0] * v.x + m[1] * v.y + m[2] * v.z,
+             m[3] * v.x + m[4] * v.y + m[5] * v.z,
+             m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+/* SetRotation(angles[3]) (src/algebra.cpp:145-151): Z(angles[2]) Y(angles[1]) X(angles[0]) */
+static void rot_set3(rotm *r, const double a[3]) { rot_null(r); rot_z(r, a[2]); rot_y(r, a[1]); rot_x(r, a[0]); }
+/* Transpose (src/algebra.cpp:202-217) marks the result as non-identity. */
+static void rot_transpose(const rotm *r, rotm *out) {
+  const double *m = r->m;
+  double t[9] = {m[0], m[3], m[6], m[1], m[4], m[7], m[2], m[5], m[8]};
+  memcpy(out->m, t, sizeof t); out->zero = 0;
+}
+
+/* ------------------------------------------------------------ derived set */
+typedef struct {
+  const mbr_scene *s;
+  rotm mb_main, mb_rot[2][3], mb_rotinv[2][3];
+} fractal_ctx;
+
+static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
+  f->s = s;
+  /* RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555) */
+  rot_set3(&f->mb_main, s->mb_rotation_main);
+  for (int fold = 0; fold < 2; fold++)
+    for (int axis = 0; axis < 3; axis++) {
+      rot_set3(&f->mb_rot[fold][axis], s->mb_rotation[fold][axis]);
+      rot_transpose(&f->mb_rot[fold][axis], &f->mb_rotinv[fold][axis]);
+    }
+}
+
+/* ----------------------------------------------------------------- hash */
+static inline uint32_t mix32(uint32_t x) {
+  /* "lowbias32" integer finaliser (C. Wellons, hash-prospector) */
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+uint32_t oracle_jitter_draw(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t step) {
+  uint32_t h = mix32(seed ^ 0x9e3779b9U);
+  h = mix32(h ^ frame);
+  h = mix32(h ^ pixel);
+  h = mix32(h ^ step);
+  return h % 1001U;
+}
+
+/* double -> int as x86-64 cvttsd2si does it: out-of-range and NaN give
+ * INT_MIN (the reference converts with a plain C cast). */
+static inline int x86_d2i(double d) {
+  if (!(d > -2147483649.0 && d < 2147483648.0)) return INT_MIN;
+  return (int)d;
+}
+static inline int x86_abs(int v) { return v == INT_MIN ? INT_MIN : (v < 0 ? -v : v); }
+/* double -> unsigned short via int, as gcc lowers (unsigned short)double */
+static inline unsigned short x86_d2u16(double d) { return (unsigned short)(unsigned)x86_d2i(d); }
+static inline unsigned char x86_d2u8(double d) { return (unsigned char)(unsigned)x86_d2i(d); }
+
+/* --------------------------------------------------------------- Compute */
+enum { MODE_NORMAL = 0, MODE_COLOURING = 1 };
+
+/* Compute<Mode> (src/fractal.cpp:40-1241) for trig_optim (:211-225),
+ * menger_sponge (:369-408) and tglad (:409-529) with the tglad and
+ * spherical pre-folds (:125-181), termination (:1129-1191) and return
+ * values (:1193-1240). */
+static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_count) {
+  const mbr_scene *s = f->s;
+  int L;
+  double distance = 0;
+  double r_dz = 1;
+  double p = s->power;
+  double fixedRadius = s->mb_fixed_radius;
+  double fR2 = fixedRadius * fixedRadius;
+  double minRadius = s->mb_min_radius;
+  double mR2 = minRadius * minRadius;
+  double tglad_factor1 = fR2 / mR2;
+  double tgladDE = 1.0;
+  double scale = s->mb_scale;
+  double tgladColor = 1.0;
+  const int formula = s->formula;
+  v3 constant = s->julia_mode ? v3from(s->julia) : v3mul(z, s->constant_factor);
+  double r = v3len(z);
+  double min = 1e200;
+
+  for (L = 0; L < N; L++) {
+    if (s->tglad_folding_mode) {
+      double lim = s->folding_limit, val = s->folding_value;
+      if (z.x > lim) { z.x = val - z.x; tgladColor *= 0.9; }
+      else if (z.x < -lim) { z.x = -val - z.x; tgladColor *= 0.9; }
+      if (z.y > lim) { z.y = val - z.y; tgladColor *= 0.9; }
+      else if (z.y < -lim) { z.y = -val - z.y; tgladColor *= 0.9; }
+      if (z.z > lim) { z.z = val - z.z; tgladColor *= 0.9; }
+      else if (z.z < -lim) { z.z = -val - z.z; tgladColor *= 0.9; }
+      r = v3len(z);
+    }
+    if (s->spherical_folding_mode) {
+      double fR2_2 = s->spherical_folding_fixed * s->spherical_folding_fixed;
+      double mR2_2 = s->spherical_folding_min * s->spherical_folding_min;
+      double r2_2 = r * r;
+      double tf1 = fR2_2 / mR2_2;
+      if (r2_2 < mR2_2) { z = v3mul(z, tf1); tgladDE *= tf1; tgladColor += 1; }
+      else if (r2_2 < fR2_2) { double tf2 = fR2_2 / r2_2; z = v3mul(z, tf2); tgladDE *= tf2; tgladColor += 10; }
+      r = v3len(z);
+    }
+
+    switch (formula) {
+      case MBR_FORMULA_TRIG_OPTIM: {
+        double th0 = asin(z.z / r);
+        double ph0 = atan2(z.y, z.x);
+        double rp = pow(r, p - 1.0);
+        double th = th0 * p;
+        double ph = ph0 * p;
+        double cth = cos(th);
+        r_dz = rp * r_dz * p + 1.0;
+        rp *= r;
+        z = v3add(v3mul(v3m(cth * cos(ph), cth * sin(ph), sin(th)), rp), constant);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_MENGER_SPONGE: {
+        double temp;
+        z.x = fabs(z.x); z.y = fabs(z.y); z.z = fabs(z.z);
+        if (z.x - z.y < 0) { temp = z.y; z.y = z.x; z.x = temp; }
+        if (z.x - z.z < 0) { temp = z.z; z.z = z.x; z.x = temp; }
+        if (z.y - z.z < 0) { temp = z.z; z.z = z.y; z.y = temp; }
+        if (mode == MODE_COLOURING) { double length2 = v3len(z); if (length2 < min) min = length2; }
+        z = v3mul(z, 3.0);
+        z.x -= 2.0; z.y -= 2.0;
+        if (z.z > 1.0) z.z -= 2.0;
+        r = v3len(z);
+        tgladDE *= 3.0;
+        break;
+      }
+      case MBR_FORMULA_TGLAD: {
+        double lim = s->mb_folding_limit, val = s->mb_folding_value;
+        if (s->mb_rotations_enabled) {
+          int lockout = 0;
+          z = rot_apply(&f->mb_rot[0][0], z);
+          if (z.x > lim) { z.x = val - z.x; tgladColor += s->mb_color_X; lockout = 1; }
+          z = rot_apply(&f->mb_rotinv[0][0], z);
+          z = rot_apply(&f->mb_rot[1][0], z);
+          if (!lockout && z.x < -lim) { z.x = -val - z.x; tgladColor += s->mb_color_X; }
+          z = rot_apply(&f->mb_rotinv[1][0], z);
+          lockout = 0;
+          z = rot_apply(&f->mb_rot[0][1], z);
+          if (z.y > lim) { z.y = val - z.y; tgladColor += s->mb_color_Y; lockout = 1; }
+          z = rot_apply(&f->mb_rotinv[0][1], z);
+          z = rot_apply(&f->mb_rot[1][1], z);
+          if (!lockout && z.y < -lim) { z.y = -val - z.y; tgladColor += s->mb_color_Y; }
+          z = rot_apply(&f->mb_rotinv[1][1], z);
+          lockout = 0;
+          z = rot_apply(&f->mb_rot[0][2], z);
+          if (z.z > lim) { z.z = val - z.z; tgladColor += s->mb_color_Z; lockout = 1; }
+          z = rot_apply(&f->mb_rotinv[0][2], z);
+          z = rot_apply(&f->mb_rot[1][2], z);
+          if (!lockout && z.z < -lim) { z.z = -val - z.z; tgladColor += s->mb_color_Z; }
+          z = rot_apply(&f->mb_rotinv[1][2], z);
+        } else {
+          if (z.x > lim) { z.x = val - z.x; tgladColor += s->mb_color_X; }
+          else if (z.x < -lim) { z.x = -val - z.x; tgladColor += s->mb_color_X; }
+          if (z.y > lim) { z.y = val - z.y; tgladColor += s->mb_color_Y; }
+          else if (z.y < -lim) { z.y = -val - z.y; tgladColor += s->mb_color_Y; }
+          if (z.z > lim) { z.z = val - z.z; tgladColor += s->mb_color_Z; }
+          else if (z.z < -lim) { z.z = -val - z.z; tgladColor += s->mb_color_Z; }
+        }
+        r = v3len(z);
+        double r2 = r * r;
+        z = v3add(z, v3from(s->mb_offset));
+        if (r2 < mR2) { z = v3mul(z, tglad_factor1); tgladDE *= tglad_factor1; tgladColor += s->mb_color_Sp1; }
+        else if (r2 < fR2) { double tf2 = fR2 / r2; z = v3mul(z, tf2); tgladDE *= tf2; tgladColor += s->mb_color_Sp2; }
+        z = v3sub(z, v3from(s->mb_offset));
+        z = rot_apply(&f->mb_main, z);
+        z = v3add(v3mul(z, scale), constant);
+        tgladDE = tgladDE * fabs(scale) + 1.0;
+        r = v3len(z);
+        break;
+      }
+      default:
+        break;
+    }
+
+    if (formula == MBR_FORMULA_MENGER_SPONGE) {
+      if (r > 1000) { distance = (r - 2.0) / tgladDE; break; }
+    } else if (formula == MBR_FORMULA_TGLAD) {
+      if (r > 1024) { distance = r / fabs(tgladDE); break; }
+    } else {
+      if (mode == MODE_NORMAL) {
+        if (r > s->bailout) { distance = 0.5 * r * log(r) / r_dz; break; }
+      } else {
+        if (s->primitive_only_plane) {
+          distance = v3len(z);
+          if (distance > 1e15) { distance = (L - log(log(r) / log((double)N)) / log(p)) / 100.0; break; }
+        } else {
+          distance = v3len(z);
+          if (distance < min) min = distance;
+          if (distance > 1e15) { distance = min; break; }
+        }
+      }
+    }
+  }
+
+  if (iter_count) *iter_count = L;
+  if (mode == MODE_NORMAL) {
+    if (L == N) distance = 0;
+    return distance;
+  }
+  if (formula == MBR_FORMULA_TGLAD) return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
+  if (formula == MBR_FORMULA_MENGER_SPONGE) return min * 1000.0;
+  return distance * 5000.0;
+}
+
+/* -------------------------------------------------------------- primitives */
+/* src/primitives.cpp:15-119 */
+static double prim_plane(v3 p, v3 centre, v3 normal) { return v3dot(v3norm(normal), v3sub(p, centre)); }
+
+static double prim_box(v3 p, v3 c, v3 size, int inverted) {
+  v3 c1 = v3m(c.x - 0.5 * size.x, c.y - 0.5 * size.y, c.z - 0.5 * size.z);
+  v3 c2 = v3m(c.x + 0.5 * size.x, c.y + 0.5 * size.y, c.z + 0.5 * size.z);
+  double sg = inverted ? 1.0 : -1.0;
+  double d[6];
+  d[0] = prim_plane(p, c1, v3m(sg, 0, 0));
+  d[1] = prim_plane(p, c2, v3m(-sg, 0, 0));
+  d[2] = prim_plane(p, c1, v3m(0, sg, 0));
+  d[3] = prim_plane(p, c2, v3m(0, -sg, 0));
+  d[4] = prim_plane(p, c1, v3m(0, 0, sg));
+  d[5] = prim_plane(p, c2, v3m(0, 0, -sg));
+  double dist = d[0];
+  for (int i = 1; i < 6; i++) {
+    if (inverted) dist = (d[i] < dist) ? d[i] : dist;
+    else dist = (d[i] > dist) ? d[i] : dist;
+  }
+  return dist;
+}
+
+static double prim_water(v3 p, double height, double amplitude, double length, double rotation,
+                         int iterations, double animSpeed, int frame) {
+  v3 plane = v3norm(v3m(0, 0, -1));
+  double planeDistance = v3dot(plane, v3sub(p, v3m(0, 0, height)));
+  if (planeDistance < amplitude * 10.0) {
+    rotm rm; rot_null(&rm); rot_z(&rm, rotation / 180 * M_PI);
+    p = rot_apply(&rm, p);
+    double phase = animSpeed * frame;
+    double k = 0.23, waveX = 0, waveY = 0, pp = 1.0, p2 = 0.05;
+    for (int i = 1; i <= iterations; i++) {
+      float p3 = (float)(pp * p2);
+      double shift = phase / (i / 3.0 + 1.0);
+      double wx = sin(i + 0.4 * (waveX) * p3 + sin(k * p.y / length * p3) + p.x / length * p3 + shift) / pp;
+      double wy = cos(i + 0.4 * (waveY) * p3 + sin(p.x / length * p3) + k * p.y / length * p3 + shift * 0.23) / pp;
+      waveX += wx; waveY += wy;
+      p2 = p2 + (1.0 - p2) * 0.7;
+      pp *= 1.872;
+    }
+    planeDistance += (waveX + waveY) * amplitude;
+  }
+  return planeDistance;
+}
+
+/* ------------------------------------------------------ CalculateDistance */
+typedef struct { uint64_t calls, iterations; } de_counter;
+
+/* CalculateDistance (src/fractal.cpp:1245-1448), analytic-DE branch (the
+ * supported formulas 2, 7 and 8 are analytic, src/interface.cpp:690-693). */
+static double calc_distance(const fractal_ctx *f, v3 point, double detailSize, int N, int minN,
+                            int normalMode, int *max_iter, int *iters_out, int *object_out,
+                            de_counter *cnt) {
+  const mbr_scene *s = f->s;
+  int L = 0, object = MBR_OBJ_FRACTAL, mi = 0;
+  double distance;
+  double limitBoxDist = 0.0;
+
+  if (s->limits_enabled) {
+    double da = fmax(point.x - s->limit_max[0], -(point.x - s->limit_min[0]));
+    double db = fmax(point.y - s->limit_max[1], -(point.y - s->limit_min[1]));
+    double dc = fmax(point.z - s->limit_max[2], -(point.z - s->limit_min[2]));
+    /* dMax (src/common_math.cpp:34-44) */
+    limitBoxDist = (da > db) ? ((da > dc) ? da : dc) : ((db > dc) ? db : dc);
+    if (limitBoxDist > detailSize) {
+      if (max_iter) *max_iter = 0;
+      if (object_out) *object_out = object;
+      return limitBoxDist;
+    }
+  }
+
+  if (!s->primitive_only_plane) {
+    distance = compute(f, MODE_NORMAL, point, N, &L);
+    if (cnt) { cnt->calls++; cnt->iterations += (L < N) ? (uint64_t)L + 1 : (uint64_t)N; }
+    mi = (L == N);
+    if (iters_out) *iters_out = L;
+    if (distance < 0) distance = 0;
+    if (L < minN && distance < detailSize) distance = detailSize;
+    if (s->interior_mode && !normalMode) {
+      if (distance < 0.5 * detailSize || L == N) { distance = detailSize; mi = 0; }
+    } else if (s->interior_mode && normalMode) {
+      if (distance < 0.9 * detailSize) { distance = detailSize - distance; mi = 0; }
+    }
+    if (s->iter_thresh && L < N) {
+      if (distance < detailSize) distance = detailSize * 1.01;
+    }
+  } else {
+    distance = 10.0;
+    mi = 0;
+  }
+
+  if (s->plane_enabled) {
+    double d = prim_plane(point, v3from(s->plane_centre), v3from(s->plane_normal));
+    if (!s->primitive_only_plane && d < distance) object = MBR_OBJ_PLANE;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->box_enabled) {
+    double d = prim_box(point, v3from(s->box_centre), v3from(s->box_size), 0);
+    if (d < distance) object = MBR_OBJ_BOX;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->inv_box_enabled) {
+    double d = prim_box(point, v3from(s->inv_box_centre), v3from(s->inv_box_size), 1);
+    if (d < distance) object = MBR_OBJ_BOX_INV;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->sphere_enabled) {
+    double d = v3len(v3sub(point, v3from(s->sphere_centre))) - s->sphere_radius;
+    if (d < distance) object = MBR_OBJ_SPHERE;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->inv_sphere_enabled) {
+    double d = s->inv_sphere_radius - v3len(v3sub(point, v3from(s->inv_sphere_centre)));
+    if (d < distance) object = MBR_OBJ_SPHERE_INV;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->water_enabled) {
+    double d = prim_water(point, s->water_height, s->water_amplitude, s->water_length,
+                          s->water_rotation, s->water_iterations, s->water_anim_speed, s->frame_no);
+    if (d < distance) object = MBR_OBJ_WATER;
+    distance = (d < distance) ? d : distance;
+  }
+  if (s->limits_enabled) {
+    if (limitBoxDist < detailSize) distance = fmax(distance, limitBoxDist);
+  }
+  if (max_iter) *max_iter = mi;
+  if (object_out) *object_out = object;
+  return distance;
+}
+
+/* ---------------------------------------------------------------- camera */
+typedef struct {
+  rotm R;
+  v3 start, light;
+  double resolution;
+  int aspect_w, aspect_h;
+} camera_t;
+
+/* MainThread set-up (src/Render3D.cpp:236-256, :316-333) plus the
+ * resolution of ReadInterface (src/interface.cpp:695). */
+static void camera_init(camera_t *c, const mbr_scene *s) {
+  rot_null(&c->R);
+  rot_z(&c->R, s->alpha);
+  rot_x(&c->R, s->beta);
+  rot_y(&c->R, s->gamma);
+  v3 baseY = rot_apply(&c->R, v3m(0.0, 1.0, 0.0));
+  v3 vec;
+  vec.x = cos(s->main_light_alpha - 0.5 * M_PI) * cos(-s->main_light_beta);
+  vec.y = sin(s->main_light_alpha - 0.5 * M_PI) * cos(-s->main_light_beta);
+  vec.z = sin(-s->main_light_beta);
+  c->light = rot_apply(&c->R, vec);
+  if (s->perspective_type == MBR_PERSP_FISH_EYE || s->perspective_type == MBR_PERSP_EQUIRECTANGULAR)
+    c->start = v3from(s->vp);
+  else
+    c->start = v3sub(v3from(s->vp), v3mul(baseY, 1.0 / s->persp * s->zoom));
+  c->resolution = 1.0 / s->image_width / s->tiles;
+}
+
+/* per-pixel view vector (src/Render3D.cpp:389-438) */
+static int view_vector(const mbr_scene *s, const camera_t *c, int x, int y, v3 *out) {
+  int width = s->image_width, height = s->image_height, tiles = s->tiles;
+  double aspectRatio = (double)width / height;
+  double tileX = (s->tile_index % tiles), tileY = (s->tile_index / tiles);
+  double fov = s->persp;
+  double x2 = ((double)x / width / tiles - 0.5 + tileX / tiles) * aspectRatio;
+  double z2 = ((double)y / height / tiles - 0.5 + tileY / tiles);
+  int cut = 0;
+  v3 v;
+  if (s->perspective_type == MBR_PERSP_FISH_EYE) {
+    if (s->fish_eye_cut && sqrt(x2 * x2 + z2 * z2) > 0.5 / fov) cut = 1;
+    double x3 = x2 * M_PI, z3 = z2 * M_PI;
+    double r = sqrt(x3 * x3 + z3 * z3);
+    if (r == 0) v = v3m(0.0, 1.0, 0.0);
+    else v = v3m(x3 / r * sin(r * fov), cos(r * fov), z3 / r * sin(r * fov));
+    v = v3norm(v);
+  } else if (s->perspective_type == MBR_PERSP_EQUIRECTANGULAR) {
+    double x3 = x2 * M_PI, z3 = z2 * M_PI;
+    v = v3m(sin(fov * x3) * cos(fov * z3), cos(fov * x3) * cos(fov * z3), sin(fov * z3));
+    v = v3norm(v);
+  } else {
+    v = v3m(x2 * fov, 1.0, z2 * fov);
+  }
+  *out = rot_apply(&c->R, v);
+  return cut;
+}
+
+/* ---------------------------------------------------------------- march */
+typedef struct { double distance, step, distThresh; v3 point; int iters; } step_rec;
+
+typedef struct {
+  int mode;               /* enum oracle_jitter */
+  uint32_t seed, frame, pixel;
+} jitter_t;
+
+/* Random(1000) (src/common_math.cpp:25-32) or its hash replacement */
+static inline int jitter_draw(const jitter_t *j, int step) {
+  if (j->mode == ORACLE_JITTER_GLIBC) return rand() % 1001;
+  if (j->mode == ORACLE_JITTER_HASH) return (int)oracle_jitter_draw(j->seed, j->frame, j->pixel, (uint32_t)step);
+  return 0;
+}
+
+typedef struct {
+  v3 point;
+  double depth, distThresh, lastDist, detail;
+  int found, buffCount, object, de_calls, missed;
+} march_result;
+
+/* RayMarching (src/Render3D.cpp:79-199) with distThreshInit = 0 (primary
+ * rays, :444) and binary search enabled. */
+static void ray_march(const fractal_ctx *f, const camera_t *c, v3 start, v3 direction, double maxScan,
+                      const jitter_t *jit, step_rec *stepBuff, march_result *out, de_counter *cnt) {
+  const mbr_scene *s = f->s;
+  v3 point = v3m(0, 0, 0);
+  int found = 0;
+  double scan = s->view_distance_min;
+  double distThresh = 0.0;
+  double dist = 0;
+  double search_accuracy = 0.01 * s->quality;
+  double search_limit = 1.0 - search_accuracy;
+  int counter = 0;
+  double step = s->view_distance_min;
+  int buffCount = 0;
+  double distThreshInit = 0.0;
+  double detail = 0.0;
+  int object = MBR_OBJ_FRACTAL;
+  int fisheye = (s->perspective_type == MBR_PERSP_FISH_EYE || s->perspective_type == MBR_PERSP_EQUIRECTANGULAR);
+  out->missed = 0;
+
+  for (int i = 0; i < 10000; i++) {
+    counter++;
+    point = v3add(start, v3mul(direction, scan));
+    int max_iter = 0, iters = 0;
+    if (s->constant_DE_threshold) distThresh = s->quality;
+    else distThresh = scan * c->resolution * s->persp / s->quality + distThreshInit;
+    if (fisheye) distThresh *= M_PI;
+    detail = distThresh;
+    dist = calc_distance(f, point, detail, s->N, s->minN, 0, &max_iter, &iters, &object, cnt);
+    stepBuff[i].distance = dist;
+    stepBuff[i].iters = iters;
+    stepBuff[i].distThresh = distThresh;
+    if (dist > 3.0) dist = 3.0;
+    if (dist < distThresh) {
+      if (dist < 0.1 * distThresh) out->missed = 1;
+      found = 1;
+      break;
+    }
+    stepBuff[i].step = step;
+    double jf = 1.0 - jitter_draw(jit, i) / 10000.0;
+    if (s->interior_mode) step = (dist - 0.8 * distThresh) * s->DE_factor * jf;
+    else step = (dist - 0.5 * distThresh) * s->DE_factor * jf;
+    stepBuff[i].point = point;
+    buffCount++;
+    scan += step;
+    if (scan > maxScan) break;
+  }
+
+  if (found) {
+    step *= 0.5;
+    for (int i = 0; i < 20; i++) {
+      if (dist < distThresh && dist > distThresh * search_limit) break;
+      if (dist > distThresh) { scan += step; point = v3add(start, v3mul(direction, scan)); }
+      else if (dist < distThresh * search_limit) { scan -= step; point = v3add(start, v3mul(direction, scan)); }
+      if (s->constant_DE_threshold) distThresh = s->quality;
+      else distThresh = scan * c->resolution * s->persp / s->quality + distThreshInit;
+      int max_iter = 0;
+      dist = calc_distance(f, point, detail, s->N, s->minN, 0, &max_iter, NULL, &object, cnt);
+      counter++;
+      step *= 0.5;
+    }
+  }
+  out->point = point;
+  out->depth = scan;
+  out->distThresh = distThresh;
+  out->lastDist = dist;
+  out->found = found;
+  out->buffCount = buffCount;
+  out->object = object;
+  out->de_calls = counter;
+  out->detail = detail;
+}
+
+/* ---------------------------------------------------------------- shaders */
+typedef struct { double R, G, B, alpha; } shade_t;
+
+typedef struct {
+  const fractal_ctx *f;
+  const camera_t *c;
+  v3 point, viewVector, normal, lightVect;
+  double dist_thresh, lastDist, delta, depth;
+  double detail;          /* calcParam->doubles.detailSize as the reference mutates it */
+  int objectType;
+  const double *ao_dirs; const int *ao_rgb; int ao_count;
+  de_counter *cnt;
+} shader_in;
+
+/* CalculateNormals (src/shaders.cpp:675-757) */
+static v3 calculate_normals(shader_in *in) {
+  const mbr_scene *s = in->f->s;
+  v3 normal = v3m(0, 0, 0);
+  int mi;
+  if (!s->slow_shading) {
+    double delta = in->delta * s->smoothness;
+    if (s->interior_mode) delta = in->dist_thresh * s->quality * 0.2 * s->smoothness;
+    int N5 = s->N * 5;
+    double s1 = calc_distance(in->f, in->point, in->detail, N5, 0, 1, &mi, NULL, NULL, in->cnt);
+    double s2 = calc_distance(in->f, v3add(in->point, v3m(delta, 0.0, 0.0)), in->detail, N5, 0, 1, &mi, NULL, NULL, in->cnt);
+    double s3 = calc_distance(in->f, v3add(in->point, v3m(0.0, delta, 0.0)), in->detail, N5, 0, 1, &mi, NULL, NULL, in->cnt);
+    double s4 = calc_distance(in->f, v3add(in->point, v3m(0.0, 0.0, delta)), in->detail, N5, 0, 1, &mi, NULL, NULL, in->cnt);
+    normal.x = s2 - s1; normal.y = s3 - s1; normal.z = s4 - s1;
+  } else {
+    double delta = in->delta * s->smoothness;
+    int N5 = s->N * 5;
+    if (s->interior_mode) delta = in->dist_thresh * s->quality * 0.2 * s->smoothness;
+    v3 p2;
+    for (p2.x = -1.0; p2.x <= 1.0; p2.x += 0.2)
+      for (p2.y = -1.0; p2.y <= 1.0; p2.y += 0.2)
+        for (p2.z = -1.0; p2.z <= 1.0; p2.z += 0.2) {
+          v3 p3 = v3add(in->point, v3mul(p2, delta));
+          double dist = calc_distance(in->f, p3, in->detail, N5, s->minN, 1, &mi, NULL, NULL, in->cnt);
+          normal = v3add(normal, v3mul(p2, dist));
+        }
+  }
+  if (normal.x == 0 && normal.y == 0 && normal.z == 0) normal.x = 1;
+  else normal = v3norm(normal);
+  return normal;
+}
+
+/* MainShadow (src/shaders.cpp:480-570) without iteration fog and
+ * volumetric lights (rejected scene features). */
+static double main_shadow(shader_in *in) {
+  const mbr_scene *s = in->f->s;
+  double factor = in->delta / in->c->resolution;
+  if (!s->penetrating_lights) factor = s->view_distance_max;
+  double dist = in->dist_thresh;
+  in->detail = in->dist_thresh;
+  double DE_factor = s->DE_factor;
+  double start = in->dist_thresh;
+  if (s->interior_mode) start = in->dist_thresh * DE_factor;
+  double shadowTemp = 1.0;
+  double softRange = tan(s->shadow_cone_angle / 180.0 * M_PI);
+  double maxSoft = 0.0;
+  int bSoft = (!s->limits_enabled && !s->iter_thresh) && softRange > 0.0;
+  int mi;
+  for (double i = start; i < factor; i += dist * DE_factor) {
+    v3 point2 = v3add(in->point, v3mul(in->lightVect, i));
+    float dist_thresh = (float)in->dist_thresh;
+    in->detail = dist_thresh;
+    dist = calc_distance(in->f, point2, in->detail, s->N, s->minN, 0, &mi, NULL, NULL, in->cnt);
+    if (bSoft) {
+      double angle = (dist - dist_thresh) / i;
+      if (angle < 0) angle = 0;
+      if (dist < dist_thresh) angle = 0;
+      double softShadow = (1.0 - angle / softRange);
+      if (s->penetrating_lights) softShadow *= (factor - i) / factor;
+      if (softShadow < 0) softShadow = 0;
+      if (softShadow > maxSoft) maxSoft = softShadow;
+    }
+    if (dist < dist_thresh || shadowTemp < 0.0) {
+      shadowTemp -= (factor - i) / factor;
+      if (!s->penetrating_lights) shadowTemp = 0.0;
+      if (shadowTemp < 0.0) shadowTemp = 0.0;
+      break;
+    }
+  }
+  return bSoft ? (1.0 - maxSoft) : shadowTemp;
+}
+
+/* FastAmbientOcclusion2 (src/shaders.cpp:585-605) */
+static double fast_ao2(shader_in *in) {
+  const mbr_scene *s = in->f->s;
+  double delta = in->dist_thresh;
+  double aoTemp = 0;
+  double quality = s->ao_quality;
+  int mi;
+  for (int i = 1; i < quality * quality; i++) {
+    double scan = i * i * delta;
+    v3 pt = v3add(in->point, v3mul(in->normal, scan));
+    double dist = calc_distance(in->f, pt, in->detail, s->N, s->minN, 0, &mi, NULL, NULL, in->cnt);
+    aoTemp += 1.0 / (pow(2.0, i)) * (scan - s->fast_ao_tune * dist) / in->dist_thresh;
+  }
+  double ao = 1.0 - 0.2 * aoTemp;
+  if (ao < 0) ao = 0;
+  return ao;
+}
+
+/* AmbientOcclusion (src/shaders.cpp:607-673) */
+static shade_t ambient_occlusion(shader_in *in) {
+  const mbr_scene *s = in->f->s;
+  shade_t AO = {0, 0, 0, 0};
+  int max_iter;
+  double start_dist = in->delta;
+  double end_dist = in->delta / in->c->resolution;
+  for (int k = 0; k < in->ao_count; k++) {
+    v3 v = v3from(in->ao_dirs + 3 * k);
+    double dist = in->lastDist;
+    double shadowTemp = 1.0;
+    for (double r = start_dist; r < end_dist; r += dist * 2.0) {
+      v3 point2 = v3add(in->point, v3mul(v, r));
+      dist = calc_distance(in->f, point2, in->detail, s->N, s->minN, 0, &max_iter, NULL, NULL, in->cnt);
+      float dist_thresh = (float)in->dist_thresh;
+      if (dist < dist_thresh || max_iter || shadowTemp < 0.0) {
+        shadowTemp -= (end_dist - r) / end_dist;
+        if (shadowTemp < 0.0) shadowTemp = 0.0;
+        break;
+      }
+    }
+    AO.R += shadowTemp * in->ao_rgb[3 * k + 0];
+    AO.G += shadowTemp * in->ao_rgb[3 * k + 1];
+    AO.B += shadowTemp * in->ao_rgb[3 * k + 2];
+  }
+  AO.R /= (in->ao_count * 256.0);
+  AO.G /= (in->ao_count * 256.0);
+  AO.B /= (in->ao_count * 256.0);
+  return AO;
+}
+
+/* IndexToColour (src/shaders.cpp:1273-1305) */
+static mbr_rgb index_to_colour(int index, const mbr_rgb *palette) {
+  mbr_rgb colour = {255, 255, 255};
+  if (index < 0) return palette[255];
+  index = index % (255 * 256);
+  int kol = index / 256;
+  if (kol < 255) {
+    double R1 = palette[kol].R, G1 = palette[kol].G, B1 = palette[kol].B;
+    double R2 = palette[kol + 1].R, G2 = palette[kol + 1].G, B2 = palette[kol + 1].B;
+    double RK = (R2 - R1) / 256.0, GK = (G2 - G1) / 256.0, BK = (B2 - B1) / 256.0;
+    int delta = index % 256;
+    colour.R = (int)(R1 + (RK * delta));
+    colour.G = (int)(G1 + (GK * delta));
+    colour.B = (int)(B1 + (BK * delta));
+  }
+  return colour;
+}
+
+/* SurfaceColour (src/shaders.cpp:813-926) */
+static shade_t surface_colour(shader_in *in) {
+  const mbr_scene *s = in->f->s;
+  shade_t out = {0, 0, 0, 0};
+  mbr_rgb pc;
+  switch (in->objectType) {
+    case MBR_OBJ_FRACTAL: {
+      int it;
+      double cval = compute(in->f, MODE_COLOURING, in->point, s->N * 10, &it);
+      if (in->cnt) { in->cnt->calls++; in->cnt->iterations += (it < s->N * 10) ? (uint64_t)it + 1 : (uint64_t)s->N * 10; }
+      int nrCol = x86_d2i(floor(cval));
+      nrCol = x86_abs(nrCol) % (248 * 256);
+      mbr_rgb colour = {256, 256, 256};
+      if (s->coloring_enabled) {
+        int color_number;
+        if (nrCol >= 248 * 256) color_number = nrCol;
+        else color_number = x86_d2i(nrCol * s->coloring_speed + 256 * s->palette_offset) % 65536;
+        colour = index_to_colour(color_number, s->palette);
+      }
+      out.R = colour.R / 256.0; out.G = colour.G / 256.0; out.B = colour.B / 256.0;
+      return out;
+    }
+    case MBR_OBJ_WATER: pc = s->water_colour; break;
+    case MBR_OBJ_BOX: pc = s->box_colour; break;
+    case MBR_OBJ_BOX_INV: pc = s->inv_box_colour; break;
+    case MBR_OBJ_SPHERE: pc = s->sphere_colour; break;
+    case MBR_OBJ_SPHERE_INV: pc = s->inv_sphere_colour; break;
+    default: pc = s->plane_colour; break;
+  }
+  out.R = pc.R / 65536.0; out.G = pc.G / 65536.0; out.B = pc.B / 65536.0;
+  return out;
+}
+
+/* ObjectShader (src/shaders.cpp:21-112) without env mapping, aux and fake
+ * lights (rejected scene features; their terms are zero). */
+static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *specularOut) {
+  const mbr_scene *s = in->f->s;
+  shade_t output;
+  v3 vn = calculate_normals(in);
+  in->normal = vn;
+  shade_t mainLight;
+  mainLight.R = s->main_light_intensity * s->main_light_colour.R / 65536.0;
+  mainLight.G = s->main_light_intensity * s->main_light_colour.G / 65536.0;
+  mainLight.B = s->main_light_intensity * s->main_light_colour.B / 65536.0;
+  /* MainShading (:759-769) */
+  double sh = v3dot(in->normal, in->lightVect);
+  if (sh < 0) sh = 0;
+  double shade = s->direct_light * ((1.0 - s->shading) + s->shading * sh);
+  double shadow = 1.0;
+  if (s->shadow) shadow = main_shadow(in);
+  /* MainSpecular (:771-784) */
+  v3 half = v3norm(v3sub(in->lightVect, in->viewVector));
+  double shade2 = v3dot(in->normal, half);
+  if (shade2 < 0.0) shade2 = 0.0;
+  shade2 = pow(shade2, 30.0) * 1.0;
+  if (shade2 > 15.0) shade2 = 15.0;
+  double specular = shade2 * s->specular;
+  shade_t colour = surface_colour(in);
+  *surfaceColour = colour;
+  shade_t ambient = {0, 0, 0, 0};
+  if (s->global_illumination) {
+    if (s->fast_global_illumination) {
+      double ao = fast_ao2(in);
+      ambient.R = ambient.G = ambient.B = ao;
+    } else {
+      ambient = ambient_occlusion(in);
+    }
+  }
+  shade_t ambient2;
+  ambient2.R = ambient.R * s->global_ilum;
+  ambient2.G = ambient.G * s->global_ilum;
+  ambient2.B = ambient.B * s->global_ilum;
+  output.R = 0.0 * ambient.R + (ambient2.R + mainLight.R * shade * shadow) * colour.R;
+  output.G = 0.0 * ambient.G + (ambient2.G + mainLight.G * shade * shadow) * colour.G;
+  output.B = 0.0 * ambient.B + (ambient2.B + mainLight.B * shade * shadow) * colour.B;
+  output.R += (0.0 + 0.0) * colour.R;
+  output.G += (0.0 + 0.0) * colour.G;
+  output.B += (0.0 + 0.0) * colour.B;
+  output.alpha = 1.0;
+  specularOut->R = 0.0 + 0.0 + mainLight.R * specular * shadow;
+  specularOut->G = 0.0 + 0.0 + mainLight.G * specular * shadow;
+  specularOut->B = 0.0 + 0.0 + mainLight.B * specular * shadow;
+  specularOut->alpha = output.alpha;
+  return output;
+}
+
+/* BackgroundShader, gradient branch (src/shaders.cpp:159-181); the
+ * gradient is stored in sRGB16 before scaling. */
+static shade_t background_shader(const mbr_scene *s, v3 viewVector) {
+  shade_t px;
+  v3 vec = v3norm(v3m(0.0, 0.0, -1.0));
+  double grad = (v3dot(viewVector, vec) + 1.0);
+  unsigned short R, G, B;
+  if (grad < 1) {
+    double Ngrad = 1.0 - grad;
+    R = x86_d2u16(s->background_color3.R * Ngrad + s->background_color2.R * grad);
+    G = x86_d2u16(s->background_color3.G * Ngrad + s->background_color2.G * grad);
+    B = x86_d2u16(s->background_color3.B * Ngrad + s->background_color2.B * grad);
+  } else {
+    grad = grad - 1;
+    double Ngrad = 1.0 - grad;
+    R = x86_d2u16(s->background_color2.R * Ngrad + s->background_color1.R * grad);
+    G = x86_d2u16(s->background_color2.G * Ngrad + s->background_color1.G * grad);
+    B = x86_d2u16(s->background_color2.B * Ngrad + s->background_color1.B * grad);
+  }
+  px.R = R / 65536.0; px.G = G / 65536.0; px.B = B / 65536.0;
+  px.alpha = 0.0;
+  return px;
+}
+
+/* VolumetricShader (src/shaders.cpp:185-478): glow, basic fog and
+ * volumetric fog (visible aux lights contribute nothing when no light is
+ * enabled; fake lights, volumetric lights and iteration fog are rejected
+ * scene features).  *opacity_written is 0 when no step group was emitted
+ * (the reference then leaves opacityOut uninitialised; we report 0). */
+static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_rec *buf, int stepCount,
+                                 shade_t oldPixel, double *opacityOut) {
+  shade_t output = oldPixel;
+  float totalOpacity = 0.0f;
+  double fogVisibility = pow(10.0, s->fog_visibility / 10 - 16.0);
+  double colourThresh = s->fog_colour1_distance;
+  double colourThresh2 = s->fog_colour2_distance;
+  double fogReduce = s->fog_distance_factor;
+  double fogIntensity = s->fog_density;
+  double glow = stepCount * s->glow_intensity / 512.0 * s->DE_factor;
+  double glowN = 1.0 - glow;
+  if (glowN < 0.0) glowN = 0.0;
+  double glowR = (s->glow_color1.R * glowN + s->glow_color2.R * glow) / 65536.0;
+  double glowG = (s->glow_color1.G * glowN + s->glow_color2.G * glow) / 65536.0;
+  double glowB = (s->glow_color1.B * glowN + s->glow_color2.B * glow) / 65536.0;
+  double totalStep = 0.0;
+  *opacityOut = 0.0;
+  for (int index = stepCount - 1; index > 0; index--) {
+    double step = buf[index].step;
+    double distance = buf[index].distance;
+    totalStep += step;
+    double dist_thresh = buf[index].distThresh;
+    if (totalStep < dist_thresh * s->quality) continue;
+    step = totalStep;
+    totalStep = 0.0;
+
+    double glowOpacity = glow * step / depth;
+    output.R = glowOpacity * glowR + (1.0 - glowOpacity) * output.R;
+    output.G = glowOpacity * glowG + (1.0 - glowOpacity) * output.G;
+    output.B = glowOpacity * glowB + (1.0 - glowOpacity) * output.B;
+    output.alpha += glowOpacity;
+
+    if (s->post_fog_enabled) {
+      double fogDensity = step / fogVisibility;
+      if (fogDensity > 1.0) fogDensity = 1.0;
+      output.R = fogDensity * s->post_fog_color.R / 65536.0 + (1.0 - fogDensity) * output.R;
+      output.G = fogDensity * s->post_fog_color.G / 65536.0 + (1.0 - fogDensity) * output.G;
+      output.B = fogDensity * s->post_fog_color.B / 65536.0 + (1.0 - fogDensity) * output.B;
+      totalOpacity = fogDensity + (1.0 - fogDensity) * totalOpacity;
+      output.alpha = fogDensity + (1.0 - fogDensity) * output.alpha;
+    }
+
+    if (fogIntensity > 0.0) {
+      double densityTemp = (step * fogReduce) / (distance * distance + fogReduce * fogReduce);
+      double k = distance / colourThresh;
+      if (k > 1) k = 1.0;
+      double kn = 1.0 - k;
+      double fogRtemp = (s->fog_colour1.R * kn + s->fog_colour2.R * k);
+      double fogGtemp = (s->fog_colour1.G * kn + s->fog_colour2.G * k);
+      double fogBtemp = (s->fog_colour1.B * kn + s->fog_colour2.B * k);
+      double k2 = distance / colourThresh2 * k;
+      if (k2 > 1) k2 = 1.0;
+      kn = 1.0 - k2;
+      fogRtemp = (fogRtemp * kn + s->fog_colour3.R * k2);
+      fogGtemp = (fogGtemp * kn + s->fog_colour3.G * k2);
+      fogBtemp = (fogBtemp * kn + s->fog_colour3.B * k2);
+      double fogDensity = 0.3 * fogIntensity * densityTemp / (1.0 + fogIntensity * densityTemp);
+      if (fogDensity > 1) fogDensity = 1.0;
+      output.R = fogDensity * fogRtemp / 65536.0 + (1.0 - fogDensity) * output.R;
+      output.G = fogDensity * fogGtemp / 65536.0 + (1.0 - fogDensity) * output.G;
+      output.B = fogDensity * fogBtemp / 65536.0 + (1.0 - fogDensity) * output.B;
+      totalOpacity = fogDensity + (1.0 - fogDensity) * totalOpacity;
+      output.alpha = fogDensity + (1.0 - fogDensity) * output.alpha;
+    }
+
+    if (totalOpacity > 1.0) totalOpacity = 1.0;
+    if (output.alpha > 1.0) output.alpha = 1.0;
+    *opacityOut = totalOpacity;
+  }
+  return output;
+}
+
+/* ------------------------------------------------------------ AO table */
+int oracle_ao_table(int AO_quality, const mbr_texture *lm, double *dirs, int *rgb, int max) {
+  /* src/Render3D.cpp:271-313; white 100x100 texture when lm == NULL */
+  int counter = 0;
+  int W = lm ? lm->width : 100, H = lm ? lm->height : 100;
+  for (double b = -0.49 * M_PI; b < 0.49 * M_PI; b += 1.0 / AO_quality) {
+    for (double a = 0.0; a < 2.0 * M_PI; a += ((2.0 / AO_quality) / cos(b))) {
+      double d[3] = {cos(a + b) * cos(b), sin(a + b) * cos(b), sin(b)};
+      int X = (int)((a + b) / (2.0 * M_PI) * W + W * 8.5) % W;
+      int Y = (int)(b / (M_PI) * H + H * 8.5) % H;
+      int R = 255, G = 255, B = 255;
+      if (lm) {
+        const uint8_t *px = lm->rgb + 3 * ((size_t)X + (size_t)Y * W);
+        R = px[0]; G = px[1]; B = px[2];
+      }
+      if (counter < max) {
+        dirs[3 * counter + 0] = d[0]; dirs[3 * counter + 1] = d[1]; dirs[3 * counter + 2] = d[2];
+        rgb[3 * counter + 0] = R; rgb[3 * counter + 1] = G; rgb[3 * counter + 2] = B;
+      }
+      if (R > 10 || G > 10 || B > 10) counter++;
+      if (counter >= 10000) break;
+    }
+    if (counter >= 10000) break;
+  }
+  if (counter == 0) {
+    counter = 1;
+    dirs[0] = dirs[1] = dirs[2] = 0.0;
+    rgb[0] = rgb[1] = rgb[2] = 0;
+  }
+  return counter;
+}
+
+/* --------------------------------------------------------- public units */
+double oracle_compute(const mbr_scene *s, int mode, const double p[3], int n_iter, int *iters) {
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  return compute(&f, mode, v3from(p), n_iter, iters);
+}
+
+double oracle_calculate_distance(const mbr_scene *s, const double p[3], double detail_size,
+                                 int n_iter, int min_n, int normal_mode,
+                                 int *max_iter, int *iters, int *object) {
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  int mi = 0, it = 0, ob = 0;
+  double d = calc_distance(&f, v3from(p), detail_size, n_iter, min_n, normal_mode, &mi, &it, &ob, NULL);
+  if (max_iter) *max_iter = mi;
+  if (iters) *iters = it;
+  if (object) *object = ob;
+  return d;
+}
+
+void oracle_camera(const mbr_scene *s, double R[9], double start[3], double light[3]) {
+  camera_t c; camera_init(&c, s);
+  v3 ex = rot_apply(&c.R, v3m(1, 0, 0)), ey = rot_apply(&c.R, v3m(0, 1, 0)), ez = rot_apply(&c.R, v3m(0, 0, 1));
+  /* columns are the images of the base vectors */
+  R[0] = ex.x; R[1] = ey.x; R[2] = ez.x;
+  R[3] = ex.y; R[4] = ey.y; R[5] = ez.y;
+  R[6] = ex.z; R[7] = ey.z; R[8] = ez.z;
+  start[0] = c.start.x; start[1] = c.start.y; start[2] = c.start.z;
+  light[0] = c.light.x; light[1] = c.light.y; light[2] = c.light.z;
+}
+
+int oracle_view_vector(const mbr_scene *s, int x, int y, double dir[3]) {
+  camera_t c; camera_init(&c, s);
+  v3 v; int cut = view_vector(s, &c, x, y, &v);
+  dir[0] = v.x; dir[1] = v.y; dir[2] = v.z;
+  return cut;
+}
+
+int oracle_march(const mbr_scene *s, const double start[3], const double dir[3], uint32_t pixel,
+                 oracle_march_out *out) {
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  camera_t c; camera_init(&c, s);
+  step_rec *buf = (step_rec *)malloc(sizeof(step_rec) * 10002);
+  if (!buf) return -1;
+  jitter_t j = {s->jitter_mode == MBR_JITTER_HASH ? ORACLE_JITTER_HASH : ORACLE_JITTER_NONE,
+                s->jitter_seed, (uint32_t)s->frame_no, pixel};
+  march_result m;
+  ray_march(&f, &c, v3from(start), v3from(dir), s->view_distance_max, &j, buf, &m, NULL);
+  out->point[0] = m.point.x; out->point[1] = m.point.y; out->point[2] = m.point.z;
+  out->depth = m.depth; out->dist_thresh = m.distThresh; out->last_dist = m.lastDist;
+  out->found = m.found; out->step_count = m.buffCount; out->object = m.object; out->de_calls = m.de_calls;
+  free(buf);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- render */
+typedef struct {
+  const mbr_scene *s;
+  const fractal_ctx *f;
+  const camera_t *c;
+  const double *ao_dirs; const int *ao_rgb; int ao_count;
+  mbr_framebuffer *fb;
+  int row_begin, row_end;
+  int jitter; uint32_t glibc_seed;
+  volatile int next_row;
+  pthread_mutex_t lock;
+  oracle_stats st;
+} render_job;
+
+/* per-pixel body of MainThread (src/Render3D.cpp:387-590) for ray 0 */
+static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_stats *st, de_counter *mc, de_counter *sc) {
+  const mbr_scene *s = job->s;
+  const camera_t *c = job->c;
+  int W = s->image_width;
+  v3 viewVector;
+  int cut = view_vector(s, c, x, y, &viewVector);
+  uint32_t pixel = (uint32_t)(x + y * W + s->tile_index * W * s->image_height);
+  jitter_t j = {job->jitter, s->jitter_seed, (uint32_t)s->frame_no, pixel};
+  march_result m;
+  memset(&m, 0, sizeof m);
+  if (!cut) ray_march(job->f, c, c->start, viewVector, s->view_distance_max, &j, buf, &m, mc);
+  st->rays++;
+  if (m.found) st->hits++;
+  st->missed_de += m.missed;
+  if ((uint64_t)m.buffCount > st->max_steps) st->max_steps = m.buffCount;
+  st->total_steps += m.buffCount;
+
+  shade_t objectShader = {0, 0, 0, 0}, backgroundShader = {0, 0, 0, 0}, specular = {0, 0, 0, 0};
+  shade_t objectColour = {0, 0, 0, 0};
+  double zdepth = m.depth;
+  shader_in in;
+  memset(&in, 0, sizeof in);
+  in.f = job->f; in.c = c;
+  in.dist_thresh = m.distThresh;
+  in.delta = s->constant_DE_threshold ? m.depth * c->resolution * s->persp : m.distThresh * s->quality;
+  in.detail = m.distThresh;
+  in.lightVect = c->light;
+  in.point = m.point;
+  in.viewVector = viewVector;
+  in.ao_dirs = job->ao_dirs; in.ao_rgb = job->ao_rgb; in.ao_count = job->ao_count;
+  in.lastDist = m.lastDist;
+  in.depth = m.depth;
+  in.objectType = m.object;
+  in.cnt = sc;
+  if (m.found) {
+    objectShader = object_shader(&in, &objectColour, &specular);
+  } else {
+    backgroundShader = background_shader(s, viewVector);
+    zdepth = 1e20;
+  }
+  shade_t pixel_sh;
+  pixel_sh.R = objectShader.R + backgroundShader.R + specular.R;
+  pixel_sh.G = objectShader.G + backgroundShader.G + specular.G;
+  pixel_sh.B = objectShader.B + backgroundShader.B + specular.B;
+  pixel_sh.alpha = objectShader.alpha + backgroundShader.alpha;
+  double opacity;
+  shade_t vol = volumetric_shader(s, m.depth, buf, m.buffCount, pixel_sh, &opacity);
+
+  size_t idx = (size_t)x + (size_t)y * W;
+  mbr_framebuffer *fb = job->fb;
+  if (fb->rgb[0]) fb->rgb[0][idx] = (float)vol.R;
+  if (fb->rgb[1]) fb->rgb[1][idx] = (float)vol.G;
+  if (fb->rgb[2]) fb->rgb[2][idx] = (float)vol.B;
+  if (fb->z) fb->z[idx] = (float)zdepth;
+  if (fb->colour[0]) fb->colour[0][idx] = x86_d2u8(objectColour.R * 255);
+  if (fb->colour[1]) fb->colour[1][idx] = x86_d2u8(objectColour.G * 255);
+  if (fb->colour[2]) fb->colour[2][idx] = x86_d2u8(objectColour.B * 255);
+  if (fb->opacity) fb->opacity[idx] = x86_d2u16(opacity * 65535.0);
+  if (fb->alpha) fb->alpha[idx] = x86_d2u16(vol.alpha * 65535.0);
+}
+
+static void *render_worker(void *arg) {
+  render_job *job = (render_job *)arg;
+  step_rec *buf = (step_rec *)malloc(sizeof(step_rec) * 10002);
+  oracle_stats st; memset(&st, 0, sizeof st);
+  de_counter mc = {0, 0}, sc = {0, 0};
+  for (;;) {
+    int y = __sync_fetch_and_add(&job->next_row, 1);
+    if (y >= job->row_end) break;
+    for (int x = 0; x < job->s->image_width; x++) render_pixel(job, x, y, buf, &st, &mc, &sc);
+  }
+  free(buf);
+  st.march_de_calls = mc.calls; st.shade_de_calls = sc.calls; st.iterations = mc.iterations + sc.iterations;
+  pthread_mutex_lock(&job->lock);
+  job->st.rays += st.rays; job->st.hits += st.hits; job->st.missed_de += st.missed_de;
+  job->st.march_de_calls += st.march_de_calls; job->st.shade_de_calls += st.shade_de_calls;
+  job->st.iterations += st.iterations; job->st.total_steps += st.total_steps;
+  if (st.max_steps > job->st.max_steps) job->st.max_steps = st.max_steps;
+  pthread_mutex_unlock(&job->lock);
+  return NULL;
+}
+
+int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin, int row_end,
+                  int nthreads, int jitter, uint32_t glibc_seed, mbr_framebuffer *fb, oracle_stats *st) {
+  if (!s || !fb || s->image_width <= 0 || s->image_height <= 0 || s->tiles < 1) return MBR_ERR_INVALID_ARG;
+  if (row_end <= 0 || row_end > s->image_height) row_end = s->image_height;
+  if (row_begin < 0) row_begin = 0;
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  camera_t c; camera_init(&c, s);
+  double *dirs = (double *)malloc(sizeof(double) * 3 * 10000);
+  int *rgb = (int *)malloc(sizeof(int) * 3 * 10000);
+  int q = s->ao_quality; if (q >= 30) q = 30; if (q <= 1) q = 1;
+  int count = oracle_ao_table(q, lightmap, dirs, rgb, 10000);
+  render_job job;
+  memset(&job, 0, sizeof job);
+  job.s = s; job.f = &f; job.c = &c; job.ao_dirs = dirs; job.ao_rgb = rgb; job.ao_count = count;
+  job.fb = fb; job.row_begin = row_begin; job.row_end = row_end; job.next_row = row_begin;
+  job.jitter = jitter; job.glibc_seed = glibc_seed;
+  pthread_mutex_init(&job.lock, NULL);
+  if (jitter == ORACLE_JITTER_GLIBC) { nthreads = 1; srand(glibc_seed); }
+  if (nthreads <= 0) { long n = sysconf(_SC_NPROCESSORS_ONLN); nthreads = n > 0 ? (int)n : 1; }
+  if (nthreads == 1) {
+    render_worker(&job);
+  } else {
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int i = 0; i < nthreads; i++) pthread_create(&th[i], NULL, render_worker, &job);
+    for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+    free(th);
+  }
+  pthread_mutex_destroy(&job.lock);
+  if (st) *st = job.st;
+  free(dirs); free(rgb);
+  return MBR_OK;
+}

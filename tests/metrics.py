@@ -1,0 +1,45 @@
+"""Image comparison metrics of the parity tests.
+
+The bars (per config): at most 0.1% of pixels may flip hit/miss, the hit
+depth (zBuffer) relative error on pixels both renders hit must be <= 1e-4,
+and the RGB PSNR of imageFloat must be >= 45 dB (peak 1.0).
+"""
+import numpy as np
+
+MISS_Z = 1e19
+
+
+def compare(a, b):
+    """a, b: frames with .rgb (3,h,w) f32 and .z (h,w) f32 planes."""
+    hit_a = a.z < MISS_Z
+    hit_b = b.z < MISS_Z
+    n = hit_a.size
+    flips = int(np.count_nonzero(hit_a != hit_b))
+    both = hit_a & hit_b
+    if both.any():
+        za = a.z[both].astype(np.float64)
+        zb = b.z[both].astype(np.float64)
+        rel = np.abs(za - zb) / np.maximum(np.abs(zb), 1e-30)
+        # agreeing pixels: hit the same surface (depths within 1%); a pixel
+        # whose ray lands on another sheet of the surface is a flip
+        same = rel < 1e-2
+        flips += int(np.count_nonzero(~same))
+        rel_same = rel[same]
+        depth_max = float(rel_same.max()) if rel_same.size else 0.0
+        depth_p999 = float(np.quantile(rel_same, 0.999)) if rel_same.size else 0.0
+    else:
+        depth_max = depth_p999 = 0.0
+    d = a.rgb.astype(np.float64) - b.rgb.astype(np.float64)
+    mse = float(np.mean(d * d))
+    psnr = float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
+    return {
+        "pixels": n, "hits": int(hit_b.sum()), "flips": flips, "flip_frac": flips / n,
+        "depth_rel_max": depth_max, "depth_rel_p999": depth_p999, "psnr_db": psnr,
+        "rgb_max_abs": float(np.abs(d).max()),
+    }
+
+
+def assert_parity(m, flip_frac=1e-3, depth_rel=1e-4, psnr_db=45.0, label=""):
+    assert m["flip_frac"] <= flip_frac, f"{label}: hit/miss flips {m}"
+    assert m["depth_rel_max"] <= depth_rel, f"{label}: depth error {m}"
+    assert m["psnr_db"] >= psnr_db, f"{label}: PSNR {m}"
