@@ -96,6 +96,7 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   f.water_iterations = s.water_iterations;
   f.frame_no = s.frame_no;
   f.power = (T)s.power;
+  f.ipow = (s.power == std::floor(s.power) && s.power >= 2 && s.power <= 32) ? (int)s.power : 0;
   f.bailout = (T)s.bailout;
   f.constant_factor = (T)s.constant_factor;
   for (int i = 0; i < 3; i++) f.julia[i] = (T)s.julia[i];
@@ -252,6 +253,11 @@ struct mbr_ctx {
   BandBufs band{};
   int band_prec = -1;
   unsigned long long *d_stats = nullptr;
+  int cu_count = 0;
+  // shading buffers (per hit; element type of the band's precision)
+  void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
+  size_t sh_ao_bytes = 0;
+  unsigned long long *d_queue = nullptr;
   hipEvent_t ev[8] = {};
   // scratch planes of mbr_render (host-buffer entry point)
   void *d_planes = nullptr;
@@ -283,6 +289,11 @@ void free_band(mbr_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   b = BandBufs{};
+  void *sp[] = {c->sh_base, c->sh_spec, c->sh_col, c->sh_fast, c->sh_ao};
+  for (void *p : sp)
+    if (p) (void)hipFree(p);
+  c->sh_base = c->sh_spec = c->sh_col = c->sh_fast = c->sh_ao = nullptr;
+  c->sh_ao_bytes = 0;
   c->band_prec = -1;
 }
 
@@ -317,6 +328,10 @@ int alloc_band(mbr_ctx *c, int precision) {
   HIPCHECK(c, hipMalloc((void **)&b.col_r, n));
   HIPCHECK(c, hipMalloc((void **)&b.col_g, n));
   HIPCHECK(c, hipMalloc((void **)&b.col_b, n));
+  HIPCHECK(c, hipMalloc(&c->sh_base, 3 * n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc(&c->sh_spec, 3 * n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc(&c->sh_col, 3 * n * b.rec_bytes));
+  HIPCHECK(c, hipMalloc(&c->sh_fast, n * b.rec_bytes));
   b.ovf_capacity = 0;
   c->band_prec = precision;
   return MBR_OK;
@@ -389,6 +404,12 @@ int mbr_init(const mbr_device_opts *opts, mbr_ctx **out) {
     return MBR_ERR_HIP;
   }
   for (hipEvent_t &e : c->ev) (void)hipEventCreate(&e);
+  if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+    c->cu_count = 0;
+  if (hipMalloc((void **)&c->d_queue, sizeof(unsigned long long)) != hipSuccess) {
+    mbr_destroy(c);
+    return MBR_ERR_HIP;
+  }
   *out = c;
   return MBR_OK;
 }
@@ -398,7 +419,7 @@ void mbr_destroy(mbr_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_band(c);
-  void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
+  void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes, c->d_queue};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev)
@@ -598,7 +619,27 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
       overflow_rays += n_ovf;
     }
     HIPCHECK(c, hipEventRecord(c->ev[1], st));
-    HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, n_hits, ao, c->d_stats, st));
+    ShadeBufs sbuf;
+    sbuf.base = c->sh_base;
+    sbuf.spec = c->sh_spec;
+    sbuf.col = c->sh_col;
+    sbuf.fast_ao = c->sh_fast;
+    sbuf.n = b.cap;
+    sbuf.queue = c->d_queue;
+    sbuf.slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
+    sbuf.formula = c->scene.formula;
+    if (sbuf.slow_ao && n_hits > 0) {
+      const size_t need = (size_t)n_hits * (size_t)c->host_sc.ao_count * b.rec_bytes;
+      if (need > c->sh_ao_bytes) {
+        if (c->sh_ao) (void)hipFree(c->sh_ao);
+        c->sh_ao = nullptr;
+        c->sh_ao_bytes = 0;
+        HIPCHECK(c, hipMalloc(&c->sh_ao, need));
+        c->sh_ao_bytes = need;
+      }
+    }
+    sbuf.ao = c->sh_ao;
+    HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, n_hits, ao, sbuf, c->cu_count, c->d_stats, st));
     if (n_hits > 0) n_shade++;
     HIPCHECK(c, hipEventRecord(c->ev[2], st));
     HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));

@@ -42,6 +42,7 @@ struct Math<double> {
   static MBR_HD double sin(double x) { return ::sin(x); }
   static MBR_HD double log(double x) { return ::log(x); }
   static MBR_HD double fabs(double x) { return ::fabs(x); }
+  static MBR_HD double copysign(double a, double b) { return ::copysign(a, b); }
 };
 
 template <>
@@ -54,6 +55,7 @@ struct Math<float> {
   static MBR_HD float sin(float x) { return ::sinf(x); }
   static MBR_HD float log(float x) { return ::logf(x); }
   static MBR_HD float fabs(float x) { return ::fabsf(x); }
+  static MBR_HD float copysign(float a, float b) { return ::copysignf(a, b); }
 };
 
 template <class T>

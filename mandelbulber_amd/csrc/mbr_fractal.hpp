@@ -218,26 +218,203 @@ struct DEInfo {
   int work;  // formula iterations executed by this call
 };
 
+// ---------------------------------------------------------------------------
+// Compute<normal> as a resumable state: de_begin sets up the orbit, each
+// de_iter call runs one pass of the iteration loop body (pre-folds, formula,
+// termination test) with the statements of src/fractal.cpp:90-1191 in order.
+// Running de_iter to completion is Compute<normal>; the persistent kernels
+// advance many such states one iteration at a time.
 template <class T>
-MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
-                       DEInfo &info) {
-  info.object = OBJ_FRACTAL;
-  info.max_iter = 0;
-  info.work = 0;
-  T distance;
-  T limitBoxDist = 0;
-  if (f.limits) {
-    T da = point.x - f.limit_max[0], db = point.y - f.limit_max[1], dc = point.z - f.limit_max[2];
-    T ta = -(point.x - f.limit_min[0]), tb = -(point.y - f.limit_min[1]), tc = -(point.z - f.limit_min[2]);
-    da = (da > ta) ? da : ta;  // MAX()
-    db = (db > tb) ? db : tb;
-    dc = (dc > tc) ? dc : tc;
-    limitBoxDist = (da > db) ? ((da > dc) ? da : dc) : ((db > dc) ? db : dc);  // dMax
-    if (limitBoxDist > detailSize) return limitBoxDist;
+struct DEIter {
+  V3<T> z, c;
+  T r, aux;  // aux: r_dz (trig formulas) or tgladDE (folding formulas)
+  int L;
+};
+
+template <class T, int F>
+MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
+  s.z = p;
+  s.c = f.julia_mode ? ld3(f.julia) : p * f.constant_factor;
+  s.r = length(p);
+  s.aux = 1;
+  s.L = 0;
+}
+
+// One loop pass.  Returns true when the call has finished; `dist` is then
+// the return value of Compute<normal> (0 when the orbit stayed bounded for
+// N iterations) and s.L the reference's iteration counter.
+template <class T, int F>
+MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
+  using M = Math<T>;
+  V3<T> z = s.z;
+  T r = s.r;
+  if (f.tglad_fold) {
+    const T lim = f.fold_limit, val = f.fold_value;
+    if (z.x > lim) z.x = val - z.x;
+    else if (z.x < -lim) z.x = -val - z.x;
+    if (z.y > lim) z.y = val - z.y;
+    else if (z.y < -lim) z.y = -val - z.y;
+    if (z.z > lim) z.z = val - z.z;
+    else if (z.z < -lim) z.z = -val - z.z;
+    r = length(z);
   }
-  if (!f.only_plane) {
-    int L;
-    distance = compute<T, MODE_NORMAL>(f, point, N, &L);
+  if (f.sph_fold) {
+    T r2 = r * r;
+    if (r2 < f.sph_mR2) { z = z * f.sph_factor1; s.aux *= f.sph_factor1; }
+    else if (r2 < f.sph_fR2) { T tf2 = f.sph_fR2 / r2; z = z * tf2; s.aux *= tf2; }
+    r = length(z);
+  }
+  bool esc;
+  if constexpr (F == 2 && sizeof(T) < 8) {
+    if (f.ipow) {
+      // FP32, integer power p: the same map without inverse trig.  With
+      // rho = |(x,y)|, (cos th, sin th) = (rho, z)/r and (cos ph, sin ph) =
+      // (x, y)/rho, so cos/sin(p th) and cos/sin(p ph) are the parts of the
+      // complex powers (rho + i z)^p / r^p and (x + i y)^p / rho^p
+      // (de Moivre), formed by binary exponentiation.
+      const int p = f.ipow;
+      const T rho = M::sqrt(z.x * z.x + z.y * z.y);
+      const T inv_r = T(1) / r;
+      T ure = rho * inv_r, uim = z.z * inv_r;  // e^{i th}
+      T wre, wim;                               // e^{i ph}
+      if (rho > T(0)) {
+        const T inv_rho = T(1) / rho;
+        wre = z.x * inv_rho;
+        wim = z.y * inv_rho;
+      } else {
+        // atan2(+-0, x) is +-0 for x = +0 and +-pi for x = -0
+        wre = Math<T>::copysign(T(1), z.x);
+        wim = T(0);
+      }
+      T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1, rb = r;
+      int e = p - 1;  // r^(p-1)
+      while (e) {
+        if (e & 1) rp1 *= rb;
+        e >>= 1;
+        if (e) rb *= rb;
+      }
+      e = p;
+      while (e) {
+        if (e & 1) {
+          T t = pure * ure - puim * uim; puim = pure * uim + puim * ure; pure = t;
+          t = pwre * wre - pwim * wim; pwim = pwre * wim + pwim * wre; pwre = t;
+        }
+        e >>= 1;
+        if (e) {
+          T t = ure * ure - uim * uim; uim = T(2) * ure * uim; ure = t;
+          t = wre * wre - wim * wim; wim = T(2) * wre * wim; wre = t;
+        }
+      }
+      s.aux = rp1 * s.aux * T(p) + T(1);
+      const T rp = rp1 * r;
+      z = V3<T>{pure * pwre, pure * pwim, puim} * rp + s.c;
+      r = length(z);
+      esc = !(r <= f.bailout);
+      if (esc) dist = T(0.5) * r * M::log(r) / s.aux;
+      s.z = z;
+      s.r = r;
+      if (esc) return true;
+      s.L++;
+      if (s.L >= N) { dist = 0; return true; }
+      return false;
+    }
+  }
+  if constexpr (F == 2) {  // trig_optim
+    const T p = f.power;
+    T th0 = M::asin(z.z / r);
+    T ph0 = M::atan2(z.y, z.x);
+    T rp = M::pow(r, p - T(1));
+    T th = th0 * p;
+    T ph = ph0 * p;
+    T cth = M::cos(th);
+    s.aux = rp * s.aux * p + T(1);
+    rp *= r;
+    z = V3<T>{cth * M::cos(ph), cth * M::sin(ph), M::sin(th)} * rp + s.c;
+    r = length(z);
+    esc = !(r <= f.bailout);
+    if (esc) dist = T(0.5) * r * M::log(r) / s.aux;
+  } else if constexpr (F == 7) {  // menger_sponge
+    T temp;
+    z.x = M::fabs(z.x); z.y = M::fabs(z.y); z.z = M::fabs(z.z);
+    if (z.x - z.y < 0) { temp = z.y; z.y = z.x; z.x = temp; }
+    if (z.x - z.z < 0) { temp = z.z; z.z = z.x; z.x = temp; }
+    if (z.y - z.z < 0) { temp = z.z; z.z = z.y; z.y = temp; }
+    z = z * T(3);
+    z.x -= T(2); z.y -= T(2);
+    if (z.z > T(1)) z.z -= T(2);
+    r = length(z);
+    s.aux *= T(3);
+    esc = !(r <= T(1000));
+    if (esc) dist = (r - T(2)) / s.aux;
+  } else {  // tglad (Mandelbox)
+    const T lim = f.mb_limit, val = f.mb_value;
+    if (f.mb_rot_on) {
+      bool lockout = false;
+      z = rot9(f.mb_rot[0][0], z);
+      if (z.x > lim) { z.x = val - z.x; lockout = true; }
+      z = rot9(f.mb_rotinv[0][0], z);
+      z = rot9(f.mb_rot[1][0], z);
+      if (!lockout && z.x < -lim) z.x = -val - z.x;
+      z = rot9(f.mb_rotinv[1][0], z);
+      lockout = false;
+      z = rot9(f.mb_rot[0][1], z);
+      if (z.y > lim) { z.y = val - z.y; lockout = true; }
+      z = rot9(f.mb_rotinv[0][1], z);
+      z = rot9(f.mb_rot[1][1], z);
+      if (!lockout && z.y < -lim) z.y = -val - z.y;
+      z = rot9(f.mb_rotinv[1][1], z);
+      lockout = false;
+      z = rot9(f.mb_rot[0][2], z);
+      if (z.z > lim) { z.z = val - z.z; lockout = true; }
+      z = rot9(f.mb_rotinv[0][2], z);
+      z = rot9(f.mb_rot[1][2], z);
+      if (!lockout && z.z < -lim) z.z = -val - z.z;
+      z = rot9(f.mb_rotinv[1][2], z);
+    } else {
+      if (z.x > lim) z.x = val - z.x;
+      else if (z.x < -lim) z.x = -val - z.x;
+      if (z.y > lim) z.y = val - z.y;
+      else if (z.y < -lim) z.y = -val - z.y;
+      if (z.z > lim) z.z = val - z.z;
+      else if (z.z < -lim) z.z = -val - z.z;
+    }
+    r = length(z);
+    T r2 = r * r;
+    z = z + ld3(f.mb_offset);
+    if (r2 < f.mb_mR2) { z = z * f.mb_factor1; s.aux *= f.mb_factor1; }
+    else if (r2 < f.mb_fR2) { T tf2 = f.mb_fR2 / r2; z = z * tf2; s.aux *= tf2; }
+    z = z - ld3(f.mb_offset);
+    if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
+    z = z * f.mb_scale + s.c;
+    s.aux = s.aux * f.mb_abs_scale + T(1);
+    r = length(z);
+    esc = !(r <= T(1024));
+    if (esc) dist = r / M::fabs(s.aux);
+  }
+  s.z = z;
+  s.r = r;
+  if (esc) return true;
+  s.L++;
+  if (s.L >= N) { dist = 0; return true; }
+  return false;
+}
+
+// CalculateDistance (src/fractal.cpp:1245-1448), analytic branch, split into
+// a start (limits early-out, primitive-only scenes) and a finish (clamp, minN
+// rule, interior mode, iteration threshold, primitives, limits) around the
+// resumable Compute<normal>.
+template <class T>
+struct DECall {
+  DEIter<T> it;
+  V3<T> p;
+  T limitBox;
+};
+
+template <class T>
+MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fractal, int N, int minN,
+                   bool normalMode, T detailSize, T limitBoxDist, DEInfo &info) {
+  info.object = OBJ_FRACTAL;
+  if (fractal) {
     info.work = (L < N) ? L + 1 : N;
     info.max_iter = (L == N);
     info.iters = L;
@@ -253,6 +430,8 @@ MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, i
     }
   } else {
     distance = 10;
+    info.max_iter = 0;
+    info.work = 0;
   }
   if (f.prim_plane) {
     T d = dot(ld3(f.plane_n), point - ld3(f.plane_c));
@@ -288,6 +467,66 @@ MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, i
     if (limitBoxDist < detailSize) distance = (distance > limitBoxDist) ? distance : limitBoxDist;
   }
   return distance;
+}
+
+// Start one CalculateDistance; true when it finished without iterating.
+template <class T, int F>
+MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
+                          DECall<T> &call, T &dist, DEInfo &info) {
+  info.object = OBJ_FRACTAL;
+  info.max_iter = 0;
+  info.work = 0;
+  info.iters = 0;
+  call.p = point;
+  call.limitBox = 0;
+  if (f.limits) {
+    T da = point.x - f.limit_max[0], db = point.y - f.limit_max[1], dc = point.z - f.limit_max[2];
+    T ta = -(point.x - f.limit_min[0]), tb = -(point.y - f.limit_min[1]), tc = -(point.z - f.limit_min[2]);
+    da = (da > ta) ? da : ta;  // MAX()
+    db = (db > tb) ? db : tb;
+    dc = (dc > tc) ? dc : tc;
+    call.limitBox = (da > db) ? ((da > dc) ? da : dc) : ((db > dc) ? db : dc);  // dMax
+    if (call.limitBox > detailSize) { dist = call.limitBox; return true; }
+  }
+  if (f.only_plane) {
+    dist = de_finish<T>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, call.limitBox, info);
+    return true;
+  }
+  de_begin<T, F>(f, point, call.it);
+  if (N <= 0) {
+    dist = de_finish<T>(f, point, T(0), 0, true, N, minN, normalMode, detailSize, call.limitBox, info);
+    return true;
+  }
+  return false;
+}
+
+// Advance one iteration; true when the call finished (dist, info valid).
+template <class T, int F>
+MBR_HD bool de_call_step(const FractalK<T> &f, DECall<T> &call, T detailSize, int N, int minN, bool normalMode,
+                         T &dist, DEInfo &info) {
+  T raw = 0;
+  if (!de_iter<T, F>(f, call.it, N, raw)) return false;
+  dist = de_finish<T>(f, call.p, raw, call.it.L, true, N, minN, normalMode, detailSize, call.limitBox, info);
+  return true;
+}
+
+template <class T, int F>
+MBR_HD T calc_distance_f(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
+                         DEInfo &info) {
+  DECall<T> call;
+  T dist = 0;
+  if (de_call_begin<T, F>(f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
+  while (!de_call_step<T, F>(f, call, detailSize, N, minN, normalMode, dist, info)) {
+  }
+  return dist;
+}
+
+template <class T>
+MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
+                       DEInfo &info) {
+  if (f.formula == 7) return calc_distance_f<T, 7>(f, point, detailSize, N, minN, normalMode, info);
+  if (f.formula == 8) return calc_distance_f<T, 8>(f, point, detailSize, N, minN, normalMode, info);
+  return calc_distance_f<T, 2>(f, point, detailSize, N, minN, normalMode, info);
 }
 
 }  // namespace mbr

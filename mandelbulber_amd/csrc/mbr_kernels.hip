@@ -262,9 +262,38 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
 }
 
 // ----------------------------------------------------------------- shade
-template <class T, class RT>
-__global__ void __launch_bounds__(kBlock) k_shade(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                  int n_hits, AODev ao, unsigned long long *stats) {
+// ObjectShader (src/shaders.cpp:21-112) runs as three launches:
+//   k_surface  one lane per hit: normals, shadow, specular, palette colour
+//              and, in fast mode, FastAmbientOcclusion2;
+//   k_ao       persistent: the AmbientOcclusion rays of all hits;
+//   k_compose  one lane per hit: AO sum in direction order and the final
+//              ObjectShader composition.
+template <class T>
+struct SurfBufs {
+  T *base;    // [3][n] mainLight * shade * shadow
+  T *spec;    // [3][n] mainLight * specular * shadow
+  T *col;     // [3][n] surface colour
+  T *fast_ao; // [n]
+  T *ao;      // [n * dirs] AO intensity per (hit, direction)
+  long long n;
+};
+
+template <class T>
+__device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, int r, ShaderIn<T> &in) {
+  const ShadeK<T> &k = sc.shade<T>();
+  in.point = v3<T>((T)b.px[r], (T)b.py[r], (T)b.pz[r]);
+  in.light = v3<T>(k.light[0], k.light[1], k.light[2]);
+  in.dist_thresh = (T)b.thr[r];
+  in.lastDist = (T)b.last[r];
+  in.depth = (T)b.depth[r];
+  in.delta = sc.constant_thresh ? (T)(b.depth[r] * sc.sd.resolution * sc.sd.persp) : (T)(b.thr[r] * sc.sd.quality);
+  in.detail = in.dist_thresh;
+  in.object = b.object[r];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                    int n_hits, SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
   const int tid = blockIdx.x * kBlock + threadIdx.x;
   LaneCount cnt = {0, 0};
@@ -276,24 +305,30 @@ __global__ void __launch_bounds__(kBlock) k_shade(const DevScene *__restrict__ s
     view_vector(sc, x, y, dir);
     const ShadeK<T> &k = sc.shade<T>();
     ShaderIn<T> in;
-    in.point = v3<T>((T)b.px[r], (T)b.py[r], (T)b.pz[r]);
+    load_hit(sc, b, r, in);
     in.view = cvt<T>(dir);
-    in.light = v3<T>(k.light[0], k.light[1], k.light[2]);
-    in.dist_thresh = (T)b.thr[r];
-    in.lastDist = (T)b.last[r];
-    in.depth = (T)b.depth[r];
-    in.delta = sc.constant_thresh ? (T)(b.depth[r] * sc.sd.resolution * sc.sd.persp) : (T)(b.thr[r] * sc.sd.quality);
-    in.detail = in.dist_thresh;
-    in.object = b.object[r];
-    AOTable<T> tab;
-    if constexpr (sizeof(T) == 8) tab.dir = ao.dir_d; else tab.dir = ao.dir_f;
-    tab.rgb = ao.rgb;
-    tab.count = ao.count;
-    RGBT<T> colour;
-    RGBT<T> px = object_shader<T>(sc, in, tab, colour, cnt);
-    ((RT *)b.sh_r)[r] = (RT)px.R;
-    ((RT *)b.sh_g)[r] = (RT)px.G;
-    ((RT *)b.sh_b)[r] = (RT)px.B;
+    in.normal = normals<T>(sc, in, cnt);
+    T sh = dot(in.normal, in.light);
+    if (sh < 0) sh = 0;
+    const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
+    T shadow = 1;
+    if (sc.shadow) shadow = main_shadow<T>(sc, in, cnt);
+    V3<T> half = normalize(in.light - in.view);
+    T shade2 = dot(in.normal, half);
+    if (shade2 < T(0)) shade2 = 0;
+    shade2 = Math<T>::pow(shade2, T(30));
+    if (shade2 > T(15)) shade2 = 15;
+    const T specular = shade2 * k.specular;
+    RGBT<T> colour = surface_colour<T>(sc, in, cnt);
+    const long long n = sb.n;
+    for (int c = 0; c < 3; c++) {
+      sb.base[c * n + tid] = k.main_light[c] * shade * shadow;
+      sb.spec[c * n + tid] = k.main_light[c] * specular * shadow;
+    }
+    sb.col[0 * n + tid] = colour.R;
+    sb.col[1 * n + tid] = colour.G;
+    sb.col[2 * n + tid] = colour.B;
+    if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T>(sc, in, cnt);
     b.col_r[r] = x86_d2u8((double)colour.R * 255);
     b.col_g[r] = x86_d2u8((double)colour.G * 255);
     b.col_b[r] = x86_d2u8((double)colour.B * 255);
@@ -302,6 +337,175 @@ __global__ void __launch_bounds__(kBlock) k_shade(const DevScene *__restrict__ s
   v[ST_SHADE_DE] = cnt.de_calls;
   v[ST_SHADE_ITERS] = cnt.iterations;
   flush_stats(stats, v);
+}
+
+// AmbientOcclusion (src/shaders.cpp:607-673) for all (hit, direction) pairs
+// of a band.  Each lane holds one AO ray and advances it one formula
+// iteration per loop pass: when its distance estimate completes it takes the
+// ray's next step, and when the ray ends it writes the ray's intensity and
+// takes the next (hit, direction) item.  Items are handed out per wave from
+// a global counter in chunks, so every lane of every wave does formula work
+// until the band's AO rays run out, whatever the spread of per-ray step and
+// iteration counts.
+constexpr unsigned kAOChunk = 256;
+
+template <class T, int F>
+__global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, int n_hits, AODev ao,
+                                               SurfBufs<T> sb, unsigned long long *queue,
+                                               unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const FractalK<T> &f = sc.frac<T>();
+  const ShadeK<T> &k = sc.shade<T>();
+  const T *dirs;
+  if constexpr (sizeof(T) == 8) dirs = ao.dir_d; else dirs = ao.dir_f;
+  const unsigned count = (unsigned)ao.count;
+  const unsigned long long total = (unsigned long long)n_hits * count;
+  const int N = sc.N, minN = sc.minN;
+  const int lane = __lane_id();
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  unsigned long long wnext = 0, wend = 0;  // wave-uniform chunk cursor
+  bool active = false, exhausted = false;
+  unsigned long long item = 0;
+  V3<T> P = v3<T>(0, 0, 0), dir = P;
+  T r = 0, end_dist = 1, detail = 0, thr_f = 0;
+  int guard = 0;
+  DECall<T> call;
+  LaneCount cnt = {0, 0};
+
+  for (;;) {
+    const bool need = !active && !exhausted;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      // hand out the next items to the lanes that need one, in lane order
+      const unsigned n = (unsigned)__popcll(mask);
+      const unsigned off = (unsigned)__popcll(mask & lt_mask);
+      const unsigned long long avail = wend - wnext;
+      unsigned long long mine = wnext + off;
+      if (n > avail) {
+        unsigned long long cbase = 0;
+        if (lane == 0) cbase = atomicAdd(queue, (unsigned long long)kAOChunk);
+        cbase = __shfl(cbase, 0);
+        if (off >= avail) mine = cbase + (off - avail);
+        wnext = cbase + (n - avail);
+        wend = cbase + kAOChunk;
+      } else {
+        wnext += n;
+      }
+      if (need) {
+        if (mine >= total) {
+          exhausted = true;
+        } else {
+          item = mine;
+          const unsigned h = (unsigned)(item / count), d = (unsigned)(item - (unsigned long long)h * count);
+          const int ray = b.hit_list[h];
+          ShaderIn<T> in;
+          load_hit(sc, b, ray, in);
+          P = in.point;
+          dir = v3<T>(dirs[4 * d + 0], dirs[4 * d + 1], dirs[4 * d + 2]);
+          r = in.delta;                      // start_dist
+          end_dist = in.delta / k.resolution;
+          detail = in.detail;
+          thr_f = (T)(float)in.dist_thresh;  // float dist_thresh (src/shaders.cpp:642)
+          guard = 0;
+          if (r < end_dist) {
+            DEInfo info;
+            T dist;
+            if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info)) {
+              // finished without iterating: treat as the step's result below
+              active = true;
+              call.it.L = -1;  // marks a completed call
+              call.it.r = dist;
+              call.it.aux = (T)info.max_iter;
+            } else {
+              active = true;
+            }
+          } else {
+            sb.ao[item] = T(1);
+          }
+        }
+      }
+    }
+    if (!__any(active || !exhausted)) break;
+    if (active) {
+      T dist = 0;
+      DEInfo info;
+      bool done;
+      int mi;
+      if (call.it.L < 0) {
+        done = true;
+        dist = call.it.r;
+        mi = call.it.aux != T(0);
+        info.work = 0;
+      } else {
+        done = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
+        mi = info.max_iter;
+      }
+      if (done) {
+        cnt.de_calls += 1;
+        cnt.iterations += (uint32_t)info.work;
+        if (dist < thr_f || mi) {
+          T shadowTemp = T(1) - (end_dist - r) / end_dist;
+          if (shadowTemp < T(0)) shadowTemp = 0;
+          sb.ao[item] = shadowTemp;
+          active = false;
+        } else {
+          r += dist * T(2);
+          guard++;
+          if (!(r < end_dist) || guard >= kMaxShadeSteps) {
+            sb.ao[item] = T(1);
+            active = false;
+          } else {
+            T d2;
+            DEInfo i2;
+            if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, d2, i2)) {
+              call.it.L = -1;
+              call.it.r = d2;
+              call.it.aux = (T)i2.max_iter;
+            }
+          }
+        }
+      }
+    }
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_SHADE_DE] = cnt.de_calls;
+  v[ST_SHADE_ITERS] = cnt.iterations;
+  flush_stats(stats, v);
+}
+
+template <class T, class RT>
+__global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__ scp, BandBufs b, int n_hits, AODev ao,
+                                                    SurfBufs<T> sb) {
+  const DevScene &sc = *scp;
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= n_hits) return;
+  const ShadeK<T> &k = sc.shade<T>();
+  const int r = b.hit_list[tid];
+  const long long n = sb.n;
+  RGBT<T> amb = {0, 0, 0};
+  if (sc.ao) {
+    if (sc.fast_ao) {
+      const T a = sb.fast_ao[tid];
+      amb = RGBT<T>{a, a, a};
+    } else {
+      const T *it = sb.ao + (long long)tid * ao.count;
+      for (int v = 0; v < ao.count; v++) {
+        const T s = it[v];
+        amb.R += s * T(ao.rgb[4 * v + 0]);
+        amb.G += s * T(ao.rgb[4 * v + 1]);
+        amb.B += s * T(ao.rgb[4 * v + 2]);
+      }
+      const T den = T(ao.count) * T(256);
+      amb.R /= den;
+      amb.G /= den;
+      amb.B /= den;
+    }
+  }
+  const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
+  ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
+  ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
+  ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
 }
 
 // ---------------------------------------------------------------- finish
@@ -374,15 +578,36 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
 }
 
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits, AODev ao,
-                        unsigned long long *stats, hipStream_t s) {
+                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s) {
   if (n_hits <= 0) return hipSuccess;
-  if (precision == 0)
-    hipLaunchKernelGGL((k_shade<double, double>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, band_first, n_hits, ao,
-                       stats);
-  else
-    hipLaunchKernelGGL((k_shade<float, float>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, band_first, n_hits, ao,
-                       stats);
-  return hipGetLastError();
+  const int formula = sbuf.formula;
+  // persistent AO grid: enough workgroups to fill every CU several times
+  const dim3 ao_grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
+  auto run = [&](auto tag) -> hipError_t {
+    using T = decltype(tag);
+    SurfBufs<T> sb;
+    sb.base = (T *)sbuf.base;
+    sb.spec = (T *)sbuf.spec;
+    sb.col = (T *)sbuf.col;
+    sb.fast_ao = (T *)sbuf.fast_ao;
+    sb.ao = (T *)sbuf.ao;
+    sb.n = sbuf.n;
+    hipLaunchKernelGGL((k_surface<T>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, band_first, n_hits, sb, stats);
+    if (sbuf.slow_ao) {
+      hipError_t e = hipMemsetAsync(sbuf.queue, 0, sizeof(unsigned long long), s);
+      if (e != hipSuccess) return e;
+      if (formula == 7)
+        hipLaunchKernelGGL((k_ao<T, 7>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_ao<T, 8>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+      else
+        hipLaunchKernelGGL((k_ao<T, 2>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+    }
+    hipLaunchKernelGGL((k_compose<T, T>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, n_hits, ao, sb);
+    return hipGetLastError();
+  };
+  if (precision == 0) return run(double{});
+  return run(float{});
 }
 
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,

@@ -56,8 +56,16 @@ struct AODev {
 // precision: 0 = FP64, 1 = FP32 march/shade with FP64 fixup
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                         const int *list, int n_list, unsigned long long *stats, hipStream_t s);
-hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits,
-                        AODev ao, unsigned long long *stats, hipStream_t s);
+// Buffers of the shading launches (element type T = the path's precision).
+struct ShadeBufs {
+  void *base, *spec, *col, *fast_ao, *ao;
+  long long n;                  // hit capacity (plane stride)
+  unsigned long long *queue;    // AO work counter
+  int slow_ao;                  // 1: run the AmbientOcclusion rays
+  int formula;
+};
+hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits, AODev ao,
+                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);
 

@@ -19,7 +19,8 @@ struct FractalK {
   int julia_mode, tglad_fold, sph_fold, mb_rot_on, mb_main_rot;
   int interior, iter_thresh, limits, only_plane;
   int prim_plane, prim_box, prim_ibox, prim_sphere, prim_isphere, prim_water;
-  int water_iterations, frame_no, water_rotated, pad0;
+  int water_iterations, frame_no, water_rotated;
+  int ipow;  // power as an integer in [2, 32] when it is one, else 0
   T power, bailout, constant_factor;
   T julia[3];
   T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;

@@ -1006,6 +1006,86 @@ int oracle_march(const mbr_scene *s, const double start[3], const double dir[3],
   return 0;
 }
 
+void oracle_rotation3(const double angles[3], double m[9]) {
+  rotm r;
+  rot_set3(&r, angles);
+  memcpy(m, r.m, sizeof r.m);
+}
+
+void oracle_volumetric(const mbr_scene *s, double depth, const double *steps, int n,
+                       const double pixel[4], double out[4], double *opacity) {
+  step_rec *buf = (step_rec *)calloc(n > 0 ? n : 1, sizeof(step_rec));
+  for (int i = 0; i < n; i++) {
+    buf[i].step = steps[3 * i + 0];
+    buf[i].distance = steps[3 * i + 1];
+    buf[i].distThresh = steps[3 * i + 2];
+  }
+  shade_t px = {pixel[0], pixel[1], pixel[2], pixel[3]};
+  shade_t o = volumetric_shader(s, depth, buf, n, px, opacity);
+  out[0] = o.R; out[1] = o.G; out[2] = o.B; out[3] = o.alpha;
+  free(buf);
+}
+
+void oracle_index_to_colour(const mbr_scene *s, int index, int rgb[3]) {
+  mbr_rgb c = index_to_colour(index, s->palette);
+  rgb[0] = c.R; rgb[1] = c.G; rgb[2] = c.B;
+}
+
+void oracle_background(const mbr_scene *s, const double view[3], double rgb[3]) {
+  shade_t b = background_shader(s, v3from(view));
+  rgb[0] = b.R; rgb[1] = b.G; rgb[2] = b.B;
+}
+
+int oracle_object_shader(const mbr_scene *s, const double point[3], const double view[3],
+                         double dist_thresh, double depth, double last_dist, int object,
+                         oracle_shade_out *out) {
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  camera_t c; camera_init(&c, s);
+  double *dirs = (double *)malloc(sizeof(double) * 3 * 10000);
+  int *rgb = (int *)malloc(sizeof(int) * 3 * 10000);
+  int q = s->ao_quality; if (q >= 30) q = 30; if (q <= 1) q = 1;
+  int count = oracle_ao_table(q, NULL, dirs, rgb, 10000);
+  shader_in in;
+  memset(&in, 0, sizeof in);
+  in.f = &f; in.c = &c;
+  in.point = v3from(point);
+  in.viewVector = v3from(view);
+  in.lightVect = c.light;
+  in.dist_thresh = dist_thresh;
+  in.delta = s->constant_DE_threshold ? depth * c.resolution * s->persp : dist_thresh * s->quality;
+  in.detail = dist_thresh;
+  in.lastDist = last_dist;
+  in.depth = depth;
+  in.objectType = object;
+  in.ao_dirs = dirs; in.ao_rgb = rgb; in.ao_count = count;
+  shade_t colour, spec;
+  shade_t o = object_shader(&in, &colour, &spec);
+  out->normal[0] = in.normal.x; out->normal[1] = in.normal.y; out->normal[2] = in.normal.z;
+  /* recompute the scalar terms for inspection (same functions, same state) */
+  double sh = v3dot(in.normal, in.lightVect);
+  if (sh < 0) sh = 0;
+  out->shade = s->direct_light * ((1.0 - s->shading) + s->shading * sh);
+  in.detail = dist_thresh;
+  out->shadow = s->shadow ? main_shadow(&in) : 1.0;
+  v3 half = v3norm(v3sub(in.lightVect, in.viewVector));
+  double s2 = v3dot(in.normal, half);
+  if (s2 < 0) s2 = 0;
+  s2 = pow(s2, 30.0);
+  if (s2 > 15.0) s2 = 15.0;
+  out->specular = s2 * s->specular;
+  shade_t ao = {0, 0, 0, 0};
+  if (s->global_illumination) {
+    if (s->fast_global_illumination) { double a = fast_ao2(&in); ao.R = ao.G = ao.B = a; }
+    else ao = ambient_occlusion(&in);
+  }
+  out->ao[0] = ao.R; out->ao[1] = ao.G; out->ao[2] = ao.B;
+  out->colour[0] = colour.R; out->colour[1] = colour.G; out->colour[2] = colour.B;
+  out->rgb[0] = o.R; out->rgb[1] = o.G; out->rgb[2] = o.B;
+  out->spec_rgb[0] = spec.R; out->spec_rgb[1] = spec.G; out->spec_rgb[2] = spec.B;
+  free(dirs); free(rgb);
+  return 0;
+}
+
 /* ---------------------------------------------------------------- render */
 typedef struct {
   const mbr_scene *s;

@@ -67,6 +67,34 @@ int oracle_ao_table(int quality, const mbr_texture *lightmap, double *dirs, int 
 int oracle_march(const mbr_scene *s, const double start[3], const double dir[3],
                  uint32_t pixel, oracle_march_out *out);
 
+/* CRotationMatrix::SetRotation(angles[3]) (src/algebra.cpp:145-151), row-major */
+void oracle_rotation3(const double angles[3], double m[9]);
+
+/* VolumetricShader (src/shaders.cpp:185-478) over a synthetic step record:
+ * steps[i*3 + {0,1,2}] = {step, distance, distThresh} for i < n.  pixel and
+ * out are {R, G, B, alpha}; *opacity receives opacityOut.R (0 when no step
+ * group was emitted). */
+void oracle_volumetric(const mbr_scene *s, double depth, const double *steps, int n,
+                       const double pixel[4], double out[4], double *opacity);
+
+/* IndexToColour (src/shaders.cpp:1273-1305) with the scene's palette */
+void oracle_index_to_colour(const mbr_scene *s, int index, int rgb[3]);
+
+/* BackgroundShader gradient (src/shaders.cpp:159-181) for a view vector */
+void oracle_background(const mbr_scene *s, const double view[3], double rgb[3]);
+
+/* ObjectShader terms at a hit point (src/shaders.cpp:21-112): normal[3],
+ * shade, shadow, specular, ao[3], colour[3] and the shader output rgb[3]
+ * (without the specular term, which is spec_rgb[3]). */
+typedef struct oracle_shade_out {
+  double normal[3];
+  double shade, shadow, specular;
+  double ao[3], colour[3], rgb[3], spec_rgb[3];
+} oracle_shade_out;
+int oracle_object_shader(const mbr_scene *s, const double point[3], const double view[3],
+                         double dist_thresh, double depth, double last_dist, int object,
+                         oracle_shade_out *out);
+
 /* Render rows [row_begin,row_end) of the scene into host planes (planes
  * sized for the whole image).  nthreads <= 0 uses the host core count. */
 int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin, int row_end,

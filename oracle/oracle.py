@@ -51,7 +51,55 @@ def _load(name):
     lib.oracle_render.argtypes = [S, P(_abi.Texture), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
                                   P(_abi.Framebuffer), P(OracleStats)]
     lib.oracle_render.restype = C.c_int
+    lib.oracle_rotation3.argtypes = [D3, C.c_double * 9]
+    lib.oracle_volumetric.argtypes = [S, C.c_double, P(C.c_double), C.c_int, C.c_double * 4, C.c_double * 4,
+                                      P(C.c_double)]
+    lib.oracle_index_to_colour.argtypes = [S, C.c_int, C.c_int * 3]
+    lib.oracle_background.argtypes = [S, D3, D3]
+    lib.oracle_object_shader.argtypes = [S, D3, D3, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         P(ShadeOut)]
+    lib.oracle_object_shader.restype = C.c_int
     return lib
+
+
+class ShadeOut(C.Structure):
+    _fields_ = [("normal", C.c_double * 3), ("shade", C.c_double), ("shadow", C.c_double),
+                ("specular", C.c_double), ("ao", C.c_double * 3), ("colour", C.c_double * 3),
+                ("rgb", C.c_double * 3), ("spec_rgb", C.c_double * 3)]
+
+
+def rotation3(angles):
+    m = (C.c_double * 9)()
+    lib().oracle_rotation3((C.c_double * 3)(*angles), m)
+    return np.array(m[:]).reshape(3, 3)
+
+
+def volumetric(scene, depth, steps, pixel):
+    steps = np.ascontiguousarray(np.asarray(steps, dtype=np.float64).reshape(-1, 3))
+    out = (C.c_double * 4)()
+    op = C.c_double(0)
+    lib().oracle_volumetric(C.byref(scene), depth, steps.ctypes.data_as(C.POINTER(C.c_double)), len(steps),
+                            (C.c_double * 4)(*pixel), out, C.byref(op))
+    return np.array(out[:]), op.value
+
+
+def index_to_colour(scene, index):
+    o = (C.c_int * 3)()
+    lib().oracle_index_to_colour(C.byref(scene), index, o)
+    return tuple(o)
+
+
+def background(scene, view):
+    o = (C.c_double * 3)()
+    lib().oracle_background(C.byref(scene), (C.c_double * 3)(*view), o)
+    return np.array(o[:])
+
+
+def object_shader(scene, point, view, dist_thresh, depth, last_dist=0.0, obj=0):
+    o = ShadeOut()
+    lib().oracle_object_shader(C.byref(scene), (C.c_double * 3)(*point), (C.c_double * 3)(*view), dist_thresh,
+                               depth, last_dist, obj, C.byref(o))
+    return o
 
 
 _libs = {}
