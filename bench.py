@@ -166,6 +166,10 @@ def main():
         "speedup_vs_fp64": value / value64,
         "pixel_diff_vs_fp64": {k: diff[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
         "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "finish": s0.ms_finish},
+        "march_counters": {"fp64_redo_rays": s0.fp64_redo_rays, "step_overflow_rays": s0.step_overflow_rays,
+                           "march_de_calls": s0.march_de_calls, "shade_de_calls": s0.shade_de_calls,
+                           "fp64_redo_rays_fp64_path": st64[-1].fp64_redo_rays,
+                           "step_overflow_rays_fp64_path": st64[-1].step_overflow_rays},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -285,6 +285,7 @@ void free_band(mbr_ctx *c) {
   BandBufs &b = c->band;
   void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
                   b.rec_thr, b.ovf_off, b.hit_list, b.ovf_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
+                  b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift,
                   b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -321,7 +322,12 @@ int alloc_band(mbr_ctx *c, int precision) {
   HIPCHECK(c, hipMalloc((void **)&b.ovf_off, n * sizeof(long long)));
   HIPCHECK(c, hipMalloc((void **)&b.hit_list, n * sizeof(int)));
   HIPCHECK(c, hipMalloc((void **)&b.ovf_list, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.counters, 4 * sizeof(unsigned long long)));
+  HIPCHECK(c, hipMalloc((void **)&b.counters, CNT_COUNT * sizeof(unsigned long long)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_ray, n * sizeof(int)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_i, n * sizeof(int)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_scan, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_step, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_drift, n * sizeof(double)));
   HIPCHECK(c, hipMalloc(&b.sh_r, n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&b.sh_g, n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&b.sh_b, n * b.rec_bytes));
@@ -589,19 +595,25 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   for (long long band_first = first; band_first < last; band_first += b.cap) {
     if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
     const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
-    HIPCHECK(c, hipMemsetAsync(b.counters, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHECK(c, hipMemsetAsync(b.counters, 0, CNT_COUNT * sizeof(unsigned long long), st));
     HIPCHECK(c, hipEventRecord(c->ev[0], st));
-    HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, nullptr, 0, c->d_stats, st));
+    HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, c->scene.formula, c->cu_count,
+                             c->d_stats, st));
     n_march++;
-    unsigned long long counters[4];
+    unsigned long long counters[CNT_COUNT];
     HIPCHECK(c, hipMemcpyAsync(counters, b.counters, sizeof counters, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
-    const int n_hits = (int)counters[0];
-    const int n_ovf = (int)counters[1];
-    const long long ovf_total = (long long)counters[2];
+    const int n_hand = (int)counters[CNT_HAND];
+    HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, n_hand, c->d_stats, st));
+    if (n_hand > 0) n_march++;
+    HIPCHECK(c, hipMemcpyAsync(counters, b.counters, sizeof counters, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    const int n_hits = (int)counters[CNT_HITS];
+    const int n_ovf = (int)counters[CNT_OVF];
+    const long long ovf_total = (long long)counters[CNT_OVF_RECS];
     if (n_ovf > 0) {
       // step records longer than the slab: re-march those rays (same
-      // kernel, same arithmetic) into the overflow runs k_march reserved
+      // step functions, same arithmetic) into the overflow runs reserved
       if (ovf_total > b.ovf_capacity) {
         if (b.ovf_step) (void)hipFree(b.ovf_step);
         if (b.ovf_dist) (void)hipFree(b.ovf_dist);
@@ -614,7 +626,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
         HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
         b.ovf_capacity = ovf_total;
       }
-      HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, b.ovf_list, n_ovf, c->d_stats, st));
+      HIPCHECK(c, launch_march_seq(precision, c->d_sc, b, (int)band_first, b.ovf_list, n_ovf, c->scene.formula, st));
       n_march++;
       overflow_rays += n_ovf;
     }

@@ -61,6 +61,36 @@ struct Math<float> {
 template <class T>
 MBR_HD T length(V3<T> a) { return Math<T>::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 
+// Hardware reciprocal / reciprocal square root / log2 of the FP32 path
+// (v_rcp_f32, v_rsq_f32, v_log_f32: about 1 ulp), in place of the
+// correctly rounded division and square-root sequences.
+// The host versions (used only by host-side checks of these templates) are
+// the plain library operations.
+MBR_HD float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+MBR_HD float fast_rsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / ::sqrtf(x);
+#endif
+}
+// a / b: exact in FP64, hardware reciprocal in FP32
+MBR_HD double fdiv(double a, double b) { return a / b; }
+MBR_HD float fdiv(float a, float b) { return a * fast_rcp(b); }
+MBR_HD float fast_ln(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_logf(x) * 0.69314718055994531f;
+#else
+  return ::logf(x);
+#endif
+}
+
 // CVector3::Normalize (src/algebra.hpp:104-111): multiply by 1/length
 template <class T>
 MBR_HD V3<T> normalize(V3<T> a) {

@@ -228,6 +228,7 @@ template <class T>
 struct DEIter {
   V3<T> z, c;
   T r, aux;  // aux: r_dz (trig formulas) or tgladDE (folding formulas)
+  T inv_r;   // 1/r, kept by the FP32 integer-power Mandelbulb path
   int L;
 };
 
@@ -238,6 +239,14 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.r = length(p);
   s.aux = 1;
   s.L = 0;
+  s.inv_r = 0;
+  if constexpr (F == 2 && sizeof(T) < 8) {
+    if (f.ipow) {
+      const T r2 = p.x * p.x + p.y * p.y + p.z * p.z;
+      s.inv_r = fast_rsq(r2);
+      s.r = r2 * s.inv_r;
+    }
+  }
 }
 
 // One loop pass.  Returns true when the call has finished; `dist` is then
@@ -261,7 +270,7 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
   if (f.sph_fold) {
     T r2 = r * r;
     if (r2 < f.sph_mR2) { z = z * f.sph_factor1; s.aux *= f.sph_factor1; }
-    else if (r2 < f.sph_fR2) { T tf2 = f.sph_fR2 / r2; z = z * tf2; s.aux *= tf2; }
+    else if (r2 < f.sph_fR2) { T tf2 = fdiv(f.sph_fR2, r2); z = z * tf2; s.aux *= tf2; }
     r = length(z);
   }
   bool esc;
@@ -271,46 +280,65 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
       // rho = |(x,y)|, (cos th, sin th) = (rho, z)/r and (cos ph, sin ph) =
       // (x, y)/rho, so cos/sin(p th) and cos/sin(p ph) are the parts of the
       // complex powers (rho + i z)^p / r^p and (x + i y)^p / rho^p
-      // (de Moivre), formed by binary exponentiation.
+      // (de Moivre), formed by binary exponentiation.  1/r is carried from
+      // the previous iteration; reciprocals and square roots use the
+      // hardware rsq/rcp.
       const int p = f.ipow;
-      const T rho = M::sqrt(z.x * z.x + z.y * z.y);
-      const T inv_r = T(1) / r;
-      T ure = rho * inv_r, uim = z.z * inv_r;  // e^{i th}
-      T wre, wim;                               // e^{i ph}
-      if (rho > T(0)) {
-        const T inv_rho = T(1) / rho;
+      T inv_r = s.inv_r;
+      if (f.tglad_fold || f.sph_fold) inv_r = fast_rsq(z.x * z.x + z.y * z.y + z.z * z.z);
+      const T rho2 = z.x * z.x + z.y * z.y;
+      T ure, uim = z.z * inv_r;  // e^{i th}
+      T wre, wim;                // e^{i ph}
+      if (rho2 > T(0)) {
+        const T inv_rho = fast_rsq(rho2);
+        ure = rho2 * inv_rho * inv_r;
         wre = z.x * inv_rho;
         wim = z.y * inv_rho;
       } else {
         // atan2(+-0, x) is +-0 for x = +0 and +-pi for x = -0
+        ure = T(0);
         wre = Math<T>::copysign(T(1), z.x);
         wim = T(0);
       }
-      T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1, rb = r;
-      int e = p - 1;  // r^(p-1)
-      while (e) {
-        if (e & 1) rp1 *= rb;
-        e >>= 1;
-        if (e) rb *= rb;
-      }
-      e = p;
-      while (e) {
-        if (e & 1) {
-          T t = pure * ure - puim * uim; puim = pure * uim + puim * ure; pure = t;
-          t = pwre * wre - pwim * wim; pwim = pwre * wim + pwim * wre; pwre = t;
-        }
-        e >>= 1;
-        if (e) {
+      T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1;
+      if (p == 8) {
+        // the standard Mandelbulb: three squarings, r^7 = r^4 r^2 r
+        for (int k = 0; k < 3; k++) {
           T t = ure * ure - uim * uim; uim = T(2) * ure * uim; ure = t;
           t = wre * wre - wim * wim; wim = T(2) * wre * wim; wre = t;
+        }
+        pure = ure; puim = uim; pwre = wre; pwim = wim;
+        const T r2 = r * r, r4 = r2 * r2;
+        rp1 = r4 * r2 * r;
+      } else {
+        T rb = r;
+        int e = p - 1;  // r^(p-1)
+        while (e) {
+          if (e & 1) rp1 *= rb;
+          e >>= 1;
+          if (e) rb *= rb;
+        }
+        e = p;
+        while (e) {
+          if (e & 1) {
+            T t = pure * ure - puim * uim; puim = pure * uim + puim * ure; pure = t;
+            t = pwre * wre - pwim * wim; pwim = pwre * wim + pwim * wre; pwre = t;
+          }
+          e >>= 1;
+          if (e) {
+            T t = ure * ure - uim * uim; uim = T(2) * ure * uim; ure = t;
+            t = wre * wre - wim * wim; wim = T(2) * wre * wim; wre = t;
+          }
         }
       }
       s.aux = rp1 * s.aux * T(p) + T(1);
       const T rp = rp1 * r;
       z = V3<T>{pure * pwre, pure * pwim, puim} * rp + s.c;
-      r = length(z);
+      const T r2 = z.x * z.x + z.y * z.y + z.z * z.z;
+      s.inv_r = fast_rsq(r2);
+      r = r2 * s.inv_r;
       esc = !(r <= f.bailout);
-      if (esc) dist = T(0.5) * r * M::log(r) / s.aux;
+      if (esc) dist = T(0.5) * r * fast_ln(r) * fast_rcp(s.aux);
       s.z = z;
       s.r = r;
       if (esc) return true;
@@ -345,7 +373,7 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
     r = length(z);
     s.aux *= T(3);
     esc = !(r <= T(1000));
-    if (esc) dist = (r - T(2)) / s.aux;
+    if (esc) dist = fdiv(r - T(2), s.aux);
   } else {  // tglad (Mandelbox)
     const T lim = f.mb_limit, val = f.mb_value;
     if (f.mb_rot_on) {
@@ -382,14 +410,14 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
     T r2 = r * r;
     z = z + ld3(f.mb_offset);
     if (r2 < f.mb_mR2) { z = z * f.mb_factor1; s.aux *= f.mb_factor1; }
-    else if (r2 < f.mb_fR2) { T tf2 = f.mb_fR2 / r2; z = z * tf2; s.aux *= tf2; }
+    else if (r2 < f.mb_fR2) { T tf2 = fdiv(f.mb_fR2, r2); z = z * tf2; s.aux *= tf2; }
     z = z - ld3(f.mb_offset);
     if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
     z = z * f.mb_scale + s.c;
     s.aux = s.aux * f.mb_abs_scale + T(1);
     r = length(z);
     esc = !(r <= T(1024));
-    if (esc) dist = r / M::fabs(s.aux);
+    if (esc) dist = fdiv(r, M::fabs(s.aux));
   }
   s.z = z;
   s.r = r;

@@ -76,6 +76,32 @@ __device__ __forceinline__ void flush_stats(unsigned long long *stats, const uns
 }
 
 // ----------------------------------------------------------------- march
+// RayMarching (src/Render3D.cpp:79-199) is split around its distance
+// estimates.  march_before_de sets up step i (point, threshold);
+// march_after_de takes step i's distance and either continues, ends the ray
+// as a miss, or hands the ray over to the FP64 continuation.  The persistent
+// kernel k_march advances many rays this way, one formula iteration per
+// pass; k_march_seq runs the same functions sequentially (re-marching rays
+// whose step record outgrew the slab), so both produce identical records.
+//
+// Hand-over rule.  The FP64 path hands a ray over when step i is a hit; the
+// continuation repeats step i in FP64 and runs the binary search.  The FP32
+// path hands a ray over when step i's FP32 distance could be a hit once FP32
+// error is allowed for: below 1.001 x threshold, or within the drift margin
+// of it.
+//
+// Drift model.  An FP32 distance differs from the FP64 distance at the same
+// point by about 1e-6 of the distance (rounding accumulated over the
+// iterations) plus the FP32 rounding of the point, 6e-8 of its largest
+// coordinate; the step taken from it (times DE_factor) carries that error
+// into the scan.  The errors of successive steps are independent roundings,
+// so the scan's deviation from the FP64 march is estimated by their root sum
+// of squares, sigma; the margin is 4 sigma.  The continuation decides every
+// later hit test and every binary-search branch in FP64; when a hit test, or
+// a branch whose step still exceeds 1e-5 of the depth (later branches move
+// the depth by less than that), lies within the margin of its threshold,
+// the FP32 march may have decided differently from the FP64 march and the
+// ray is re-marched in FP64 from the start.
 struct MarchOut {
   V3<double> point;
   double scan, distThresh, dist;
@@ -83,86 +109,96 @@ struct MarchOut {
   bool found, missed;
 };
 
-// RayMarching (src/Render3D.cpp:79-199) for one ray.  T is the arithmetic
-// of the DE during the march; scan, ray points and thresholds are FP64, and
-// the binary search evaluates the DE in FP64.
-//
-// For T = float the march also bounds its own drift: every FP32 distance
-// differs from the FP64 one by at most a few ulp of the distance plus the
-// rounding of the point to FP32, and the step taken from it moves the scan
-// by that much.  A hit test or binary-search decision whose FP64 margin to
-// its threshold is inside twice the accumulated bound could have gone the
-// other way in the FP64 march; the function then reports `ambiguous` and the
-// caller re-marches the ray in FP64.
-template <class T, class RT>
-__device__ __forceinline__ void march_ray(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
-                                          const RecPlanes<RT> &rec, MarchOut &o, LaneCount &cnt, bool &ambiguous) {
-  constexpr bool kF32 = sizeof(T) < 8;
-  const FractalK<T> &f = sc.frac<T>();
+struct RayState {
+  V3<double> point;
+  double scan, step, thr, drift, dist;
+  int i, object;
+};
+
+enum : int { SO_CONTINUE = 0, SO_MISS = 1, SO_HANDOFF = 2 };
+
+__device__ __forceinline__ void march_before_de(const DevScene &sc, V3<double> start, V3<double> dir, RayState &s) {
   const ShadeK<double> &k = sc.sd;
-  const double quality = k.quality, res = k.resolution, persp = k.persp, DEf = k.DE_factor;
-  const double search_limit = 1.0 - 0.01 * quality;
-  const double maxScan = sc.view_distance_max;
-  const bool fisheye = sc.persp_type == 1 || sc.persp_type == 2;
-  const double stepFactor = sc.interior ? 0.8 : 0.5;
-  double scan = sc.view_distance_min, distThresh = 0.0, dist = 0.0, step = sc.view_distance_min;
-  double detail = 0.0;
-  double drift = 0.0;  // bound on |scan_fp32 - scan_fp64|
-  int buffCount = 0, object = OBJ_FRACTAL;
+  s.point = start + dir * s.scan;
+  s.thr = sc.constant_thresh ? k.quality : s.scan * k.resolution * k.persp / k.quality + 0.0;
+  if (sc.persp_type == 1 || sc.persp_type == 2) s.thr *= kPi;
+}
+
+template <class T, class RT>
+__device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, double d, int object, uint32_t pixel,
+                                              const RecPlanes<RT> &rec) {
+  const ShadeK<double> &k = sc.sd;
+  s.object = object;
+  const double dist = d > 3.0 ? 3.0 : d;
+  double err = 0.0;
+  if constexpr (sizeof(T) < 8) {
+    // s.drift holds the variance of the scan deviation
+    const double pmax = fmax(fabs(s.point.x), fmax(fabs(s.point.y), fabs(s.point.z)));
+    err = 1e-6 * fabs(d) + 6e-8 * pmax;
+    if (dist < 1.001 * s.thr || dist - s.thr < 4.0 * sqrt(s.drift + err * err)) return SO_HANDOFF;
+  } else {
+    if (dist < s.thr) return SO_HANDOFF;
+  }
+  const double jf = sc.jitter_mode ? 1.0 - jitter_draw(sc.jitter_seed, sc.frame_no, pixel, (uint32_t)s.i) / 10000.0 : 1.0;
+  const double old_step = s.step;
+  s.step = (dist - (sc.interior ? 0.8 : 0.5) * s.thr) * k.DE_factor * jf;
+  rec.put(s.i, old_step, d, s.thr);
+  s.scan += s.step;
+  s.drift += (err * k.DE_factor) * (err * k.DE_factor);
+  s.dist = dist;
+  s.i++;
+  if (s.scan > sc.view_distance_max || s.i >= 10000) return SO_MISS;
+  return SO_CONTINUE;
+}
+
+// The reference loop from step s.i on, in FP64, then the binary search
+// (src/Render3D.cpp:145-183).  Sets `ambiguous` when a decision lies within
+// the drift margin of its threshold (see the hand-over rule above).
+template <class RT>
+__device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
+                                   const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
+                                   bool &ambiguous) {
+  const ShadeK<double> &k = sc.sd;
+  const double search_limit = 1.0 - 0.01 * k.quality;
+  const double margin = 4.0 * sqrt(s.drift);  // s.drift: variance of the FP32 scan deviation
   bool found = false, missed = false;
-  V3<double> point = v3<double>(0, 0, 0);
-  DEInfo info;
+  double dist = 0.0, detail = 0.0;
   ambiguous = false;
-  for (int i = 0; i < 10000; i++) {
-    point = start + dir * scan;
-    distThresh = sc.constant_thresh ? quality : scan * res * persp / quality + 0.0;
-    if (fisheye) distThresh *= kPi;
-    detail = distThresh;
-    dist = (double)calc_distance<T>(f, cvt<T>(point), (T)detail, sc.N, sc.minN, false, info);
+  for (; s.i < 10000; s.i++) {
+    march_before_de(sc, start, dir, s);
+    detail = s.thr;
+    DEInfo info;
+    dist = calc_distance<double>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
     cnt.add(info);
-    double err = 0.0;
-    if constexpr (kF32) {
-      // FP32 error of this distance: 4 ulp of the distance + the FP32
-      // rounding of the point (the DE moves at most ~1 per unit of position)
-      const double pmax = fmax(fabs(point.x), fmax(fabs(point.y), fabs(point.z)));
-      err = 4.8e-7 * fabs(dist) + 1.2e-7 * pmax;
-      // tentative hits and near-threshold steps are decided in FP64
-      if (dist < 1.001 * distThresh || dist - distThresh < 2.0 * (drift + err)) {
-        dist = calc_distance<double>(sc.fd, point, detail, sc.N, sc.minN, false, info);
-        cnt.add(info);
-        err = 0.0;
-        if (fabs(dist - distThresh) < 2.0 * drift) ambiguous = true;
-      }
-    }
-    object = info.object;
+    s.object = info.object;
     const double dist_raw = dist;
     if (dist > 3.0) dist = 3.0;
-    if (dist < distThresh) {
-      if (dist < 0.1 * distThresh) missed = true;
+    if (fabs(dist - s.thr) < margin) ambiguous = true;
+    if (dist < s.thr) {
+      if (dist < 0.1 * s.thr) missed = true;
       found = true;
       break;
     }
-    const double jf = sc.jitter_mode ? 1.0 - jitter_draw(sc.jitter_seed, sc.frame_no, pixel, i) / 10000.0 : 1.0;
-    const double old_step = step;
-    step = (dist - stepFactor * distThresh) * DEf * jf;
-    rec.put(i, old_step, dist_raw, distThresh);
-    buffCount++;
-    scan += step;
-    if constexpr (kF32) drift += err * DEf;
-    if (scan > maxScan) break;
+    const double jf = sc.jitter_mode ? 1.0 - jitter_draw(sc.jitter_seed, sc.frame_no, pixel, (uint32_t)s.i) / 10000.0 : 1.0;
+    const double old_step = s.step;
+    s.step = (dist - (sc.interior ? 0.8 : 0.5) * s.thr) * k.DE_factor * jf;
+    rec.put(s.i, old_step, dist_raw, s.thr);
+    s.scan += s.step;
+    if (s.scan > sc.view_distance_max) { s.i++; break; }
   }
+  double distThresh = s.thr, scan = s.scan;
+  V3<double> point = s.point;
+  int object = s.object;
   if (found) {
-    // binary search into [thr*(1-0.01q), thr], FP64 (src/Render3D.cpp:145-183)
-    step *= 0.5;
-    for (int i = 0; i < 20; i++) {
-      if constexpr (kF32) {
-        if (fabs(dist - distThresh) < 2.0 * drift || fabs(dist - distThresh * search_limit) < 2.0 * drift)
-          ambiguous = true;
-      }
+    double step = s.step * 0.5;
+    for (int j = 0; j < 20; j++) {
+      if (step > 1e-5 * scan &&
+          (fabs(dist - distThresh) < margin || fabs(dist - distThresh * search_limit) < margin))
+        ambiguous = true;
       if (dist < distThresh && dist > distThresh * search_limit) break;
       if (dist > distThresh) { scan += step; point = start + dir * scan; }
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
-      distThresh = sc.constant_thresh ? quality : scan * res * persp / quality + 0.0;
+      distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + 0.0;
       DEInfo bi;
       dist = calc_distance<double>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
       cnt.add(bi);
@@ -174,90 +210,262 @@ __device__ __forceinline__ void march_ray(const DevScene &sc, V3<double> start, 
   o.scan = scan;
   o.distThresh = distThresh;
   o.dist = dist;
-  o.buffCount = buffCount;
+  o.buffCount = s.i;
   o.object = object;
   o.found = found;
   o.missed = missed;
 }
 
-// One lane per primary ray (or, in re-march mode, per listed ray).  The
-// FP32 instantiation re-marches a ray in FP64 when march_ray reports that
-// its discrete decisions are within the FP32 error bound.
-template <class T, class RT>
+__device__ __forceinline__ RayState ray_start_state(const DevScene &sc) {
+  RayState s;
+  s.point = v3<double>(0, 0, 0);
+  s.scan = sc.view_distance_min;
+  s.step = sc.view_distance_min;
+  s.thr = 0.0;
+  s.drift = 0.0;
+  s.dist = 0.0;
+  s.i = 0;
+  s.object = OBJ_FRACTAL;
+  return s;
+}
+
+// A ray whose march ended without the FP64 continuation (a miss).
+__device__ __forceinline__ void write_ray(const BandBufs &b, int r, const MarchOut &o) {
+  b.found[r] = o.found;
+  b.object[r] = (uint8_t)o.object;
+  b.count[r] = o.buffCount;
+  b.depth[r] = o.scan;
+  b.thr[r] = o.distThresh;
+  b.last[r] = o.dist;
+  b.px[r] = o.point.x;
+  b.py[r] = o.point.y;
+  b.pz[r] = o.point.z;
+  if (o.found) {
+    const unsigned long long h = atomicAdd(&b.counters[CNT_HITS], 1ull);
+    b.hit_list[h] = r;
+  }
+  if (o.buffCount > b.slab) {
+    // reserve an exactly sized run of overflow records for the re-march
+    const unsigned long long ov = atomicAdd(&b.counters[CNT_OVF], 1ull);
+    b.ovf_list[ov] = r;
+    b.ovf_off[r] = (long long)atomicAdd(&b.counters[CNT_OVF_RECS], (unsigned long long)o.buffCount);
+  }
+}
+
+template <class RT>
+__device__ __forceinline__ RecPlanes<RT> slab_planes(const BandBufs &b, int r) {
+  RecPlanes<RT> rec;
+  rec.step = (RT *)b.rec_step;
+  rec.dist = (RT *)b.rec_dist;
+  rec.thr = (RT *)b.rec_thr;
+  rec.base = r;
+  rec.stride = b.cap;
+  rec.limit = b.slab;
+  return rec;
+}
+
+// Persistent march: one primary ray per lane, one formula iteration per
+// pass, lane states and amortised bookkeeping as in k_ao.
+enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
+
+template <class T, class RT, int F>
 __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                  int band_rays, const int *__restrict__ list, int n_list,
-                                                  unsigned long long *stats) {
+                                                  int band_rays, unsigned long long *stats) {
   const DevScene &sc = *scp;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
-  const bool remarch = list != nullptr;
-  int r = -1;
-  if (remarch) { if (tid < n_list) r = list[tid]; }
-  else if (tid < band_rays) r = tid;
+  const FractalK<T> &f = sc.frac<T>();
+  const int N = sc.N, minN = sc.minN;
+  const int lane = __lane_id();
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
 
+  int state = MS_NEED;
+  int r = 0;
+  uint32_t pixel = 0;
+  V3<double> dir = v3<double>(0, 0, 0);
+  RayState s = ray_start_state(sc);
+  DECall<T> call;
+  T dist = 0;
+  int object = OBJ_FRACTAL;
   LaneCount cnt = {0, 0};
-  int hit = 0, missed = 0, redo = 0;
-  if (r >= 0) {
-    const int p = band_first + r;
-    const int x = p % sc.width, y = p / sc.width;
-    V3<double> dir;
-    const bool cut = view_vector(sc, x, y, dir);
-    const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
-    const uint32_t pixel = sc.pixel_base + (uint32_t)p;
+  int missed = 0;
 
-    RecPlanes<RT> rec;
-    rec.step = (RT *)(remarch ? b.ovf_step : b.rec_step);
-    rec.dist = (RT *)(remarch ? b.ovf_dist : b.rec_dist);
-    rec.thr = (RT *)(remarch ? b.ovf_thr : b.rec_thr);
-    if (remarch) { rec.base = b.ovf_off[r]; rec.stride = 1; rec.limit = b.count[r]; }
-    else { rec.base = r; rec.stride = b.cap; rec.limit = b.slab; }
+  auto begin_de = [&]() {
+    march_before_de(sc, start, dir, s);
+    DEInfo info;
+    state = MS_ITER;
+    if (de_call_begin<T, F>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
+      state = MS_DONE;
+      object = info.object;
+      cnt.de_calls += 1;
+    }
+  };
 
-    MarchOut o;
-    o.point = v3<double>(0, 0, 0);
-    o.scan = 0; o.distThresh = 0; o.dist = 0; o.buffCount = 0; o.object = OBJ_FRACTAL;
-    o.found = false; o.missed = false;
-    if (!cut) {
-      bool ambiguous = false;
-      march_ray<T, RT>(sc, start, dir, pixel, rec, o, cnt, ambiguous);
-      if constexpr (sizeof(T) < 8) {
-        if (ambiguous) {
-          bool unused;
-          march_ray<double, RT>(sc, start, dir, pixel, rec, o, cnt, unused);
-          redo = 1;
+  for (;;) {
+    const unsigned long long m_need = __ballot(state == MS_NEED);
+    const unsigned long long m_iter = __ballot(state == MS_ITER);
+    const unsigned long long m_done = __ballot(state == MS_DONE);
+    if (!(m_need | m_iter | m_done)) break;
+    const int n_need = __popcll(m_need), n_iter = __popcll(m_iter), n_done = __popcll(m_done);
+
+    if (n_need && (4 * n_need >= n_need + n_iter + n_done || n_iter + n_done == 0)) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&b.counters[CNT_RAYQ], (unsigned long long)n_need);
+      base = __shfl(base, 0);
+      if (state == MS_NEED) {
+        const unsigned long long mine = base + (unsigned long long)__popcll(m_need & lt_mask);
+        if (mine >= (unsigned long long)band_rays) {
+          state = MS_EXHAUSTED;
+        } else {
+          r = (int)mine;
+          const int p = band_first + r;
+          const int x = p % sc.width, y = p / sc.width;
+          const bool cut = view_vector(sc, x, y, dir);
+          pixel = sc.pixel_base + (uint32_t)p;
+          s = ray_start_state(sc);
+          if (cut) {
+            MarchOut o;
+            o.point = v3<double>(0, 0, 0);
+            o.scan = 0; o.distThresh = 0; o.dist = 0; o.buffCount = 0; o.object = OBJ_FRACTAL;
+            o.found = false; o.missed = false;
+            write_ray(b, r, o);
+          } else {
+            begin_de();
+          }
         }
       }
     }
-    if (!remarch) {
-      b.found[r] = o.found;
-      b.object[r] = (uint8_t)o.object;
-      b.count[r] = o.buffCount;
-      b.depth[r] = o.scan;
-      b.thr[r] = o.distThresh;
-      b.last[r] = o.dist;
-      b.px[r] = o.point.x;
-      b.py[r] = o.point.y;
-      b.pz[r] = o.point.z;
-      missed = o.missed;
-      if (o.found) {
-        const unsigned long long h = atomicAdd(&b.counters[0], 1ull);
-        b.hit_list[h] = r;
-        hit = 1;
+
+    if (state == MS_ITER) {
+      DEInfo info;
+      if (de_call_step<T, F>(f, call, (T)s.thr, N, minN, false, dist, info)) {
+        state = MS_DONE;
+        object = info.object;
+        cnt.de_calls += 1;
+        cnt.iterations += (uint32_t)info.work;
       }
-      if (o.buffCount > b.slab) {
-        // reserve an exactly sized run of overflow records for the re-march
-        const unsigned long long ov = atomicAdd(&b.counters[1], 1ull);
-        b.ovf_list[ov] = r;
-        b.ovf_off[r] = (long long)atomicAdd(&b.counters[2], (unsigned long long)o.buffCount);
+    }
+
+    const unsigned long long m_done2 = __ballot(state == MS_DONE);
+    const unsigned long long m_iter2 = __ballot(state == MS_ITER);
+    const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
+    if (n_done2 && (4 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == MS_DONE) {
+      const int out = march_after_de<T, RT>(sc, s, (double)dist, object, pixel, slab_planes<RT>(b, r));
+      if (out == SO_CONTINUE) {
+        begin_de();
+      } else if (out == SO_MISS) {
+        MarchOut o;
+        o.point = s.point;
+        o.scan = s.scan;
+        o.distThresh = s.thr;
+        o.dist = s.dist;
+        o.buffCount = s.i;
+        o.object = s.object;
+        o.found = false;
+        o.missed = false;
+        write_ray(b, r, o);
+        state = MS_NEED;
+      } else {
+        const unsigned long long h = atomicAdd(&b.counters[CNT_HAND], 1ull);
+        b.hand_ray[h] = r;
+        b.hand_i[h] = s.i;
+        b.hand_scan[h] = s.scan;
+        b.hand_step[h] = s.step;
+        b.hand_drift[h] = s.drift;
+        state = MS_NEED;
       }
     }
   }
-  if (!remarch) {
-    unsigned long long v[ST_COUNT] = {0};
-    v[ST_MARCH_DE] = cnt.de_calls;
-    v[ST_MARCH_ITERS] = cnt.iterations;
-    v[ST_HITS] = (unsigned long long)hit;
-    v[ST_MISSED] = (unsigned long long)missed;
-    v[ST_REDO] = (unsigned long long)redo;
-    flush_stats(stats, v);
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_MARCH_DE] = cnt.de_calls;
+  v[ST_MARCH_ITERS] = cnt.iterations;
+  flush_stats(stats, v);
+  (void)missed;
+}
+
+// FP64 continuation of the handed-over rays: one lane per ray.
+template <class RT>
+__global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                      int n_hand, unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  LaneCount cnt = {0, 0};
+  int hit = 0, missed = 0, redo = 0;
+  if (tid < n_hand) {
+    const int r = b.hand_ray[tid];
+    const int p = band_first + r;
+    const int x = p % sc.width, y = p / sc.width;
+    V3<double> dir;
+    view_vector(sc, x, y, dir);
+    const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
+    const uint32_t pixel = sc.pixel_base + (uint32_t)p;
+    RayState s = ray_start_state(sc);
+    s.i = b.hand_i[tid];
+    s.scan = b.hand_scan[tid];
+    s.step = b.hand_step[tid];
+    s.drift = b.hand_drift[tid];
+    const RecPlanes<RT> rec = slab_planes<RT>(b, r);
+    MarchOut o;
+    bool ambiguous = false;
+    march_continue_f64<RT>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
+    if (ambiguous) {
+      bool unused;
+      march_continue_f64<RT>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
+      redo = 1;
+    }
+    write_ray(b, r, o);
+    hit = o.found;
+    missed = o.missed;
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_MARCH_DE] = cnt.de_calls;
+  v[ST_MARCH_ITERS] = cnt.iterations;
+  v[ST_HITS] = (unsigned long long)hit;
+  v[ST_MISSED] = (unsigned long long)missed;
+  v[ST_REDO] = (unsigned long long)redo;
+  flush_stats(stats, v);
+}
+
+// Sequential re-march of the listed rays with their full step record into
+// the overflow runs reserved for them: the same step functions as
+// k_march + k_march_fix, so the same records.
+template <class T, class RT, int F>
+__global__ void __launch_bounds__(kBlock) k_march_seq(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                      const int *__restrict__ list, int n_list) {
+  const DevScene &sc = *scp;
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= n_list) return;
+  const int r = list[tid];
+  const int p = band_first + r;
+  const int x = p % sc.width, y = p / sc.width;
+  V3<double> dir;
+  view_vector(sc, x, y, dir);
+  const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
+  const uint32_t pixel = sc.pixel_base + (uint32_t)p;
+  RecPlanes<RT> rec;
+  rec.step = (RT *)b.ovf_step;
+  rec.dist = (RT *)b.ovf_dist;
+  rec.thr = (RT *)b.ovf_thr;
+  rec.base = b.ovf_off[r];
+  rec.stride = 1;
+  rec.limit = b.count[r];
+  LaneCount cnt = {0, 0};
+  RayState s = ray_start_state(sc);
+  for (;;) {
+    march_before_de(sc, start, dir, s);
+    DEInfo info;
+    const T d = calc_distance_f<T, F>(sc.frac<T>(), cvt<T>(s.point), (T)s.thr, sc.N, sc.minN, false, info);
+    const int out = march_after_de<T, RT>(sc, s, (double)d, info.object, pixel, rec);
+    if (out == SO_CONTINUE) continue;
+    if (out == SO_HANDOFF) {
+      MarchOut o;
+      bool ambiguous = false;
+      march_continue_f64<RT>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
+      if (ambiguous) {
+        bool unused;
+        march_continue_f64<RT>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
+      }
+    }
+    break;
   }
 }
 
@@ -347,7 +555,18 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 // a global counter in chunks, so every lane of every wave does formula work
 // until the band's AO rays run out, whatever the spread of per-ray step and
 // iteration counts.
-constexpr unsigned kAOChunk = 256;
+// Items are (hit, direction) pairs in hit-major order.  A wave walks the
+// directions of one hit at a time and takes the next hit index from the
+// global counter when they run out, so an item needs no division and a lane
+// reloads the hit's data only when its hit changes.
+//
+// A lane is in one of four states: NEED (wants an item), ITER (advancing a
+// distance estimate), DONE (estimate complete, AO step pending) or EXHAUSTED.
+// Every pass advances the ITER lanes by one formula iteration.  The AO step
+// logic of DONE lanes and the item hand-out of NEED lanes run only once a
+// quarter of the wave waits for them (or nothing else can run), so their
+// cost is shared by many lanes instead of being paid on almost every pass.
+enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
 template <class T, int F>
 __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, int n_hits, AODev ao,
@@ -359,111 +578,118 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   const T *dirs;
   if constexpr (sizeof(T) == 8) dirs = ao.dir_d; else dirs = ao.dir_f;
   const unsigned count = (unsigned)ao.count;
-  const unsigned long long total = (unsigned long long)n_hits * count;
   const int N = sc.N, minN = sc.minN;
   const int lane = __lane_id();
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-  unsigned long long wnext = 0, wend = 0;  // wave-uniform chunk cursor
-  bool active = false, exhausted = false;
+  // wave-uniform cursor: hit wh, next direction wd
+  long long wh = -1;
+  unsigned wd = count;
+  int state = AO_NEED;
+  long long h_cur = -1;  // hit whose data this lane holds
   unsigned long long item = 0;
   V3<T> P = v3<T>(0, 0, 0), dir = P;
-  T r = 0, end_dist = 1, detail = 0, thr_f = 0;
-  int guard = 0;
+  T r = 0, start_dist = 0, end_dist = 1, detail = 0, thr_f = 0;
+  T dist = 0;
+  int mi = 0, guard = 0;
   DECall<T> call;
   LaneCount cnt = {0, 0};
 
+  // start the distance estimate at P + dir * r; DONE when no iteration runs
+  auto begin_de = [&]() {
+    DEInfo info;
+    state = AO_ITER;
+    if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info)) {
+      state = AO_DONE;
+      mi = info.max_iter;
+      cnt.de_calls += 1;
+    }
+  };
+
   for (;;) {
-    const bool need = !active && !exhausted;
-    const unsigned long long mask = __ballot(need);
-    if (mask) {
+    const unsigned long long m_need = __ballot(state == AO_NEED);
+    const unsigned long long m_iter = __ballot(state == AO_ITER);
+    const unsigned long long m_done = __ballot(state == AO_DONE);
+    if (!(m_need | m_iter | m_done)) break;
+    const int n_need = __popcll(m_need), n_iter = __popcll(m_iter), n_done = __popcll(m_done);
+
+    if (n_need && (4 * n_need >= n_need + n_iter + n_done || n_iter + n_done == 0)) {
       // hand out the next items to the lanes that need one, in lane order
-      const unsigned n = (unsigned)__popcll(mask);
-      const unsigned off = (unsigned)__popcll(mask & lt_mask);
-      const unsigned long long avail = wend - wnext;
-      unsigned long long mine = wnext + off;
+      const unsigned n = (unsigned)n_need;
+      const unsigned off = (unsigned)__popcll(m_need & lt_mask);
+      const unsigned avail = count - wd;
+      long long h = wh;
+      unsigned d = wd + off;
       if (n > avail) {
-        unsigned long long cbase = 0;
-        if (lane == 0) cbase = atomicAdd(queue, (unsigned long long)kAOChunk);
-        cbase = __shfl(cbase, 0);
-        if (off >= avail) mine = cbase + (off - avail);
-        wnext = cbase + (n - avail);
-        wend = cbase + kAOChunk;
+        // new hits for the lanes beyond this hit's remaining directions
+        const unsigned extra = n - avail;
+        const unsigned khits = (extra + count - 1) / count;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(queue, (unsigned long long)khits);
+        base = __shfl(base, 0);
+        if (off >= avail) {
+          const unsigned j = off - avail;
+          const unsigned q = count >= 64 ? 0u : j / count;  // several hits only when count < 64
+          h = (long long)base + q;
+          d = j - q * count;
+        }
+        wh = (long long)base + (khits - 1);
+        wd = extra - (khits - 1) * count;
       } else {
-        wnext += n;
+        wd += n;
       }
-      if (need) {
-        if (mine >= total) {
-          exhausted = true;
+      if (state == AO_NEED) {
+        if (h >= n_hits) {
+          state = AO_EXHAUSTED;
         } else {
-          item = mine;
-          const unsigned h = (unsigned)(item / count), d = (unsigned)(item - (unsigned long long)h * count);
-          const int ray = b.hit_list[h];
-          ShaderIn<T> in;
-          load_hit(sc, b, ray, in);
-          P = in.point;
-          dir = v3<T>(dirs[4 * d + 0], dirs[4 * d + 1], dirs[4 * d + 2]);
-          r = in.delta;                      // start_dist
-          end_dist = in.delta / k.resolution;
-          detail = in.detail;
-          thr_f = (T)(float)in.dist_thresh;  // float dist_thresh (src/shaders.cpp:642)
-          guard = 0;
-          if (r < end_dist) {
-            DEInfo info;
-            T dist;
-            if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info)) {
-              // finished without iterating: treat as the step's result below
-              active = true;
-              call.it.L = -1;  // marks a completed call
-              call.it.r = dist;
-              call.it.aux = (T)info.max_iter;
-            } else {
-              active = true;
-            }
-          } else {
-            sb.ao[item] = T(1);
+          if (h != h_cur) {
+            h_cur = h;
+            ShaderIn<T> in;
+            load_hit(sc, b, b.hit_list[h], in);
+            P = in.point;
+            start_dist = in.delta;
+            end_dist = in.delta / k.resolution;
+            detail = in.detail;
+            thr_f = (T)(float)in.dist_thresh;  // float dist_thresh (src/shaders.cpp:642)
           }
+          item = (unsigned long long)h * count + d;
+          dir = v3<T>(dirs[4 * d + 0], dirs[4 * d + 1], dirs[4 * d + 2]);
+          r = start_dist;
+          guard = 0;
+          if (r < end_dist) begin_de();
+          else sb.ao[item] = T(1);
         }
       }
     }
-    if (!__any(active || !exhausted)) break;
-    if (active) {
-      T dist = 0;
+
+    if (state == AO_ITER) {
       DEInfo info;
-      bool done;
-      int mi;
-      if (call.it.L < 0) {
-        done = true;
-        dist = call.it.r;
-        mi = call.it.aux != T(0);
-        info.work = 0;
-      } else {
-        done = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
+      if (de_call_step<T, F>(f, call, detail, N, minN, false, dist, info)) {
+        state = AO_DONE;
         mi = info.max_iter;
-      }
-      if (done) {
         cnt.de_calls += 1;
         cnt.iterations += (uint32_t)info.work;
-        if (dist < thr_f || mi) {
-          T shadowTemp = T(1) - (end_dist - r) / end_dist;
-          if (shadowTemp < T(0)) shadowTemp = 0;
-          sb.ao[item] = shadowTemp;
-          active = false;
+      }
+    }
+
+    const unsigned long long m_done2 = __ballot(state == AO_DONE);
+    const unsigned long long m_iter2 = __ballot(state == AO_ITER);
+    const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
+    if (n_done2 && (4 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
+      // one step of the AO ray (src/shaders.cpp:625-662)
+      if (dist < thr_f || mi) {
+        T shadowTemp = T(1) - (end_dist - r) / end_dist;
+        if (shadowTemp < T(0)) shadowTemp = 0;
+        sb.ao[item] = shadowTemp;
+        state = AO_NEED;
+      } else {
+        r += dist * T(2);
+        guard++;
+        if (!(r < end_dist) || guard >= kMaxShadeSteps) {
+          sb.ao[item] = T(1);
+          state = AO_NEED;
         } else {
-          r += dist * T(2);
-          guard++;
-          if (!(r < end_dist) || guard >= kMaxShadeSteps) {
-            sb.ao[item] = T(1);
-            active = false;
-          } else {
-            T d2;
-            DEInfo i2;
-            if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, d2, i2)) {
-              call.it.L = -1;
-              call.it.r = d2;
-              call.it.aux = (T)i2.max_iter;
-            }
-          }
+          begin_de();
         }
       }
     }
@@ -565,15 +791,47 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        const int *list, int n_list, unsigned long long *stats, hipStream_t s) {
-  const int n = list ? n_list : band_rays;
-  if (n <= 0) return hipSuccess;
+                        int formula, int cu_count, unsigned long long *stats, hipStream_t s) {
+  if (band_rays <= 0) return hipSuccess;
+  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    if (formula == 7)
+      hipLaunchKernelGGL((k_march<T, T, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+    else if (formula == 8)
+      hipLaunchKernelGGL((k_march<T, T, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+    else
+      hipLaunchKernelGGL((k_march<T, T, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+  };
+  if (precision == 0) run(double{});
+  else run(float{});
+  return hipGetLastError();
+}
+
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hand,
+                            unsigned long long *stats, hipStream_t s) {
+  if (n_hand <= 0) return hipSuccess;
   if (precision == 0)
-    hipLaunchKernelGGL((k_march<double, double>), grid_for(n), dim3(kBlock), 0, s, sc, b, band_first, band_rays, list,
-                       n_list, stats);
+    hipLaunchKernelGGL((k_march_fix<double>), grid_for(n_hand), dim3(kBlock), 0, s, sc, b, band_first, n_hand, stats);
   else
-    hipLaunchKernelGGL((k_march<float, float>), grid_for(n), dim3(kBlock), 0, s, sc, b, band_first, band_rays, list,
-                       n_list, stats);
+    hipLaunchKernelGGL((k_march_fix<float>), grid_for(n_hand), dim3(kBlock), 0, s, sc, b, band_first, n_hand, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_march_seq(int precision, const DevScene *sc, const BandBufs &b, int band_first, const int *list,
+                            int n_list, int formula, hipStream_t s) {
+  if (n_list <= 0) return hipSuccess;
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    if (formula == 7)
+      hipLaunchKernelGGL((k_march_seq<T, T, 7>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
+    else if (formula == 8)
+      hipLaunchKernelGGL((k_march_seq<T, T, 8>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
+    else
+      hipLaunchKernelGGL((k_march_seq<T, T, 2>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
+  };
+  if (precision == 0) run(double{});
+  else run(float{});
   return hipGetLastError();
 }
 

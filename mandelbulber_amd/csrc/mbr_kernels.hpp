@@ -14,6 +14,9 @@ enum StatIdx {
   ST_COUNT = 8
 };
 
+// Band counters (BandBufs::counters).
+enum CounterIdx { CNT_HITS = 0, CNT_OVF = 1, CNT_OVF_RECS = 2, CNT_HAND = 3, CNT_RAYQ = 4, CNT_COUNT = 8 };
+
 // Per-band device buffers.  A band is a run of consecutive pixels of the
 // image (row-major), at most `cap` rays.  Step records are step-major
 // structure-of-arrays: record i of ray r lives at [i * cap + r], so the
@@ -30,7 +33,10 @@ struct BandBufs {
   long long *ovf_off;                    // per ray: offset into overflow planes or -1
   // compaction lists
   int *hit_list, *ovf_list;
-  unsigned long long *counters;          // [0] hits, [1] overflow rays, [2] overflow records
+  unsigned long long *counters;          // CNT_* below
+  // rays handed from the march to its FP64 continuation
+  int *hand_ray, *hand_i;
+  double *hand_scan, *hand_step, *hand_drift;
   // overflow records (sized on demand)
   void *ovf_step, *ovf_dist, *ovf_thr;
   long long ovf_capacity;
@@ -55,7 +61,11 @@ struct AODev {
 
 // precision: 0 = FP64, 1 = FP32 march/shade with FP64 fixup
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        const int *list, int n_list, unsigned long long *stats, hipStream_t s);
+                        int formula, int cu_count, unsigned long long *stats, hipStream_t s);
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hand,
+                            unsigned long long *stats, hipStream_t s);
+hipError_t launch_march_seq(int precision, const DevScene *sc, const BandBufs &b, int band_first, const int *list,
+                            int n_list, int formula, hipStream_t s);
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;

@@ -47,7 +47,9 @@ def _check(renderer, oracle_mod, sc, label, fp64_exact=True):
         assert_parity(m, label=f"{label} {name}")
         if prec == _abi.PREC_FP64 and fp64_exact:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12 and m["psnr_db"] >= 120, m
-            assert st.march_de_calls == ost.march_de_calls
+            # the FP64 continuation repeats the distance estimate of the hit
+            # step once per hit ray; every other call is the reference's
+            assert st.march_de_calls == ost.march_de_calls + ost.hits
 
 
 @pytest.mark.parametrize("bailout", [100.0, 10.0])
