@@ -127,6 +127,9 @@ def main():
     dt64, st64 = timed(torch, r, planes, _abi.PREC_FP64, args.fp64_steps, 0)
     f64 = to_frame(torch, planes)
     diff = compare(f32, f64)
+    dtm, stm = timed(torch, r, planes, _abi.PREC_FP32_MARCH, 1, 1)
+    fm = to_frame(torch, planes)
+    diffm = compare(fm, f64)
 
     value = rays * args.steps / dt32 / 1e6
     value64 = rays * args.fp64_steps / dt64 / 1e6
@@ -157,13 +160,17 @@ def main():
             "workload": f"Mandelbulb power 8, {args.width}x{args.height}, N 250, hard shadow, AO quality 8 "
                         f"({s0.shade_de_calls // max(1, s0.hits)} DE calls per hit), volumetric fog 0.5, "
                         f"light seed {args.seed}",
-            "precision": "FP32 DE iteration and shading; FP64 ray origins, scan and final bisection",
+            "precision": "FP64 primary-ray march (origins, scan, hit decisions, binary search); FP32 DE "
+                     "iteration in normals, shadow, AO, colouring; FP32 fog and composition",
             "rays": rays, "hits": s0.hits,
         },
         "fp64": {"value": value64, "unit": "Mrays/s", "ms_per_step": dt64 / args.fp64_steps * 1e3,
                  "shade_ms_per_launch": shade64_ms,
                  "shade_achieved_tflops": shade64_flops / (shade64_ms * 1e-3) / 1e12},
         "speedup_vs_fp64": value / value64,
+        "fp32_march": {"value": rays / dtm / 1e6, "unit": "Mrays/s", "march_ms": stm[-1].ms_march,
+                       "fp64_redo_rays": stm[-1].fp64_redo_rays,
+                       "pixel_diff_vs_fp64": {k: diffm[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")}},
         "pixel_diff_vs_fp64": {k: diff[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
         "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "finish": s0.ms_finish},
         "march_counters": {"fp64_redo_rays": s0.fp64_redo_rays, "step_overflow_rays": s0.step_overflow_rays,

@@ -72,12 +72,19 @@ enum mbr_jitter {
   MBR_JITTER_HASH = 1
 };
 
-/* Arithmetic of the render: plain FP64 everywhere (the reference's CPU
- * arithmetic), or FP32 distance-estimator iteration and shading with FP64
- * ray origins / scan and an FP64 final bisection. */
+/* Arithmetic of the render.
+ *   FP64        FP64 everywhere: the reference's CPU arithmetic.
+ *   FP32_FIXUP  primary-ray march (ray origins, scan, every hit decision and
+ *               the binary search) in FP64, so hits and depths are those of
+ *               the FP64 march; normals, shadow, AO, colouring, fog in FP32.
+ *   FP32_MARCH  as FP32_FIXUP, but the march iterates in FP32 until FP32
+ *               error could change a hit decision, continues in FP64 from
+ *               there, and re-marches in FP64 the rays whose result the FP32
+ *               part could have changed. */
 enum mbr_precision {
   MBR_PREC_FP64 = 0,
-  MBR_PREC_FP32_FIXUP = 1
+  MBR_PREC_FP32_FIXUP = 1,
+  MBR_PREC_FP32_MARCH = 2
 };
 
 typedef struct mbr_rgb { int R, G, B; } mbr_rgb; /* sRGB (src/cimage.hpp:39-44) */

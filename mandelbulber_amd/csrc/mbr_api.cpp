@@ -5,8 +5,10 @@
 // (src/interface.cpp:688-730, src/callbacks.cpp:1538-1555) into the device
 // scene block, after the parameter clamps of CheckPrameters
 // (src/interface.cpp:3587-3602).  mbr_render[_device] runs the frame as
-// bands of at most kBandRays rays: march -> (overflow re-march) -> shade
-// over compacted hits -> finish.
+// bands of at most kBandRays rays: reset -> march -> FP64 continuation ->
+// shade over compacted hits -> finish, all enqueued on one stream with the
+// work counts passed between launches in device memory; the host waits once
+// per frame.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +30,7 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kBandRays = 1 << 21;   // rays per band (one 1080p frame)
 constexpr int kSlab = 96;            // step records per ray kept in the slab
+constexpr size_t kAOBudget = size_t(8) << 30;  // bytes of per-(hit, direction) AO intensities per band
 
 // ---------------------------------------------------- rotation matrices
 // CRotationMatrix (src/algebra.cpp:80-218): row-major, post-multiplied
@@ -257,8 +260,7 @@ struct mbr_ctx {
   // shading buffers (per hit; element type of the band's precision)
   void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
   size_t sh_ao_bytes = 0;
-  unsigned long long *d_queue = nullptr;
-  hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> ev;  // 4 per band of the last frame
   // scratch planes of mbr_render (host-buffer entry point)
   void *d_planes = nullptr;
   size_t d_planes_bytes = 0;
@@ -284,8 +286,8 @@ namespace {
 void free_band(mbr_ctx *c) {
   BandBufs &b = c->band;
   void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
-                  b.rec_thr, b.ovf_off, b.hit_list, b.ovf_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
-                  b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift,
+                  b.rec_thr, b.ovf_off, b.hit_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
+                  b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift, b.hand_dist,
                   b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -298,11 +300,28 @@ void free_band(mbr_ctx *c) {
   c->band_prec = -1;
 }
 
-int alloc_band(mbr_ctx *c, int precision) {
-  if (c->band_prec == precision) return MBR_OK;
+// Overflow pool of `runs` runs (freeing the previous one).
+int alloc_pool(mbr_ctx *c, long long runs) {
+  BandBufs &b = c->band;
+  if (b.ovf_step) (void)hipFree(b.ovf_step);
+  if (b.ovf_dist) (void)hipFree(b.ovf_dist);
+  if (b.ovf_thr) (void)hipFree(b.ovf_thr);
+  b.ovf_step = b.ovf_dist = b.ovf_thr = nullptr;
+  b.ovf_runs = 0;
+  const size_t bytes = (size_t)runs * (size_t)(kMaxSteps - b.slab) * b.rec_bytes;
+  HIPCHECK(c, hipMalloc(&b.ovf_step, bytes));
+  HIPCHECK(c, hipMalloc(&b.ovf_dist, bytes));
+  HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
+  b.ovf_runs = runs;
+  return MBR_OK;
+}
+
+// Band buffers for `cap` rays; kept while precision and cap stay the same.
+int alloc_band(mbr_ctx *c, int precision, int cap) {
+  if (c->band_prec == precision && c->band.cap == cap) return MBR_OK;
   free_band(c);
   BandBufs &b = c->band;
-  b.cap = kBandRays;
+  b.cap = cap;
   b.slab = kSlab;
   b.rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
   const size_t n = (size_t)b.cap;
@@ -321,13 +340,13 @@ int alloc_band(mbr_ctx *c, int precision) {
   HIPCHECK(c, hipMalloc(&b.rec_thr, rec));
   HIPCHECK(c, hipMalloc((void **)&b.ovf_off, n * sizeof(long long)));
   HIPCHECK(c, hipMalloc((void **)&b.hit_list, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.ovf_list, n * sizeof(int)));
   HIPCHECK(c, hipMalloc((void **)&b.counters, CNT_COUNT * sizeof(unsigned long long)));
   HIPCHECK(c, hipMalloc((void **)&b.hand_ray, n * sizeof(int)));
   HIPCHECK(c, hipMalloc((void **)&b.hand_i, n * sizeof(int)));
   HIPCHECK(c, hipMalloc((void **)&b.hand_scan, n * sizeof(double)));
   HIPCHECK(c, hipMalloc((void **)&b.hand_step, n * sizeof(double)));
   HIPCHECK(c, hipMalloc((void **)&b.hand_drift, n * sizeof(double)));
+  HIPCHECK(c, hipMalloc((void **)&b.hand_dist, n * sizeof(double)));
   HIPCHECK(c, hipMalloc(&b.sh_r, n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&b.sh_g, n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&b.sh_b, n * b.rec_bytes));
@@ -338,7 +357,9 @@ int alloc_band(mbr_ctx *c, int precision) {
   HIPCHECK(c, hipMalloc(&c->sh_spec, 3 * n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&c->sh_col, 3 * n * b.rec_bytes));
   HIPCHECK(c, hipMalloc(&c->sh_fast, n * b.rec_bytes));
-  b.ovf_capacity = 0;
+  // one overflow run per 256 rays to begin with; grown when a frame needs more
+  int rc = alloc_pool(c, std::max<long long>(64, n / 256));
+  if (rc != MBR_OK) return rc;
   c->band_prec = precision;
   return MBR_OK;
 }
@@ -409,13 +430,8 @@ int mbr_init(const mbr_device_opts *opts, mbr_ctx **out) {
     delete c;
     return MBR_ERR_HIP;
   }
-  for (hipEvent_t &e : c->ev) (void)hipEventCreate(&e);
   if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
     c->cu_count = 0;
-  if (hipMalloc((void **)&c->d_queue, sizeof(unsigned long long)) != hipSuccess) {
-    mbr_destroy(c);
-    return MBR_ERR_HIP;
-  }
   *out = c;
   return MBR_OK;
 }
@@ -425,7 +441,7 @@ void mbr_destroy(mbr_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_band(c);
-  void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes, c->d_queue};
+  void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev)
@@ -564,7 +580,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   if (fb->width != s.image_width || fb->height != s.image_height)
     return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
   const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
-  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP)
+  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP && precision != MBR_PREC_FP32_MARCH)
     return c->fail(MBR_ERR_INVALID_ARG, "unknown precision");
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
   if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
@@ -572,9 +588,28 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   if (row0 >= row1) return c->fail(MBR_ERR_INVALID_ARG, "empty row range");
   hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
   HIPCHECK(c, hipSetDevice(c->device));
-  int rc = alloc_band(c, precision);
+  // band size: every ray of a band may hit, and the AO pass keeps one
+  // intensity per (hit, direction)
+  const bool slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
+  const int rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
+  int cap = kBandRays;
+  if (slow_ao) {
+    const size_t per_ray = (size_t)c->host_sc.ao_count * rec_bytes;
+    cap = (int)std::min<size_t>((size_t)kBandRays, std::max<size_t>(4096, kAOBudget / per_ray));
+  }
+  int rc = alloc_band(c, precision, cap);
   if (rc != MBR_OK) return rc;
   BandBufs &b = c->band;
+  if (slow_ao) {
+    const size_t need = (size_t)b.cap * (size_t)c->host_sc.ao_count * b.rec_bytes;
+    if (need > c->sh_ao_bytes) {
+      if (c->sh_ao) (void)hipFree(c->sh_ao);
+      c->sh_ao = nullptr;
+      c->sh_ao_bytes = 0;
+      HIPCHECK(c, hipMalloc(&c->sh_ao, need));
+      c->sh_ao_bytes = need;
+    }
+  }
 
   FrameOut out;
   for (int i = 0; i < 3; i++) {
@@ -585,91 +620,64 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   out.alpha = fb->alpha;
   out.opacity = fb->opacity;
   AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
+  ShadeBufs sbuf;
+  sbuf.base = c->sh_base;
+  sbuf.spec = c->sh_spec;
+  sbuf.col = c->sh_col;
+  sbuf.fast_ao = c->sh_fast;
+  sbuf.ao = c->sh_ao;
+  sbuf.n = b.cap;
+  sbuf.slow_ao = slow_ao;
+  sbuf.formula = c->scene.formula;
 
-  HIPCHECK(c, hipMemsetAsync(c->d_stats, 0, ST_COUNT * sizeof(unsigned long long), st));
-  double ms_march = 0, ms_shade = 0, ms_finish = 0;
-  unsigned long long overflow_rays = 0;
-  uint64_t n_march = 0, n_shade = 0, n_finish = 0;
   const long long first = (long long)row0 * s.image_width;
   const long long last = (long long)row1 * s.image_width;
-  for (long long band_first = first; band_first < last; band_first += b.cap) {
-    if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
-    const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
-    HIPCHECK(c, hipMemsetAsync(b.counters, 0, CNT_COUNT * sizeof(unsigned long long), st));
-    HIPCHECK(c, hipEventRecord(c->ev[0], st));
-    HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, c->scene.formula, c->cu_count,
-                             c->d_stats, st));
-    n_march++;
-    unsigned long long counters[CNT_COUNT];
-    HIPCHECK(c, hipMemcpyAsync(counters, b.counters, sizeof counters, hipMemcpyDeviceToHost, st));
-    HIPCHECK(c, hipStreamSynchronize(st));
-    const int n_hand = (int)counters[CNT_HAND];
-    HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, n_hand, c->d_stats, st));
-    if (n_hand > 0) n_march++;
-    HIPCHECK(c, hipMemcpyAsync(counters, b.counters, sizeof counters, hipMemcpyDeviceToHost, st));
-    HIPCHECK(c, hipStreamSynchronize(st));
-    const int n_hits = (int)counters[CNT_HITS];
-    const int n_ovf = (int)counters[CNT_OVF];
-    const long long ovf_total = (long long)counters[CNT_OVF_RECS];
-    if (n_ovf > 0) {
-      // step records longer than the slab: re-march those rays (same
-      // step functions, same arithmetic) into the overflow runs reserved
-      if (ovf_total > b.ovf_capacity) {
-        if (b.ovf_step) (void)hipFree(b.ovf_step);
-        if (b.ovf_dist) (void)hipFree(b.ovf_dist);
-        if (b.ovf_thr) (void)hipFree(b.ovf_thr);
-        b.ovf_step = b.ovf_dist = b.ovf_thr = nullptr;
-        b.ovf_capacity = 0;
-        const size_t bytes = (size_t)ovf_total * b.rec_bytes;
-        HIPCHECK(c, hipMalloc(&b.ovf_step, bytes));
-        HIPCHECK(c, hipMalloc(&b.ovf_dist, bytes));
-        HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
-        b.ovf_capacity = ovf_total;
-      }
-      HIPCHECK(c, launch_march_seq(precision, c->d_sc, b, (int)band_first, b.ovf_list, n_ovf, c->scene.formula, st));
-      n_march++;
-      overflow_rays += n_ovf;
+  const int n_bands = (int)((last - first + b.cap - 1) / b.cap);
+  while ((int)c->ev.size() < 4 * n_bands) {
+    hipEvent_t e;
+    HIPCHECK(c, hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  unsigned long long v[ST_COUNT];
+  for (int attempt = 0;; attempt++) {
+    HIPCHECK(c, hipMemsetAsync(c->d_stats, 0, ST_COUNT * sizeof(unsigned long long), st));
+    int band = 0;
+    for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
+      if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
+      const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
+      hipEvent_t *e = &c->ev[4 * band];
+      HIPCHECK(c, launch_band_reset(b, st));
+      HIPCHECK(c, hipEventRecord(e[0], st));
+      HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, c->scene.formula, c->cu_count,
+                               c->d_stats, st));
+      HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, c->cu_count, c->d_stats, st));
+      HIPCHECK(c, hipEventRecord(e[1], st));
+      HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, ao, sbuf, c->cu_count, c->d_stats, st));
+      HIPCHECK(c, hipEventRecord(e[2], st));
+      HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));
+      HIPCHECK(c, hipEventRecord(e[3], st));
     }
-    HIPCHECK(c, hipEventRecord(c->ev[1], st));
-    ShadeBufs sbuf;
-    sbuf.base = c->sh_base;
-    sbuf.spec = c->sh_spec;
-    sbuf.col = c->sh_col;
-    sbuf.fast_ao = c->sh_fast;
-    sbuf.n = b.cap;
-    sbuf.queue = c->d_queue;
-    sbuf.slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
-    sbuf.formula = c->scene.formula;
-    if (sbuf.slow_ao && n_hits > 0) {
-      const size_t need = (size_t)n_hits * (size_t)c->host_sc.ao_count * b.rec_bytes;
-      if (need > c->sh_ao_bytes) {
-        if (c->sh_ao) (void)hipFree(c->sh_ao);
-        c->sh_ao = nullptr;
-        c->sh_ao_bytes = 0;
-        HIPCHECK(c, hipMalloc(&c->sh_ao, need));
-        c->sh_ao_bytes = need;
-      }
-    }
-    sbuf.ao = c->sh_ao;
-    HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, n_hits, ao, sbuf, c->cu_count, c->d_stats, st));
-    if (n_hits > 0) n_shade++;
-    HIPCHECK(c, hipEventRecord(c->ev[2], st));
-    HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));
-    n_finish++;
-    HIPCHECK(c, hipEventRecord(c->ev[3], st));
-    HIPCHECK(c, hipEventSynchronize(c->ev[3]));
+    HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    if (v[ST_POOL_MISS] == 0) break;
+    // some ray found the overflow pool full: grow it to hold every
+    // overflowing ray of the frame and render again
+    if (attempt > 0) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
+    rc = alloc_pool(c, (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
+    if (rc != MBR_OK) return rc;
+  }
+  double ms_march = 0, ms_shade = 0, ms_finish = 0;
+  for (int band = 0; band < n_bands; band++) {
+    const hipEvent_t *e = &c->ev[4 * band];
     float t0 = 0, t1 = 0, t2 = 0;
-    HIPCHECK(c, hipEventElapsedTime(&t0, c->ev[0], c->ev[1]));
-    HIPCHECK(c, hipEventElapsedTime(&t1, c->ev[1], c->ev[2]));
-    HIPCHECK(c, hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+    HIPCHECK(c, hipEventElapsedTime(&t0, e[0], e[1]));
+    HIPCHECK(c, hipEventElapsedTime(&t1, e[1], e[2]));
+    HIPCHECK(c, hipEventElapsedTime(&t2, e[2], e[3]));
     ms_march += t0;
     ms_shade += t1;
     ms_finish += t2;
   }
   if (stats) {
-    unsigned long long v[ST_COUNT];
-    HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
-    HIPCHECK(c, hipStreamSynchronize(st));
     std::memset(stats, 0, sizeof *stats);
     stats->ms_march = ms_march;
     stats->ms_shade = ms_shade;
@@ -682,11 +690,11 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->iterations = v[ST_MARCH_ITERS] + v[ST_SHADE_ITERS];
     stats->march_iterations = v[ST_MARCH_ITERS];
     stats->shade_iterations = v[ST_SHADE_ITERS];
-    stats->march_launches = n_march;
-    stats->shade_launches = n_shade;
-    stats->finish_launches = n_finish;
+    stats->march_launches = n_bands;
+    stats->shade_launches = n_bands;
+    stats->finish_launches = n_bands;
     stats->missed_de = v[ST_MISSED];
-    stats->step_overflow_rays = overflow_rays;
+    stats->step_overflow_rays = v[ST_OVF_RAYS];
     stats->fp64_redo_rays = v[ST_REDO];
   }
   return MBR_OK;

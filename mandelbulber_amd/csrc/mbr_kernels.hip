@@ -46,19 +46,42 @@ __device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3
   return cut;
 }
 
+// Step record writer of one ray: records below the slab size go to the
+// step-major slab, later ones to the ray's run in the overflow pool, which
+// the ray takes (one atomic) on its first overflowing record.  A re-march of
+// the same ray reuses its run.  When the pool is full the ray's later records
+// are dropped and ST_POOL_MISS tells the host to grow the pool and render
+// the band again.
 template <class RT>
 struct RecPlanes {
-  RT *step, *dist, *thr;
-  long long base;   // element offset of record 0
-  long long stride; // element distance between records i and i+1
-  int limit;        // records beyond this index are not stored
+  RT *step, *dist, *thr;     // slab planes
+  RT *ostep, *odist, *othr;  // overflow pool planes
+  long long *ovf_off;
+  unsigned long long *runs, *stats;
+  long long cap, ovf_runs;
+  int slab, r;
   __device__ __forceinline__ void put(int i, double step_v, double dist_v, double thr_v) const {
-    if (i < limit) {
-      const long long o = base + (long long)i * stride;
-      step[o] = (RT)step_v;
-      dist[o] = (RT)dist_v;
-      thr[o] = (RT)thr_v;
+    long long o;
+    RT *st, *di, *th;
+    if (i < slab) {
+      o = (long long)i * cap + r;
+      st = step; di = dist; th = thr;
+    } else {
+      long long base = ovf_off[r];
+      if (base == -1) {
+        const unsigned long long run = atomicAdd(runs, 1ull);
+        base = run < (unsigned long long)ovf_runs ? (long long)run * (kMaxSteps - slab) : -2;
+        ovf_off[r] = base;
+        atomicAdd(&stats[ST_OVF_RAYS], 1ull);
+        if (base < 0) atomicAdd(&stats[ST_POOL_MISS], 1ull);
+      }
+      if (base < 0) return;
+      o = base + (i - slab);
+      st = ostep; di = odist; th = othr;
     }
+    st[o] = (RT)step_v;
+    di[o] = (RT)dist_v;
+    th[o] = (RT)thr_v;
   }
 };
 
@@ -81,8 +104,8 @@ __device__ __forceinline__ void flush_stats(unsigned long long *stats, const uns
 // march_after_de takes step i's distance and either continues, ends the ray
 // as a miss, or hands the ray over to the FP64 continuation.  The persistent
 // kernel k_march advances many rays this way, one formula iteration per
-// pass; k_march_seq runs the same functions sequentially (re-marching rays
-// whose step record outgrew the slab), so both produce identical records.
+// pass; the FP64 redo of k_march_fix runs the same functions sequentially,
+// so both produce identical records.
 //
 // Hand-over rule.  The FP64 path hands a ray over when step i is a hit; the
 // continuation repeats step i in FP64 and runs the binary search.  The FP32
@@ -97,11 +120,20 @@ __device__ __forceinline__ void flush_stats(unsigned long long *stats, const uns
 // into the scan.  The errors of successive steps are independent roundings,
 // so the scan's deviation from the FP64 march is estimated by their root sum
 // of squares, sigma; the margin is 4 sigma.  The continuation decides every
-// later hit test and every binary-search branch in FP64; when a hit test, or
-// a branch whose step still exceeds 1e-5 of the depth (later branches move
-// the depth by less than that), lies within the margin of its threshold,
-// the FP32 march may have decided differently from the FP64 march and the
-// ray is re-marched in FP64 from the start.
+// later hit test and every binary-search branch in FP64.  A decision that
+// lies within the margin of its threshold may differ from the FP64 march's.
+// Either way the binary search ends in the band where the distance lies in
+// (thr (1 - 0.01 quality), thr), next to the point where the decision was
+// taken; along the ray the distance changes at rate g (estimated from the
+// last two distance estimates), so that band spans about
+//   W = (0.01 quality thr + 2 margin) / g
+// of depth, and both outcomes end within W of each other.  A branch whose
+// step is below 5e-6 of the depth moves the result by less than 2e-5 of the
+// depth and is not counted.  The ray is re-marched in FP64 from the start
+// when a counted decision was ambiguous and W exceeds kRedoTol of the
+// depth, i.e. on grazing incidence.
+constexpr double kRedoTol = 3e-5;
+
 struct MarchOut {
   V3<double> point;
   double scan, distThresh, dist;
@@ -163,6 +195,8 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
   const double margin = 4.0 * sqrt(s.drift);  // s.drift: variance of the FP32 scan deviation
   bool found = false, missed = false;
   double dist = 0.0, detail = 0.0;
+  double prev_dist = s.dist, prev_scan = s.scan - s.step;  // the step before s.i
+  bool have_prev = s.i > 0;
   ambiguous = false;
   for (; s.i < 10000; s.i++) {
     march_before_de(sc, start, dir, s);
@@ -183,6 +217,9 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
     const double old_step = s.step;
     s.step = (dist - (sc.interior ? 0.8 : 0.5) * s.thr) * k.DE_factor * jf;
     rec.put(s.i, old_step, dist_raw, s.thr);
+    prev_dist = dist;
+    prev_scan = s.scan;
+    have_prev = true;
     s.scan += s.step;
     if (s.scan > sc.view_distance_max) { s.i++; break; }
   }
@@ -192,10 +229,13 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
   if (found) {
     double step = s.step * 0.5;
     for (int j = 0; j < 20; j++) {
-      if (step > 1e-5 * scan &&
+      if (step > 5e-6 * scan &&
           (fabs(dist - distThresh) < margin || fabs(dist - distThresh * search_limit) < margin))
         ambiguous = true;
       if (dist < distThresh && dist > distThresh * search_limit) break;
+      prev_dist = dist;
+      prev_scan = scan;
+      have_prev = true;
       if (dist > distThresh) { scan += step; point = start + dir * scan; }
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
       distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + 0.0;
@@ -204,6 +244,11 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       cnt.add(bi);
       object = bi.object;
       step *= 0.5;
+    }
+    if (ambiguous && have_prev) {
+      const double g = fabs(dist - prev_dist) / fmax(fabs(scan - prev_scan), 1e-300);
+      const double W = (0.01 * k.quality * distThresh + 2.0 * margin) / fmax(g, 1e-12);
+      ambiguous = W > kRedoTol * scan;
     }
   }
   o.point = point;
@@ -244,23 +289,24 @@ __device__ __forceinline__ void write_ray(const BandBufs &b, int r, const MarchO
     const unsigned long long h = atomicAdd(&b.counters[CNT_HITS], 1ull);
     b.hit_list[h] = r;
   }
-  if (o.buffCount > b.slab) {
-    // reserve an exactly sized run of overflow records for the re-march
-    const unsigned long long ov = atomicAdd(&b.counters[CNT_OVF], 1ull);
-    b.ovf_list[ov] = r;
-    b.ovf_off[r] = (long long)atomicAdd(&b.counters[CNT_OVF_RECS], (unsigned long long)o.buffCount);
-  }
 }
 
 template <class RT>
-__device__ __forceinline__ RecPlanes<RT> slab_planes(const BandBufs &b, int r) {
+__device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, unsigned long long *stats) {
   RecPlanes<RT> rec;
   rec.step = (RT *)b.rec_step;
   rec.dist = (RT *)b.rec_dist;
   rec.thr = (RT *)b.rec_thr;
-  rec.base = r;
-  rec.stride = b.cap;
-  rec.limit = b.slab;
+  rec.ostep = (RT *)b.ovf_step;
+  rec.odist = (RT *)b.ovf_dist;
+  rec.othr = (RT *)b.ovf_thr;
+  rec.ovf_off = b.ovf_off;
+  rec.runs = &b.counters[CNT_RUNS];
+  rec.stats = stats;
+  rec.cap = b.cap;
+  rec.ovf_runs = b.ovf_runs;
+  rec.slab = b.slab;
+  rec.r = r;
   return rec;
 }
 
@@ -317,6 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
           state = MS_EXHAUSTED;
         } else {
           r = (int)mine;
+          b.ovf_off[r] = -1;
           const int p = band_first + r;
           const int x = p % sc.width, y = p / sc.width;
           const bool cut = view_vector(sc, x, y, dir);
@@ -349,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
     const unsigned long long m_iter2 = __ballot(state == MS_ITER);
     const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
     if (n_done2 && (4 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == MS_DONE) {
-      const int out = march_after_de<T, RT>(sc, s, (double)dist, object, pixel, slab_planes<RT>(b, r));
+      const int out = march_after_de<T, RT>(sc, s, (double)dist, object, pixel, ray_records<RT>(b, r, stats));
       if (out == SO_CONTINUE) {
         begin_de();
       } else if (out == SO_MISS) {
@@ -371,6 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
         b.hand_scan[h] = s.scan;
         b.hand_step[h] = s.step;
         b.hand_drift[h] = s.drift;
+        b.hand_dist[h] = s.dist;
         state = MS_NEED;
       }
     }
@@ -382,100 +430,52 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
   (void)missed;
 }
 
-// FP64 continuation of the handed-over rays: one lane per ray.
+// FP64 continuation of the handed-over rays: one lane per ray, a grid-stride
+// loop over the hand-over list whose length the march left in the band
+// counters.
 template <class RT>
 __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                      int n_hand, unsigned long long *stats) {
+                                                      unsigned long long *stats) {
   const DevScene &sc = *scp;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  const long long n_hand = (long long)b.counters[CNT_HAND];
   LaneCount cnt = {0, 0};
-  int hit = 0, missed = 0, redo = 0;
-  if (tid < n_hand) {
-    const int r = b.hand_ray[tid];
+  unsigned long long hits = 0, missed = 0, redo = 0;
+  const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
+  for (long long h = (long long)blockIdx.x * kBlock + threadIdx.x; h < n_hand; h += (long long)gridDim.x * kBlock) {
+    const int r = b.hand_ray[h];
     const int p = band_first + r;
     const int x = p % sc.width, y = p / sc.width;
     V3<double> dir;
     view_vector(sc, x, y, dir);
-    const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
     const uint32_t pixel = sc.pixel_base + (uint32_t)p;
     RayState s = ray_start_state(sc);
-    s.i = b.hand_i[tid];
-    s.scan = b.hand_scan[tid];
-    s.step = b.hand_step[tid];
-    s.drift = b.hand_drift[tid];
-    const RecPlanes<RT> rec = slab_planes<RT>(b, r);
+    s.i = b.hand_i[h];
+    s.scan = b.hand_scan[h];
+    s.step = b.hand_step[h];
+    s.drift = b.hand_drift[h];
+    s.dist = b.hand_dist[h];
+    const RecPlanes<RT> rec = ray_records<RT>(b, r, stats);
     MarchOut o;
     bool ambiguous = false;
     march_continue_f64<RT>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
     if (ambiguous) {
       bool unused;
       march_continue_f64<RT>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
-      redo = 1;
+      redo++;
     }
     write_ray(b, r, o);
-    hit = o.found;
-    missed = o.missed;
+    hits += o.found;
+    missed += o.missed;
   }
   unsigned long long v[ST_COUNT] = {0};
   v[ST_MARCH_DE] = cnt.de_calls;
   v[ST_MARCH_ITERS] = cnt.iterations;
-  v[ST_HITS] = (unsigned long long)hit;
-  v[ST_MISSED] = (unsigned long long)missed;
-  v[ST_REDO] = (unsigned long long)redo;
+  v[ST_HITS] = hits;
+  v[ST_MISSED] = missed;
+  v[ST_REDO] = redo;
   flush_stats(stats, v);
 }
 
-// Sequential re-march of the listed rays with their full step record into
-// the overflow runs reserved for them: the same step functions as
-// k_march + k_march_fix, so the same records.
-template <class T, class RT, int F>
-__global__ void __launch_bounds__(kBlock) k_march_seq(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                      const int *__restrict__ list, int n_list) {
-  const DevScene &sc = *scp;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
-  if (tid >= n_list) return;
-  const int r = list[tid];
-  const int p = band_first + r;
-  const int x = p % sc.width, y = p / sc.width;
-  V3<double> dir;
-  view_vector(sc, x, y, dir);
-  const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
-  const uint32_t pixel = sc.pixel_base + (uint32_t)p;
-  RecPlanes<RT> rec;
-  rec.step = (RT *)b.ovf_step;
-  rec.dist = (RT *)b.ovf_dist;
-  rec.thr = (RT *)b.ovf_thr;
-  rec.base = b.ovf_off[r];
-  rec.stride = 1;
-  rec.limit = b.count[r];
-  LaneCount cnt = {0, 0};
-  RayState s = ray_start_state(sc);
-  for (;;) {
-    march_before_de(sc, start, dir, s);
-    DEInfo info;
-    const T d = calc_distance_f<T, F>(sc.frac<T>(), cvt<T>(s.point), (T)s.thr, sc.N, sc.minN, false, info);
-    const int out = march_after_de<T, RT>(sc, s, (double)d, info.object, pixel, rec);
-    if (out == SO_CONTINUE) continue;
-    if (out == SO_HANDOFF) {
-      MarchOut o;
-      bool ambiguous = false;
-      march_continue_f64<RT>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
-      if (ambiguous) {
-        bool unused;
-        march_continue_f64<RT>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
-      }
-    }
-    break;
-  }
-}
-
-// ----------------------------------------------------------------- shade
-// ObjectShader (src/shaders.cpp:21-112) runs as three launches:
-//   k_surface  one lane per hit: normals, shadow, specular, palette colour
-//              and, in fast mode, FastAmbientOcclusion2;
-//   k_ao       persistent: the AmbientOcclusion rays of all hits;
-//   k_compose  one lane per hit: AO sum in direction order and the final
-//              ObjectShader composition.
 template <class T>
 struct SurfBufs {
   T *base;    // [3][n] mainLight * shade * shadow
@@ -501,11 +501,12 @@ __device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, 
 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                    int n_hits, SurfBufs<T> sb, unsigned long long *stats) {
+                                                    SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  const long long n_hits = (long long)b.counters[CNT_HITS];
   LaneCount cnt = {0, 0};
-  if (tid < n_hits) {
+  for (long long tid = (long long)blockIdx.x * kBlock + threadIdx.x; tid < n_hits;
+       tid += (long long)gridDim.x * kBlock) {
     const int r = b.hit_list[tid];
     const int p = band_first + r;
     const int x = p % sc.width, y = p / sc.width;
@@ -569,10 +570,11 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
 template <class T, int F>
-__global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, int n_hits, AODev ao,
-                                               SurfBufs<T> sb, unsigned long long *queue,
-                                               unsigned long long *stats) {
+__global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
+                                               SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
+  const long long n_hits = (long long)b.counters[CNT_HITS];
+  unsigned long long *queue = &b.counters[CNT_AOQ];
   const FractalK<T> &f = sc.frac<T>();
   const ShadeK<T> &k = sc.shade<T>();
   const T *dirs;
@@ -701,37 +703,52 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
 }
 
 template <class T, class RT>
-__global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__ scp, BandBufs b, int n_hits, AODev ao,
+__global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
                                                     SurfBufs<T> sb) {
   const DevScene &sc = *scp;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
-  if (tid >= n_hits) return;
+  const long long n_hits = (long long)b.counters[CNT_HITS];
   const ShadeK<T> &k = sc.shade<T>();
-  const int r = b.hit_list[tid];
   const long long n = sb.n;
-  RGBT<T> amb = {0, 0, 0};
-  if (sc.ao) {
-    if (sc.fast_ao) {
-      const T a = sb.fast_ao[tid];
-      amb = RGBT<T>{a, a, a};
-    } else {
-      const T *it = sb.ao + (long long)tid * ao.count;
-      for (int v = 0; v < ao.count; v++) {
-        const T s = it[v];
-        amb.R += s * T(ao.rgb[4 * v + 0]);
-        amb.G += s * T(ao.rgb[4 * v + 1]);
-        amb.B += s * T(ao.rgb[4 * v + 2]);
+  const int lane = __lane_id();
+  const long long wave0 = ((long long)blockIdx.x * kBlock + threadIdx.x) / 64;
+  const long long waves = (long long)gridDim.x * (kBlock / 64);
+  // one wave per hit: the lanes read the hit's AO intensities coalesced,
+  // lane j summing directions j, j + 64, ... in order, then the 64 partial
+  // sums are added pairwise (a fixed order, so results are reproducible)
+  for (long long tid = wave0; tid < n_hits; tid += waves) {
+    const int r = b.hit_list[tid];
+    RGBT<T> amb = {0, 0, 0};
+    if (sc.ao) {
+      if (sc.fast_ao) {
+        const T a = sb.fast_ao[tid];
+        amb = RGBT<T>{a, a, a};
+      } else {
+        const T *it = sb.ao + tid * ao.count;
+        for (int v = lane; v < ao.count; v += 64) {
+          const T s = it[v];
+          amb.R += s * T(ao.rgb[4 * v + 0]);
+          amb.G += s * T(ao.rgb[4 * v + 1]);
+          amb.B += s * T(ao.rgb[4 * v + 2]);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          amb.R += __shfl_xor(amb.R, off);
+          amb.G += __shfl_xor(amb.G, off);
+          amb.B += __shfl_xor(amb.B, off);
+        }
+        const T den = T(ao.count) * T(256);
+        amb.R /= den;
+        amb.G /= den;
+        amb.B /= den;
       }
-      const T den = T(ao.count) * T(256);
-      amb.R /= den;
-      amb.G /= den;
-      amb.B /= den;
+    }
+    if (lane == 0) {
+      const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
+      ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
+      ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
+      ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
     }
   }
-  const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
-  ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
-  ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
-  ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
 }
 
 // ---------------------------------------------------------------- finish
@@ -761,19 +778,29 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
   const int n = b.count[r];
   const T depth = (T)b.depth[r];
   VolOut<T> v;
-  if (n <= b.slab) {
-    const RT *st = (const RT *)b.rec_step, *di = (const RT *)b.rec_dist, *th = (const RT *)b.rec_thr;
-    const long long cap = b.cap;
+  const RT *st = (const RT *)b.rec_step, *di = (const RT *)b.rec_dist, *th = (const RT *)b.rec_thr;
+  const long long cap = b.cap;
+  const int slab = b.slab;
+  if (n <= slab) {
     v = volumetric<T>(sc, depth, n, px, alpha, [&](int i, T &s, T &d, T &t) {
       const long long o = (long long)i * cap + r;
       s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
     });
   } else {
-    const RT *st = (const RT *)b.ovf_step, *di = (const RT *)b.ovf_dist, *th = (const RT *)b.ovf_thr;
+    // the host renders a band again when a ray found the pool full, so every
+    // ray that reaches this point holds a run
+    const RT *ost = (const RT *)b.ovf_step, *odi = (const RT *)b.ovf_dist, *oth = (const RT *)b.ovf_thr;
     const long long base = b.ovf_off[r];
     v = volumetric<T>(sc, depth, n, px, alpha, [&](int i, T &s, T &d, T &t) {
-      const long long o = base + i;
-      s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
+      if (i < slab) {
+        const long long o = (long long)i * cap + r;
+        s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
+      } else if (base >= 0) {
+        const long long o = base + (i - slab);
+        s = (T)ost[o]; d = (T)odi[o]; t = (T)oth[o];
+      } else {
+        s = 0; d = 0; t = 0;
+      }
     });
   }
   if (out.rgb[0]) out.rgb[0][p] = (float)v.rgb.R;
@@ -794,53 +821,47 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
                         int formula, int cu_count, unsigned long long *stats, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
   const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
-  auto run = [&](auto tag) {
+  auto run = [&](auto tag, auto rtag) {
     using T = decltype(tag);
+    using RT = decltype(rtag);
     if (formula == 7)
-      hipLaunchKernelGGL((k_march<T, T, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+      hipLaunchKernelGGL((k_march<T, RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
     else if (formula == 8)
-      hipLaunchKernelGGL((k_march<T, T, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+      hipLaunchKernelGGL((k_march<T, RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
     else
-      hipLaunchKernelGGL((k_march<T, T, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+      hipLaunchKernelGGL((k_march<T, RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
   };
-  if (precision == 0) run(double{});
-  else run(float{});
+  if (precision == 0) run(double{}, double{});
+  else if (precision == 1) run(double{}, float{});
+  else run(float{}, float{});
   return hipGetLastError();
 }
 
-hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hand,
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int cu_count,
                             unsigned long long *stats, hipStream_t s) {
-  if (n_hand <= 0) return hipSuccess;
+  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 4 : 1024));
   if (precision == 0)
-    hipLaunchKernelGGL((k_march_fix<double>), grid_for(n_hand), dim3(kBlock), 0, s, sc, b, band_first, n_hand, stats);
+    hipLaunchKernelGGL((k_march_fix<double>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
   else
-    hipLaunchKernelGGL((k_march_fix<float>), grid_for(n_hand), dim3(kBlock), 0, s, sc, b, band_first, n_hand, stats);
+    hipLaunchKernelGGL((k_march_fix<float>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
   return hipGetLastError();
 }
 
-hipError_t launch_march_seq(int precision, const DevScene *sc, const BandBufs &b, int band_first, const int *list,
-                            int n_list, int formula, hipStream_t s) {
-  if (n_list <= 0) return hipSuccess;
-  auto run = [&](auto tag) {
-    using T = decltype(tag);
-    if (formula == 7)
-      hipLaunchKernelGGL((k_march_seq<T, T, 7>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
-    else if (formula == 8)
-      hipLaunchKernelGGL((k_march_seq<T, T, 8>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
-    else
-      hipLaunchKernelGGL((k_march_seq<T, T, 2>), grid_for(n_list), dim3(kBlock), 0, s, sc, b, band_first, list, n_list);
-  };
-  if (precision == 0) run(double{});
-  else run(float{});
+__global__ void k_band_reset(unsigned long long *counters) {
+  if (threadIdx.x < CNT_COUNT) counters[threadIdx.x] = 0;
+}
+
+hipError_t launch_band_reset(const BandBufs &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_band_reset, dim3(1), dim3(64), 0, s, b.counters);
   return hipGetLastError();
 }
 
-hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits, AODev ao,
+hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
                         ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s) {
-  if (n_hits <= 0) return hipSuccess;
   const int formula = sbuf.formula;
-  // persistent AO grid: enough workgroups to fill every CU several times
-  const dim3 ao_grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
+  // persistent grids: enough workgroups to fill every CU several times; the
+  // kernels loop over the band's hits
+  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
   auto run = [&](auto tag) -> hipError_t {
     using T = decltype(tag);
     SurfBufs<T> sb;
@@ -850,18 +871,16 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     sb.fast_ao = (T *)sbuf.fast_ao;
     sb.ao = (T *)sbuf.ao;
     sb.n = sbuf.n;
-    hipLaunchKernelGGL((k_surface<T>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, band_first, n_hits, sb, stats);
+    hipLaunchKernelGGL((k_surface<T>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
     if (sbuf.slow_ao) {
-      hipError_t e = hipMemsetAsync(sbuf.queue, 0, sizeof(unsigned long long), s);
-      if (e != hipSuccess) return e;
       if (formula == 7)
-        hipLaunchKernelGGL((k_ao<T, 7>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+        hipLaunchKernelGGL((k_ao<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else if (formula == 8)
-        hipLaunchKernelGGL((k_ao<T, 8>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+        hipLaunchKernelGGL((k_ao<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else
-        hipLaunchKernelGGL((k_ao<T, 2>), ao_grid, dim3(kBlock), 0, s, sc, b, n_hits, ao, sb, sbuf.queue, stats);
+        hipLaunchKernelGGL((k_ao<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
     }
-    hipLaunchKernelGGL((k_compose<T, T>), grid_for(n_hits), dim3(kBlock), 0, s, sc, b, n_hits, ao, sb);
+    hipLaunchKernelGGL((k_compose<T, T>), grid, dim3(kBlock), 0, s, sc, b, ao, sb);
     return hipGetLastError();
   };
   if (precision == 0) return run(double{});

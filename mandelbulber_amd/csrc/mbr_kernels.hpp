@@ -11,16 +11,24 @@ namespace mbr {
 // Counters accumulated on the device (one atomic add per workgroup).
 enum StatIdx {
   ST_MARCH_DE = 0, ST_SHADE_DE, ST_MARCH_ITERS, ST_SHADE_ITERS, ST_HITS, ST_MISSED, ST_REDO,
-  ST_COUNT = 8
+  ST_OVF_RAYS,   // rays whose step record outgrew the slab
+  ST_POOL_MISS,  // of those, rays that found the overflow pool full
+  ST_COUNT = 10
 };
 
-// Band counters (BandBufs::counters).
-enum CounterIdx { CNT_HITS = 0, CNT_OVF = 1, CNT_OVF_RECS = 2, CNT_HAND = 3, CNT_RAYQ = 4, CNT_COUNT = 8 };
+// Band counters (BandBufs::counters), zeroed on the device at band start:
+// hits appended, rays handed to the FP64 continuation, primary-ray queue,
+// overflow runs handed out, AO work queue.
+enum CounterIdx { CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_RUNS = 3, CNT_AOQ = 4, CNT_COUNT = 8 };
+
+// Maximum step records of one ray (the loop bound of src/Render3D.cpp:95).
+constexpr int kMaxSteps = 10000;
 
 // Per-band device buffers.  A band is a run of consecutive pixels of the
 // image (row-major), at most `cap` rays.  Step records are step-major
-// structure-of-arrays: record i of ray r lives at [i * cap + r], so the
-// march writes of one wave are coalesced.
+// structure-of-arrays: record i < slab of ray r lives at [i * cap + r];
+// records from `slab` on go to a run of kMaxSteps - slab records of the
+// overflow pool, taken by the ray when it first needs one.
 struct BandBufs {
   int cap;          // rays per band
   int slab;         // step records kept per ray in the slab
@@ -30,16 +38,16 @@ struct BandBufs {
   int *count;       // step record length (buffCount)
   double *depth, *thr, *last, *px, *py, *pz;
   void *rec_step, *rec_dist, *rec_thr;   // slab planes [slab][cap]
-  long long *ovf_off;                    // per ray: offset into overflow planes or -1
-  // compaction lists
-  int *hit_list, *ovf_list;
+  long long *ovf_off;                    // per ray: first pool record of its run, -1 none, -2 pool full
+  // compaction list of the hits
+  int *hit_list;
   unsigned long long *counters;          // CNT_* below
   // rays handed from the march to its FP64 continuation
   int *hand_ray, *hand_i;
-  double *hand_scan, *hand_step, *hand_drift;
-  // overflow records (sized on demand)
+  double *hand_scan, *hand_step, *hand_drift, *hand_dist;
+  // overflow pool: ovf_runs runs of kMaxSteps - slab records
   void *ovf_step, *ovf_dist, *ovf_thr;
-  long long ovf_capacity;
+  long long ovf_runs;
   // shade output per ray
   void *sh_r, *sh_g, *sh_b;
   uint8_t *col_r, *col_g, *col_b;
@@ -59,22 +67,23 @@ struct AODev {
   int count;
 };
 
-// precision: 0 = FP64, 1 = FP32 march/shade with FP64 fixup
+// precision: 0 = FP64; 1 = FP64 march, FP32 records and shading; 2 = FP32
+// march with FP64 continuation, FP32 records and shading (mbr_precision)
+// All launches of a band read their work counts (hits, hand-overs) from
+// b.counters on the device: a band is enqueued without host round trips.
+hipError_t launch_band_reset(const BandBufs &b, hipStream_t s);
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                         int formula, int cu_count, unsigned long long *stats, hipStream_t s);
-hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hand,
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int cu_count,
                             unsigned long long *stats, hipStream_t s);
-hipError_t launch_march_seq(int precision, const DevScene *sc, const BandBufs &b, int band_first, const int *list,
-                            int n_list, int formula, hipStream_t s);
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
   long long n;                  // hit capacity (plane stride)
-  unsigned long long *queue;    // AO work counter
   int slow_ao;                  // 1: run the AmbientOcclusion rays
   int formula;
 };
-hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, int n_hits, AODev ao,
+hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
                         ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);

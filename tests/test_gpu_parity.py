@@ -31,7 +31,7 @@ def oracle_mod():
     return oracle
 
 
-PRECISIONS = [(_abi.PREC_FP64, "fp64"), (_abi.PREC_FP32_FIXUP, "fp32")]
+PRECISIONS = [(_abi.PREC_FP64, "fp64"), (_abi.PREC_FP32_FIXUP, "fp32"), (_abi.PREC_FP32_MARCH, "fp32-march")]
 
 
 def _check(renderer, oracle_mod, sc, label, fp64_exact=True):
@@ -43,10 +43,14 @@ def _check(renderer, oracle_mod, sc, label, fp64_exact=True):
         m = compare(frame, ref)
         print(f"  {m}", flush=True)
         assert st.rays == sc.image_width * sc.image_height
-        assert st.hits == ost.hits or prec != _abi.PREC_FP64
+        assert st.hits == ost.hits or prec == _abi.PREC_FP32_MARCH
         assert_parity(m, label=f"{label} {name}")
         if prec == _abi.PREC_FP64 and fp64_exact:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12 and m["psnr_db"] >= 120, m
+        if prec == _abi.PREC_FP32_FIXUP:
+            # the march is the FP64 march: same hits, depths from the same
+            # arithmetic (stored as float)
+            assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-7, m
             # the FP64 continuation repeats the distance estimate of the hit
             # step once per hit ray; every other call is the reference's
             assert st.march_de_calls == ost.march_de_calls + ost.hits
