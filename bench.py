@@ -29,6 +29,12 @@ METRIC = "Mrays/s at 1080p Mandelbulb: FP32+FP64-fixup vs plain FP64, ≤0.1% px
 # the FP64 one.
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 
+# HBM bytes per k_ao launch on this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1,
+# the factors that make a 1 GiB device copy read 1 GiB and write 1 GiB),
+# from the rocprofv3 --pmc runs summarised in the file named here.
+AO_TRAFFIC_BYTES = None
+AO_TRAFFIC_SOURCE = None
+
 
 def planes_on_gpu(torch, w, h):
     dev = torch.device("cuda:0")
@@ -134,17 +140,22 @@ def main():
     value = rays * args.steps / dt32 / 1e6
     value64 = rays * args.fp64_steps / dt64 / 1e6
 
-    # roofline of the dominant kernel (k_shade): algorithmic FLOPs per
-    # launch / average launch duration from the library's HIP events on the
-    # launch stream
-    shade_ms = sum(s.ms_shade for s in st32) / max(1, sum(s.shade_launches for s in st32))
-    shade_flops = sum(algorithmic_flops(sc, s, "shade") for s in st32) / max(1, sum(s.shade_launches for s in st32))
-    achieved = shade_flops / (shade_ms * 1e-3) / 1e12
+    # roofline of the dominant kernel (k_ao, the AmbientOcclusion rays):
+    # algorithmic FLOPs per launch / average launch duration, the duration
+    # from the library's HIP events recorded around the launch on its stream
+    launches = max(1, sum(s.shade_launches for s in st32))
+    ao_ms = sum(s.ms_ao for s in st32) / launches
+    ao_flops = sum(algorithmic_flops(sc, s, "ao") for s in st32) / launches
+    achieved = ao_flops / (ao_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS["f32"], "traffic": None, "kernel": "k_shade<float>",
-            "flops_per_launch": shade_flops, "ms_per_launch": shade_ms}
-    shade64_ms = sum(s.ms_shade for s in st64) / max(1, sum(s.shade_launches for s in st64))
-    shade64_flops = sum(algorithmic_flops(sc, s, "shade") for s in st64) / max(1, sum(s.shade_launches for s in st64))
+            "frac": achieved / PEAK_TFLOPS["f32"], "traffic": AO_TRAFFIC_BYTES,
+            "traffic_source": AO_TRAFFIC_SOURCE, "kernel": "k_ao<float, 2>",
+            "flops_per_launch": ao_flops, "ms_per_launch": ao_ms,
+            "ao_de_calls": st32[-1].ao_de_calls, "ao_iterations": st32[-1].ao_iterations,
+            "ao_steps": st32[-1].ao_steps}
+    launches64 = max(1, sum(s.shade_launches for s in st64))
+    ao64_ms = sum(s.ms_ao for s in st64) / launches64
+    ao64_flops = sum(algorithmic_flops(sc, s, "ao") for s in st64) / launches64
 
     cpu = None
     if not args.no_cpu:
@@ -165,14 +176,16 @@ def main():
             "rays": rays, "hits": s0.hits,
         },
         "fp64": {"value": value64, "unit": "Mrays/s", "ms_per_step": dt64 / args.fp64_steps * 1e3,
-                 "shade_ms_per_launch": shade64_ms,
-                 "shade_achieved_tflops": shade64_flops / (shade64_ms * 1e-3) / 1e12},
+                 "kernel_ms": {"march": st64[-1].ms_march, "shade": st64[-1].ms_shade, "ao": st64[-1].ms_ao,
+                               "finish": st64[-1].ms_finish},
+                 "ao_achieved_tflops": ao64_flops / (ao64_ms * 1e-3) / 1e12,
+                 "ao_frac_of_f64_peak": ao64_flops / (ao64_ms * 1e-3) / 1e12 / PEAK_TFLOPS["f64"]},
         "speedup_vs_fp64": value / value64,
         "fp32_march": {"value": rays / dtm / 1e6, "unit": "Mrays/s", "march_ms": stm[-1].ms_march,
                        "fp64_redo_rays": stm[-1].fp64_redo_rays,
                        "pixel_diff_vs_fp64": {k: diffm[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")}},
         "pixel_diff_vs_fp64": {k: diff[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
-        "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "finish": s0.ms_finish},
+        "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "ao": s0.ms_ao, "finish": s0.ms_finish},
         "march_counters": {"fp64_redo_rays": s0.fp64_redo_rays, "step_overflow_rays": s0.step_overflow_rays,
                            "march_de_calls": s0.march_de_calls, "shade_de_calls": s0.shade_de_calls,
                            "fp64_redo_rays_fp64_path": st64[-1].fp64_redo_rays,

@@ -262,10 +262,15 @@ typedef struct mbr_stats {
   uint64_t shade_iterations;       /* ... of which in the shade kernel */
   uint64_t march_launches, shade_launches, finish_launches;
   uint64_t missed_de;              /* hits with dist < 0.1*thresh */
-  uint64_t step_overflow_rays;     /* rays re-marched for the step record */
-  uint64_t fp64_redo_rays;         /* FP32-path rays re-marched in FP64 because a
+  uint64_t step_overflow_rays;     /* rays whose step record outgrew the slab
+                                      (their later records use the overflow pool) */
+  uint64_t fp64_redo_rays;         /* FP32_MARCH rays re-marched in FP64 because a
                                       hit or bisection decision fell inside the
                                       FP32 error bound */
+  double ms_ao;                    /* part of ms_shade spent in the AO kernel */
+  uint64_t ao_de_calls;            /* DE calls of the AO rays (in shade_de_calls) */
+  uint64_t ao_iterations;          /* formula iterations of those calls */
+  uint64_t ao_steps;               /* AO ray steps taken */
 } mbr_stats;
 
 typedef struct mbr_device_opts {

@@ -128,6 +128,8 @@ class Stats(C.Structure):
         ("march_launches", C.c_uint64), ("shade_launches", C.c_uint64), ("finish_launches", C.c_uint64),
         ("missed_de", C.c_uint64),
         ("step_overflow_rays", C.c_uint64), ("fp64_redo_rays", C.c_uint64),
+        ("ms_ao", C.c_double), ("ao_de_calls", C.c_uint64), ("ao_iterations", C.c_uint64),
+        ("ao_steps", C.c_uint64),
     ]
 
 

@@ -633,7 +633,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   const long long first = (long long)row0 * s.image_width;
   const long long last = (long long)row1 * s.image_width;
   const int n_bands = (int)((last - first + b.cap - 1) / b.cap);
-  while ((int)c->ev.size() < 4 * n_bands) {
+  while ((int)c->ev.size() < 6 * n_bands) {
     hipEvent_t e;
     HIPCHECK(c, hipEventCreate(&e));
     c->ev.push_back(e);
@@ -645,14 +645,16 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
       if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
       const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
-      hipEvent_t *e = &c->ev[4 * band];
+      hipEvent_t *e = &c->ev[6 * band];
       HIPCHECK(c, launch_band_reset(b, st));
       HIPCHECK(c, hipEventRecord(e[0], st));
       HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, c->scene.formula, c->cu_count,
                                c->d_stats, st));
-      HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, c->cu_count, c->d_stats, st));
+      HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, c->scene.formula, c->cu_count, c->d_stats,
+                                   st));
       HIPCHECK(c, hipEventRecord(e[1], st));
-      HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, ao, sbuf, c->cu_count, c->d_stats, st));
+      HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, ao, sbuf, c->cu_count, c->d_stats, st, e[4],
+                               e[5]));
       HIPCHECK(c, hipEventRecord(e[2], st));
       HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));
       HIPCHECK(c, hipEventRecord(e[3], st));
@@ -666,16 +668,18 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     rc = alloc_pool(c, (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
     if (rc != MBR_OK) return rc;
   }
-  double ms_march = 0, ms_shade = 0, ms_finish = 0;
+  double ms_march = 0, ms_shade = 0, ms_finish = 0, ms_ao = 0;
   for (int band = 0; band < n_bands; band++) {
-    const hipEvent_t *e = &c->ev[4 * band];
-    float t0 = 0, t1 = 0, t2 = 0;
+    const hipEvent_t *e = &c->ev[6 * band];
+    float t0 = 0, t1 = 0, t2 = 0, t3 = 0;
     HIPCHECK(c, hipEventElapsedTime(&t0, e[0], e[1]));
     HIPCHECK(c, hipEventElapsedTime(&t1, e[1], e[2]));
     HIPCHECK(c, hipEventElapsedTime(&t2, e[2], e[3]));
+    HIPCHECK(c, hipEventElapsedTime(&t3, e[4], e[5]));
     ms_march += t0;
     ms_shade += t1;
     ms_finish += t2;
+    ms_ao += t3;
   }
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
@@ -696,6 +700,10 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->missed_de = v[ST_MISSED];
     stats->step_overflow_rays = v[ST_OVF_RAYS];
     stats->fp64_redo_rays = v[ST_REDO];
+    stats->ms_ao = ms_ao;
+    stats->ao_de_calls = v[ST_AO_DE];
+    stats->ao_iterations = v[ST_AO_ITERS];
+    stats->ao_steps = v[ST_AO_STEPS];
   }
   return MBR_OK;
 }

@@ -186,7 +186,7 @@ __device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, d
 // The reference loop from step s.i on, in FP64, then the binary search
 // (src/Render3D.cpp:145-183).  Sets `ambiguous` when a decision lies within
 // the drift margin of its threshold (see the hand-over rule above).
-template <class RT>
+template <class RT, int F>
 __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
                                    const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
                                    bool &ambiguous) {
@@ -202,7 +202,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
     march_before_de(sc, start, dir, s);
     detail = s.thr;
     DEInfo info;
-    dist = calc_distance<double>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
+    dist = calc_distance_f<double, F>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
     cnt.add(info);
     s.object = info.object;
     const double dist_raw = dist;
@@ -240,7 +240,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
       distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + 0.0;
       DEInfo bi;
-      dist = calc_distance<double>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
+      dist = calc_distance_f<double, F>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
       cnt.add(bi);
       object = bi.object;
       step *= 0.5;
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
 // FP64 continuation of the handed-over rays: one lane per ray, a grid-stride
 // loop over the hand-over list whose length the march left in the band
 // counters.
-template <class RT>
+template <class RT, int F>
 __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                       unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -456,11 +456,13 @@ __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict
     s.dist = b.hand_dist[h];
     const RecPlanes<RT> rec = ray_records<RT>(b, r, stats);
     MarchOut o;
-    bool ambiguous = false;
-    march_continue_f64<RT>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
-    if (ambiguous) {
-      bool unused;
-      march_continue_f64<RT>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
+    // the continuation, then (when it reports an ambiguous decision) the
+    // FP64 march from the start: one inlined copy of the march loop
+    for (int pass = 0;; pass++) {
+      bool ambiguous = false;
+      march_continue_f64<RT, F>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
+      if (!ambiguous || pass > 0) break;
+      s = ray_start_state(sc);
       redo++;
     }
     write_ray(b, r, o);
@@ -499,7 +501,7 @@ __device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, 
   in.object = b.object[r];
 }
 
-template <class T>
+template <class T, int F>
 __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                     SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -516,12 +518,12 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     ShaderIn<T> in;
     load_hit(sc, b, r, in);
     in.view = cvt<T>(dir);
-    in.normal = normals<T>(sc, in, cnt);
+    in.normal = normals<T, F>(sc, in, cnt);
     T sh = dot(in.normal, in.light);
     if (sh < 0) sh = 0;
     const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
     T shadow = 1;
-    if (sc.shadow) shadow = main_shadow<T>(sc, in, cnt);
+    if (sc.shadow) shadow = main_shadow<T, F>(sc, in, cnt);
     V3<T> half = normalize(in.light - in.view);
     T shade2 = dot(in.normal, half);
     if (shade2 < T(0)) shade2 = 0;
@@ -537,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     sb.col[0 * n + tid] = colour.R;
     sb.col[1 * n + tid] = colour.G;
     sb.col[2 * n + tid] = colour.B;
-    if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T>(sc, in, cnt);
+    if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T, F>(sc, in, cnt);
     b.col_r[r] = x86_d2u8((double)colour.R * 255);
     b.col_g[r] = x86_d2u8((double)colour.G * 255);
     b.col_b[r] = x86_d2u8((double)colour.B * 255);
@@ -596,6 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   int mi = 0, guard = 0;
   DECall<T> call;
   LaneCount cnt = {0, 0};
+  unsigned long long ao_steps = 0;
 
   // start the distance estimate at P + dir * r; DONE when no iteration runs
   auto begin_de = [&]() {
@@ -679,6 +682,7 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
     const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
     if (n_done2 && (4 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
       // one step of the AO ray (src/shaders.cpp:625-662)
+      ao_steps++;
       if (dist < thr_f || mi) {
         T shadowTemp = T(1) - (end_dist - r) / end_dist;
         if (shadowTemp < T(0)) shadowTemp = 0;
@@ -699,6 +703,9 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   unsigned long long v[ST_COUNT] = {0};
   v[ST_SHADE_DE] = cnt.de_calls;
   v[ST_SHADE_ITERS] = cnt.iterations;
+  v[ST_AO_DE] = cnt.de_calls;
+  v[ST_AO_ITERS] = cnt.iterations;
+  v[ST_AO_STEPS] = ao_steps;
   flush_stats(stats, v);
 }
 
@@ -837,13 +844,20 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
   return hipGetLastError();
 }
 
-hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int cu_count,
-                            unsigned long long *stats, hipStream_t s) {
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
+                            int cu_count, unsigned long long *stats, hipStream_t s) {
   const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 4 : 1024));
-  if (precision == 0)
-    hipLaunchKernelGGL((k_march_fix<double>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
-  else
-    hipLaunchKernelGGL((k_march_fix<float>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+  auto run = [&](auto rtag) {
+    using RT = decltype(rtag);
+    if (formula == 7)
+      hipLaunchKernelGGL((k_march_fix<RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    else if (formula == 8)
+      hipLaunchKernelGGL((k_march_fix<RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    else
+      hipLaunchKernelGGL((k_march_fix<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+  };
+  if (precision == 0) run(double{});
+  else run(float{});
   return hipGetLastError();
 }
 
@@ -857,7 +871,8 @@ hipError_t launch_band_reset(const BandBufs &b, hipStream_t s) {
 }
 
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
-                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s) {
+                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s,
+                        hipEvent_t ev_ao0, hipEvent_t ev_ao1) {
   const int formula = sbuf.formula;
   // persistent grids: enough workgroups to fill every CU several times; the
   // kernels loop over the band's hits
@@ -871,7 +886,16 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     sb.fast_ao = (T *)sbuf.fast_ao;
     sb.ao = (T *)sbuf.ao;
     sb.n = sbuf.n;
-    hipLaunchKernelGGL((k_surface<T>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    if (formula == 7)
+      hipLaunchKernelGGL((k_surface<T, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    else if (formula == 8)
+      hipLaunchKernelGGL((k_surface<T, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    else
+      hipLaunchKernelGGL((k_surface<T, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    if (ev_ao0) {
+      hipError_t e = hipEventRecord(ev_ao0, s);
+      if (e != hipSuccess) return e;
+    }
     if (sbuf.slow_ao) {
       if (formula == 7)
         hipLaunchKernelGGL((k_ao<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
@@ -879,6 +903,10 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
         hipLaunchKernelGGL((k_ao<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else
         hipLaunchKernelGGL((k_ao<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+    }
+    if (ev_ao1) {
+      hipError_t e = hipEventRecord(ev_ao1, s);
+      if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_compose<T, T>), grid, dim3(kBlock), 0, s, sc, b, ao, sb);
     return hipGetLastError();

@@ -13,7 +13,8 @@ enum StatIdx {
   ST_MARCH_DE = 0, ST_SHADE_DE, ST_MARCH_ITERS, ST_SHADE_ITERS, ST_HITS, ST_MISSED, ST_REDO,
   ST_OVF_RAYS,   // rays whose step record outgrew the slab
   ST_POOL_MISS,  // of those, rays that found the overflow pool full
-  ST_COUNT = 10
+  ST_AO_DE, ST_AO_ITERS, ST_AO_STEPS,
+  ST_COUNT = 13
 };
 
 // Band counters (BandBufs::counters), zeroed on the device at band start:
@@ -74,8 +75,8 @@ struct AODev {
 hipError_t launch_band_reset(const BandBufs &b, hipStream_t s);
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                         int formula, int cu_count, unsigned long long *stats, hipStream_t s);
-hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int cu_count,
-                            unsigned long long *stats, hipStream_t s);
+hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
+                            int cu_count, unsigned long long *stats, hipStream_t s);
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
@@ -83,8 +84,12 @@ struct ShadeBufs {
   int slow_ao;                  // 1: run the AmbientOcclusion rays
   int formula;
 };
+// Shading in three launches: surface (normals, shadow, specular, colour,
+// fast AO), AO rays, composition; ev_ao (when not null) is recorded around
+// the AO launch.
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
-                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s);
+                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s,
+                        hipEvent_t ev_ao0, hipEvent_t ev_ao1);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);
 

@@ -45,15 +45,19 @@ struct RGBT {
   T R, G, B;
 };
 
-template <class T>
+// Distance estimate of the shaders.  F is the formula when the caller was
+// compiled for one (a kernel per formula), 0 for a run-time choice.
+template <class T, int F = 0>
 MBR_HD T de(const DevScene &sc, V3<T> p, T detail, int N, int minN, bool normalMode, DEInfo &info,
             LaneCount &cnt) {
-  T d = calc_distance<T>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
+  T d;
+  if constexpr (F == 0) d = calc_distance<T>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
+  else d = calc_distance_f<T, F>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
   cnt.add(info);
   return d;
 }
 
-template <class T>
+template <class T, int F = 0>
 MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   V3<T> n = v3<T>(0, 0, 0);
@@ -62,10 +66,10 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   if (!sc.slow_shading) {
     T delta = in.delta * k.smoothness;
     if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
-    T s1 = de<T>(sc, in.point, in.detail, N5, 0, true, info, cnt);
-    T s2 = de<T>(sc, in.point + v3<T>(delta, 0, 0), in.detail, N5, 0, true, info, cnt);
-    T s3 = de<T>(sc, in.point + v3<T>(0, delta, 0), in.detail, N5, 0, true, info, cnt);
-    T s4 = de<T>(sc, in.point + v3<T>(0, 0, delta), in.detail, N5, 0, true, info, cnt);
+    T s1 = de<T, F>(sc, in.point, in.detail, N5, 0, true, info, cnt);
+    T s2 = de<T, F>(sc, in.point + v3<T>(delta, 0, 0), in.detail, N5, 0, true, info, cnt);
+    T s3 = de<T, F>(sc, in.point + v3<T>(0, delta, 0), in.detail, N5, 0, true, info, cnt);
+    T s4 = de<T, F>(sc, in.point + v3<T>(0, 0, delta), in.detail, N5, 0, true, info, cnt);
     n = v3<T>(s2 - s1, s3 - s1, s4 - s1);
   } else {
     T delta = in.delta * k.smoothness;
@@ -75,7 +79,7 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
       for (double py = -1.0; py <= 1.0; py += 0.2)
         for (double pz = -1.0; pz <= 1.0; pz += 0.2) {
           V3<T> p2 = v3<T>((T)px, (T)py, (T)pz);
-          T d = de<T>(sc, in.point + p2 * delta, in.detail, N5, sc.minN, true, info, cnt);
+          T d = de<T, F>(sc, in.point + p2 * delta, in.detail, N5, sc.minN, true, info, cnt);
           n = n + p2 * d;
         }
   }
@@ -84,7 +88,7 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   return n;
 }
 
-template <class T>
+template <class T, int F = 0>
 MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   T factor = in.delta / k.resolution;
@@ -102,7 +106,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
     V3<T> point2 = in.point + in.light * i;
     const float dist_thresh = (float)in.dist_thresh;
     in.detail = (T)dist_thresh;
-    dist = de<T>(sc, point2, in.detail, sc.N, sc.minN, false, info, cnt);
+    dist = de<T, F>(sc, point2, in.detail, sc.N, sc.minN, false, info, cnt);
     if (sc.bsoft) {
       T angle = (dist - (T)dist_thresh) / i;
       if (angle < 0) angle = 0;
@@ -122,7 +126,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   return sc.bsoft ? (T(1) - maxSoft) : shadowTemp;
 }
 
-template <class T>
+template <class T, int F = 0>
 MBR_HD T fast_ao2(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   T delta = in.dist_thresh;
@@ -133,7 +137,7 @@ MBR_HD T fast_ao2(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   for (int i = 1; i < q * q; i++) {
     inv2i *= T(0.5);  // 1/pow(2,i), exact
     T scan = T(i * i) * delta;
-    T dist = de<T>(sc, in.point + in.normal * scan, in.detail, sc.N, sc.minN, false, info, cnt);
+    T dist = de<T, F>(sc, in.point + in.normal * scan, in.detail, sc.N, sc.minN, false, info, cnt);
     aoTemp += inv2i * (scan - k.fast_ao_tune * dist) / in.dist_thresh;
   }
   T ao = T(1) - T(0.2) * aoTemp;
@@ -148,7 +152,7 @@ struct AOTable {
   int count;
 };
 
-template <class T>
+template <class T, int F = 0>
 MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   RGBT<T> ao = {0, 0, 0};
@@ -162,7 +166,7 @@ MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTa
     T shadowTemp = 1;
     int guard = 0;
     for (T r = start_dist; r < end_dist && guard < kMaxShadeSteps; r += dist * T(2), guard++) {
-      dist = de<T>(sc, in.point + dir * r, in.detail, sc.N, sc.minN, false, info, cnt);
+      dist = de<T, F>(sc, in.point + dir * r, in.detail, sc.N, sc.minN, false, info, cnt);
       if (dist < (T)dist_thresh || info.max_iter || shadowTemp < T(0)) {
         shadowTemp -= (end_dist - r) / end_dist;
         if (shadowTemp < T(0)) shadowTemp = 0;
@@ -235,16 +239,16 @@ MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCou
 
 // ObjectShader: returns object + specular terms; *colour_out is the surface
 // colour (colourBuffer source).
-template <class T>
+template <class T, int F = 0>
 MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, RGBT<T> &colour_out,
                              LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
-  in.normal = normals<T>(sc, in, cnt);
+  in.normal = normals<T, F>(sc, in, cnt);
   T sh = dot(in.normal, in.light);
   if (sh < 0) sh = 0;
   const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
   T shadow = 1;
-  if (sc.shadow) shadow = main_shadow<T>(sc, in, cnt);
+  if (sc.shadow) shadow = main_shadow<T, F>(sc, in, cnt);
   V3<T> half = normalize(in.light - in.view);
   T shade2 = dot(in.normal, half);
   if (shade2 < T(0)) shade2 = 0;
@@ -256,10 +260,10 @@ MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<
   RGBT<T> ambient = {0, 0, 0};
   if (sc.ao) {
     if (sc.fast_ao) {
-      T a = fast_ao2<T>(sc, in, cnt);
+      T a = fast_ao2<T, F>(sc, in, cnt);
       ambient = RGBT<T>{a, a, a};
     } else {
-      ambient = ambient_occlusion<T>(sc, in, tab, cnt);
+      ambient = ambient_occlusion<T, F>(sc, in, tab, cnt);
     }
   }
   RGBT<T> out;

@@ -19,6 +19,8 @@ F_STEP = 8
 def algorithmic_flops(scene, stats, kernel):
     f_iter = F_ITER[scene.formula]
     f_call = F_TAIL[scene.formula] + F_STEP
+    if kernel == "ao":
+        return stats.ao_iterations * f_iter + stats.ao_de_calls * f_call
     if kernel == "shade":
         return stats.shade_iterations * f_iter + stats.shade_de_calls * f_call
     if kernel == "march":
