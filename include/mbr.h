@@ -319,6 +319,39 @@ int mbr_render(mbr_ctx *ctx, const mbr_render_opts *opts,
 int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
                       mbr_framebuffer *fb, mbr_stats *stats);
 
+/* Batched rendering of many frames (animation frames, or unrelated scenes of
+ * mixed formulas and sizes) as ONE launch of a HIP graph: the frames' band
+ * launches are captured once, spread over `slots` concurrent chains, and
+ * the graph is launched `replays` times.  The reference renders a flight
+ * with one Render() per frame (src/Render3D.cpp:2305-2521, keyframe loop);
+ * mbr_render_batch(scenes, n) is n such renders. */
+typedef struct mbr_batch_opts {
+  int precision;                   /* enum mbr_precision */
+  void *stream;                    /* hipStream_t the graph is launched on; NULL = the context's */
+  int slots;                       /* concurrent frame chains 1..4; 0 = tuned or 2 */
+  int blocks_per_cu;               /* workgroups per CU of the persistent kernels; 0 = tuned or 8 */
+  int autotune;                    /* 1: time every (slots, blocks_per_cu) candidate on this batch,
+                                      keep the fastest in the context for later batches */
+  int replays;                     /* graph launches (>= 1); the last one's frames are returned */
+} mbr_batch_opts;
+
+typedef struct mbr_batch_result {
+  double ms_per_launch;            /* GPU time of one graph launch (HIP events), mean over replays */
+  double ms_build;                 /* host time of capture + instantiate */
+  int slots, blocks_per_cu;        /* launch shape used */
+  int graph_nodes;
+  int tuned_candidates;            /* candidates timed when autotune was set */
+  double tuned_ms[16];             /* their launch times, in candidate order */
+} mbr_batch_result;
+
+/* Render n scenes into n caller-owned DEVICE framebuffers (fbs[i] sized as
+ * scenes[i]).  Stats sum over the frames of the last replay.  Frames are
+ * independent: results equal n calls of mbr_set_scene + mbr_render_device. */
+int mbr_render_batch(mbr_ctx *ctx, const mbr_scene *scenes, int n,
+                     const mbr_texture *lightmap, mbr_framebuffer *fbs,
+                     const mbr_batch_opts *opts, mbr_stats *stats,
+                     mbr_batch_result *result);
+
 
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,

@@ -109,6 +109,20 @@ class Framebuffer(C.Structure):
     ]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("stream", C.c_void_p), ("slots", C.c_int), ("blocks_per_cu", C.c_int),
+        ("autotune", C.c_int), ("replays", C.c_int),
+    ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("ms_per_launch", C.c_double), ("ms_build", C.c_double), ("slots", C.c_int), ("blocks_per_cu", C.c_int),
+        ("graph_nodes", C.c_int), ("tuned_candidates", C.c_int), ("tuned_ms", C.c_double * 16),
+    ]
+
+
 class RenderOpts(C.Structure):
     _fields_ = [
         ("precision", C.c_int),

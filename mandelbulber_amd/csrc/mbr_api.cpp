@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -241,6 +242,18 @@ int ao_table(int AO_quality, const mbr_texture *lm, std::vector<double> &dirs, s
 
 }  // namespace
 
+// Device buffers of one chain of band launches: the single-frame path uses
+// slot 0; a batch runs up to kMaxSlots frame chains concurrently.
+struct Slot {
+  BandBufs band{};
+  int prec = -1;
+  // shading buffers (per hit; element type of the band's precision)
+  void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
+  size_t sh_ao_bytes = 0;
+};
+
+constexpr int kMaxSlots = 4;
+
 struct mbr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -253,17 +266,22 @@ struct mbr_ctx {
   double *d_ao_d = nullptr;
   int *d_ao_rgb = nullptr;
   int ao_count = 0;
-  BandBufs band{};
-  int band_prec = -1;
+  Slot slots[kMaxSlots];
   unsigned long long *d_stats = nullptr;
   int cu_count = 0;
-  // shading buffers (per hit; element type of the band's precision)
-  void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
-  size_t sh_ao_bytes = 0;
-  std::vector<hipEvent_t> ev;  // 4 per band of the last frame
+  std::vector<hipEvent_t> ev;  // 6 per band of the last frame
   // scratch planes of mbr_render (host-buffer entry point)
   void *d_planes = nullptr;
   size_t d_planes_bytes = 0;
+  // batch: per-frame scene blocks and AO tables, streams, tuned launch shape
+  DevScene *d_batch_sc = nullptr;
+  int batch_sc_cap = 0;
+  float *d_batch_ao_f = nullptr;
+  double *d_batch_ao_d = nullptr;
+  int *d_batch_ao_rgb = nullptr;
+  size_t batch_ao_cap = 0;
+  hipStream_t side[kMaxSlots] = {};
+  int tuned_slots = 0, tuned_bpc = 0;
 
   int fail(int rc, const std::string &msg) {
     err = msg;
@@ -283,26 +301,26 @@ struct mbr_ctx {
 
 namespace {
 
-void free_band(mbr_ctx *c) {
-  BandBufs &b = c->band;
+void free_band(Slot &sl) {
+  BandBufs &b = sl.band;
   void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
-                  b.rec_thr, b.ovf_off, b.hit_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
+                  b.rec_thr, b.ovf_off, b.hit_list, b.redo_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
                   b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift, b.hand_dist,
                   b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   b = BandBufs{};
-  void *sp[] = {c->sh_base, c->sh_spec, c->sh_col, c->sh_fast, c->sh_ao};
+  void *sp[] = {sl.sh_base, sl.sh_spec, sl.sh_col, sl.sh_fast, sl.sh_ao};
   for (void *p : sp)
     if (p) (void)hipFree(p);
-  c->sh_base = c->sh_spec = c->sh_col = c->sh_fast = c->sh_ao = nullptr;
-  c->sh_ao_bytes = 0;
-  c->band_prec = -1;
+  sl.sh_base = sl.sh_spec = sl.sh_col = sl.sh_fast = sl.sh_ao = nullptr;
+  sl.sh_ao_bytes = 0;
+  sl.prec = -1;
 }
 
 // Overflow pool of `runs` runs (freeing the previous one).
-int alloc_pool(mbr_ctx *c, long long runs) {
-  BandBufs &b = c->band;
+int alloc_pool(mbr_ctx *c, Slot &sl, long long runs) {
+  BandBufs &b = sl.band;
   if (b.ovf_step) (void)hipFree(b.ovf_step);
   if (b.ovf_dist) (void)hipFree(b.ovf_dist);
   if (b.ovf_thr) (void)hipFree(b.ovf_thr);
@@ -316,52 +334,71 @@ int alloc_pool(mbr_ctx *c, long long runs) {
   return MBR_OK;
 }
 
-// Band buffers for `cap` rays; kept while precision and cap stay the same.
-int alloc_band(mbr_ctx *c, int precision, int cap) {
-  if (c->band_prec == precision && c->band.cap == cap) return MBR_OK;
-  free_band(c);
-  BandBufs &b = c->band;
-  b.cap = cap;
-  b.slab = kSlab;
-  b.rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
-  const size_t n = (size_t)b.cap;
-  HIPCHECK(c, hipMalloc((void **)&b.found, n));
-  HIPCHECK(c, hipMalloc((void **)&b.object, n));
-  HIPCHECK(c, hipMalloc((void **)&b.count, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.depth, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.thr, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.last, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.px, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.py, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.pz, n * sizeof(double)));
-  const size_t rec = n * (size_t)b.slab * b.rec_bytes;
-  HIPCHECK(c, hipMalloc(&b.rec_step, rec));
-  HIPCHECK(c, hipMalloc(&b.rec_dist, rec));
-  HIPCHECK(c, hipMalloc(&b.rec_thr, rec));
-  HIPCHECK(c, hipMalloc((void **)&b.ovf_off, n * sizeof(long long)));
-  HIPCHECK(c, hipMalloc((void **)&b.hit_list, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.counters, CNT_COUNT * sizeof(unsigned long long)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_ray, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_i, n * sizeof(int)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_scan, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_step, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_drift, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc((void **)&b.hand_dist, n * sizeof(double)));
-  HIPCHECK(c, hipMalloc(&b.sh_r, n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc(&b.sh_g, n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc(&b.sh_b, n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc((void **)&b.col_r, n));
-  HIPCHECK(c, hipMalloc((void **)&b.col_g, n));
-  HIPCHECK(c, hipMalloc((void **)&b.col_b, n));
-  HIPCHECK(c, hipMalloc(&c->sh_base, 3 * n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc(&c->sh_spec, 3 * n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc(&c->sh_col, 3 * n * b.rec_bytes));
-  HIPCHECK(c, hipMalloc(&c->sh_fast, n * b.rec_bytes));
-  // one overflow run per 256 rays to begin with; grown when a frame needs more
-  int rc = alloc_pool(c, std::max<long long>(64, n / 256));
-  if (rc != MBR_OK) return rc;
-  c->band_prec = precision;
+// Band buffers for `cap` rays and AO intensities for `ao_count` directions
+// per hit; kept while precision and cap stay the same.
+int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count) {
+  if (sl.prec != precision || sl.band.cap != cap) {
+    free_band(sl);
+    BandBufs &b = sl.band;
+    b.cap = cap;
+    b.slab = kSlab;
+    b.rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
+    const size_t n = (size_t)b.cap;
+    HIPCHECK(c, hipMalloc((void **)&b.found, n));
+    HIPCHECK(c, hipMalloc((void **)&b.object, n));
+    HIPCHECK(c, hipMalloc((void **)&b.count, n * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.depth, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.thr, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.last, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.px, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.py, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.pz, n * sizeof(double)));
+    const size_t rec = n * (size_t)b.slab * b.rec_bytes;
+    HIPCHECK(c, hipMalloc(&b.rec_step, rec));
+    HIPCHECK(c, hipMalloc(&b.rec_dist, rec));
+    HIPCHECK(c, hipMalloc(&b.rec_thr, rec));
+    HIPCHECK(c, hipMalloc((void **)&b.ovf_off, n * sizeof(long long)));
+    HIPCHECK(c, hipMalloc((void **)&b.hit_list, n * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.redo_list, n * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.counters, CNT_COUNT * sizeof(unsigned long long)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_ray, n * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_i, n * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_scan, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_step, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_drift, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&b.hand_dist, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&b.sh_r, n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&b.sh_g, n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&b.sh_b, n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc((void **)&b.col_r, n));
+    HIPCHECK(c, hipMalloc((void **)&b.col_g, n));
+    HIPCHECK(c, hipMalloc((void **)&b.col_b, n));
+    HIPCHECK(c, hipMalloc(&sl.sh_base, 3 * n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&sl.sh_spec, 3 * n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&sl.sh_col, 3 * n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&sl.sh_fast, n * b.rec_bytes));
+    // one overflow run per 256 rays to begin with; grown when a frame needs more
+    int rc = alloc_pool(c, sl, std::max<long long>(64, (long long)n / 256));
+    if (rc != MBR_OK) return rc;
+    sl.prec = precision;
+  }
+  const size_t need = (size_t)sl.band.cap * (size_t)std::max(1, ao_count) * sl.band.rec_bytes;
+  if (ao_count > 0 && need > sl.sh_ao_bytes) {
+    if (sl.sh_ao) (void)hipFree(sl.sh_ao);
+    sl.sh_ao = nullptr;
+    sl.sh_ao_bytes = 0;
+    HIPCHECK(c, hipMalloc(&sl.sh_ao, need));
+    sl.sh_ao_bytes = need;
+  }
   return MBR_OK;
+}
+
+// Rays per band: every ray of a band may hit, and the AO pass keeps one
+// intensity per (hit, direction).
+int band_cap(int precision, int ao_count) {
+  if (ao_count <= 0) return kBandRays;
+  const size_t per_ray = (size_t)ao_count * (precision == MBR_PREC_FP64 ? 8 : 4);
+  return (int)std::min<size_t>((size_t)kBandRays, std::max<size_t>(4096, kAOBudget / per_ray));
 }
 
 int check_supported(mbr_ctx *c, const mbr_scene &s) {
@@ -440,7 +477,12 @@ void mbr_destroy(mbr_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  free_band(c);
+  for (Slot &sl : c->slots) free_band(sl);
+  for (hipStream_t st : c->side)
+    if (st) (void)hipStreamDestroy(st);
+  void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb};
+  for (void *p : bp)
+    if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -452,9 +494,22 @@ void mbr_destroy(mbr_ctx *c) {
 
 const char *mbr_last_error(const mbr_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap) {
-  if (!c || !scene_in) return MBR_ERR_INVALID_ARG;
-  mbr_scene s = *scene_in;
+}  // extern "C"
+
+namespace {
+
+// A scene after CheckPrameters and derivation: the device scene block and
+// the AO direction table (directions as x, y, z, 0; lightmap colours).
+struct Prepared {
+  mbr_scene s;
+  DevScene d;
+  std::vector<double> dirs;
+  std::vector<int> rgb;
+};
+
+int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap, Prepared &p) {
+  mbr_scene &s = p.s;
+  s = *scene_in;
   int rc = check_supported(c, s);
   if (rc != MBR_OK) return rc;
   // CheckPrameters (src/interface.cpp:3587-3602)
@@ -470,7 +525,7 @@ int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   if (lightmap && (!lightmap->rgb || lightmap->width <= 0 || lightmap->height <= 0))
     return c->fail(MBR_ERR_INVALID_ARG, "bad lightmap texture");
 
-  DevScene &d = c->host_sc;
+  DevScene &d = p.d;
   std::memset(&d, 0, sizeof d);
   d.width = s.image_width;
   d.height = s.image_height;
@@ -546,11 +601,23 @@ int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   fill_shade(d.sd, s, light, resolution);
   fill_shade(d.sf, s, light, resolution);
 
-  std::vector<double> dirs;
-  std::vector<int> rgb;
-  const int count = ao_table(s.ao_quality, lightmap, dirs, rgb);
-  d.ao_count = count;
+  d.ao_count = ao_table(s.ao_quality, lightmap, p.dirs, p.rgb);
+  return MBR_OK;
+}
 
+}  // namespace
+
+extern "C" {
+
+int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap) {
+  if (!c || !scene_in) return MBR_ERR_INVALID_ARG;
+  Prepared p;
+  int rc = prepare_scene(c, scene_in, lightmap, p);
+  if (rc != MBR_OK) return rc;
+  const DevScene &d = p.d;
+  const std::vector<double> &dirs = p.dirs;
+  const std::vector<int> &rgb = p.rgb;
+  const int count = d.ao_count;
   HIPCHECK(c, hipSetDevice(c->device));
   if (count > c->ao_count) {
     if (c->d_ao_f) (void)hipFree(c->d_ao_f);
@@ -568,49 +635,54 @@ int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   HIPCHECK(c, hipMemcpyAsync(c->d_ao_rgb, rgb.data(), rgb.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(c->d_sc, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->scene = s;
+  c->host_sc = d;
+  c->scene = p.s;
   c->has_scene = true;
   return MBR_OK;
 }
 
-int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr_stats *stats) {
-  if (!c || !fb) return MBR_ERR_INVALID_ARG;
-  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
-  const mbr_scene &s = c->scene;
-  if (fb->width != s.image_width || fb->height != s.image_height)
-    return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
-  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
-  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP && precision != MBR_PREC_FP32_MARCH)
-    return c->fail(MBR_ERR_INVALID_ARG, "unknown precision");
-  int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
-  if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
-  if (row0 < 0) row0 = 0;
-  if (row0 >= row1) return c->fail(MBR_ERR_INVALID_ARG, "empty row range");
-  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
-  HIPCHECK(c, hipSetDevice(c->device));
-  // band size: every ray of a band may hit, and the AO pass keeps one
-  // intensity per (hit, direction)
-  const bool slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
-  const int rec_bytes = precision == MBR_PREC_FP64 ? 8 : 4;
-  int cap = kBandRays;
-  if (slow_ao) {
-    const size_t per_ray = (size_t)c->host_sc.ao_count * rec_bytes;
-    cap = (int)std::min<size_t>((size_t)kBandRays, std::max<size_t>(4096, kAOBudget / per_ray));
-  }
-  int rc = alloc_band(c, precision, cap);
-  if (rc != MBR_OK) return rc;
-  BandBufs &b = c->band;
-  if (slow_ao) {
-    const size_t need = (size_t)b.cap * (size_t)c->host_sc.ao_count * b.rec_bytes;
-    if (need > c->sh_ao_bytes) {
-      if (c->sh_ao) (void)hipFree(c->sh_ao);
-      c->sh_ao = nullptr;
-      c->sh_ao_bytes = 0;
-      HIPCHECK(c, hipMalloc(&c->sh_ao, need));
-      c->sh_ao_bytes = need;
-    }
-  }
+}  // extern "C"
 
+namespace {
+
+// Enqueue every band of one frame (rays [first, last) of the image) on
+// stream st, with the scene block d_sc, through the buffers of slot sl.
+// `persist` is the workgroup count of the persistent launches.  With ev
+// (6 events per band) the launches are bracketed for timing; a captured
+// batch passes none.
+int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, const DevScene &host, int formula,
+                  AODev ao, long long first, long long last, const FrameOut &out, int persist, hipStream_t st,
+                  hipEvent_t *ev, volatile const int *cancel) {
+  BandBufs &b = sl.band;
+  ShadeBufs sbuf;
+  sbuf.base = sl.sh_base;
+  sbuf.spec = sl.sh_spec;
+  sbuf.col = sl.sh_col;
+  sbuf.fast_ao = sl.sh_fast;
+  sbuf.ao = sl.sh_ao;
+  sbuf.n = b.cap;
+  sbuf.slow_ao = host.ao && !host.fast_ao;
+  sbuf.formula = formula;
+  int band = 0;
+  for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
+    if (cancel && *cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
+    const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
+    hipEvent_t *e = ev ? ev + 6 * band : nullptr;
+    HIPCHECK(c, launch_band_reset(b, st));
+    if (e) HIPCHECK(c, hipEventRecord(e[0], st));
+    HIPCHECK(c, launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, persist, c->d_stats, st));
+    HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, c->d_stats, st));
+    if (e) HIPCHECK(c, hipEventRecord(e[1], st));
+    HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, c->d_stats, st,
+                             e ? e[4] : nullptr, e ? e[5] : nullptr));
+    if (e) HIPCHECK(c, hipEventRecord(e[2], st));
+    HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st));
+    if (e) HIPCHECK(c, hipEventRecord(e[3], st));
+  }
+  return MBR_OK;
+}
+
+FrameOut frame_out(const mbr_framebuffer *fb) {
   FrameOut out;
   for (int i = 0; i < 3; i++) {
     out.rgb[i] = fb->rgb[i];
@@ -619,53 +691,83 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   out.z = fb->z;
   out.alpha = fb->alpha;
   out.opacity = fb->opacity;
-  AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
-  ShadeBufs sbuf;
-  sbuf.base = c->sh_base;
-  sbuf.spec = c->sh_spec;
-  sbuf.col = c->sh_col;
-  sbuf.fast_ao = c->sh_fast;
-  sbuf.ao = c->sh_ao;
-  sbuf.n = b.cap;
-  sbuf.slow_ao = slow_ao;
-  sbuf.formula = c->scene.formula;
+  return out;
+}
 
+void fill_stats(mbr_stats *stats, const unsigned long long *v, long long rays, int launches) {
+  std::memset(stats, 0, sizeof *stats);
+  stats->rays = (uint64_t)rays;
+  stats->hits = v[ST_HITS];
+  stats->march_de_calls = v[ST_MARCH_DE];
+  stats->shade_de_calls = v[ST_SHADE_DE];
+  stats->iterations = v[ST_MARCH_ITERS] + v[ST_SHADE_ITERS];
+  stats->march_iterations = v[ST_MARCH_ITERS];
+  stats->shade_iterations = v[ST_SHADE_ITERS];
+  stats->march_launches = launches;
+  stats->shade_launches = launches;
+  stats->finish_launches = launches;
+  stats->missed_de = v[ST_MISSED];
+  stats->step_overflow_rays = v[ST_OVF_RAYS];
+  stats->fp64_redo_rays = v[ST_REDO];
+  stats->ao_de_calls = v[ST_AO_DE];
+  stats->ao_iterations = v[ST_AO_ITERS];
+  stats->ao_steps = v[ST_AO_STEPS];
+}
+
+int check_precision(mbr_ctx *c, int precision) {
+  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP && precision != MBR_PREC_FP32_MARCH)
+    return c->fail(MBR_ERR_INVALID_ARG, "unknown precision");
+  return MBR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr_stats *stats) {
+  if (!c || !fb) return MBR_ERR_INVALID_ARG;
+  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
+  const mbr_scene &s = c->scene;
+  if (fb->width != s.image_width || fb->height != s.image_height)
+    return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
+  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  int rc = check_precision(c, precision);
+  if (rc != MBR_OK) return rc;
+  int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
+  if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
+  if (row0 < 0) row0 = 0;
+  if (row0 >= row1) return c->fail(MBR_ERR_INVALID_ARG, "empty row range");
+  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  HIPCHECK(c, hipSetDevice(c->device));
+  const bool slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
+  const int ao_count = slow_ao ? c->host_sc.ao_count : 0;
+  Slot &sl = c->slots[0];
+  rc = alloc_slot(c, sl, precision, band_cap(precision, ao_count), ao_count);
+  if (rc != MBR_OK) return rc;
+  const FrameOut out = frame_out(fb);
+  const AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
   const long long first = (long long)row0 * s.image_width;
   const long long last = (long long)row1 * s.image_width;
-  const int n_bands = (int)((last - first + b.cap - 1) / b.cap);
+  const int n_bands = (int)((last - first + sl.band.cap - 1) / sl.band.cap);
   while ((int)c->ev.size() < 6 * n_bands) {
     hipEvent_t e;
     HIPCHECK(c, hipEventCreate(&e));
     c->ev.push_back(e);
   }
+  const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
   unsigned long long v[ST_COUNT];
   for (int attempt = 0;; attempt++) {
-    HIPCHECK(c, hipMemsetAsync(c->d_stats, 0, ST_COUNT * sizeof(unsigned long long), st));
-    int band = 0;
-    for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
-      if (opts && opts->cancel && *opts->cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
-      const int band_rays = (int)std::min<long long>(b.cap, last - band_first);
-      hipEvent_t *e = &c->ev[6 * band];
-      HIPCHECK(c, launch_band_reset(b, st));
-      HIPCHECK(c, hipEventRecord(e[0], st));
-      HIPCHECK(c, launch_march(precision, c->d_sc, b, (int)band_first, band_rays, c->scene.formula, c->cu_count,
-                               c->d_stats, st));
-      HIPCHECK(c, launch_march_fix(precision, c->d_sc, b, (int)band_first, c->scene.formula, c->cu_count, c->d_stats,
-                                   st));
-      HIPCHECK(c, hipEventRecord(e[1], st));
-      HIPCHECK(c, launch_shade(precision, c->d_sc, b, (int)band_first, ao, sbuf, c->cu_count, c->d_stats, st, e[4],
-                               e[5]));
-      HIPCHECK(c, hipEventRecord(e[2], st));
-      HIPCHECK(c, launch_finish(precision, c->d_sc, b, (int)band_first, band_rays, out, st));
-      HIPCHECK(c, hipEventRecord(e[3], st));
-    }
+    HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, st));
+    rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
+                       c->ev.data(), opts ? opts->cancel : nullptr);
+    if (rc != MBR_OK) return rc;
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
     if (v[ST_POOL_MISS] == 0) break;
     // some ray found the overflow pool full: grow it to hold every
     // overflowing ray of the frame and render again
     if (attempt > 0) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
-    rc = alloc_pool(c, (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
+    rc = alloc_pool(c, sl, (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
     if (rc != MBR_OK) return rc;
   }
   double ms_march = 0, ms_shade = 0, ms_finish = 0, ms_ao = 0;
@@ -682,28 +784,12 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     ms_ao += t3;
   }
   if (stats) {
-    std::memset(stats, 0, sizeof *stats);
+    fill_stats(stats, v, last - first, n_bands);
     stats->ms_march = ms_march;
     stats->ms_shade = ms_shade;
     stats->ms_finish = ms_finish;
-    stats->ms_total = ms_march + ms_shade + ms_finish;
-    stats->rays = (uint64_t)(last - first);
-    stats->hits = v[ST_HITS];
-    stats->march_de_calls = v[ST_MARCH_DE];
-    stats->shade_de_calls = v[ST_SHADE_DE];
-    stats->iterations = v[ST_MARCH_ITERS] + v[ST_SHADE_ITERS];
-    stats->march_iterations = v[ST_MARCH_ITERS];
-    stats->shade_iterations = v[ST_SHADE_ITERS];
-    stats->march_launches = n_bands;
-    stats->shade_launches = n_bands;
-    stats->finish_launches = n_bands;
-    stats->missed_de = v[ST_MISSED];
-    stats->step_overflow_rays = v[ST_OVF_RAYS];
-    stats->fp64_redo_rays = v[ST_REDO];
     stats->ms_ao = ms_ao;
-    stats->ao_de_calls = v[ST_AO_DE];
-    stats->ao_iterations = v[ST_AO_ITERS];
-    stats->ao_steps = v[ST_AO_STEPS];
+    stats->ms_total = ms_march + ms_shade + ms_finish;
   }
   return MBR_OK;
 }
@@ -748,6 +834,238 @@ int mbr_render(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr
   if (fb->alpha) HIPCHECK(c, hipMemcpyAsync(fb->alpha + o0, dfb.alpha + o0, cnt * 2, hipMemcpyDeviceToHost, st));
   if (fb->opacity) HIPCHECK(c, hipMemcpyAsync(fb->opacity + o0, dfb.opacity + o0, cnt * 2, hipMemcpyDeviceToHost, st));
   HIPCHECK(c, hipStreamSynchronize(st));
+  return MBR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+namespace {
+
+struct BatchFrame {
+  Prepared p;
+  long long ao_off = 0;   // first direction of the frame's AO table in the batch table
+  long long rays = 0;
+};
+
+// Capture every frame's band launches into one graph: frames go to the
+// `slots` chains greedily by ray count (largest first, to the least loaded
+// chain), chains fork from and join back to stream s0.
+int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_framebuffer *fbs, int precision,
+                      int slots, int bpc, hipStream_t s0, hipGraphExec_t *exec, int *nodes) {
+  const int persist = (c->cu_count > 0 ? c->cu_count : 256) * bpc;
+  std::vector<int> order(frames.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return frames[a].rays > frames[b].rays; });
+  std::vector<long long> load(slots, 0);
+  std::vector<int> chain(frames.size(), 0);
+  for (int f : order) {
+    int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    chain[f] = k;
+    load[k] += frames[f].rays;
+  }
+  hipEvent_t fork = nullptr, join[kMaxSlots] = {};
+  HIPCHECK(c, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  for (int k = 1; k < slots; k++) HIPCHECK(c, hipEventCreateWithFlags(&join[k], hipEventDisableTiming));
+  hipGraph_t graph = nullptr;
+  int rc = MBR_OK;
+  HIPCHECK(c, hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+  auto body = [&]() -> int {
+    HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, s0));
+    HIPCHECK(c, hipEventRecord(fork, s0));
+    for (int k = 1; k < slots; k++) HIPCHECK(c, hipStreamWaitEvent(c->side[k], fork, 0));
+    for (int k = 0; k < slots; k++) {
+      hipStream_t st = k == 0 ? s0 : c->side[k];
+      for (int f : order) {
+        if (chain[f] != k) continue;
+        const BatchFrame &fr = frames[f];
+        const AODev ao{c->d_batch_ao_f + 4 * fr.ao_off, c->d_batch_ao_d + 4 * fr.ao_off,
+                       c->d_batch_ao_rgb + 4 * fr.ao_off, fr.p.d.ao_count};
+        int r = enqueue_frame(c, c->slots[k], precision, c->d_batch_sc + f, fr.p.d, fr.p.s.formula, ao, 0, fr.rays,
+                              frame_out(&fbs[f]), persist, st, nullptr, nullptr);
+        if (r != MBR_OK) return r;
+      }
+    }
+    for (int k = 1; k < slots; k++) {
+      HIPCHECK(c, hipEventRecord(join[k], c->side[k]));
+      HIPCHECK(c, hipStreamWaitEvent(s0, join[k], 0));
+    }
+    return MBR_OK;
+  };
+  rc = body();
+  hipError_t ec = hipStreamEndCapture(s0, &graph);
+  if (rc == MBR_OK && ec != hipSuccess) rc = c->hip_fail(ec, "hipStreamEndCapture");
+  if (rc == MBR_OK) {
+    hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) rc = c->hip_fail(ei, "hipGraphInstantiate");
+    size_t nn = 0;
+    if (rc == MBR_OK && hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess) *nodes = (int)nn;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(fork);
+  for (int k = 1; k < slots; k++) (void)hipEventDestroy(join[k]);
+  return rc;
+}
+
+// Launch the graph `replays` times; mean GPU ms per launch.
+int time_graph(mbr_ctx *c, hipGraphExec_t exec, hipStream_t s0, int replays, double *ms) {
+  hipEvent_t e0, e1;
+  HIPCHECK(c, hipEventCreate(&e0));
+  HIPCHECK(c, hipEventCreate(&e1));
+  HIPCHECK(c, hipEventRecord(e0, s0));
+  for (int i = 0; i < replays; i++) HIPCHECK(c, hipGraphLaunch(exec, s0));
+  HIPCHECK(c, hipEventRecord(e1, s0));
+  HIPCHECK(c, hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHECK(c, hipEventElapsedTime(&t, e0, e1));
+  *ms = t / replays;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return MBR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_texture *lightmap, mbr_framebuffer *fbs,
+                     const mbr_batch_opts *opts, mbr_stats *stats, mbr_batch_result *result) {
+  if (!c || !scenes || !fbs || n <= 0) return MBR_ERR_INVALID_ARG;
+  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  int rc = check_precision(c, precision);
+  if (rc != MBR_OK) return rc;
+  HIPCHECK(c, hipSetDevice(c->device));
+  // derive every scene on the host
+  std::vector<BatchFrame> frames(n);
+  long long ao_total = 0;
+  int max_ao = 0;
+  for (int i = 0; i < n; i++) {
+    rc = prepare_scene(c, &scenes[i], lightmap, frames[i].p);
+    if (rc != MBR_OK) return c->fail(rc, "scene " + std::to_string(i) + ": " + c->err);
+    const mbr_scene &s = frames[i].p.s;
+    if (fbs[i].width != s.image_width || fbs[i].height != s.image_height)
+      return c->fail(MBR_ERR_INVALID_ARG, "framebuffer " + std::to_string(i) + " differs from its scene's size");
+    frames[i].rays = (long long)s.image_width * s.image_height;
+    frames[i].ao_off = ao_total;
+    ao_total += frames[i].p.d.ao_count;
+    const DevScene &d = frames[i].p.d;
+    if (d.ao && !d.fast_ao) max_ao = std::max(max_ao, d.ao_count);
+  }
+  // upload scene blocks and AO tables
+  if (n > c->batch_sc_cap) {
+    if (c->d_batch_sc) (void)hipFree(c->d_batch_sc);
+    c->d_batch_sc = nullptr;
+    c->batch_sc_cap = 0;
+    HIPCHECK(c, hipMalloc((void **)&c->d_batch_sc, (size_t)n * sizeof(DevScene)));
+    c->batch_sc_cap = n;
+  }
+  if ((size_t)ao_total > c->batch_ao_cap) {
+    void *p[] = {c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb};
+    for (void *q : p)
+      if (q) (void)hipFree(q);
+    c->d_batch_ao_f = nullptr; c->d_batch_ao_d = nullptr; c->d_batch_ao_rgb = nullptr;
+    c->batch_ao_cap = 0;
+    HIPCHECK(c, hipMalloc((void **)&c->d_batch_ao_f, (size_t)ao_total * 4 * sizeof(float)));
+    HIPCHECK(c, hipMalloc((void **)&c->d_batch_ao_d, (size_t)ao_total * 4 * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&c->d_batch_ao_rgb, (size_t)ao_total * 4 * sizeof(int)));
+    c->batch_ao_cap = (size_t)ao_total;
+  }
+  {
+    std::vector<DevScene> ds(n);
+    std::vector<float> af((size_t)ao_total * 4);
+    std::vector<double> ad((size_t)ao_total * 4);
+    std::vector<int> ar((size_t)ao_total * 4);
+    for (int i = 0; i < n; i++) {
+      ds[i] = frames[i].p.d;
+      const size_t o = (size_t)frames[i].ao_off * 4;
+      for (size_t j = 0; j < frames[i].p.dirs.size(); j++) {
+        ad[o + j] = frames[i].p.dirs[j];
+        af[o + j] = (float)frames[i].p.dirs[j];
+      }
+      std::copy(frames[i].p.rgb.begin(), frames[i].p.rgb.end(), ar.begin() + o);
+    }
+    HIPCHECK(c, hipMemcpy(c->d_batch_sc, ds.data(), (size_t)n * sizeof(DevScene), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_batch_ao_f, af.data(), af.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_batch_ao_d, ad.data(), ad.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_batch_ao_rgb, ar.data(), ar.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  hipStream_t s0 = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  for (int k = 1; k < kMaxSlots; k++)
+    if (!c->side[k]) HIPCHECK(c, hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+  const int cap = band_cap(precision, max_ao);
+
+  // launch shapes to time (autotune) or the one to use
+  std::vector<std::pair<int, int>> cands;
+  int slots = opts && opts->slots ? opts->slots : (c->tuned_slots ? c->tuned_slots : 2);
+  int bpc = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (c->tuned_bpc ? c->tuned_bpc : 8);
+  if (opts && opts->autotune) {
+    for (int sl : {1, 2, 4})
+      for (int b : {4, 8, 16}) cands.push_back({sl, b});
+  }
+  if (slots < 1 || slots > kMaxSlots || bpc < 1 || bpc > 32)
+    return c->fail(MBR_ERR_INVALID_ARG, "slots must be 1..4 and blocks_per_cu 1..32");
+  const int n_slots = cands.empty() ? slots : kMaxSlots;
+  for (int k = 0; k < n_slots; k++) {
+    rc = alloc_slot(c, c->slots[k], precision, cap, max_ao);
+    if (rc != MBR_OK) return rc;
+  }
+  mbr_batch_result res;
+  std::memset(&res, 0, sizeof res);
+  if (!cands.empty()) {
+    double best = 1e300;
+    for (size_t i = 0; i < cands.size(); i++) {
+      hipGraphExec_t ex = nullptr;
+      int nodes = 0;
+      rc = build_batch_graph(c, frames, fbs, precision, cands[i].first, cands[i].second, s0, &ex, &nodes);
+      if (rc != MBR_OK) return rc;
+      double ms = 0;
+      HIPCHECK(c, hipGraphLaunch(ex, s0));  // warm-up
+      rc = time_graph(c, ex, s0, 1, &ms);
+      (void)hipGraphExecDestroy(ex);
+      if (rc != MBR_OK) return rc;
+      if (i < 16) res.tuned_ms[i] = ms;
+      if (ms < best) {
+        best = ms;
+        slots = cands[i].first;
+        bpc = cands[i].second;
+      }
+    }
+    res.tuned_candidates = (int)cands.size();
+    c->tuned_slots = slots;
+    c->tuned_bpc = bpc;
+  }
+  const int replays = opts && opts->replays > 0 ? opts->replays : 1;
+  unsigned long long v[ST_COUNT];
+  for (int attempt = 0;; attempt++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    hipGraphExec_t ex = nullptr;
+    int nodes = 0;
+    rc = build_batch_graph(c, frames, fbs, precision, slots, bpc, s0, &ex, &nodes);
+    if (rc != MBR_OK) return rc;
+    res.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    res.graph_nodes = nodes;
+    double ms = 0;
+    rc = time_graph(c, ex, s0, replays, &ms);
+    (void)hipGraphExecDestroy(ex);
+    if (rc != MBR_OK) return rc;
+    res.ms_per_launch = ms;
+    HIPCHECK(c, hipMemcpy(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost));
+    if (v[ST_POOL_MISS] == 0) break;
+    if (attempt > 0) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
+    for (int k = 0; k < slots; k++) {
+      rc = alloc_pool(c, c->slots[k], (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
+      if (rc != MBR_OK) return rc;
+    }
+  }
+  res.slots = slots;
+  res.blocks_per_cu = bpc;
+  if (result) *result = res;
+  if (stats) {
+    long long rays = 0;
+    for (auto &f : frames) rays += f.rays;
+    fill_stats(stats, v, rays, n);
+    stats->ms_total = res.ms_per_launch;
+  }
   return MBR_OK;
 }
 

@@ -456,15 +456,52 @@ __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict
     s.dist = b.hand_dist[h];
     const RecPlanes<RT> rec = ray_records<RT>(b, r, stats);
     MarchOut o;
-    // the continuation, then (when it reports an ambiguous decision) the
-    // FP64 march from the start: one inlined copy of the march loop
-    for (int pass = 0;; pass++) {
-      bool ambiguous = false;
-      march_continue_f64<RT, F>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
-      if (!ambiguous || pass > 0) break;
-      s = ray_start_state(sc);
+    bool ambiguous = false;
+    march_continue_f64<RT, F>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
+    if (ambiguous) {
+      // re-marched from the start by k_march_redo
+      b.redo_list[atomicAdd(&b.counters[CNT_REDO], 1ull)] = r;
       redo++;
+    } else {
+      write_ray(b, r, o);
+      hits += o.found;
+      missed += o.missed;
     }
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_MARCH_DE] = cnt.de_calls;
+  v[ST_MARCH_ITERS] = cnt.iterations;
+  v[ST_HITS] = hits;
+  v[ST_MISSED] = missed;
+  v[ST_REDO] = redo;
+  flush_stats(stats, v);
+}
+
+// FP64 march from the start of the rays k_march_fix listed (FP32_MARCH rays
+// whose result FP32 error could have changed).  A kernel of its own: with
+// the redo as a second inlined pass of the march loop inside k_march_fix,
+// the gfx950 build returned found = false for some lanes whose rays were
+// found, depending on which rays shared a wave (the Mandelbox FP32_MARCH
+// parity test covers it).
+template <class RT, int F>
+__global__ void __launch_bounds__(kBlock) k_march_redo(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                       unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const long long n_redo = (long long)b.counters[CNT_REDO];
+  LaneCount cnt = {0, 0};
+  unsigned long long hits = 0, missed = 0;
+  const V3<double> start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
+  for (long long h = (long long)blockIdx.x * kBlock + threadIdx.x; h < n_redo; h += (long long)gridDim.x * kBlock) {
+    const int r = b.redo_list[h];
+    const int p = band_first + r;
+    const int x = p % sc.width, y = p / sc.width;
+    V3<double> dir;
+    view_vector(sc, x, y, dir);
+    const uint32_t pixel = sc.pixel_base + (uint32_t)p;
+    const RecPlanes<RT> rec = ray_records<RT>(b, r, stats);
+    MarchOut o;
+    bool unused = false;
+    march_continue_f64<RT, F>(sc, start, dir, pixel, rec, ray_start_state(sc), o, cnt, unused);
     write_ray(b, r, o);
     hits += o.found;
     missed += o.missed;
@@ -474,7 +511,6 @@ __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict
   v[ST_MARCH_ITERS] = cnt.iterations;
   v[ST_HITS] = hits;
   v[ST_MISSED] = missed;
-  v[ST_REDO] = redo;
   flush_stats(stats, v);
 }
 
@@ -825,9 +861,9 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int cu_count, unsigned long long *stats, hipStream_t s) {
+                        int formula, int persist_blocks, unsigned long long *stats, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
-  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
+  const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto tag, auto rtag) {
     using T = decltype(tag);
     using RT = decltype(rtag);
@@ -845,8 +881,8 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
 }
 
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
-                            int cu_count, unsigned long long *stats, hipStream_t s) {
-  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 4 : 1024));
+                            int persist_blocks, unsigned long long *stats, hipStream_t s) {
+  const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto rtag) {
     using RT = decltype(rtag);
     if (formula == 7)
@@ -855,6 +891,14 @@ hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b
       hipLaunchKernelGGL((k_march_fix<RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
     else
       hipLaunchKernelGGL((k_march_fix<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    if (precision == 2) {
+      if (formula == 7)
+        hipLaunchKernelGGL((k_march_redo<RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_march_redo<RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+      else
+        hipLaunchKernelGGL((k_march_redo<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    }
   };
   if (precision == 0) run(double{});
   else run(float{});
@@ -870,13 +914,22 @@ hipError_t launch_band_reset(const BandBufs &b, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ void k_zero(unsigned long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_zero(unsigned long long *p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
-                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s,
+                        ShadeBufs sbuf, int persist_blocks, unsigned long long *stats, hipStream_t s,
                         hipEvent_t ev_ao0, hipEvent_t ev_ao1) {
   const int formula = sbuf.formula;
   // persistent grids: enough workgroups to fill every CU several times; the
   // kernels loop over the band's hits
-  const dim3 grid((unsigned)(cu_count > 0 ? cu_count * 8 : 2048));
+  const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto tag) -> hipError_t {
     using T = decltype(tag);
     SurfBufs<T> sb;

@@ -19,8 +19,9 @@ enum StatIdx {
 
 // Band counters (BandBufs::counters), zeroed on the device at band start:
 // hits appended, rays handed to the FP64 continuation, primary-ray queue,
-// overflow runs handed out, AO work queue.
-enum CounterIdx { CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_RUNS = 3, CNT_AOQ = 4, CNT_COUNT = 8 };
+// overflow runs handed out, AO work queue,
+// rays listed for the FP64 redo.
+enum CounterIdx { CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_RUNS = 3, CNT_AOQ = 4, CNT_REDO = 5, CNT_COUNT = 8 };
 
 // Maximum step records of one ray (the loop bound of src/Render3D.cpp:95).
 constexpr int kMaxSteps = 10000;
@@ -40,8 +41,8 @@ struct BandBufs {
   double *depth, *thr, *last, *px, *py, *pz;
   void *rec_step, *rec_dist, *rec_thr;   // slab planes [slab][cap]
   long long *ovf_off;                    // per ray: first pool record of its run, -1 none, -2 pool full
-  // compaction list of the hits
-  int *hit_list;
+  // compaction lists: hits, rays to re-march in FP64
+  int *hit_list, *redo_list;
   unsigned long long *counters;          // CNT_* below
   // rays handed from the march to its FP64 continuation
   int *hand_ray, *hand_i;
@@ -72,11 +73,13 @@ struct AODev {
 // march with FP64 continuation, FP32 records and shading (mbr_precision)
 // All launches of a band read their work counts (hits, hand-overs) from
 // b.counters on the device: a band is enqueued without host round trips.
+// persist_blocks: workgroups of the persistent launches (a few per CU).
 hipError_t launch_band_reset(const BandBufs &b, hipStream_t s);
+hipError_t launch_zero(unsigned long long *p, int n, hipStream_t s);
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int cu_count, unsigned long long *stats, hipStream_t s);
+                        int formula, int persist_blocks, unsigned long long *stats, hipStream_t s);
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
-                            int cu_count, unsigned long long *stats, hipStream_t s);
+                            int persist_blocks, unsigned long long *stats, hipStream_t s);
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
@@ -88,7 +91,7 @@ struct ShadeBufs {
 // fast AO), AO rays, composition; ev_ao (when not null) is recorded around
 // the AO launch.
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
-                        ShadeBufs sbuf, int cu_count, unsigned long long *stats, hipStream_t s,
+                        ShadeBufs sbuf, int persist_blocks, unsigned long long *stats, hipStream_t s,
                         hipEvent_t ev_ao0, hipEvent_t ev_ao1);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);

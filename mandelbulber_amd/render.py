@@ -91,6 +91,30 @@ class Renderer:
               self._ctx)
         return frame, st
 
+    def render_batch(self, scenes, planes_list, precision=_abi.PREC_FP32_FIXUP, slots=0, blocks_per_cu=0,
+                     autotune=False, replays=1, stream=None, lightmap=None):
+        """Render many scenes into device planes (one dict per scene, see
+        device_framebuffer) as one captured HIP graph; returns
+        (Stats summed over the frames, BatchResult)."""
+        n = len(scenes)
+        arr = (_abi.Scene * n)()
+        for i, s in enumerate(scenes):
+            C.memmove(C.byref(arr[i]), C.byref(s), C.sizeof(_abi.Scene))
+        fbs = (_abi.Framebuffer * n)()
+        for i, (s, p) in enumerate(zip(scenes, planes_list)):
+            fbs[i] = device_framebuffer(p, s.image_width, s.image_height)
+        tex = None
+        if lightmap is not None:
+            self._lm = np.ascontiguousarray(lightmap, dtype=np.uint8)
+            tex = _abi.Texture(self._lm.shape[1], self._lm.shape[0],
+                               self._lm.ctypes.data_as(C.POINTER(C.c_uint8)))
+        o = _abi.BatchOpts(precision, stream, slots, blocks_per_cu, 1 if autotune else 0, replays)
+        st = _abi.Stats()
+        res = _abi.BatchResult()
+        check(lib().mbr_render_batch(self._ctx, arr, n, C.byref(tex) if tex is not None else None, fbs, C.byref(o),
+                                     C.byref(st), C.byref(res)), self._ctx)
+        return st, res
+
     def render_device(self, planes, precision=_abi.PREC_FP32_FIXUP, rows=None, stream=None):
         """Render into device planes (see device_framebuffer); returns Stats."""
         s = self.scene

@@ -97,3 +97,92 @@ def mandelbulb_full(width=1920, height=1080, seed=1):
         "volumetric_fog_density": 0.5,
     })
     return s
+
+
+MANDELBOX_KEYS = {
+    "formula": 8, "mandelbox_scale": 2, "mandelbox_folding_limit": 1, "mandelbox_folding_value": 2,
+    "mandelbox_folding_min_radius": 0.5, "mandelbox_folding_fixed_radius": 1, "mandelbox_rotation_enabled": 0,
+    "N": 250,
+}
+
+
+def mandelbox_view(width=320, height=240, shaded=True):
+    """Mandelbox scale 2, fold limit 1 (value 2), min radius 0.5, fixed
+    radius 1, N 250, seen whole from (0,-12,0), yaw 30, pitch 20; with
+    shaded, hard shadow, AO quality 4 and fog 0.5."""
+    s = mandelbulb_plain(width, height)
+    apply(s, dict(MANDELBOX_KEYS, view_point_y=-12, angle_alfa=30, angle_beta=20))
+    if shaded:
+        _shade(s, seed=2, ao_quality=4)
+    return s
+
+
+def menger_view(width=320, height=240, shaded=True):
+    """Menger sponge, N 15, seen from (0,-3.5,0), yaw 30, pitch 20; with
+    shaded, hard shadow, AO quality 4 and fog 0.5."""
+    s = mandelbulb_plain(width, height)
+    apply(s, {"formula": 7, "N": 15, "view_point_y": -3.5, "angle_alfa": 30, "angle_beta": 20})
+    if shaded:
+        _shade(s, seed=3, ao_quality=4)
+    return s
+
+
+def _shade(s, seed, ao_quality):
+    a, b = light_from_seed(seed)
+    apply(s, {
+        "shadows_enabled": 1, "shadows_cone_angle": 0, "penetrating_lights": 0,
+        "main_light_alfa": a, "main_light_beta": b,
+        "ambient_occlusion_enabled": 1, "fast_ambient_occlusion_mode": 0, "ambient_occlusion_quality": ao_quality,
+        "volumetric_fog_density": 0.5,
+    })
+
+
+def look_at_origin(scene, vp, roll=0.0):
+    """Place the camera at vp looking at the origin.  The forward vector of
+    the reference camera is M (0,1,0) with M = Rz(alfa) Rx(beta) Ry(gamma)
+    (src/algebra.cpp:94-143), i.e. (-sin a cos b, cos a cos b, sin b)."""
+    vp = np.asarray(vp, dtype=np.float64)
+    f = -vp / np.linalg.norm(vp)
+    beta = math.asin(max(-1.0, min(1.0, f[2])))
+    alpha = math.atan2(-f[0], f[1])
+    apply(scene, {"view_point_x": float(vp[0]), "view_point_y": float(vp[1]), "view_point_z": float(vp[2]),
+                  "angle_alfa": math.degrees(alpha), "angle_beta": math.degrees(beta),
+                  "angle_gamma": math.degrees(roll)})
+    return scene
+
+
+FLIGHT_RESOLUTIONS = ((640, 480), (800, 600), (1280, 720), (1920, 1080))
+
+
+def flight(n=64, seed=0, res_div=1):
+    """A batched flight of n frames: formulas cycling Mandelbulb (power
+    uniform in [6,10]), Mandelbox (scale 2, fold limit 1, min radius 0.5,
+    fixed radius 1), Menger sponge (N 15); resolution drawn from 640x480,
+    800x600, 1280x720, 1920x1080; camera on a seeded direction at distance
+    uniform in [1.5,5] looking at the origin; one hard-shadow ray per hit.
+    The distance is in units of the fractal's size: the scale-2 Mandelbox
+    reaches about twice as far from the origin as the Mandelbulb and the
+    sponge, so its cameras sit at twice the drawn distance (at the drawn
+    distance half of them would start inside the solid, where every ray
+    hits at depth ~0).  res_div divides every resolution (small copies for
+    tests)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        w, h = FLIGHT_RESOLUTIONS[int(rng.integers(len(FLIGHT_RESOLUTIONS)))]
+        s = mandelbulb_plain(w // res_div, h // res_div)
+        kind = k % 3
+        if kind == 0:
+            apply(s, {"formula": 2, "power": float(rng.uniform(6.0, 10.0))})
+        elif kind == 1:
+            apply(s, MANDELBOX_KEYS)
+        else:
+            apply(s, {"formula": 7, "N": 15})
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        look_at_origin(s, u * rng.uniform(1.5, 5.0) * (2.0 if kind == 1 else 1.0))
+        a, b = light_from_seed(int(rng.integers(1 << 30)))
+        apply(s, {"shadows_enabled": 1, "shadows_cone_angle": 0, "penetrating_lights": 0,
+                  "main_light_alfa": a, "main_light_beta": b, "frame_no": k})
+        out.append(s)
+    return out

@@ -67,6 +67,16 @@ def test_mandelbulb_shadow_ao_fog_small(renderer, oracle_mod, seed):
     _check(renderer, oracle_mod, sc, f"shadow+AO+fog seed {seed}")
 
 
+@pytest.mark.parametrize("shaded", [False, True])
+def test_mandelbox(renderer, oracle_mod, shaded):
+    _check(renderer, oracle_mod, scene.mandelbox_view(160, 120, shaded=shaded), f"mandelbox shaded={shaded}")
+
+
+@pytest.mark.parametrize("shaded", [False, True])
+def test_menger_sponge(renderer, oracle_mod, shaded):
+    _check(renderer, oracle_mod, scene.menger_view(160, 120, shaded=shaded), f"menger shaded={shaded}")
+
+
 def test_framebuffer_planes_consistent(renderer):
     sc = scene.mandelbulb_plain(128, 96)
     renderer.set_scene(sc)
