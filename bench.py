@@ -97,6 +97,50 @@ def cpu_baseline(sc, seconds, threads):
     }
 
 
+def bench_flight(torch, r, _abi, scene, n_frames, replays):
+    """The batched flight: n frames of mixed formulas and resolutions as one
+    captured HIP graph (autotuned launch shape) against the same frames
+    rendered one mbr_set_scene + mbr_render_device call after another (the
+    reference renders a flight one Render() per frame)."""
+    fl = scene.flight(n_frames, seed=0)
+    rays = sum(s.image_width * s.image_height for s in fl)
+    seq_planes = [planes_on_gpu(torch, s.image_width, s.image_height) for s in fl]
+    bat_planes = [planes_on_gpu(torch, s.image_width, s.image_height) for s in fl]
+    # sequential: warm-up pass, then a timed pass
+    for timed_pass in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s, p in zip(fl, seq_planes):
+            r.set_scene(s)
+            r.render_device(p, precision=_abi.PREC_FP32_FIXUP)
+        torch.cuda.synchronize()
+        dt_seq = time.perf_counter() - t0
+    # batch: autotune (times 9 launch shapes), then `replays` launches of the tuned graph
+    _, tune = r.render_batch(fl, bat_planes, autotune=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, res = r.render_batch(fl, bat_planes, replays=replays)
+    torch.cuda.synchronize()
+    dt_call = time.perf_counter() - t0
+    same = all(torch.equal(a[k], b[k]) for a, b in zip(seq_planes, bat_planes) for k in ("z", "colour", "alpha"))
+    same = same and all(torch.equal(torch.nan_to_num(a["rgb"]), torch.nan_to_num(b["rgb"]))
+                        for a, b in zip(seq_planes, bat_planes))
+    return {
+        "workload": f"{n_frames} frames, formulas cycling Mandelbulb (power U[6,10]) / Mandelbox / Menger (N 15), "
+                    f"640x480..1920x1080, camera U[1.5,5] (x2 for the Mandelbox) looking at the origin, hard shadow",
+        "rays": rays, "hits": st.hits,
+        "graph_value": rays / (res.ms_per_launch * 1e-3) / 1e6, "unit": "Mrays/s",
+        "graph_ms_per_launch": res.ms_per_launch, "graph_nodes": res.graph_nodes,
+        "graph_build_ms": res.ms_build, "call_ms_incl_build": dt_call * 1e3,
+        "sequential_value": rays / dt_seq / 1e6, "sequential_ms": dt_seq * 1e3,
+        "speedup_vs_sequential": dt_seq * 1e3 / res.ms_per_launch,
+        "tuned": {"slots": res.slots, "blocks_per_cu": res.blocks_per_cu,
+                  "candidates_ms": {f"{a}x{b}": tune.tuned_ms[i]
+                                    for i, (a, b) in enumerate((a, b) for a in (1, 2, 4) for b in (4, 8, 16))}},
+        "identical_to_sequential": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +152,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-flight", action="store_true")
+    ap.add_argument("--flight-frames", type=int, default=64)
+    ap.add_argument("--flight-replays", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this render core runs on one GPU; multi-GPU runs are not part of this build "
@@ -157,6 +204,8 @@ def main():
     ao64_ms = sum(s.ms_ao for s in st64) / launches64
     ao64_flops = sum(algorithmic_flops(sc, s, "ao") for s in st64) / launches64
 
+    flight = None if args.no_flight else bench_flight(torch, r, _abi, scene, args.flight_frames, args.flight_replays)
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -191,6 +240,7 @@ def main():
                            "fp64_redo_rays_fp64_path": st64[-1].fp64_redo_rays,
                            "step_overflow_rays_fp64_path": st64[-1].step_overflow_rays},
         "roofline": roof,
+        "flight": flight,
         "cpu_baseline": cpu,
     }
     r.close()

@@ -304,7 +304,8 @@ namespace {
 void free_band(Slot &sl) {
   BandBufs &b = sl.band;
   void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
-                  b.rec_thr, b.ovf_off, b.hit_list, b.redo_list, b.counters, b.ovf_step, b.ovf_dist, b.ovf_thr,
+                  b.rec_thr, b.ovf_off, b.hit_list, b.redo_list, b.counters, b.ovf_dir, b.ovf_step, b.ovf_dist,
+                  b.ovf_thr,
                   b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift, b.hand_dist,
                   b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
   for (void *p : ptrs)
@@ -318,21 +319,37 @@ void free_band(Slot &sl) {
   sl.prec = -1;
 }
 
-// Overflow pool of `runs` runs (freeing the previous one).
-int alloc_pool(mbr_ctx *c, Slot &sl, long long runs) {
+// Overflow pool of `dirs` directories and `chunks` chunks (freeing the
+// previous one).
+int alloc_pool(mbr_ctx *c, Slot &sl, long long dirs, long long chunks) {
   BandBufs &b = sl.band;
-  if (b.ovf_step) (void)hipFree(b.ovf_step);
-  if (b.ovf_dist) (void)hipFree(b.ovf_dist);
-  if (b.ovf_thr) (void)hipFree(b.ovf_thr);
+  void *old[] = {b.ovf_dir, b.ovf_step, b.ovf_dist, b.ovf_thr};
+  for (void *p : old)
+    if (p) (void)hipFree(p);
+  b.ovf_dir = nullptr;
   b.ovf_step = b.ovf_dist = b.ovf_thr = nullptr;
-  b.ovf_runs = 0;
-  const size_t bytes = (size_t)runs * (size_t)(kMaxSteps - b.slab) * b.rec_bytes;
+  b.ovf_dirs = b.ovf_chunks = 0;
+  const size_t bytes = (size_t)chunks * kChunk * b.rec_bytes;
+  HIPCHECK(c, hipMalloc((void **)&b.ovf_dir, (size_t)dirs * kDirSlots * sizeof(int)));
   HIPCHECK(c, hipMalloc(&b.ovf_step, bytes));
   HIPCHECK(c, hipMalloc(&b.ovf_dist, bytes));
   HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
-  b.ovf_runs = runs;
+  b.ovf_dirs = dirs;
+  b.ovf_chunks = chunks;
   return MBR_OK;
 }
+
+// Grow the pool after a render that found it full and asked for `dirs`
+// directories and `chunks` chunks: at least double it, and at least 1.25 x
+// what was asked for (rays that found no directory asked for no chunk, so a
+// second render may ask for more).
+int grow_pool(mbr_ctx *c, Slot &sl, unsigned long long dirs, unsigned long long chunks) {
+  const BandBufs &b = sl.band;
+  return alloc_pool(c, sl, std::max<long long>(2 * b.ovf_dirs, (long long)(dirs + dirs / 4 + 64)),
+                    std::max<long long>(2 * b.ovf_chunks, (long long)(chunks + chunks / 4 + 64)));
+}
+
+constexpr int kPoolAttempts = 8;  // renders of a frame while its overflow pool grows
 
 // Band buffers for `cap` rays and AO intensities for `ao_count` directions
 // per hit; kept while precision and cap stay the same.
@@ -377,8 +394,9 @@ int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count) {
     HIPCHECK(c, hipMalloc(&sl.sh_spec, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_col, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_fast, n * b.rec_bytes));
-    // one overflow run per 256 rays to begin with; grown when a frame needs more
-    int rc = alloc_pool(c, sl, std::max<long long>(64, (long long)n / 256));
+    // one directory and one chunk per 64 rays to begin with; grown when a
+    // frame needs more
+    int rc = alloc_pool(c, sl, std::max<long long>(256, (long long)n / 64), std::max<long long>(256, (long long)n / 64));
     if (rc != MBR_OK) return rc;
     sl.prec = precision;
   }
@@ -766,8 +784,8 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     if (v[ST_POOL_MISS] == 0) break;
     // some ray found the overflow pool full: grow it to hold every
     // overflowing ray of the frame and render again
-    if (attempt > 0) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
-    rc = alloc_pool(c, sl, (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
+    if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
+    rc = grow_pool(c, sl, v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
     if (rc != MBR_OK) return rc;
   }
   double ms_march = 0, ms_shade = 0, ms_finish = 0, ms_ao = 0;
@@ -1011,17 +1029,39 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   }
   mbr_batch_result res;
   std::memset(&res, 0, sizeof res);
+  unsigned long long v[ST_COUNT];
+  // Build the graph of one launch shape and launch it `replays` times; when
+  // a record found the overflow pool full, grow every slot's pool to what the
+  // batch asked for and do it again.
+  auto run_shape = [&](int sl_n, int b_n, int reps, bool warm, double *ms) -> int {
+    for (int attempt = 0;; attempt++) {
+      const auto t0 = std::chrono::steady_clock::now();
+      hipGraphExec_t ex = nullptr;
+      int nodes = 0;
+      int r2 = build_batch_graph(c, frames, fbs, precision, sl_n, b_n, s0, &ex, &nodes);
+      if (r2 != MBR_OK) return r2;
+      res.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      res.graph_nodes = nodes;
+      hipError_t e = warm ? hipGraphLaunch(ex, s0) : hipSuccess;
+      if (e == hipSuccess) r2 = time_graph(c, ex, s0, reps, ms);
+      else r2 = c->hip_fail(e, "hipGraphLaunch");
+      (void)hipGraphExecDestroy(ex);
+      if (r2 != MBR_OK) return r2;
+      HIPCHECK(c, hipMemcpy(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost));
+      if (v[ST_POOL_MISS] == 0) return MBR_OK;
+      if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
+      for (int k = 0; k < kMaxSlots; k++) {
+        if (c->slots[k].prec < 0) continue;
+        r2 = grow_pool(c, c->slots[k], v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
+        if (r2 != MBR_OK) return r2;
+      }
+    }
+  };
   if (!cands.empty()) {
     double best = 1e300;
     for (size_t i = 0; i < cands.size(); i++) {
-      hipGraphExec_t ex = nullptr;
-      int nodes = 0;
-      rc = build_batch_graph(c, frames, fbs, precision, cands[i].first, cands[i].second, s0, &ex, &nodes);
-      if (rc != MBR_OK) return rc;
       double ms = 0;
-      HIPCHECK(c, hipGraphLaunch(ex, s0));  // warm-up
-      rc = time_graph(c, ex, s0, 1, &ms);
-      (void)hipGraphExecDestroy(ex);
+      rc = run_shape(cands[i].first, cands[i].second, 1, true, &ms);
       if (rc != MBR_OK) return rc;
       if (i < 16) res.tuned_ms[i] = ms;
       if (ms < best) {
@@ -1035,28 +1075,8 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
     c->tuned_bpc = bpc;
   }
   const int replays = opts && opts->replays > 0 ? opts->replays : 1;
-  unsigned long long v[ST_COUNT];
-  for (int attempt = 0;; attempt++) {
-    const auto t0 = std::chrono::steady_clock::now();
-    hipGraphExec_t ex = nullptr;
-    int nodes = 0;
-    rc = build_batch_graph(c, frames, fbs, precision, slots, bpc, s0, &ex, &nodes);
-    if (rc != MBR_OK) return rc;
-    res.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    res.graph_nodes = nodes;
-    double ms = 0;
-    rc = time_graph(c, ex, s0, replays, &ms);
-    (void)hipGraphExecDestroy(ex);
-    if (rc != MBR_OK) return rc;
-    res.ms_per_launch = ms;
-    HIPCHECK(c, hipMemcpy(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost));
-    if (v[ST_POOL_MISS] == 0) break;
-    if (attempt > 0) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
-    for (int k = 0; k < slots; k++) {
-      rc = alloc_pool(c, c->slots[k], (long long)(v[ST_OVF_RAYS] + v[ST_OVF_RAYS] / 4 + 64));
-      if (rc != MBR_OK) return rc;
-    }
-  }
+  rc = run_shape(slots, bpc, replays, false, &res.ms_per_launch);
+  if (rc != MBR_OK) return rc;
   res.slots = slots;
   res.blocks_per_cu = bpc;
   if (result) *result = res;

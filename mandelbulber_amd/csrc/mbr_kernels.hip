@@ -47,18 +47,20 @@ __device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3
 }
 
 // Step record writer of one ray: records below the slab size go to the
-// step-major slab, later ones to the ray's run in the overflow pool, which
-// the ray takes (one atomic) on its first overflowing record.  A re-march of
-// the same ray reuses its run.  When the pool is full the ray's later records
-// are dropped and ST_POOL_MISS tells the host to grow the pool and render
-// the band again.
+// step-major slab, later ones to chunks of the overflow pool listed in the
+// ray's directory; the ray takes its directory (one atomic) on its first
+// overflowing record and each chunk on the chunk's first record.  A
+// re-march of the same ray reuses them.  When the pool is full the record
+// is dropped and ST_POOL_MISS tells the host to grow the pool and render
+// the band again.  All records of one ray are written by one lane.
 template <class RT>
 struct RecPlanes {
   RT *step, *dist, *thr;     // slab planes
   RT *ostep, *odist, *othr;  // overflow pool planes
   long long *ovf_off;
-  unsigned long long *runs, *stats;
-  long long cap, ovf_runs;
+  int *dir;
+  unsigned long long *dirs, *chunks, *stats;
+  long long cap, max_dirs, max_chunks;
   int slab, r;
   __device__ __forceinline__ void put(int i, double step_v, double dist_v, double thr_v) const {
     long long o;
@@ -67,16 +69,33 @@ struct RecPlanes {
       o = (long long)i * cap + r;
       st = step; di = dist; th = thr;
     } else {
-      long long base = ovf_off[r];
-      if (base == -1) {
-        const unsigned long long run = atomicAdd(runs, 1ull);
-        base = run < (unsigned long long)ovf_runs ? (long long)run * (kMaxSteps - slab) : -2;
-        ovf_off[r] = base;
+      long long dj = ovf_off[r];
+      if (dj == -1) {
+        const unsigned long long n = atomicAdd(dirs, 1ull);
         atomicAdd(&stats[ST_OVF_RAYS], 1ull);
-        if (base < 0) atomicAdd(&stats[ST_POOL_MISS], 1ull);
+        dj = n < (unsigned long long)max_dirs ? (long long)n : -2;
+        if (dj >= 0)
+          for (int k = 0; k < kDirSlots; k++) dir[dj * kDirSlots + k] = -1;
+        ovf_off[r] = dj;
       }
-      if (base < 0) return;
-      o = base + (i - slab);
+      if (dj < 0) {
+        atomicAdd(&stats[ST_POOL_MISS], 1ull);
+        return;
+      }
+      const int rel = i - slab;
+      int *slot = &dir[dj * kDirSlots + rel / kChunk];
+      int ch = *slot;
+      if (ch == -1) {
+        const unsigned long long n = atomicAdd(chunks, 1ull);
+        atomicAdd(&stats[ST_OVF_CHUNKS], 1ull);
+        ch = n < (unsigned long long)max_chunks ? (int)n : -2;
+        *slot = ch;
+      }
+      if (ch < 0) {
+        atomicAdd(&stats[ST_POOL_MISS], 1ull);
+        return;
+      }
+      o = (long long)ch * kChunk + rel % kChunk;
       st = ostep; di = odist; th = othr;
     }
     st[o] = (RT)step_v;
@@ -301,10 +320,13 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
   rec.odist = (RT *)b.ovf_dist;
   rec.othr = (RT *)b.ovf_thr;
   rec.ovf_off = b.ovf_off;
-  rec.runs = &b.counters[CNT_RUNS];
+  rec.dir = b.ovf_dir;
+  rec.dirs = &b.counters[CNT_DIRS];
+  rec.chunks = &b.counters[CNT_CHUNKS];
   rec.stats = stats;
   rec.cap = b.cap;
-  rec.ovf_runs = b.ovf_runs;
+  rec.max_dirs = b.ovf_dirs;
+  rec.max_chunks = b.ovf_chunks;
   rec.slab = b.slab;
   rec.r = r;
   return rec;
@@ -830,16 +852,18 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
       s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
     });
   } else {
-    // the host renders a band again when a ray found the pool full, so every
-    // ray that reaches this point holds a run
+    // records from the slab on sit in the chunks the ray's directory lists
+    // (the host renders a band again when a record found the pool full)
     const RT *ost = (const RT *)b.ovf_step, *odi = (const RT *)b.ovf_dist, *oth = (const RT *)b.ovf_thr;
-    const long long base = b.ovf_off[r];
+    const long long dj = b.ovf_off[r];
+    const int *dir = b.ovf_dir + (dj >= 0 ? dj : 0) * kDirSlots;
     v = volumetric<T>(sc, depth, n, px, alpha, [&](int i, T &s, T &d, T &t) {
+      const int ch = (i >= slab && dj >= 0) ? dir[(i - slab) / kChunk] : -1;
       if (i < slab) {
         const long long o = (long long)i * cap + r;
         s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
-      } else if (base >= 0) {
-        const long long o = base + (i - slab);
+      } else if (ch >= 0) {
+        const long long o = (long long)ch * kChunk + (i - slab) % kChunk;
         s = (T)ost[o]; d = (T)odi[o]; t = (T)oth[o];
       } else {
         s = 0; d = 0; t = 0;

@@ -12,25 +12,32 @@ namespace mbr {
 enum StatIdx {
   ST_MARCH_DE = 0, ST_SHADE_DE, ST_MARCH_ITERS, ST_SHADE_ITERS, ST_HITS, ST_MISSED, ST_REDO,
   ST_OVF_RAYS,   // rays whose step record outgrew the slab
-  ST_POOL_MISS,  // of those, rays that found the overflow pool full
+  ST_POOL_MISS,  // records dropped because the overflow pool was full
+  ST_OVF_CHUNKS, // overflow chunks taken
   ST_AO_DE, ST_AO_ITERS, ST_AO_STEPS,
-  ST_COUNT = 13
+  ST_COUNT = 14
 };
 
 // Band counters (BandBufs::counters), zeroed on the device at band start:
 // hits appended, rays handed to the FP64 continuation, primary-ray queue,
 // overflow runs handed out, AO work queue,
 // rays listed for the FP64 redo.
-enum CounterIdx { CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_RUNS = 3, CNT_AOQ = 4, CNT_REDO = 5, CNT_COUNT = 8 };
+enum CounterIdx {
+  CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_DIRS = 3, CNT_AOQ = 4, CNT_REDO = 5, CNT_CHUNKS = 6, CNT_COUNT = 8
+};
 
 // Maximum step records of one ray (the loop bound of src/Render3D.cpp:95).
 constexpr int kMaxSteps = 10000;
+// Overflow records come in chunks of kChunk; a ray's directory lists the
+// chunks of its records slab, slab + kChunk, ... (at most kDirSlots).
+constexpr int kChunk = 256;
+constexpr int kDirSlots = 40;  // >= (kMaxSteps - slab) / kChunk for any slab >= 0
 
 // Per-band device buffers.  A band is a run of consecutive pixels of the
 // image (row-major), at most `cap` rays.  Step records are step-major
 // structure-of-arrays: record i < slab of ray r lives at [i * cap + r];
-// records from `slab` on go to a run of kMaxSteps - slab records of the
-// overflow pool, taken by the ray when it first needs one.
+// records from `slab` on go to chunks of kChunk records of the overflow
+// pool, listed in a per-ray directory, both taken when first needed.
 struct BandBufs {
   int cap;          // rays per band
   int slab;         // step records kept per ray in the slab
@@ -40,16 +47,18 @@ struct BandBufs {
   int *count;       // step record length (buffCount)
   double *depth, *thr, *last, *px, *py, *pz;
   void *rec_step, *rec_dist, *rec_thr;   // slab planes [slab][cap]
-  long long *ovf_off;                    // per ray: first pool record of its run, -1 none, -2 pool full
+  long long *ovf_off;                    // per ray: its directory, -1 none, -2 pool full
   // compaction lists: hits, rays to re-march in FP64
   int *hit_list, *redo_list;
   unsigned long long *counters;          // CNT_* below
   // rays handed from the march to its FP64 continuation
   int *hand_ray, *hand_i;
   double *hand_scan, *hand_step, *hand_drift, *hand_dist;
-  // overflow pool: ovf_runs runs of kMaxSteps - slab records
+  // overflow pool: ovf_dirs directories of kDirSlots chunk indices,
+  // ovf_chunks chunks of kChunk records
+  int *ovf_dir;
   void *ovf_step, *ovf_dist, *ovf_thr;
-  long long ovf_runs;
+  long long ovf_dirs, ovf_chunks;
   // shade output per ray
   void *sh_r, *sh_g, *sh_b;
   uint8_t *col_r, *col_g, *col_b;
