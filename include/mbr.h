@@ -220,6 +220,12 @@ typedef struct mbr_scene {
   /* settings of the render core that have no .fract key */
   int jitter_mode;                 /* enum mbr_jitter */
   uint32_t jitter_seed;
+  int hit_refine;                  /* 1: after the binary search, Newton steps move
+                                      the hit to the middle of the refinement band
+                                      (distance = (1 - 0.005 quality) x threshold),
+                                      so depth is a smooth function of the scene
+                                      parameters (the differentiable renders);
+                                      0: the reference's binary-search result */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
@@ -318,6 +324,19 @@ int mbr_render(mbr_ctx *ctx, const mbr_render_opts *opts,
  * enqueued on the stream and complete when the call returns. */
 int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
                       mbr_framebuffer *fb, mbr_stats *stats);
+
+/* Gradient of a pixel loss (no counterpart in the reference, which renders
+ * only forward): given the image adjoint dL/dC as three DEVICE float planes
+ * of the scene's size (dL/dR, dL/dG, dL/dB of imageFloat), writes dL/dtheta
+ * to grad[9] (host) for theta = (power, vp[0], vp[1], vp[2], alpha, beta,
+ * gamma, main_light_alpha, main_light_beta), angles in radians.  The image
+ * is the FP64 render of the current scene, which must be a trig_optim
+ * Mandelbulb with hit_refine = 1 and no shadows, AO, glow, fog, post fog,
+ * primitives, limits, interior or constant-threshold mode (their
+ * contributions are steps in theta); the surface colour index counts as a
+ * constant.  Rays are marched again (FP64) inside the call. */
+int mbr_adjoint(mbr_ctx *ctx, const mbr_render_opts *opts,
+                const float *const grad_rgb[3], double grad[9], mbr_stats *stats);
 
 /* Batched rendering of many frames (animation frames, or unrelated scenes of
  * mixed formulas and sizes) as ONE launch of a HIP graph: the frames' band

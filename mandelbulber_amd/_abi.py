@@ -86,7 +86,7 @@ class Scene(C.Structure):
         ("volumetric_light_enabled", C.c_int * 5), ("iter_fog_enabled", C.c_int),
         ("fake_lights_enabled", C.c_int), ("raytraced_reflections", C.c_int),
         ("reflections_max", C.c_int),
-        ("jitter_mode", C.c_int), ("jitter_seed", C.c_uint32),
+        ("jitter_mode", C.c_int), ("jitter_seed", C.c_uint32), ("hit_refine", C.c_int),
     ]
 
 

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmbr.so")
 EXPORTS = (
     "mbr_version", "mbr_scene_defaults", "mbr_load_settings", "mbr_set_setting", "mbr_init",
     "mbr_destroy", "mbr_last_error", "mbr_set_scene", "mbr_render", "mbr_render_device",
-    "mbr_scene_layout", "mbr_jitter_draw", "mbr_render_batch",
+    "mbr_scene_layout", "mbr_jitter_draw", "mbr_render_batch", "mbr_adjoint",
 )
 
 _lib = None
@@ -45,6 +45,7 @@ def _declare(lib):
     lib.mbr_scene_layout.restype = C.c_size_t
     lib.mbr_jitter_draw.argtypes = [C.c_uint32] * 4
     lib.mbr_jitter_draw.restype = C.c_uint32
+    lib.mbr_adjoint.argtypes = [C.c_void_p, P(_abi.RenderOpts), C.c_void_p * 3, C.c_double * 9, P(_abi.Stats)]
     lib.mbr_render_batch.argtypes = [C.c_void_p, P(_abi.Scene), C.c_int, P(_abi.Texture), P(_abi.Framebuffer),
                                      P(_abi.BatchOpts), P(_abi.Stats), P(_abi.BatchResult)]
 

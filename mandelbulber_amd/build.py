@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmbr.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_settings.cpp"]
+SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_settings.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

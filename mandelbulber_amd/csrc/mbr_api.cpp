@@ -281,6 +281,7 @@ struct mbr_ctx {
   int *d_batch_ao_rgb = nullptr;
   size_t batch_ao_cap = 0;
   hipStream_t side[kMaxSlots] = {};
+  double *d_grad = nullptr;  // mbr_adjoint accumulator
   int tuned_slots = 0, tuned_bpc = 0;
 
   int fail(int rc, const std::string &msg) {
@@ -498,7 +499,7 @@ void mbr_destroy(mbr_ctx *c) {
   for (Slot &sl : c->slots) free_band(sl);
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
-  void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb};
+  void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
@@ -554,6 +555,7 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   d.frame_no = s.frame_no;
   d.jitter_mode = s.jitter_mode;
   d.jitter_seed = s.jitter_seed;
+  d.hit_refine = s.hit_refine;
   d.pixel_base = (uint32_t)((long long)s.tile_index * s.image_width * s.image_height);
   d.aspect = (double)s.image_width / s.image_height;
   d.fov = s.persp;
@@ -1086,6 +1088,68 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
     fill_stats(stats, v, rays, n);
     stats->ms_total = res.ms_per_launch;
   }
+  return MBR_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad_rgb[3], double grad[9],
+                mbr_stats *stats) {
+  if (!c || !grad_rgb || !grad || !grad_rgb[0] || !grad_rgb[1] || !grad_rgb[2]) return MBR_ERR_INVALID_ARG;
+  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
+  const mbr_scene &s = c->scene;
+  const DevScene &h = c->host_sc;
+  if (s.formula != MBR_FORMULA_TRIG_OPTIM) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: formula must be 2");
+  if (!s.hit_refine) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: the scene needs hit_refine = 1");
+  if (h.shadow || h.ao || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
+      s.perspective_type != MBR_PERSP_THREE_POINT || s.tiles != 1 || s.constant_DE_threshold || s.interior_mode ||
+      s.limits_enabled || s.iter_thresh || h.fd.prim_plane || h.fd.prim_box || h.fd.prim_ibox ||
+      h.fd.prim_sphere || h.fd.prim_isphere || h.fd.prim_water || h.fd.tglad_fold || h.fd.sph_fold)
+    return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: shadows, AO, glow, fog, post fog, fish-eye, tiles, "
+                                        "constant threshold, interior mode, limits, iteration threshold, "
+                                        "primitives and pre-folds have no gradient here");
+  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  HIPCHECK(c, hipSetDevice(c->device));
+  Slot &sl = c->slots[0];
+  int rc = alloc_slot(c, sl, MBR_PREC_FP64, kBandRays, 0);
+  if (rc != MBR_OK) return rc;
+  if (!c->d_grad) HIPCHECK(c, hipMalloc((void **)&c->d_grad, 16 * sizeof(double)));
+  AdjParams p;
+  p.power = s.power;
+  for (int i = 0; i < 3; i++) p.vp[i] = s.vp[i];
+  p.alpha = s.alpha;
+  p.beta = s.beta;
+  p.gamma = s.gamma;
+  p.light_alpha = s.main_light_alpha;
+  p.light_beta = s.main_light_beta;
+  p.zoom_persp = s.zoom / s.persp;
+  const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
+  const long long last = (long long)s.image_width * s.image_height;
+  unsigned long long v[ST_COUNT];
+  for (int attempt = 0;; attempt++) {
+    HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, st));
+    HIPCHECK(c, launch_zero((unsigned long long *)c->d_grad, 16, st));
+    for (long long band_first = 0; band_first < last; band_first += sl.band.cap) {
+      const int band_rays = (int)std::min<long long>(sl.band.cap, last - band_first);
+      HIPCHECK(c, launch_band_reset(sl.band, st));
+      HIPCHECK(c, launch_march(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, band_rays, s.formula, persist,
+                               c->d_stats, st));
+      HIPCHECK(c, launch_march_fix(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, s.formula, persist / 2,
+                                   c->d_stats, st));
+      HIPCHECK(c, launch_adjoint(c->d_sc, sl.band, (int)band_first, band_rays, p, grad_rgb[0], grad_rgb[1],
+                                 grad_rgb[2], c->d_grad, persist, st));
+    }
+    HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    if (v[ST_POOL_MISS] == 0) break;
+    if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
+    rc = grow_pool(c, sl, v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
+    if (rc != MBR_OK) return rc;
+  }
+  HIPCHECK(c, hipMemcpy(grad, c->d_grad, 9 * sizeof(double), hipMemcpyDeviceToHost));
+  if (stats) fill_stats(stats, v, last, (int)((last + sl.band.cap - 1) / sl.band.cap));
   return MBR_OK;
 }
 

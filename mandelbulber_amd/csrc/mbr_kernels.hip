@@ -19,32 +19,6 @@ namespace mbr {
 constexpr int kBlock = 256;
 constexpr double kPi = 3.14159265358979323846;
 
-// per-pixel view vector (src/Render3D.cpp:389-438); true when the fish-eye
-// cut removes the pixel
-__device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3<double> &out) {
-  const double tiles = sc.tiles;
-  const double x2 = ((double)x / sc.width / tiles - 0.5 + sc.tile_x / tiles) * sc.aspect;
-  const double z2 = ((double)y / sc.height / tiles - 0.5 + sc.tile_y / tiles);
-  const double fov = sc.fov;
-  bool cut = false;
-  V3<double> v;
-  if (sc.persp_type == 1) {
-    if (sc.fish_eye_cut && sqrt(x2 * x2 + z2 * z2) > 0.5 / fov) cut = true;
-    const double x3 = x2 * kPi, z3 = z2 * kPi;
-    const double r = sqrt(x3 * x3 + z3 * z3);
-    if (r == 0) v = v3<double>(0.0, 1.0, 0.0);
-    else v = v3<double>(x3 / r * sin(r * fov), cos(r * fov), z3 / r * sin(r * fov));
-    v = normalize(v);
-  } else if (sc.persp_type == 2) {
-    const double x3 = x2 * kPi, z3 = z2 * kPi;
-    v = v3<double>(sin(fov * x3) * cos(fov * z3), cos(fov * x3) * cos(fov * z3), sin(fov * z3));
-    v = normalize(v);
-  } else {
-    v = v3<double>(x2 * fov, 1.0, z2 * fov);
-  }
-  out = sc.R_identity ? v : mat_apply(sc.R, v);
-  return cut;
-}
 
 // Step record writer of one ray: records below the slab size go to the
 // step-major slab, later ones to chunks of the overflow pool listed in the
@@ -268,6 +242,33 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       const double g = fabs(dist - prev_dist) / fmax(fabs(scan - prev_scan), 1e-300);
       const double W = (0.01 * k.quality * distThresh + 2.0 * margin) / fmax(g, 1e-12);
       ambiguous = W > kRedoTol * scan;
+    }
+    if (sc.hit_refine && !sc.constant_thresh) {
+      // Newton steps on F(t) = dist(t) - c0 thr(t), c0 the middle of the
+      // band, with a central-difference slope; each step at most one
+      // threshold long
+      const double c0 = 1.0 - 0.005 * k.quality;
+      const double kthr = k.resolution * k.persp / k.quality;
+      double t = scan, f = dist - c0 * kthr * t;
+      for (int it = 0; it < 8 && fabs(f) > 1e-13 * kthr * t; it++) {
+        const double eps = 1e-3 * kthr * t;
+        DEInfo bi;
+        const double dp = calc_distance_f<double, F>(sc.fd, start + dir * (t + eps), detail, sc.N, sc.minN, false, bi);
+        const double dm = calc_distance_f<double, F>(sc.fd, start + dir * (t - eps), detail, sc.N, sc.minN, false, bi);
+        const double g = (dp - dm) / (2.0 * eps) - c0 * kthr;
+        if (!(fabs(g) > 0.0)) break;
+        double dt = -f / g;
+        const double lim = kthr * t;
+        dt = dt > lim ? lim : (dt < -lim ? -lim : dt);
+        t += dt;
+        dist = calc_distance_f<double, F>(sc.fd, start + dir * t, detail, sc.N, sc.minN, false, bi);
+        cnt.add(bi);
+        object = bi.object;
+        f = dist - c0 * kthr * t;
+      }
+      scan = t;
+      point = start + dir * scan;
+      distThresh = scan * kthr;
     }
   }
   o.point = point;

@@ -102,6 +102,16 @@ struct ShadeBufs {
 hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, int band_first, AODev ao,
                         ShadeBufs sbuf, int persist_blocks, unsigned long long *stats, hipStream_t s,
                         hipEvent_t ev_ao0, hipEvent_t ev_ao1);
+// Scene parameters the adjoint differentiates (radians), and zoom/persp.
+struct AdjParams {
+  double power, vp[3], alpha, beta, gamma, light_alpha, light_beta;
+  double zoom_persp;  // the ray start sits zoom/persp behind vp
+};
+// dL/dtheta of the band's pixels added into grad[9] (device doubles); g* are
+// the image adjoint planes dL/dC (full image, row-major).
+hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
+                          const float *gR, const float *gG, const float *gB, double *grad, int blocks,
+                          hipStream_t s);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);
 

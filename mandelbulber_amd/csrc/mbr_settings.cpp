@@ -212,6 +212,7 @@ std::vector<KeyDef> build_table() {
   D("bailout", OFF(bailout));
   I("jitter_mode", OFF(jitter_mode));
   t.push_back({"jitter_seed", KT_UINT, OFF(jitter_seed), 1.0});
+  I("hit_refine", OFF(hit_refine));
   return t;
 }
 

@@ -371,4 +371,31 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
   return o;
 }
 
+// per-pixel view vector (src/Render3D.cpp:389-438); true when the fish-eye
+// cut removes the pixel
+__device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3<double> &out) {
+  const double tiles = sc.tiles;
+  const double x2 = ((double)x / sc.width / tiles - 0.5 + sc.tile_x / tiles) * sc.aspect;
+  const double z2 = ((double)y / sc.height / tiles - 0.5 + sc.tile_y / tiles);
+  const double fov = sc.fov;
+  bool cut = false;
+  V3<double> v;
+  if (sc.persp_type == 1) {
+    if (sc.fish_eye_cut && sqrt(x2 * x2 + z2 * z2) > 0.5 / fov) cut = true;
+    const double x3 = x2 * 3.14159265358979323846, z3 = z2 * 3.14159265358979323846;
+    const double r = sqrt(x3 * x3 + z3 * z3);
+    if (r == 0) v = v3<double>(0.0, 1.0, 0.0);
+    else v = v3<double>(x3 / r * sin(r * fov), cos(r * fov), z3 / r * sin(r * fov));
+    v = normalize(v);
+  } else if (sc.persp_type == 2) {
+    const double x3 = x2 * 3.14159265358979323846, z3 = z2 * 3.14159265358979323846;
+    v = v3<double>(sin(fov * x3) * cos(fov * z3), cos(fov * x3) * cos(fov * z3), sin(fov * z3));
+    v = normalize(v);
+  } else {
+    v = v3<double>(x2 * fov, 1.0, z2 * fov);
+  }
+  out = sc.R_identity ? v : mat_apply(sc.R, v);
+  return cut;
+}
+
 }  // namespace mbr

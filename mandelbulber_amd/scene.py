@@ -186,3 +186,41 @@ def flight(n=64, seed=0, res_div=1):
                   "main_light_alfa": a, "main_light_beta": b, "frame_no": k})
         out.append(s)
     return out
+
+
+DIFF_PARAMS = ("power", "vp_x", "vp_y", "vp_z", "alpha", "beta", "gamma", "light_alpha", "light_beta")
+
+
+def get_params(scene):
+    """The differentiable parameters of a scene (radians for the angles)."""
+    return [scene.power, scene.vp[0], scene.vp[1], scene.vp[2], scene.alpha, scene.beta, scene.gamma,
+            scene.main_light_alpha, scene.main_light_beta]
+
+
+def set_params(scene, theta):
+    """Write the nine differentiable parameters into a scene (in place)."""
+    t = [float(v) for v in theta]
+    scene.power = t[0]
+    scene.vp[0], scene.vp[1], scene.vp[2] = t[1], t[2], t[3]
+    scene.alpha, scene.beta, scene.gamma = t[4], t[5], t[6]
+    scene.main_light_alpha, scene.main_light_beta = t[7], t[8]
+    return scene
+
+
+def differentiable(width=512, height=512, seed=0):
+    """A Mandelbulb scene for gradient work: power uniform in [6,10], the
+    camera on a seeded direction at distance uniform in [2.5,4] looking at
+    the origin with a seeded roll, a seeded light direction; Lambert and
+    specular shading only (no shadow, AO, glow or fog, whose contributions
+    are steps in the parameters), surface colouring off (its palette index
+    is an integer), no step jitter, refined hits (hit_refine)."""
+    rng = np.random.default_rng(seed)
+    s = mandelbulb_plain(width, height)
+    apply(s, {"power": float(rng.uniform(6.0, 10.0)), "glow_intensity": 0, "fractal_color": 0,
+              "jitter_mode": 0, "hit_refine": 1})
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    look_at_origin(s, u * rng.uniform(2.5, 4.0), roll=float(rng.uniform(-0.5, 0.5)))
+    a, b = light_from_seed(int(rng.integers(1 << 30)))
+    apply(s, {"main_light_alfa": a, "main_light_beta": b})
+    return s
