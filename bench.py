@@ -141,6 +141,84 @@ def bench_flight(torch, r, _abi, scene, n_frames, replays):
     }
 
 
+def bench_adjoint(torch, r, scene, n_scenes, size):
+    """Differentiable rendering: n Mandelbulb scenes (power U[6,10], seeded
+    camera and light) at size x size; MSE against the image at power 7.3;
+    dL/d(power, camera, light) from the adjoint kernel through the
+    torch.autograd.Function, timed, and checked against central
+    differences (h = 1e-4) of the FP64 forward on every scene."""
+    from mandelbulber_amd import diff
+    fwd_ms, bwd_ms = [], []
+    steps = (1e-4, 1e-5)
+    errs = {h: {"rel": [], "abs": [], "left": []} for h in steps}
+    for i in range(n_scenes):
+        base = scene.differentiable(size, size, seed=100 + i)
+        theta = torch.tensor(scene.get_params(base), dtype=torch.float64, requires_grad=True)
+        tt = theta.detach().clone()
+        tt[0] = 7.3
+        with torch.no_grad():
+            target = diff.render(tt, r, base).double()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = diff.render(theta, r, base)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = ((img.double() - target) ** 2).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd_ms.append((t1 - t0) * 1e3)
+        bwd_ms.append((t2 - t1) * 1e3)
+        for h in steps:
+            adj, fd, out, region = diff.check_gradient(r, base, h=h)
+            scale = max(abs(fd).max(), 1e-300)
+            err = abs(adj - fd)
+            big = abs(fd) > 0.05 * scale
+            errs[h]["abs"].append(float((err / scale).max()))
+            errs[h]["rel"].append(float((err[big] / abs(fd[big])).max()) if big.any() else 0.0)
+            errs[h]["left"].append(out / max(region, 1))
+    import statistics
+    fd_check = {f"h={h:g}": {"median_rel_err_large_components": statistics.median(errs[h]["rel"]),
+                              "max_rel_err_large_components": max(errs[h]["rel"]),
+                              "median_err_over_largest_component": statistics.median(errs[h]["abs"]),
+                              "max_err_over_largest_component": max(errs[h]["abs"]),
+                              "mean_hit_pixels_left_out": sum(errs[h]["left"]) / n_scenes}
+                for h in steps}
+    rays = size * size
+    return {
+        "workload": f"{n_scenes} Mandelbulb scenes {size}x{size}, power U[6,10], seeded camera and light, "
+                    f"MSE vs power 7.3; gradients in power, vp (3), alfa/beta/gamma, light alfa/beta",
+        "forward_ms_per_scene": sum(fwd_ms) / n_scenes, "backward_ms_per_scene": sum(bwd_ms) / n_scenes,
+        "backward_Mrays_per_s": rays / (sum(bwd_ms) / n_scenes * 1e-3) / 1e6,
+        "fd_check": fd_check,
+    }
+
+
+def bench_deep_zoom(torch, r, _abi, scene, compare, width, height):
+    """Deep-zoom Mandelbox (frame width 1e-6 at a seeded surface point),
+    FP32_FIXUP against the FP64 frame: time and the error of FP32 shading
+    (whose distance estimates start in FP64 at the hit point)."""
+    sc = scene.deep_zoom(width, height, seed=1)
+    r.set_scene(sc)
+    planes = planes_on_gpu(torch, width, height)
+    out = {"workload": f"Mandelbox scale 2 (fold limit 1, min radius 0.5, fixed radius 1), {width}x{height}, "
+                       f"frame width 1e-6 at a seeded surface point, hard shadow", "rays": width * height}
+    frames = {}
+    for prec, name in ((_abi.PREC_FP32_FIXUP, "fp32_fixup"), (_abi.PREC_FP64, "fp64")):
+        r.render_device(planes, precision=prec)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = r.render_device(planes, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        frames[name] = to_frame(torch, planes)
+        out[name] = {"value": width * height / dt / 1e6, "unit": "Mrays/s", "ms": dt * 1e3, "hits": st.hits}
+    d = compare(frames["fp32_fixup"], frames["fp64"])
+    out["fp32_vs_fp64"] = {k: d[k] for k in ("flip_frac", "depth_rel_max", "psnr_db", "rgb_max_abs")}
+    out["speedup_vs_fp64"] = out["fp32_fixup"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +233,10 @@ def main():
     ap.add_argument("--no-flight", action="store_true")
     ap.add_argument("--flight-frames", type=int, default=64)
     ap.add_argument("--flight-replays", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-deep-zoom", action="store_true")
+    ap.add_argument("--adjoint-scenes", type=int, default=16)
+    ap.add_argument("--adjoint-size", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this render core runs on one GPU; multi-GPU runs are not part of this build "
@@ -205,6 +287,8 @@ def main():
     ao64_flops = sum(algorithmic_flops(sc, s, "ao") for s in st64) / launches64
 
     flight = None if args.no_flight else bench_flight(torch, r, _abi, scene, args.flight_frames, args.flight_replays)
+    adjoint = None if args.no_adjoint else bench_adjoint(torch, r, scene, args.adjoint_scenes, args.adjoint_size)
+    deep = None if args.no_deep_zoom else bench_deep_zoom(torch, r, _abi, scene, compare, 3840, 2160)
 
     cpu = None
     if not args.no_cpu:
@@ -241,6 +325,8 @@ def main():
                            "step_overflow_rays_fp64_path": st64[-1].step_overflow_rays},
         "roofline": roof,
         "flight": flight,
+        "adjoint": adjoint,
+        "deep_zoom": deep,
         "cpu_baseline": cpu,
     }
     r.close()

@@ -334,9 +334,16 @@ int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
  * Mandelbulb with hit_refine = 1 and no shadows, AO, glow, fog, post fog,
  * primitives, limits, interior or constant-threshold mode (their
  * contributions are steps in theta); the surface colour index counts as a
- * constant.  Rays are marched again (FP64) inside the call. */
+ * constant.  Hit pixels whose ray meets the surface within about 3 degrees
+ * of grazing (where the depth derivative diverges), or whose colour tangent
+ * exceeds 1e5 per unit parameter (a degenerate normal), contribute nothing; when
+ * `skipped` (a DEVICE uint8 plane of the image's size, may be NULL) is
+ * given, it is set to 1 on those pixels and 0 on the others the call
+ * visits (pixels whose adjoint is zero are not visited).  Rays are marched
+ * again (FP64) inside the call. */
 int mbr_adjoint(mbr_ctx *ctx, const mbr_render_opts *opts,
-                const float *const grad_rgb[3], double grad[9], mbr_stats *stats);
+                const float *const grad_rgb[3], double grad[9], uint8_t *skipped,
+                mbr_stats *stats);
 
 /* Batched rendering of many frames (animation frames, or unrelated scenes of
  * mixed formulas and sizes) as ONE launch of a HIP graph: the frames' band

@@ -45,7 +45,8 @@ def _declare(lib):
     lib.mbr_scene_layout.restype = C.c_size_t
     lib.mbr_jitter_draw.argtypes = [C.c_uint32] * 4
     lib.mbr_jitter_draw.restype = C.c_uint32
-    lib.mbr_adjoint.argtypes = [C.c_void_p, P(_abi.RenderOpts), C.c_void_p * 3, C.c_double * 9, P(_abi.Stats)]
+    lib.mbr_adjoint.argtypes = [C.c_void_p, P(_abi.RenderOpts), C.c_void_p * 3, C.c_double * 9, C.c_void_p,
+                                P(_abi.Stats)]
     lib.mbr_render_batch.argtypes = [C.c_void_p, P(_abi.Scene), C.c_int, P(_abi.Texture), P(_abi.Framebuffer),
                                      P(_abi.BatchOpts), P(_abi.Stats), P(_abi.BatchResult)]
 

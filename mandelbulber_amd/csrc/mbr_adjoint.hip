@@ -30,7 +30,8 @@
 // src/shaders.cpp:855-870) and the background is stored as 16-bit integers:
 // both are steps in theta whose derivative is zero almost everywhere; the
 // kernel uses zero for the colour and the slope of the gradient before the
-// 16-bit rounding for the background.
+// 16-bit rounding for the background.  Pixels whose depth or normal is
+// ill-conditioned (grazing hits, degenerate normals) are left out.
 #include "mbr_kernels.hpp"
 #include "mbr_shade.hpp"
 
@@ -38,6 +39,7 @@ namespace mbr {
 
 constexpr int kAdjBlock = 256;
 constexpr int kParams = 9;
+constexpr double kTangentMax = 1e5;
 
 // Value and gradient in (p.x, p.y, p.z, power).
 struct G4 {
@@ -176,7 +178,7 @@ template <class RT>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                        int band_rays, AdjParams P, const float *__restrict__ gR,
                                                        const float *__restrict__ gG, const float *__restrict__ gB,
-                                                       double *grad) {
+                                                       double *grad, uint8_t *skipped) {
   const DevScene &sc = *scp;
   const ShadeK<double> &k = sc.sd;
   const FractalK<double> &f = sc.fd;
@@ -223,6 +225,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
       dd[4 + a] = mv(dR[a], v0);
       dO[4 + a] = mv(dR[a], yz) * -1.0;
     }
+    if (skipped) skipped[p] = 0;
     dO[1] = v3<double>(1, 0, 0);
     dO[2] = v3<double>(0, 1, 0);
     dO[3] = v3<double>(0, 0, 1);
@@ -246,6 +249,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
       const G4 D = de_grad(f, ph, P.power, thr, sc.N, sc.minN);
       const V3<double> gD = v3<double>(D.g[0], D.g[1], D.g[2]);
       const double Ft = dot(gD, d) - c0 * kthr;
+      // The depth is well conditioned where the ray meets the level set
+      // D = c0 thr going down it: Ft < 0 with a margin.  Near a grazing
+      // tangency Ft -> 0 and dt/dtheta diverges: the refined hit then sits
+      // next to a fold of t(theta), where the forward jumps for a small
+      // change of theta; such pixels are left out (and flagged).
+      if (!(Ft < -0.05 * length(gD) * length(d))) {
+        if (skipped) skipped[p] = 1;
+        continue;
+      }
+      if (skipped) skipped[p] = 0;
       double dt[kParams];
       V3<double> dp[kParams];
       for (int i = 0; i < kParams; i++) {
@@ -275,6 +288,8 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
       // forward's colour, recomputed at the same point
       ShaderIn<double> in;
       in.point = ph;
+      in.point64 = ph;
+      in.detail = thr;
       in.object = b.object[r];
       LaneCount cnt = {0, 0};
       const RGBT<double> colour = surface_colour<double>(sc, in, cnt);
@@ -299,6 +314,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
         dC[2][i] = mlB * dshade * col[2] + mlB * dspec;
       }
     }
+    // A tangent beyond kTangentMax colour units per unit parameter comes from
+    // a degenerate normal (the four estimates nearly equal) or a fold of the
+    // hit: the pixel is ill-conditioned and left out (and flagged).
+    bool finite = true;
+    for (int i = 0; i < kParams; i++)
+      for (int ch = 0; ch < 3; ch++) finite = finite && fabs(dC[ch][i]) <= kTangentMax;
+    if (!finite) {
+      if (skipped) skipped[p] = 1;
+      continue;
+    }
     for (int i = 0; i < kParams; i++) acc[i] += aR * dC[0][i] + aG * dC[1][i] + aB * dC[2][i];
   }
   // block reduction, one atomic per parameter and workgroup
@@ -318,15 +343,15 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
 }
 
 hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
-                          const float *gR, const float *gG, const float *gB, double *grad, int blocks,
-                          hipStream_t s) {
+                          const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
+                          int blocks, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
   if (b.rec_bytes == 8)
     hipLaunchKernelGGL((k_adjoint<double>), dim3(std::max(1, blocks)), dim3(kAdjBlock), 0, s, sc, b, band_first,
-                       band_rays, p, gR, gG, gB, grad);
+                       band_rays, p, gR, gG, gB, grad, skipped);
   else
     hipLaunchKernelGGL((k_adjoint<float>), dim3(std::max(1, blocks)), dim3(kAdjBlock), 0, s, sc, b, band_first,
-                       band_rays, p, gR, gG, gB, grad);
+                       band_rays, p, gR, gG, gB, grad, skipped);
   return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 
 #include "mbr.h"
 #include "mbr_core.hpp"
+#include "mbr_fractal.hpp"
 #include "mbr_kernels.hpp"
 #include "mbr_scene_dev.hpp"
 
@@ -622,6 +623,18 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   fill_shade(d.sf, s, light, resolution);
 
   d.ao_count = ao_table(s.ao_quality, lightmap, p.dirs, p.rgb);
+  // Deep zoom: every hit lies at least the distance estimate at the ray
+  // start away, so its threshold is at least thr_min; when that is below
+  // what FP32 resolves at the hit's coordinates (|start| + a few units of
+  // march), FP32 shading must start its estimates in FP64.
+  {
+    const V3<double> st = v3<double>(d.start[0], d.start[1], d.start[2]);
+    DEInfo info;
+    const double de0 = calc_distance<double>(d.fd, st, 0.0, s.N, s.minN, false, info);
+    const double near = std::max(de0, s.view_distance_min);
+    const double thr_min = s.constant_DE_threshold ? s.quality : near * resolution * s.persp / s.quality;
+    d.deep = thr_min < 2.0 * kMixedSwitch * (length(st) + 3.0);
+  }
   return MBR_OK;
 }
 
@@ -682,6 +695,7 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
   sbuf.ao = sl.sh_ao;
   sbuf.n = b.cap;
   sbuf.slow_ao = host.ao && !host.fast_ao;
+  sbuf.deep = host.deep;
   sbuf.formula = formula;
   int band = 0;
   for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
@@ -1096,7 +1110,7 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
 extern "C" {
 
 int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad_rgb[3], double grad[9],
-                mbr_stats *stats) {
+                uint8_t *skipped, mbr_stats *stats) {
   if (!c || !grad_rgb || !grad || !grad_rgb[0] || !grad_rgb[1] || !grad_rgb[2]) return MBR_ERR_INVALID_ARG;
   if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
   const mbr_scene &s = c->scene;
@@ -1139,7 +1153,7 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
       HIPCHECK(c, launch_march_fix(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, s.formula, persist / 2,
                                    c->d_stats, st));
       HIPCHECK(c, launch_adjoint(c->d_sc, sl.band, (int)band_first, band_rays, p, grad_rgb[0], grad_rgb[1],
-                                 grad_rgb[2], c->d_grad, persist, st));
+                                 grad_rgb[2], c->d_grad, skipped, persist, st));
     }
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));

@@ -549,6 +549,68 @@ MBR_HD T calc_distance_f(const FractalK<T> &f, V3<T> point, T detailSize, int N,
   return dist;
 }
 
+// Distance estimate at an FP64 point for an FP32 caller (camera-relative
+// precision).  An FP32 iteration rounds z and c by about 6e-8 of their size;
+// once the orbit's derivative aux has grown so that such a rounding moves the
+// estimate by less than 1/170 of the threshold (aux * detail >= kMixedSwitch
+// * max(|z|, |c|)), the rest of the iteration runs in FP32.  At ordinary
+// zoom that holds at the start and the call is the FP32 one; at deep zoom
+// (detail far below FP32 resolution of the coordinates) the first
+// iterations run in FP64 from the exact point.
+constexpr double kMixedSwitch = 1e-5;
+
+template <int F>
+MBR_HD bool de_call_begin_p64(const FractalK<double> &fd, const FractalK<float> &ff, V3<double> p, float detail,
+                              int N, int minN, bool normalMode, DECall<float> &call, float &dist, DEInfo &info) {
+  const V3<float> pf = cvt<float>(p);
+  const float cmax = length(pf) * (ff.julia_mode ? 0.0f : (ff.constant_factor > 1.0f ? ff.constant_factor : 1.0f));
+  const float zc = ff.julia_mode ? length(ld3(ff.julia)) : cmax;
+  if ((double)detail >= kMixedSwitch * (double)(length(pf) > zc ? length(pf) : zc))
+    return de_call_begin<float, F>(ff, pf, detail, N, minN, normalMode, call, dist, info);
+  DECall<double> c64;
+  double d64 = 0.0;
+  if (de_call_begin<double, F>(fd, p, (double)detail, N, minN, normalMode, c64, d64, info)) {
+    dist = (float)d64;
+    return true;
+  }
+  const double cm = length(c64.it.c);
+  for (;;) {
+    const double zm = c64.it.r;
+    if (Math<double>::fabs(c64.it.aux) * (double)detail >= kMixedSwitch * (zm > cm ? zm : cm)) break;
+    if (de_call_step<double, F>(fd, c64, (double)detail, N, minN, normalMode, d64, info)) {
+      dist = (float)d64;
+      return true;
+    }
+  }
+  call.p = pf;
+  call.limitBox = (float)c64.limitBox;
+  call.it.z = cvt<float>(c64.it.z);
+  call.it.c = cvt<float>(c64.it.c);
+  call.it.r = (float)c64.it.r;
+  call.it.aux = (float)c64.it.aux;
+  call.it.L = c64.it.L;
+  const float r2 = call.it.z.x * call.it.z.x + call.it.z.y * call.it.z.y + call.it.z.z * call.it.z.z;
+  call.it.inv_r = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+  return false;
+}
+
+// Whole CalculateDistance at an FP64 point in the caller's precision T.
+template <class T, int F>
+MBR_HD T calc_distance_p64(const FractalK<double> &fd, const FractalK<T> &f, V3<double> point, T detailSize, int N,
+                           int minN, bool normalMode, DEInfo &info) {
+  if constexpr (sizeof(T) == 8) {
+    return calc_distance_f<double, F>(fd, point, detailSize, N, minN, normalMode, info);
+  } else {
+    DECall<float> call;
+    float dist = 0;
+    if (de_call_begin_p64<F>(fd, f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
+    while (!de_call_step<float, F>(f, call, detailSize, N, minN, normalMode, dist, info)) {
+    }
+    return dist;
+  }
+}
+
+
 template <class T>
 MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                        DEInfo &info) {

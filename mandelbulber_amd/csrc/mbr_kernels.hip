@@ -11,6 +11,8 @@
 //             (glow, fog, volumetric fog) over the step record read back to
 //             front, and coalesced writes of the framebuffer planes
 //             (src/Render3D.cpp:530-589).
+#include <type_traits>
+
 #include "mbr_kernels.hpp"
 #include "mbr_shade.hpp"
 
@@ -550,7 +552,8 @@ struct SurfBufs {
 template <class T>
 __device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, int r, ShaderIn<T> &in) {
   const ShadeK<T> &k = sc.shade<T>();
-  in.point = v3<T>((T)b.px[r], (T)b.py[r], (T)b.pz[r]);
+  in.point64 = v3<double>(b.px[r], b.py[r], b.pz[r]);
+  in.point = cvt<T>(in.point64);
   in.light = v3<T>(k.light[0], k.light[1], k.light[2]);
   in.dist_thresh = (T)b.thr[r];
   in.lastDist = (T)b.last[r];
@@ -560,7 +563,7 @@ __device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, 
   in.object = b.object[r];
 }
 
-template <class T, int F>
+template <class T, int F, bool Deep>
 __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                     SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -577,19 +580,19 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     ShaderIn<T> in;
     load_hit(sc, b, r, in);
     in.view = cvt<T>(dir);
-    in.normal = normals<T, F>(sc, in, cnt);
+    in.normal = normals<T, F, Deep>(sc, in, cnt);
     T sh = dot(in.normal, in.light);
     if (sh < 0) sh = 0;
     const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
     T shadow = 1;
-    if (sc.shadow) shadow = main_shadow<T, F>(sc, in, cnt);
+    if (sc.shadow) shadow = main_shadow<T, F, Deep>(sc, in, cnt);
     V3<T> half = normalize(in.light - in.view);
     T shade2 = dot(in.normal, half);
     if (shade2 < T(0)) shade2 = 0;
     shade2 = Math<T>::pow(shade2, T(30));
     if (shade2 > T(15)) shade2 = 15;
     const T specular = shade2 * k.specular;
-    RGBT<T> colour = surface_colour<T>(sc, in, cnt);
+    RGBT<T> colour = surface_colour<T, Deep>(sc, in, cnt);
     const long long n = sb.n;
     for (int c = 0; c < 3; c++) {
       sb.base[c * n + tid] = k.main_light[c] * shade * shadow;
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     sb.col[0 * n + tid] = colour.R;
     sb.col[1 * n + tid] = colour.G;
     sb.col[2 * n + tid] = colour.B;
-    if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T, F>(sc, in, cnt);
+    if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T, F, Deep>(sc, in, cnt);
     b.col_r[r] = x86_d2u8((double)colour.R * 255);
     b.col_g[r] = x86_d2u8((double)colour.G * 255);
     b.col_b[r] = x86_d2u8((double)colour.B * 255);
@@ -630,7 +633,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 // cost is shared by many lanes instead of being paid on almost every pass.
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
-template <class T, int F>
+template <class T, int F, bool Deep>
 __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
                                                SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -652,6 +655,8 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   long long h_cur = -1;  // hit whose data this lane holds
   unsigned long long item = 0;
   V3<T> P = v3<T>(0, 0, 0), dir = P;
+  V3<double> P64 = v3<double>(0, 0, 0);
+  bool deep = false;  // FP32 path: the hit's detail is below FP32 resolution of its coordinates
   T r = 0, start_dist = 0, end_dist = 1, detail = 0, thr_f = 0;
   T dist = 0;
   int mi = 0, guard = 0;
@@ -663,7 +668,17 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   auto begin_de = [&]() {
     DEInfo info;
     state = AO_ITER;
-    if (de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info)) {
+    bool done;
+    if constexpr (Deep && sizeof(T) < 8) {
+      if (deep)
+        done = de_call_begin_p64<F>(sc.fd, f, P64 + cvt<double>(dir) * (double)r, detail, N, minN, false, call, dist,
+                                    info);
+      else
+        done = de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+    } else {
+      done = de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+    }
+    if (done) {
       state = AO_DONE;
       mi = info.max_iter;
       cnt.de_calls += 1;
@@ -711,6 +726,8 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
             ShaderIn<T> in;
             load_hit(sc, b, b.hit_list[h], in);
             P = in.point;
+            P64 = in.point64;
+            if constexpr (Deep) deep = (double)in.detail < 2.0 * kMixedSwitch * length(P64);
             start_dist = in.delta;
             end_dist = in.delta / k.resolution;
             detail = in.detail;
@@ -964,23 +981,35 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     sb.fast_ao = (T *)sbuf.fast_ao;
     sb.ao = (T *)sbuf.ao;
     sb.n = sbuf.n;
-    if (formula == 7)
-      hipLaunchKernelGGL((k_surface<T, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
-    else if (formula == 8)
-      hipLaunchKernelGGL((k_surface<T, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
-    else
-      hipLaunchKernelGGL((k_surface<T, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    auto surf = [&](auto deep_tag) {
+      constexpr bool D = decltype(deep_tag)::value;
+      if (formula == 7)
+        hipLaunchKernelGGL((k_surface<T, 7, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_surface<T, 8, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+      else
+        hipLaunchKernelGGL((k_surface<T, 2, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+    };
+    auto aok = [&](auto deep_tag) {
+      constexpr bool D = decltype(deep_tag)::value;
+      if (formula == 7)
+        hipLaunchKernelGGL((k_ao<T, 7, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_ao<T, 8, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+      else
+        hipLaunchKernelGGL((k_ao<T, 2, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+    };
+    // deep-zoom variants only for FP32 shading of scenes that need them
+    const bool deep = sbuf.deep && sizeof(T) < 8;
+    if (deep) surf(std::true_type{});
+    else surf(std::false_type{});
     if (ev_ao0) {
       hipError_t e = hipEventRecord(ev_ao0, s);
       if (e != hipSuccess) return e;
     }
     if (sbuf.slow_ao) {
-      if (formula == 7)
-        hipLaunchKernelGGL((k_ao<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
-      else if (formula == 8)
-        hipLaunchKernelGGL((k_ao<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
-      else
-        hipLaunchKernelGGL((k_ao<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+      if (deep) aok(std::true_type{});
+      else aok(std::false_type{});
     }
     if (ev_ao1) {
       hipError_t e = hipEventRecord(ev_ao1, s);

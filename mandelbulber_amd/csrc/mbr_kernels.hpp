@@ -94,6 +94,7 @@ struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
   long long n;                  // hit capacity (plane stride)
   int slow_ao;                  // 1: run the AmbientOcclusion rays
+  int deep;                     // 1: FP32 shading starts distance estimates in FP64 where needed
   int formula;
 };
 // Shading in three launches: surface (normals, shadow, specular, colour,
@@ -110,8 +111,8 @@ struct AdjParams {
 // dL/dtheta of the band's pixels added into grad[9] (device doubles); g* are
 // the image adjoint planes dL/dC (full image, row-major).
 hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
-                          const float *gR, const float *gG, const float *gB, double *grad, int blocks,
-                          hipStream_t s);
+                          const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
+                          int blocks, hipStream_t s);
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);
 

@@ -55,6 +55,7 @@ struct DevScene {
   // image / camera (FP64: ray origins and scan stay FP64 in every path)
   int width, height, tiles, tile_index;
   int persp_type, fish_eye_cut, frame_no, jitter_mode, hit_refine;
+  int deep;  // the pixel footprint can fall below FP32 resolution of the coordinates
   uint32_t jitter_seed, pixel_base;  // pixel index offset of this tile
   double aspect, fov, tile_x, tile_y;
   double R[9];

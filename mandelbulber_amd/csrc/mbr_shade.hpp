@@ -35,6 +35,7 @@ struct LaneCount {
 template <class T>
 struct ShaderIn {
   V3<T> point, view, normal, light;
+  V3<double> point64;  // the hit point in FP64 (distance estimates start from it)
   T dist_thresh, lastDist, delta, depth;
   T detail;  // calcParam->doubles.detailSize as the reference mutates it
   int object;
@@ -57,7 +58,24 @@ MBR_HD T de(const DevScene &sc, V3<T> p, T detail, int N, int minN, bool normalM
   return d;
 }
 
-template <class T, int F = 0>
+// Distance estimate at the hit point plus an offset.  Deep (FP32 callers
+// of deep-zoom scenes): the point is formed in FP64 and the estimate starts
+// in FP64 (calc_distance_p64); otherwise the point and the estimate are in
+// the caller's precision.
+template <class T, int F, bool Deep>
+MBR_HD T de_at(const DevScene &sc, const ShaderIn<T> &in, V3<T> off, T detail, int N, int minN, bool normalMode,
+               DEInfo &info, LaneCount &cnt) {
+  if constexpr (Deep && sizeof(T) < 8 && F != 0) {
+    const T d = calc_distance_p64<T, F>(sc.fd, sc.frac<T>(), in.point64 + cvt<double>(off), detail, N, minN,
+                                        normalMode, info);
+    cnt.add(info);
+    return d;
+  } else {
+    return de<T, F>(sc, in.point + off, detail, N, minN, normalMode, info, cnt);
+  }
+}
+
+template <class T, int F = 0, bool Deep = false>
 MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   V3<T> n = v3<T>(0, 0, 0);
@@ -66,10 +84,10 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   if (!sc.slow_shading) {
     T delta = in.delta * k.smoothness;
     if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
-    T s1 = de<T, F>(sc, in.point, in.detail, N5, 0, true, info, cnt);
-    T s2 = de<T, F>(sc, in.point + v3<T>(delta, 0, 0), in.detail, N5, 0, true, info, cnt);
-    T s3 = de<T, F>(sc, in.point + v3<T>(0, delta, 0), in.detail, N5, 0, true, info, cnt);
-    T s4 = de<T, F>(sc, in.point + v3<T>(0, 0, delta), in.detail, N5, 0, true, info, cnt);
+    T s1 = de_at<T, F, Deep>(sc, in, v3<T>(0, 0, 0), in.detail, N5, 0, true, info, cnt);
+    T s2 = de_at<T, F, Deep>(sc, in, v3<T>(delta, 0, 0), in.detail, N5, 0, true, info, cnt);
+    T s3 = de_at<T, F, Deep>(sc, in, v3<T>(0, delta, 0), in.detail, N5, 0, true, info, cnt);
+    T s4 = de_at<T, F, Deep>(sc, in, v3<T>(0, 0, delta), in.detail, N5, 0, true, info, cnt);
     n = v3<T>(s2 - s1, s3 - s1, s4 - s1);
   } else {
     T delta = in.delta * k.smoothness;
@@ -79,7 +97,7 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
       for (double py = -1.0; py <= 1.0; py += 0.2)
         for (double pz = -1.0; pz <= 1.0; pz += 0.2) {
           V3<T> p2 = v3<T>((T)px, (T)py, (T)pz);
-          T d = de<T, F>(sc, in.point + p2 * delta, in.detail, N5, sc.minN, true, info, cnt);
+          T d = de_at<T, F, Deep>(sc, in, p2 * delta, in.detail, N5, sc.minN, true, info, cnt);
           n = n + p2 * d;
         }
   }
@@ -88,7 +106,7 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   return n;
 }
 
-template <class T, int F = 0>
+template <class T, int F = 0, bool Deep = false>
 MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   T factor = in.delta / k.resolution;
@@ -103,10 +121,10 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   DEInfo info;
   int guard = 0;
   for (T i = start; i < factor && guard < kMaxShadeSteps; i += dist * DE_factor, guard++) {
-    V3<T> point2 = in.point + in.light * i;
+
     const float dist_thresh = (float)in.dist_thresh;
     in.detail = (T)dist_thresh;
-    dist = de<T, F>(sc, point2, in.detail, sc.N, sc.minN, false, info, cnt);
+    dist = de_at<T, F, Deep>(sc, in, in.light * i, in.detail, sc.N, sc.minN, false, info, cnt);
     if (sc.bsoft) {
       T angle = (dist - (T)dist_thresh) / i;
       if (angle < 0) angle = 0;
@@ -126,7 +144,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   return sc.bsoft ? (T(1) - maxSoft) : shadowTemp;
 }
 
-template <class T, int F = 0>
+template <class T, int F = 0, bool Deep = false>
 MBR_HD T fast_ao2(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   T delta = in.dist_thresh;
@@ -137,7 +155,7 @@ MBR_HD T fast_ao2(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   for (int i = 1; i < q * q; i++) {
     inv2i *= T(0.5);  // 1/pow(2,i), exact
     T scan = T(i * i) * delta;
-    T dist = de<T, F>(sc, in.point + in.normal * scan, in.detail, sc.N, sc.minN, false, info, cnt);
+    T dist = de_at<T, F, Deep>(sc, in, in.normal * scan, in.detail, sc.N, sc.minN, false, info, cnt);
     aoTemp += inv2i * (scan - k.fast_ao_tune * dist) / in.dist_thresh;
   }
   T ao = T(1) - T(0.2) * aoTemp;
@@ -152,7 +170,7 @@ struct AOTable {
   int count;
 };
 
-template <class T, int F = 0>
+template <class T, int F = 0, bool Deep = false>
 MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
   RGBT<T> ao = {0, 0, 0};
@@ -166,7 +184,7 @@ MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTa
     T shadowTemp = 1;
     int guard = 0;
     for (T r = start_dist; r < end_dist && guard < kMaxShadeSteps; r += dist * T(2), guard++) {
-      dist = de<T, F>(sc, in.point + dir * r, in.detail, sc.N, sc.minN, false, info, cnt);
+      dist = de_at<T, F, Deep>(sc, in, dir * r, in.detail, sc.N, sc.minN, false, info, cnt);
       if (dist < (T)dist_thresh || info.max_iter || shadowTemp < T(0)) {
         shadowTemp -= (end_dist - r) / end_dist;
         if (shadowTemp < T(0)) shadowTemp = 0;
@@ -202,7 +220,7 @@ MBR_HD void index_to_colour(int index, const int (*palette)[3], int out[3]) {
   }
 }
 
-template <class T>
+template <class T, bool Deep = false>
 MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCount &cnt) {
   RGBT<T> out;
   const int *pc;
@@ -210,7 +228,12 @@ MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCou
     case OBJ_FRACTAL: {
       int L;
       const int N10 = sc.N * 10;
-      T cval = compute<T, MODE_COLOURING>(sc.frac<T>(), in.point, N10, &L);
+      T cval;
+      if (Deep && sizeof(T) < 8 && (double)in.detail < 2.0 * kMixedSwitch * length(in.point64))
+        // deep zoom: the FP32 point would merge neighbouring pixels' orbits
+        cval = (T)compute<double, MODE_COLOURING>(sc.fd, in.point64, N10, &L);
+      else
+        cval = compute<T, MODE_COLOURING>(sc.frac<T>(), in.point, N10, &L);
       cnt.de_calls += 1;
       cnt.iterations += (uint32_t)((L < N10) ? L + 1 : N10);
       int nrCol = x86_d2i(floor((double)cval));
@@ -239,31 +262,31 @@ MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCou
 
 // ObjectShader: returns object + specular terms; *colour_out is the surface
 // colour (colourBuffer source).
-template <class T, int F = 0>
+template <class T, int F = 0, bool Deep = false>
 MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<T> &tab, RGBT<T> &colour_out,
                              LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
-  in.normal = normals<T, F>(sc, in, cnt);
+  in.normal = normals<T, F, Deep>(sc, in, cnt);
   T sh = dot(in.normal, in.light);
   if (sh < 0) sh = 0;
   const T shade = k.direct_light * ((T(1) - k.shading) + k.shading * sh);
   T shadow = 1;
-  if (sc.shadow) shadow = main_shadow<T, F>(sc, in, cnt);
+  if (sc.shadow) shadow = main_shadow<T, F, Deep>(sc, in, cnt);
   V3<T> half = normalize(in.light - in.view);
   T shade2 = dot(in.normal, half);
   if (shade2 < T(0)) shade2 = 0;
   shade2 = Math<T>::pow(shade2, T(30));
   if (shade2 > T(15)) shade2 = 15;
   const T specular = shade2 * k.specular;
-  RGBT<T> colour = surface_colour<T>(sc, in, cnt);
+  RGBT<T> colour = surface_colour<T, Deep>(sc, in, cnt);
   colour_out = colour;
   RGBT<T> ambient = {0, 0, 0};
   if (sc.ao) {
     if (sc.fast_ao) {
-      T a = fast_ao2<T, F>(sc, in, cnt);
+      T a = fast_ao2<T, F, Deep>(sc, in, cnt);
       ambient = RGBT<T>{a, a, a};
     } else {
-      ambient = ambient_occlusion<T, F>(sc, in, tab, cnt);
+      ambient = ambient_occlusion<T, F, Deep>(sc, in, tab, cnt);
     }
   }
   RGBT<T> out;

@@ -115,16 +115,22 @@ class Renderer:
                                      C.byref(st), C.byref(res)), self._ctx)
         return st, res
 
-    def adjoint(self, grad_rgb, stream=None):
+    def adjoint(self, grad_rgb, stream=None, skipped=None):
         """dL/dtheta (9 floats: power, vp x/y/z, alpha, beta, gamma, light
         alpha, light beta) of the current scene for the image adjoint
-        grad_rgb: a contiguous (3, H, W) float32 device tensor dL/dC."""
+        grad_rgb: a contiguous (3, H, W) float32 device tensor dL/dC.
+        skipped: optional (H, W) uint8 device tensor, set to 1 on the
+        grazing hit pixels the adjoint leaves out."""
         s = self.scene
         assert tuple(grad_rgb.shape) == (3, s.image_height, s.image_width) and grad_rgb.is_contiguous()
         planes = (C.c_void_p * 3)(*[grad_rgb[c].data_ptr() for c in range(3)])
         out = (C.c_double * 9)()
         st = _abi.Stats()
-        check(lib().mbr_adjoint(self._ctx, C.byref(self._opts(_abi.PREC_FP64, None, stream)), planes, out,
+        sk = None
+        if skipped is not None:
+            assert tuple(skipped.shape) == (s.image_height, s.image_width) and skipped.is_contiguous()
+            sk = skipped.data_ptr()
+        check(lib().mbr_adjoint(self._ctx, C.byref(self._opts(_abi.PREC_FP64, None, stream)), planes, out, sk,
                                 C.byref(st)), self._ctx)
         return list(out), st
 

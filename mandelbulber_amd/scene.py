@@ -224,3 +224,58 @@ def differentiable(width=512, height=512, seed=0):
     a, b = light_from_seed(int(rng.integers(1 << 30)))
     apply(s, {"main_light_alfa": a, "main_light_beta": b})
     return s
+
+
+def _mandelbox_distance(p, n_iter=250):
+    """Distance estimate of the Mandelbox of MANDELBOX_KEYS (scale 2, fold
+    limit 1, fold value 2, min radius 0.5, fixed radius 1, escape 1024),
+    FP64, for placing a camera on its surface; the renders use the
+    library's kernels."""
+    c = np.array(p, dtype=np.float64)
+    z = c.copy()
+    dz = 1.0
+    for _ in range(n_iter):
+        z = np.where(z > 1.0, 2.0 - z, np.where(z < -1.0, -2.0 - z, z))
+        r2 = float(z @ z)
+        if r2 < 0.25:
+            z *= 4.0
+            dz *= 4.0
+        elif r2 < 1.0:
+            z *= 1.0 / r2
+            dz *= 1.0 / r2
+        z = z * 2.0 + c
+        dz = dz * 2.0 + 1.0
+        r = float(np.sqrt(z @ z))
+        if r > 1024.0:
+            return r / abs(dz)
+    return 0.0
+
+
+def deep_zoom(width=3840, height=2160, seed=0, frame_width=1e-6):
+    """Deep-zoom Mandelbox: the camera sits 10 zoom units in front of a
+    seeded surface point (reached by marching a seeded direction from
+    radius 8 toward the origin), looking along that direction with a seeded
+    roll; zoom = frame_width / aspect, so the frame is frame_width wide at
+    the view point.  Hard shadow, no AO, no fog."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    p = 8.0 * u
+    for _ in range(100000):
+        d = _mandelbox_distance(p)
+        if d < 1e-13:
+            break
+        p = p - u * d
+    zoom = frame_width * height / width
+    s = mandelbulb_plain(width, height)
+    apply(s, dict(MANDELBOX_KEYS, zoom=zoom))
+    fwd = -u
+    vp = p - fwd * (10.0 * zoom)
+    # orientation: forward along -u (look_at_origin from a point on the ray
+    # through the origin), plus a seeded roll
+    look_at_origin(s, u * np.linalg.norm(vp), roll=float(rng.uniform(-np.pi, np.pi)))
+    apply(s, {"view_point_x": float(vp[0]), "view_point_y": float(vp[1]), "view_point_z": float(vp[2])})
+    a, b = light_from_seed(int(rng.integers(1 << 30)))
+    apply(s, {"shadows_enabled": 1, "shadows_cone_angle": 0, "penetrating_lights": 0,
+              "main_light_alfa": a, "main_light_beta": b})
+    return s

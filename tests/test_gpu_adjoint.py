@@ -3,16 +3,10 @@ forward, and through torch.autograd.
 
 Loss: MSE of the image against a target rendered at power 7.3 with the same
 camera and light.  FD: central differences with h = 1e-4 of the same FP64
-forward (refined hits, no jitter).  The image is piecewise smooth in theta:
-it steps where a silhouette crosses a pixel centre and where the escape
-iteration count of one of the distance estimates behind a pixel changes (the
-normal and the refined depth jump there).  A central difference across such
-a step measures jump / 2h, not a derivative, so pixels whose hit/miss status
-differs between any of the 19 renders, or whose two one-sided differences
-disagree (a step inside [theta - h, theta + h]), are left out of both the
-loss and the adjoint.  Bar: every component within 1e-3 relative of the FD
-gradient where it is not small, and within 1e-3 of the largest component
-everywhere."""
+forward (refined hits, no jitter), with the pixels whose image steps inside
+[theta - h, theta + h] left out of both sides (diff.check_gradient).  Bar:
+every component within 1e-3 relative of the FD gradient where it is not
+small, and within 1e-3 of the largest component everywhere."""
 import numpy as np
 import pytest
 
@@ -33,51 +27,9 @@ def renderer():
     r.close()
 
 
-def _render(r, base, theta):
-    import torch
-    s = scene.set_params(scene.copy(base), theta)
-    r.set_scene(s)
-    dev = torch.device("cuda:0")
-    p = {"rgb": torch.empty((3, s.image_height, s.image_width), dtype=torch.float32, device=dev),
-         "z": torch.empty((s.image_height, s.image_width), dtype=torch.float32, device=dev)}
-    r.render_device(p, precision=_abi.PREC_FP64)
-    return p["rgb"].double(), p["z"] < 1e19
-
-
 def fd_check(r, base, h=H_FD, region="hits"):
-    import torch
-    theta0 = np.array(scene.get_params(base))
-    target_theta = theta0.copy()
-    target_theta[0] = 7.3
-    T, _ = _render(r, base, target_theta)
-    C0, hit0 = _render(r, base, theta0)
-    plus, minus = [], []
-    mask = torch.ones_like(hit0)
-    for k in range(9):
-        e = np.zeros(9)
-        e[k] = h
-        cp, hp = _render(r, base, theta0 + e)
-        cm, hm = _render(r, base, theta0 - e)
-        plus.append(cp)
-        minus.append(cm)
-        mask &= (hp == hit0) & (hm == hit0)
-        # one-sided differences (per unit theta) agree on smooth pixels to
-        # O(h); the floor covers the rounding of the image / h (float32 on
-        # hits, the 16-bit background on misses)
-        floor = 5e-3 if region == "hits" else 3.0 / 65536.0 / h
-        dp, dm = (cp - C0) / h, (C0 - cm) / h
-        smooth = (dp - dm).abs() <= 0.05 * torch.maximum(dp.abs(), dm.abs()) + floor
-        mask &= smooth.all(dim=0)
-    mask &= hit0 if region == "hits" else ~hit0
-    nel = C0.numel()
-    m = mask.double()
-    loss = lambda C: float(((C - T) ** 2 * m).sum() / nel)
-    fd = np.array([(loss(plus[k]) - loss(minus[k])) / (2 * h) for k in range(9)])
-    gimg = (2.0 * (C0 - T) * m / nel).float().contiguous()
-    r.set_scene(scene.set_params(scene.copy(base), theta0))
-    adj, _ = r.adjoint(gimg)
-    region_px = int(hit0.sum()) if region == "hits" else int((~hit0).sum())
-    return np.array(adj), fd, region_px - int(mask.sum()), region_px
+    from mandelbulber_amd import diff
+    return diff.check_gradient(r, base, h=h, region=region)
 
 
 def _compare(adj, fd, label, names=scene.DIFF_PARAMS):
