@@ -226,6 +226,14 @@ typedef struct mbr_scene {
                                       so depth is a smooth function of the scene
                                       parameters (the differentiable renders);
                                       0: the reference's binary-search result */
+
+  /* image output (sImageAdjustments, src/fractparams.h; CompileImage,
+   * src/cimage.cpp:132-213) */
+  double brightness;               /* brightness */
+  double contrast;                 /* contrast */
+  double image_gamma;              /* gamma */
+  int hdr;                         /* hdr */
+  char file_destination[512];      /* file_destination (path prefix of the images) */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
@@ -378,6 +386,17 @@ int mbr_render_batch(mbr_ctx *ctx, const mbr_scene *scenes, int n,
                      const mbr_batch_opts *opts, mbr_stats *stats,
                      mbr_batch_result *result);
 
+
+/* CompileImage (src/cimage.cpp:132-213): tone-map the float planes of a
+ * render (HOST memory) with the scene's brightness, contrast, hdr and gamma
+ * into interleaved 16-bit RGB (image16: 3 * width * height). */
+int mbr_compile_image(const mbr_scene *scene, int width, int height,
+                      const float *const rgb[3], uint16_t *image16);
+
+/* SavePNG / SavePNG16 / SavePNG16Alpha (src/files.cpp): write image16 (and
+ * the alpha plane when not NULL) as an 8- or 16-bit PNG. */
+int mbr_write_png(const char *path, int width, int height, const uint16_t *image16,
+                  const uint16_t *alpha, int bits);
 
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,

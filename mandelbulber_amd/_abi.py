@@ -87,6 +87,8 @@ class Scene(C.Structure):
         ("fake_lights_enabled", C.c_int), ("raytraced_reflections", C.c_int),
         ("reflections_max", C.c_int),
         ("jitter_mode", C.c_int), ("jitter_seed", C.c_uint32), ("hit_refine", C.c_int),
+        ("brightness", C.c_double), ("contrast", C.c_double), ("image_gamma", C.c_double), ("hdr", C.c_int),
+        ("file_destination", C.c_char * 512),
     ]
 
 

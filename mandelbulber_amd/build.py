@@ -1,4 +1,5 @@
-"""Build the render-core library mandelbulber_amd/libmbr.so with hipcc for gfx950.
+"""Build the render-core library mandelbulber_amd/libmbr.so with hipcc for gfx950
+and the -nogui driver mandelbulber_amd/mbr_nogui.
 
 Usage: python -m mandelbulber_amd.build   (or __graft_entry__.build())
 """
@@ -11,9 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmbr.so")
+CLI = os.path.join(HERE, "mbr_nogui")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_settings.cpp"]
+SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_settings.cpp", "mbr_image.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -40,13 +42,21 @@ def build(verbose=True):
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    # the -nogui driver: host C++ over the C ABI, found next to the library
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "mbr_nogui.cpp"),
+           "-o", tmp, "-L" + HERE, "-lmbr", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mbr_nogui build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {CLI}")
     return OUT
 
 

@@ -1,0 +1,210 @@
+// mbr_nogui.cpp — command-line renderer over the C ABI, the -nogui driver of
+// the reference (ReadComandlineParams, src/interface.cpp:3334-3561; the
+// still-image branch of MainRender, src/Render3D.cpp:2305-2470): load a
+// settings file with overrides, render it, compile the image, write it.
+//
+//   mbr_nogui [options...] settings_file
+//     -nogui              accepted (this driver has no GUI)
+//     -o key=value        override a setting of the file
+//     -res WIDTHxHEIGHT   override the image resolution
+//     -fpk N              override frames_per_keyframe
+//     -format FORMAT      jpg | png | png16 | png16alpha
+//     -start N, -end N    frame range (animation modes)
+//     -precision P        fp32 (default) | fp64 | fp32march   (this driver)
+//     -out PATH           write the image to PATH instead of
+//                         file_destination + 00000 + extension (this driver)
+//
+// The image goes to file_destination + frame number (%05d) + ".png", as
+// IndexFilename (src/files.cpp:30-35) names it.  There is no JPEG encoder in
+// this build: -format jpg writes an 8-bit PNG.  Flight and keyframe
+// animation (-flight, -keyframe) and network rendering (-ip, -port) are not
+// part of the render core and are rejected.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mbr.h"
+
+namespace {
+
+void usage() {
+  std::printf(
+      "Syntax:\nmbr_nogui [options...] settings_file\noptions:\n"
+      "  -nogui            - accepted; this driver never opens a GUI\n"
+      "  -o key=value      - override item 'key' from settings file with new value 'value'\n"
+      "  -res WIDTHxHEIGHT - override image resolution\n"
+      "  -fpk N            - override frames per key parameter\n"
+      "  -start N          - start rendering from frame number N\n"
+      "  -end N            - rendering will end on frame number N\n"
+      "  -format FORMAT    - image output format: jpg (written as 8-bit PNG), png, png16, png16alpha\n"
+      "  -precision P      - fp32 (FP64 march, FP32 shading; default), fp64, fp32march\n"
+      "  -out PATH         - output file (default: file_destination + frame + .png)\n");
+}
+
+void make_parent(const std::string &path) {
+  for (size_t i = 1; i < path.size(); i++) {
+    if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::vector<std::string> overrides;
+  std::string settings, format = "jpg", out;
+  int precision = MBR_PREC_FP32_FIXUP;
+  int start_frame = 0;
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto need = [&](const char *what) -> const char * {
+      if (i + 1 >= argc) {
+        std::printf("commandline: %s needs an argument\n", what);
+        usage();
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (a == "-nogui") continue;
+    if (a == "-help" || a == "--help") {
+      usage();
+      return 0;
+    }
+    if (a == "-flight" || a == "-keyframe" || a == "-ip" || a == "-port") {
+      std::printf("commandline: %s (animation / network rendering) is not part of this render core\n", a.c_str());
+      return 2;
+    }
+    if (a == "-o") { overrides.push_back(need("-o")); continue; }
+    if (a == "-res") {
+      int w = 0, h = 0;
+      if (std::sscanf(need("-res"), "%dx%d", &w, &h) != 2 || w <= 0 || h <= 0) {
+        std::printf("commandline: Argument must be -res WWWWxHHHH\n");
+        return 2;
+      }
+      overrides.push_back("image_width=" + std::to_string(w));
+      overrides.push_back("image_height=" + std::to_string(h));
+      continue;
+    }
+    if (a == "-fpk") {
+      const int fpk = std::atoi(need("-fpk"));
+      if (fpk <= 0) {
+        std::printf("commandline: argument must be -fpk NNN\n");
+        return 2;
+      }
+      overrides.push_back("frames_per_keyframe=" + std::to_string(fpk));
+      continue;
+    }
+    if (a == "-start") { start_frame = std::atoi(need("-start")); continue; }
+    if (a == "-end") { (void)need("-end"); continue; }
+    if (a == "-format") {
+      format = need("-format");
+      if (format != "jpg" && format != "png" && format != "png16" && format != "png16alpha") {
+        std::printf("commandline: wrong image format\n");
+        return 2;
+      }
+      continue;
+    }
+    if (a == "-precision") {
+      const std::string p = need("-precision");
+      if (p == "fp64") precision = MBR_PREC_FP64;
+      else if (p == "fp32") precision = MBR_PREC_FP32_FIXUP;
+      else if (p == "fp32march") precision = MBR_PREC_FP32_MARCH;
+      else {
+        std::printf("commandline: precision must be fp32, fp64 or fp32march\n");
+        return 2;
+      }
+      continue;
+    }
+    if (a == "-out") { out = need("-out"); continue; }
+    if (!a.empty() && a[0] == '-') {
+      std::printf("commandline: wrong parameters\n");
+      usage();
+      return 2;
+    }
+    settings = a;
+  }
+  if (settings.empty()) {
+    usage();
+    return 2;
+  }
+  std::vector<const char *> ov;
+  for (const std::string &o : overrides) ov.push_back(o.c_str());
+  mbr_scene s;
+  int rc = mbr_load_settings(settings.c_str(), ov.empty() ? nullptr : ov.data(), (int)ov.size(), &s);
+  if (rc != MBR_OK) {
+    std::printf("cannot load settings %s (status %d)\n", settings.c_str(), rc);
+    return 1;
+  }
+  s.frame_no = start_frame;
+  mbr_ctx *ctx = nullptr;
+  rc = mbr_init(nullptr, &ctx);
+  if (rc != MBR_OK) {
+    std::printf("mbr_init failed (status %d): no usable GPU\n", rc);
+    return 1;
+  }
+  rc = mbr_set_scene(ctx, &s, nullptr);
+  if (rc != MBR_OK) {
+    std::printf("scene rejected: %s\n", mbr_last_error(ctx));
+    mbr_destroy(ctx);
+    return 1;
+  }
+  const int w = s.image_width, h = s.image_height;
+  const size_t n = (size_t)w * h;
+  std::vector<float> r(n), g(n), b(n), z(n);
+  std::vector<uint8_t> cr(n), cg(n), cb(n);
+  std::vector<uint16_t> alpha(n), opacity(n);
+  mbr_framebuffer fb;
+  fb.width = w;
+  fb.height = h;
+  fb.rgb[0] = r.data();
+  fb.rgb[1] = g.data();
+  fb.rgb[2] = b.data();
+  fb.z = z.data();
+  fb.colour[0] = cr.data();
+  fb.colour[1] = cg.data();
+  fb.colour[2] = cb.data();
+  fb.alpha = alpha.data();
+  fb.opacity = opacity.data();
+  mbr_render_opts opts;
+  std::memset(&opts, 0, sizeof opts);
+  opts.precision = precision;
+  mbr_stats st;
+  const auto t0 = std::chrono::steady_clock::now();
+  rc = mbr_render(ctx, &opts, &fb, &st);
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (rc != MBR_OK) {
+    std::printf("render failed: %s\n", mbr_last_error(ctx));
+    mbr_destroy(ctx);
+    return 1;
+  }
+  // the reference's end-of-render report (src/Render3D.cpp:638-652)
+  std::printf("rendering time: %.3f s (GPU %.3f ms), %.2f Mrays/s\n", sec, st.ms_total, n / sec / 1e6);
+  std::printf("hits %llu of %llu rays, DE calls %llu, iterations %llu (%.1f per DE call), missed DE %.4f%%\n",
+              (unsigned long long)st.hits, (unsigned long long)st.rays,
+              (unsigned long long)(st.march_de_calls + st.shade_de_calls), (unsigned long long)st.iterations,
+              (double)st.iterations / (double)std::max<uint64_t>(1, st.march_de_calls + st.shade_de_calls),
+              100.0 * (double)st.missed_de / (double)std::max<uint64_t>(1, st.hits));
+  std::vector<uint16_t> img16(3 * n);
+  const float *planes[3] = {r.data(), g.data(), b.data()};
+  rc = mbr_compile_image(&s, w, h, planes, img16.data());
+  if (out.empty()) {
+    char idx[16];
+    std::snprintf(idx, sizeof idx, "%05d", start_frame);
+    out = std::string(s.file_destination) + idx + ".png";
+  }
+  make_parent(out);
+  const int bits = (format == "png16" || format == "png16alpha") ? 16 : 8;
+  if (rc == MBR_OK) rc = mbr_write_png(out.c_str(), w, h, img16.data(), format == "png16alpha" ? alpha.data() : nullptr,
+                                       bits);
+  mbr_destroy(ctx);
+  if (rc != MBR_OK) {
+    std::printf("cannot write %s (status %d)\n", out.c_str(), rc);
+    return 1;
+  }
+  std::printf("image saved: %s%s\n", out.c_str(), format == "jpg" ? " (8-bit PNG: no JPEG encoder in this build)" : "");
+  return 0;
+}

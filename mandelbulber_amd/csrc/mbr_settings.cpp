@@ -21,7 +21,7 @@
 
 namespace {
 
-enum KeyType { KT_DOUBLE, KT_INT, KT_UINT, KT_IGNORE, KT_PALETTE };
+enum KeyType { KT_DOUBLE, KT_INT, KT_UINT, KT_IGNORE, KT_PALETTE, KT_STRING };
 
 struct KeyDef {
   std::string name;
@@ -69,7 +69,8 @@ std::vector<KeyDef> build_table() {
   I("iteration_threshold_mode", OFF(iter_thresh));
   G("analityc_DE_mode");  // derived from the formula (src/interface.cpp:690-693)
   D("DE_factor", OFF(DE_factor));
-  G("brightness"); G("contrast"); G("gamma"); G("hdr"); G("ambient");
+  D("brightness", OFF(brightness)); D("contrast", OFF(contrast)); D("gamma", OFF(image_gamma)); I("hdr", OFF(hdr));
+  G("ambient");
   D("reflect", OFF(reflect));
   D("shadows_intensity", OFF(direct_light)); D("shadows_cone_angle", OFF(shadow_cone_angle));
   D("ambient_occlusion", OFF(global_ilum)); I("ambient_occlusion_quality", OFF(ao_quality));
@@ -204,8 +205,8 @@ std::vector<KeyDef> build_table() {
     G(k);
   G("ocl_custom_DE_mode"); G("ocl_custom_formula_name"); G("ocl_delta_DE_step"); G("ocl_DOF_method");
   for (int i = 0; i < 15; i++) G("ocl_custom_par_" + std::to_string(i));
-  for (const char *k : {"file_destination", "file_background", "file_envmap", "file_lightmap",
-                        "file_animation_path", "file_keyframes"})
+  t.push_back({"file_destination", KT_STRING, OFF(file_destination), 1.0});
+  for (const char *k : {"file_background", "file_envmap", "file_lightmap", "file_animation_path", "file_keyframes"})
     G(k);
   t.push_back({"palette", KT_PALETTE, 0, 1.0});
   // render-core settings with no reference key
@@ -285,6 +286,12 @@ int set_one(mbr_scene *s, const char *key, const char *value, LoadState *st) {
         if (st && ok) st->palette_loaded = true;
         break;
       }
+      case KT_STRING: {
+        char *dst = base + k.offset;
+        std::strncpy(dst, value, sizeof(s->file_destination) - 1);
+        dst[sizeof(s->file_destination) - 1] = 0;
+        break;
+      }
       case KT_IGNORE: break;
     }
     return MBR_OK;
@@ -362,6 +369,11 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->bailout = 100.0;                // src/fractal.cpp:1153
   s->jitter_mode = MBR_JITTER_HASH;
   s->jitter_seed = 1234;             // aux_light_random_seed, the reference's srand seed
+  s->brightness = 1.0;
+  s->contrast = 1.0;
+  s->image_gamma = 1.0;
+  s->hdr = 0;
+  std::strncpy(s->file_destination, "images/image", sizeof(s->file_destination) - 1);
   return MBR_OK;
 }
 

@@ -142,3 +142,22 @@ class Renderer:
         check(lib().mbr_render_device(self._ctx, C.byref(self._opts(precision, rows, stream)), C.byref(fb),
                                       C.byref(st)), self._ctx)
         return st
+
+
+def compile_image(scene, rgb):
+    """CompileImage: tone-map float planes (3, H, W) with the scene's
+    brightness, contrast, hdr and gamma into 16-bit RGB (H, W, 3)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    _, h, w = rgb.shape
+    out = np.empty((h, w, 3), np.uint16)
+    planes = (C.c_void_p * 3)(*[rgb[c].ctypes.data for c in range(3)])
+    check(lib().mbr_compile_image(C.byref(scene), w, h, planes, out.ctypes.data))
+    return out
+
+
+def write_png(path, image16, alpha=None, bits=16):
+    """Write (H, W, 3) uint16 (and an optional (H, W) uint16 alpha) as PNG."""
+    image16 = np.ascontiguousarray(image16, dtype=np.uint16)
+    h, w, _ = image16.shape
+    a = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.uint16)
+    check(lib().mbr_write_png(path.encode(), w, h, image16.ctypes.data, None if a is None else a.ctypes.data, bits))

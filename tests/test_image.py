@@ -1,0 +1,109 @@
+"""Image compile (CompileImage, src/cimage.cpp:132-213) and the PNG writer,
+on the CPU; the -nogui driver's command line."""
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+from mandelbulber_amd import scene
+from mandelbulber_amd.render import compile_image, write_png
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLI = os.path.join(os.path.dirname(HERE), "mandelbulber_amd", "mbr_nogui")
+
+
+def reference_compile(s, rgb):
+    """CalculatePixel restated in numpy: float32 channel arithmetic with the
+    double constants of the source, 16-bit truncation, gamma table."""
+    gamma = (np.power(np.arange(65536) / 65536.0, 1.0 / s.image_gamma) * 65535).astype(np.int64).astype(np.uint16)
+    v = (rgb.astype(np.float32) * np.float32(s.brightness)).astype(np.float32)
+    v = ((v.astype(np.float64) - 0.5) * s.contrast + 0.5).astype(np.float32)
+    v = np.maximum(v, np.float32(0))
+    if s.hdr:
+        v = np.tanh(v).astype(np.float32)
+    v = np.minimum(v, np.float32(1))
+    q = (v.astype(np.float64) * 65535.0).astype(np.uint16)
+    return np.moveaxis(gamma[q], 0, -1)
+
+
+@pytest.mark.parametrize("adj", [dict(), dict(brightness=1.7, contrast=0.8), dict(gamma=2.2),
+                                 dict(brightness=0.6, contrast=1.3, gamma=0.7)])
+def test_compile_image_matches_restatement(adj):
+    s = scene.defaults()
+    for k, v in adj.items():
+        scene.set(s, k, float(v))
+    rng = np.random.default_rng(7)
+    rgb = rng.uniform(-0.2, 1.4, size=(3, 24, 40)).astype(np.float32)
+    got = compile_image(s, rgb)
+    ref = reference_compile(s, rgb)
+    assert np.array_equal(got, ref)
+
+
+def test_compile_image_identity_adjustments_and_hdr():
+    s = scene.defaults()
+    rgb = np.linspace(0, 1, 3 * 5 * 7, dtype=np.float32).reshape(3, 5, 7)
+    got = compile_image(s, rgb).astype(np.int64)
+    # brightness 1, contrast 1, gamma 1: 65535 x v truncated, then the gamma
+    # table pow(i/65536, 1) * 65535 truncated (one less than i for i > 0)
+    q = (rgb.astype(np.float64) * 65535.0).astype(np.int64)
+    expect = np.where(q > 0, (q / 65536.0 * 65535).astype(np.int64), 0)
+    assert np.array_equal(got, np.moveaxis(expect, 0, -1))
+    scene.set(s, "hdr", 1)
+    h = compile_image(s, np.full((3, 2, 2), 10.0, np.float32))
+    assert np.all(h >= 65530)  # tanh(10) ~ 1
+
+
+def _read_png(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        t = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(t + body) & 0xFFFFFFFF
+        chunks.setdefault(t, b"")
+        chunks[t] += body
+        pos += 12 + n
+    w, h, bits, ctype = struct.unpack(">IIBB", chunks[b"IHDR"][:10])
+    raw = zlib.decompress(chunks[b"IDAT"])
+    ch = 4 if ctype == 6 else 3
+    bpp = ch * bits // 8
+    rows = [raw[y * (w * bpp + 1):(y + 1) * (w * bpp + 1)] for y in range(h)]
+    assert all(r[0] == 0 for r in rows)
+    dt = ">u2" if bits == 16 else "u1"
+    px = np.frombuffer(b"".join(r[1:] for r in rows), dtype=dt).reshape(h, w, ch)
+    return px, bits
+
+
+@pytest.mark.parametrize("bits,alpha", [(16, False), (16, True), (8, False)])
+def test_png_round_trip(tmp_path, bits, alpha):
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 65536, size=(9, 13, 3), dtype=np.uint16)
+    a = rng.integers(0, 65536, size=(9, 13), dtype=np.uint16) if alpha else None
+    path = str(tmp_path / "t.png")
+    write_png(path, img, a, bits)
+    px, b = _read_png(path)
+    assert b == bits
+    want = img if a is None else np.concatenate([img, a[..., None]], axis=-1)
+    if bits == 8:
+        want = want >> 8
+    assert np.array_equal(px.astype(np.uint16), want)
+
+
+def test_cli_help_and_argument_errors():
+    assert os.path.exists(CLI), "mbr_nogui is built by mandelbulber_amd.build"
+    r = subprocess.run([CLI, "-help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "-res WIDTHxHEIGHT" in r.stdout
+    r = subprocess.run([CLI, "-res", "12x"], capture_output=True, text=True)
+    assert r.returncode == 2 and "-res" in r.stdout
+    r = subprocess.run([CLI, "-format", "tiff", "x.fract"], capture_output=True, text=True)
+    assert r.returncode == 2 and "wrong image format" in r.stdout
+    r = subprocess.run([CLI, "-flight", "x.fract"], capture_output=True, text=True)
+    assert r.returncode == 2
+    r = subprocess.run([CLI, "/nonexistent.fract"], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot load settings" in r.stdout
