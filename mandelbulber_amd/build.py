@@ -23,6 +23,10 @@ FLAGS = [
     # packed FP32 instructions stay disabled: the compiler forms them from
     # ordinary float code, and some of their forms are unreliable here
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 square roots and divisions use the hardware v_sqrt_f32 / v_rcp_f32
+    # (1 ulp) instead of the correctly rounded sequences: the FP32 paths are
+    # held to tolerances; FP64 arithmetic is not affected
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
