@@ -627,10 +627,12 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 //
 // A lane is in one of four states: NEED (wants an item), ITER (advancing a
 // distance estimate), DONE (estimate complete, AO step pending) or EXHAUSTED.
-// Every pass advances the ITER lanes by one formula iteration.  The AO step
-// logic of DONE lanes and the item hand-out of NEED lanes run only once a
-// quarter of the wave waits for them (or nothing else can run), so their
-// cost is shared by many lanes instead of being paid on almost every pass.
+// Every pass advances the ITER lanes by up to two formula iterations (an AO
+// distance estimate takes about 2.3 on average, so the pass overhead is paid
+// about once per estimate).  The AO step logic of DONE lanes runs once a
+// third of the wave waits for it, the item hand-out of NEED lanes once a
+// quarter does (or nothing else can run), so their cost is shared by many
+// lanes instead of being paid on almost every pass.
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
 template <class T, int F, bool Deep>
@@ -745,7 +747,9 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
 
     if (state == AO_ITER) {
       DEInfo info;
-      if (de_call_step<T, F>(f, call, detail, N, minN, false, dist, info)) {
+      bool fin = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
+      if (!fin) fin = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
+      if (fin) {
         state = AO_DONE;
         mi = info.max_iter;
         cnt.de_calls += 1;
@@ -756,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
     const unsigned long long m_done2 = __ballot(state == AO_DONE);
     const unsigned long long m_iter2 = __ballot(state == AO_ITER);
     const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
-    if (n_done2 && (4 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
+    if (n_done2 && (3 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
       // one step of the AO ray (src/shaders.cpp:625-662)
       ao_steps++;
       if (dist < thr_f || mi) {
