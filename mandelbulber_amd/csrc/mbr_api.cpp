@@ -635,6 +635,11 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
     const double thr_min = s.constant_DE_threshold ? s.quality : near * resolution * s.persp / s.quality;
     d.deep = thr_min < 2.0 * kMixedSwitch * (length(st) + 3.0);
   }
+  // Plain scenes run kernel variants compiled without the branches (and the
+  // live scene constants) of the optional distance-estimate terms.
+  d.plain = d.fd.limits == 0 && d.fd.only_plane == 0 && d.fd.interior == 0 && d.fd.iter_thresh == 0 &&
+            !d.fd.prim_plane && !d.fd.prim_box && !d.fd.prim_ibox && !d.fd.prim_sphere && !d.fd.prim_isphere &&
+            !d.fd.prim_water && !d.fd.tglad_fold && !d.fd.sph_fold && !d.fd.julia_mode;
   return MBR_OK;
 }
 
@@ -696,6 +701,7 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
   sbuf.n = b.cap;
   sbuf.slow_ao = host.ao && !host.fast_ao;
   sbuf.deep = host.deep;
+  sbuf.plain = host.plain;
   sbuf.formula = formula;
   int band = 0;
   for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
@@ -704,7 +710,8 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     hipEvent_t *e = ev ? ev + 6 * band : nullptr;
     HIPCHECK(c, launch_band_reset(b, st));
     if (e) HIPCHECK(c, hipEventRecord(e[0], st));
-    HIPCHECK(c, launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, persist, c->d_stats, st));
+    HIPCHECK(c,
+             launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, host.plain, persist, c->d_stats, st));
     HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, c->d_stats, st));
     if (e) HIPCHECK(c, hipEventRecord(e[1], st));
     HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, c->d_stats, st,
@@ -1148,8 +1155,8 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
     for (long long band_first = 0; band_first < last; band_first += sl.band.cap) {
       const int band_rays = (int)std::min<long long>(sl.band.cap, last - band_first);
       HIPCHECK(c, launch_band_reset(sl.band, st));
-      HIPCHECK(c, launch_march(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, band_rays, s.formula, persist,
-                               c->d_stats, st));
+      HIPCHECK(c, launch_march(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, band_rays, s.formula,
+                               c->host_sc.plain, persist, c->d_stats, st));
       HIPCHECK(c, launch_march_fix(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, s.formula, persist / 2,
                                    c->d_stats, st));
       HIPCHECK(c, launch_adjoint(c->d_sc, sl.band, (int)band_first, band_rays, p, grad_rgb[0], grad_rgb[1],

@@ -232,10 +232,10 @@ struct DEIter {
   int L;
 };
 
-template <class T, int F>
+template <class T, int F, bool P = false>
 MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.z = p;
-  s.c = f.julia_mode ? ld3(f.julia) : p * f.constant_factor;
+  s.c = (!P && f.julia_mode) ? ld3(f.julia) : p * f.constant_factor;
   s.r = length(p);
   s.aux = 1;
   s.L = 0;
@@ -252,12 +252,12 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
 // One loop pass.  Returns true when the call has finished; `dist` is then
 // the return value of Compute<normal> (0 when the orbit stayed bounded for
 // N iterations) and s.L the reference's iteration counter.
-template <class T, int F>
+template <class T, int F, bool P = false>
 MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
   using M = Math<T>;
   V3<T> z = s.z;
   T r = s.r;
-  if (f.tglad_fold) {
+  if ((!P && f.tglad_fold)) {
     const T lim = f.fold_limit, val = f.fold_value;
     if (z.x > lim) z.x = val - z.x;
     else if (z.x < -lim) z.x = -val - z.x;
@@ -267,7 +267,7 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
     else if (z.z < -lim) z.z = -val - z.z;
     r = length(z);
   }
-  if (f.sph_fold) {
+  if ((!P && f.sph_fold)) {
     T r2 = r * r;
     if (r2 < f.sph_mR2) { z = z * f.sph_factor1; s.aux *= f.sph_factor1; }
     else if (r2 < f.sph_fR2) { T tf2 = fdiv(f.sph_fR2, r2); z = z * tf2; s.aux *= tf2; }
@@ -285,7 +285,7 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
       // hardware rsq/rcp.
       const int p = f.ipow;
       T inv_r = s.inv_r;
-      if (f.tglad_fold || f.sph_fold) inv_r = fast_rsq(z.x * z.x + z.y * z.y + z.z * z.z);
+      if ((!P && f.tglad_fold) || (!P && f.sph_fold)) inv_r = fast_rsq(z.x * z.x + z.y * z.y + z.z * z.z);
       const T rho2 = z.x * z.x + z.y * z.y;
       T ure, uim = z.z * inv_r;  // e^{i th}
       T wre, wim;                // e^{i ph}
@@ -438,7 +438,7 @@ struct DECall {
   T limitBox;
 };
 
-template <class T>
+template <class T, bool P = false>
 MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fractal, int N, int minN,
                    bool normalMode, T detailSize, T limitBoxDist, DEInfo &info) {
   info.object = OBJ_FRACTAL;
@@ -448,12 +448,12 @@ MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fr
     info.iters = L;
     if (distance < 0) distance = 0;
     if (L < minN && distance < detailSize) distance = detailSize;
-    if (f.interior && !normalMode) {
+    if ((!P && f.interior) && !normalMode) {
       if (distance < T(0.5) * detailSize || L == N) { distance = detailSize; info.max_iter = 0; }
-    } else if (f.interior && normalMode) {
+    } else if ((!P && f.interior) && normalMode) {
       if (distance < T(0.9) * detailSize) { distance = detailSize - distance; info.max_iter = 0; }
     }
-    if (f.iter_thresh && L < N) {
+    if ((!P && f.iter_thresh) && L < N) {
       if (distance < detailSize) distance = detailSize * T(1.01);
     }
   } else {
@@ -461,44 +461,44 @@ MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fr
     info.max_iter = 0;
     info.work = 0;
   }
-  if (f.prim_plane) {
+  if ((!P && f.prim_plane)) {
     T d = dot(ld3(f.plane_n), point - ld3(f.plane_c));
-    if (!f.only_plane && d < distance) info.object = OBJ_PLANE;
+    if (!(!P && f.only_plane) && d < distance) info.object = OBJ_PLANE;
     distance = (d < distance) ? d : distance;
   }
-  if (f.prim_box) {
+  if ((!P && f.prim_box)) {
     T d = prim_box(point, f.box_c1, f.box_c2);
     if (d < distance) info.object = OBJ_BOX;
     distance = (d < distance) ? d : distance;
   }
-  if (f.prim_ibox) {
+  if ((!P && f.prim_ibox)) {
     T d = prim_ibox(point, f.ibox_c1, f.ibox_c2);
     if (d < distance) info.object = OBJ_BOX_INV;
     distance = (d < distance) ? d : distance;
   }
-  if (f.prim_sphere) {
+  if ((!P && f.prim_sphere)) {
     T d = length(point - ld3(f.sph_c)) - f.sph_r;
     if (d < distance) info.object = OBJ_SPHERE;
     distance = (d < distance) ? d : distance;
   }
-  if (f.prim_isphere) {
+  if ((!P && f.prim_isphere)) {
     T d = f.isph_r - length(point - ld3(f.isph_c));
     if (d < distance) info.object = OBJ_SPHERE_INV;
     distance = (d < distance) ? d : distance;
   }
-  if (f.prim_water) {
+  if ((!P && f.prim_water)) {
     T d = prim_water(f, point);
     if (d < distance) info.object = OBJ_WATER;
     distance = (d < distance) ? d : distance;
   }
-  if (f.limits) {
+  if ((!P && f.limits)) {
     if (limitBoxDist < detailSize) distance = (distance > limitBoxDist) ? distance : limitBoxDist;
   }
   return distance;
 }
 
 // Start one CalculateDistance; true when it finished without iterating.
-template <class T, int F>
+template <class T, int F, bool P = false>
 MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                           DECall<T> &call, T &dist, DEInfo &info) {
   info.object = OBJ_FRACTAL;
@@ -507,7 +507,7 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
   info.iters = 0;
   call.p = point;
   call.limitBox = 0;
-  if (f.limits) {
+  if ((!P && f.limits)) {
     T da = point.x - f.limit_max[0], db = point.y - f.limit_max[1], dc = point.z - f.limit_max[2];
     T ta = -(point.x - f.limit_min[0]), tb = -(point.y - f.limit_min[1]), tc = -(point.z - f.limit_min[2]);
     da = (da > ta) ? da : ta;  // MAX()
@@ -516,25 +516,25 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
     call.limitBox = (da > db) ? ((da > dc) ? da : dc) : ((db > dc) ? db : dc);  // dMax
     if (call.limitBox > detailSize) { dist = call.limitBox; return true; }
   }
-  if (f.only_plane) {
-    dist = de_finish<T>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, call.limitBox, info);
+  if ((!P && f.only_plane)) {
+    dist = de_finish<T, P>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
   }
-  de_begin<T, F>(f, point, call.it);
+  de_begin<T, F, P>(f, point, call.it);
   if (N <= 0) {
-    dist = de_finish<T>(f, point, T(0), 0, true, N, minN, normalMode, detailSize, call.limitBox, info);
+    dist = de_finish<T, P>(f, point, T(0), 0, true, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
   }
   return false;
 }
 
 // Advance one iteration; true when the call finished (dist, info valid).
-template <class T, int F>
+template <class T, int F, bool P = false>
 MBR_HD bool de_call_step(const FractalK<T> &f, DECall<T> &call, T detailSize, int N, int minN, bool normalMode,
                          T &dist, DEInfo &info) {
   T raw = 0;
-  if (!de_iter<T, F>(f, call.it, N, raw)) return false;
-  dist = de_finish<T>(f, call.p, raw, call.it.L, true, N, minN, normalMode, detailSize, call.limitBox, info);
+  if (!de_iter<T, F, P>(f, call.it, N, raw)) return false;
+  dist = de_finish<T, P>(f, call.p, raw, call.it.L, true, N, minN, normalMode, detailSize, call.limitBox, info);
   return true;
 }
 

@@ -339,7 +339,7 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
 // pass, lane states and amortised bookkeeping as in k_ao.
 enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
 
-template <class T, class RT, int F>
+template <class T, class RT, int F, bool Plain>
 __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                   int band_rays, unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
     march_before_de(sc, start, dir, s);
     DEInfo info;
     state = MS_ITER;
-    if (de_call_begin<T, F>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
+    if (de_call_begin<T, F, Plain>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
       state = MS_DONE;
       object = info.object;
       cnt.de_calls += 1;
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
 
     if (state == MS_ITER) {
       DEInfo info;
-      if (de_call_step<T, F>(f, call, (T)s.thr, N, minN, false, dist, info)) {
+      if (de_call_step<T, F, Plain>(f, call, (T)s.thr, N, minN, false, dist, info)) {
         state = MS_DONE;
         object = info.object;
         cnt.de_calls += 1;
@@ -635,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 // lanes instead of being paid on almost every pass.
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
-template <class T, int F, bool Deep>
+template <class T, int F, bool Deep, bool Plain>
 __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
                                                SurfBufs<T> sb, unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -676,9 +676,9 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
         done = de_call_begin_p64<F>(sc.fd, f, P64 + cvt<double>(dir) * (double)r, detail, N, minN, false, call, dist,
                                     info);
       else
-        done = de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+        done = de_call_begin<T, F, Plain>(f, P + dir * r, detail, N, minN, false, call, dist, info);
     } else {
-      done = de_call_begin<T, F>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+      done = de_call_begin<T, F, Plain>(f, P + dir * r, detail, N, minN, false, call, dist, info);
     }
     if (done) {
       state = AO_DONE;
@@ -747,8 +747,8 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
 
     if (state == AO_ITER) {
       DEInfo info;
-      bool fin = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
-      if (!fin) fin = de_call_step<T, F>(f, call, detail, N, minN, false, dist, info);
+      bool fin = de_call_step<T, F, Plain>(f, call, detail, N, minN, false, dist, info);
+      if (!fin) fin = de_call_step<T, F, Plain>(f, call, detail, N, minN, false, dist, info);
       if (fin) {
         state = AO_DONE;
         mi = info.max_iter;
@@ -907,18 +907,23 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int persist_blocks, unsigned long long *stats, hipStream_t s) {
+                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
   const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto tag, auto rtag) {
     using T = decltype(tag);
     using RT = decltype(rtag);
-    if (formula == 7)
-      hipLaunchKernelGGL((k_march<T, RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
-    else if (formula == 8)
-      hipLaunchKernelGGL((k_march<T, RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
-    else
-      hipLaunchKernelGGL((k_march<T, RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+    auto go = [&](auto plain_tag) {
+      constexpr bool Pl = decltype(plain_tag)::value;
+      if (formula == 7)
+        hipLaunchKernelGGL((k_march<T, RT, 7, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_march<T, RT, 8, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+      else
+        hipLaunchKernelGGL((k_march<T, RT, 2, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
+    };
+    if (plain) go(std::true_type{});
+    else go(std::false_type{});
   };
   if (precision == 0) run(double{}, double{});
   else if (precision == 1) run(double{}, float{});
@@ -994,14 +999,15 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
       else
         hipLaunchKernelGGL((k_surface<T, 2, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
     };
-    auto aok = [&](auto deep_tag) {
+    auto aok = [&](auto deep_tag, auto plain_tag) {
       constexpr bool D = decltype(deep_tag)::value;
+      constexpr bool Pl = decltype(plain_tag)::value;
       if (formula == 7)
-        hipLaunchKernelGGL((k_ao<T, 7, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        hipLaunchKernelGGL((k_ao<T, 7, D, Pl>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else if (formula == 8)
-        hipLaunchKernelGGL((k_ao<T, 8, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        hipLaunchKernelGGL((k_ao<T, 8, D, Pl>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else
-        hipLaunchKernelGGL((k_ao<T, 2, D>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        hipLaunchKernelGGL((k_ao<T, 2, D, Pl>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
     };
     // deep-zoom variants only for FP32 shading of scenes that need them
     const bool deep = sbuf.deep && sizeof(T) < 8;
@@ -1012,8 +1018,12 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
       if (e != hipSuccess) return e;
     }
     if (sbuf.slow_ao) {
-      if (deep) aok(std::true_type{});
-      else aok(std::false_type{});
+      // plain scenes (no primitives, limits, interior mode, iteration
+      // threshold, pre-folds or Julia constant) take a variant without those
+      // branches and the scene constants they keep live
+      if (deep) aok(std::true_type{}, std::false_type{});
+      else if (sbuf.plain) aok(std::false_type{}, std::true_type{});
+      else aok(std::false_type{}, std::false_type{});
     }
     if (ev_ao1) {
       hipError_t e = hipEventRecord(ev_ao1, s);

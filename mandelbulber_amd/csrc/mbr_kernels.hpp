@@ -86,7 +86,7 @@ struct AODev {
 hipError_t launch_band_reset(const BandBufs &b, hipStream_t s);
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t s);
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int persist_blocks, unsigned long long *stats, hipStream_t s);
+                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s);
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
                             int persist_blocks, unsigned long long *stats, hipStream_t s);
 // Buffers of the shading launches (element type T = the path's precision).
@@ -95,6 +95,7 @@ struct ShadeBufs {
   long long n;                  // hit capacity (plane stride)
   int slow_ao;                  // 1: run the AmbientOcclusion rays
   int deep;                     // 1: FP32 shading starts distance estimates in FP64 where needed
+  int plain;                    // 1: no primitives, limits, interior, iteration threshold, pre-folds, Julia
   int formula;
 };
 // Shading in three launches: surface (normals, shadow, specular, colour,

@@ -56,6 +56,7 @@ struct DevScene {
   int width, height, tiles, tile_index;
   int persp_type, fish_eye_cut, frame_no, jitter_mode, hit_refine;
   int deep;  // the pixel footprint can fall below FP32 resolution of the coordinates
+  int plain; // no primitives, limits, interior mode, iteration threshold, pre-folds or Julia constant
   uint32_t jitter_seed, pixel_base;  // pixel index offset of this tile
   double aspect, fov, tile_x, tile_y;
   double R[9];
