@@ -80,6 +80,11 @@ MBR_HD float fast_rsq(float x) {
   return 1.0f / ::sqrtf(x);
 #endif
 }
+// FP64 forms of the same operations for templates shared by both
+// precisions: the correctly rounded division, square root and log.
+MBR_HD double fast_rcp(double x) { return 1.0 / x; }
+MBR_HD double fast_rsq(double x) { return 1.0 / Math<double>::sqrt(x); }
+MBR_HD double fast_ln(double x) { return Math<double>::log(x); }
 // a / b: exact in FP64, hardware reciprocal in FP32
 MBR_HD double fdiv(double a, double b) { return a / b; }
 MBR_HD float fdiv(float a, float b) { return a * fast_rcp(b); }

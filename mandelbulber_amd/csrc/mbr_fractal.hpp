@@ -232,7 +232,7 @@ struct DEIter {
   int L;
 };
 
-template <class T, int F, bool P = false>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
 MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.z = p;
   s.c = (!P && f.julia_mode) ? ld3(f.julia) : p * f.constant_factor;
@@ -240,7 +240,7 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.aux = 1;
   s.L = 0;
   s.inv_r = 0;
-  if constexpr (F == 2 && sizeof(T) < 8) {
+  if constexpr (F == 2 && A) {
     if (f.ipow) {
       const T r2 = p.x * p.x + p.y * p.y + p.z * p.z;
       s.inv_r = fast_rsq(r2);
@@ -252,7 +252,7 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
 // One loop pass.  Returns true when the call has finished; `dist` is then
 // the return value of Compute<normal> (0 when the orbit stayed bounded for
 // N iterations) and s.L the reference's iteration counter.
-template <class T, int F, bool P = false>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
 MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
   using M = Math<T>;
   V3<T> z = s.z;
@@ -274,14 +274,15 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
     r = length(z);
   }
   bool esc;
-  if constexpr (F == 2 && sizeof(T) < 8) {
+  if constexpr (F == 2 && A) {
     if (f.ipow) {
-      // FP32, integer power p: the same map without inverse trig.  With
+      // Integer power p (A: the FP32 kernels, and FP64 marching of the mixed
+      // precision modes): the same map without inverse trig.  With
       // rho = |(x,y)|, (cos th, sin th) = (rho, z)/r and (cos ph, sin ph) =
       // (x, y)/rho, so cos/sin(p th) and cos/sin(p ph) are the parts of the
       // complex powers (rho + i z)^p / r^p and (x + i y)^p / rho^p
       // (de Moivre), formed by binary exponentiation.  1/r is carried from
-      // the previous iteration; reciprocals and square roots use the
+      // the previous iteration; in FP32 reciprocals and square roots use the
       // hardware rsq/rcp.
       const int p = f.ipow;
       T inv_r = s.inv_r;
@@ -498,7 +499,7 @@ MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fr
 }
 
 // Start one CalculateDistance; true when it finished without iterating.
-template <class T, int F, bool P = false>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
 MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                           DECall<T> &call, T &dist, DEInfo &info) {
   info.object = OBJ_FRACTAL;
@@ -520,7 +521,7 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
     dist = de_finish<T, P>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
   }
-  de_begin<T, F, P>(f, point, call.it);
+  de_begin<T, F, P, A>(f, point, call.it);
   if (N <= 0) {
     dist = de_finish<T, P>(f, point, T(0), 0, true, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
@@ -529,11 +530,11 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
 }
 
 // Advance one iteration; true when the call finished (dist, info valid).
-template <class T, int F, bool P = false>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
 MBR_HD bool de_call_step(const FractalK<T> &f, DECall<T> &call, T detailSize, int N, int minN, bool normalMode,
                          T &dist, DEInfo &info) {
   T raw = 0;
-  if (!de_iter<T, F, P>(f, call.it, N, raw)) return false;
+  if (!de_iter<T, F, P, A>(f, call.it, N, raw)) return false;
   dist = de_finish<T, P>(f, call.p, raw, call.it.L, true, N, minN, normalMode, detailSize, call.limitBox, info);
   return true;
 }

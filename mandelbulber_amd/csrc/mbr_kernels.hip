@@ -342,6 +342,9 @@ enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
 template <class T, class RT, int F, bool Plain>
 __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                   int band_rays, unsigned long long *stats) {
+  // integer-power Mandelbulb iterations without inverse trig wherever the
+  // records are FP32; the FP64 mode keeps the trigonometric form
+  constexpr bool kAlg = sizeof(RT) < 8;
   const DevScene &sc = *scp;
   const FractalK<T> &f = sc.frac<T>();
   const int N = sc.N, minN = sc.minN;
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
     march_before_de(sc, start, dir, s);
     DEInfo info;
     state = MS_ITER;
-    if (de_call_begin<T, F, Plain>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
+    if (de_call_begin<T, F, Plain, kAlg>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
       state = MS_DONE;
       object = info.object;
       cnt.de_calls += 1;
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
 
     if (state == MS_ITER) {
       DEInfo info;
-      if (de_call_step<T, F, Plain>(f, call, (T)s.thr, N, minN, false, dist, info)) {
+      if (de_call_step<T, F, Plain, kAlg>(f, call, (T)s.thr, N, minN, false, dist, info)) {
         state = MS_DONE;
         object = info.object;
         cnt.de_calls += 1;
