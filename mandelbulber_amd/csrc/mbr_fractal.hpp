@@ -539,13 +539,13 @@ MBR_HD bool de_call_step(const FractalK<T> &f, DECall<T> &call, T detailSize, in
   return true;
 }
 
-template <class T, int F>
+template <class T, int F, bool A = (sizeof(T) < 8)>
 MBR_HD T calc_distance_f(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                          DEInfo &info) {
   DECall<T> call;
   T dist = 0;
-  if (de_call_begin<T, F>(f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
-  while (!de_call_step<T, F>(f, call, detailSize, N, minN, normalMode, dist, info)) {
+  if (de_call_begin<T, F, false, A>(f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
+  while (!de_call_step<T, F, false, A>(f, call, detailSize, N, minN, normalMode, dist, info)) {
   }
   return dist;
 }

@@ -185,6 +185,9 @@ template <class RT, int F>
 __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
                                    const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
                                    bool &ambiguous) {
+  // trig-free integer-power iteration where the records are FP32, as in
+  // k_march
+  constexpr bool kAlg = sizeof(RT) < 8;
   const ShadeK<double> &k = sc.sd;
   const double search_limit = 1.0 - 0.01 * k.quality;
   const double margin = 4.0 * sqrt(s.drift);  // s.drift: variance of the FP32 scan deviation
@@ -197,7 +200,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
     march_before_de(sc, start, dir, s);
     detail = s.thr;
     DEInfo info;
-    dist = calc_distance_f<double, F>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
+    dist = calc_distance_f<double, F, kAlg>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
     cnt.add(info);
     s.object = info.object;
     const double dist_raw = dist;
@@ -235,7 +238,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
       distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + 0.0;
       DEInfo bi;
-      dist = calc_distance_f<double, F>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
+      dist = calc_distance_f<double, F, kAlg>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
       cnt.add(bi);
       object = bi.object;
       step *= 0.5;
@@ -255,15 +258,15 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       for (int it = 0; it < 8 && fabs(f) > 1e-13 * kthr * t; it++) {
         const double eps = 1e-3 * kthr * t;
         DEInfo bi;
-        const double dp = calc_distance_f<double, F>(sc.fd, start + dir * (t + eps), detail, sc.N, sc.minN, false, bi);
-        const double dm = calc_distance_f<double, F>(sc.fd, start + dir * (t - eps), detail, sc.N, sc.minN, false, bi);
+        const double dp = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * (t + eps), detail, sc.N, sc.minN, false, bi);
+        const double dm = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * (t - eps), detail, sc.N, sc.minN, false, bi);
         const double g = (dp - dm) / (2.0 * eps) - c0 * kthr;
         if (!(fabs(g) > 0.0)) break;
         double dt = -f / g;
         const double lim = kthr * t;
         dt = dt > lim ? lim : (dt < -lim ? -lim : dt);
         t += dt;
-        dist = calc_distance_f<double, F>(sc.fd, start + dir * t, detail, sc.N, sc.minN, false, bi);
+        dist = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * t, detail, sc.N, sc.minN, false, bi);
         cnt.add(bi);
         object = bi.object;
         f = dist - c0 * kthr * t;
