@@ -234,6 +234,14 @@ typedef struct mbr_scene {
   double image_gamma;              /* gamma */
   int hdr;                         /* hdr */
   char file_destination[512];      /* file_destination (path prefix of the images) */
+
+  /* post effects on the compiled image (src/image.cpp; order in the
+   * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
+  int ssao_enabled;                /* post_SSAO_enabled */
+  int ssao_quality;                /* post_SSAO_quality */
+  int dof_enabled;                 /* post_DOF_enabled */
+  double dof_focus;                /* post_DOF_focus */
+  double dof_radius;               /* post_DOF_radius */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
@@ -397,6 +405,28 @@ int mbr_compile_image(const mbr_scene *scene, int width, int height,
  * the alpha plane when not NULL) as an 8- or 16-bit PNG. */
 int mbr_write_png(const char *path, int width, int height, const uint16_t *image16,
                   const uint16_t *alpha, int bits);
+
+/* Planes of one rendered frame for the post effects.  image16 is the
+ * compiled image (mbr_compile_image), interleaved RGB, 3*width*height. */
+typedef struct mbr_post_planes {
+  int width, height;
+  const float *z;                  /* zBuffer (1e20 on a miss) */
+  const uint16_t *opacity;         /* opacity plane */
+  const uint8_t *colour[3];        /* colour planes (palette colour of the hit) */
+  uint16_t *image16;               /* in/out */
+  uint16_t *alpha;                 /* in/out for DOF; unused by SSAO (may be NULL) */
+} mbr_post_planes;
+
+/* Screen-space ambient occlusion (PostRendering_SSAO / ThreadSSAO,
+ * src/image.cpp:249-415; called from src/Render3D.cpp:1096-1099) with the
+ * scene's ssao_quality, persp, perspective_type and global_ilum: for every
+ * hit pixel, the horizon angle over ssao_quality screen directions of the
+ * zBuffer, added to image16 as colour x AO x intensity x (1 - opacity).
+ * on_device = 1: the planes are device memory and the kernel is queued on
+ * `stream` (NULL = the context's) without a host wait; 0: host memory,
+ * copied in and out, returns when done.  Needs no mbr_set_scene. */
+int mbr_post_ssao(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *planes,
+                  int on_device, void *stream);
 
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,

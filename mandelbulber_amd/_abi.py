@@ -89,6 +89,8 @@ class Scene(C.Structure):
         ("jitter_mode", C.c_int), ("jitter_seed", C.c_uint32), ("hit_refine", C.c_int),
         ("brightness", C.c_double), ("contrast", C.c_double), ("image_gamma", C.c_double), ("hdr", C.c_int),
         ("file_destination", C.c_char * 512),
+        ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
+        ("dof_focus", C.c_double), ("dof_radius", C.c_double),
     ]
 
 
@@ -108,6 +110,14 @@ class Framebuffer(C.Structure):
         ("colour", C.c_void_p * 3),
         ("alpha", C.c_void_p),
         ("opacity", C.c_void_p),
+    ]
+
+
+class PostPlanes(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("z", C.c_void_p), ("opacity", C.c_void_p), ("colour", C.c_void_p * 3),
+        ("image16", C.c_void_p), ("alpha", C.c_void_p),
     ]
 
 

@@ -284,6 +284,12 @@ struct mbr_ctx {
   hipStream_t side[kMaxSlots] = {};
   double *d_grad = nullptr;  // mbr_adjoint accumulator
   int tuned_slots = 0, tuned_bpc = 0;
+  // post effects: cos/sin table of the last SSAO quality, staging planes of
+  // the host-memory calls
+  double *d_ssao_tab = nullptr;
+  int ssao_tab_q = 0;
+  void *d_post = nullptr;
+  size_t d_post_bytes = 0;
 
   int fail(int rc, const std::string &msg) {
     err = msg;
@@ -500,7 +506,8 @@ void mbr_destroy(mbr_ctx *c) {
   for (Slot &sl : c->slots) free_band(sl);
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
-  void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad};
+  void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad, c->d_ssao_tab,
+                c->d_post};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
@@ -832,6 +839,76 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->ms_ao = ms_ao;
     stats->ms_total = ms_march + ms_shade + ms_finish;
   }
+  return MBR_OK;
+}
+
+int mbr_post_ssao(mbr_ctx *c, const mbr_scene *s, const mbr_post_planes *pl, int on_device, void *stream) {
+  if (!c || !s || !pl) return MBR_ERR_INVALID_ARG;
+  const int w = pl->width, h = pl->height, q = s->ssao_quality;
+  if (w <= 0 || h <= 0 || !pl->z || !pl->opacity || !pl->colour[0] || !pl->colour[1] || !pl->colour[2] ||
+      !pl->image16)
+    return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_ssao: planes missing");
+  if (q < 1 || q > 1024) return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_ssao: ssao_quality must be in [1, 1024]");
+  if (s->perspective_type < MBR_PERSP_THREE_POINT || s->perspective_type > MBR_PERSP_EQUIRECTANGULAR)
+    return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_ssao: unknown perspective_type");
+  HIPCHECK(c, hipSetDevice(c->device));
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  if (q != c->ssao_tab_q) {
+    // the direction table of src/image.cpp:266-271, from the host libm
+    std::vector<double> tab(2 * (size_t)q);
+    for (int i = 0; i < q; i++) {
+      tab[i] = cos((double)i / q * 2.0 * M_PI);
+      tab[q + i] = sin((double)i / q * 2.0 * M_PI);
+    }
+    // an earlier device-plane call may still read the old table on another stream
+    if (c->d_ssao_tab) HIPCHECK(c, hipDeviceSynchronize());
+    if (c->d_ssao_tab) HIPCHECK(c, hipFree(c->d_ssao_tab));
+    c->d_ssao_tab = nullptr;
+    c->ssao_tab_q = 0;
+    HIPCHECK(c, hipMalloc(&c->d_ssao_tab, tab.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(c->d_ssao_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    c->ssao_tab_q = q;
+  }
+  SsaoParams p;
+  p.width = w;
+  p.height = h;
+  p.quality = q;
+  p.type = s->perspective_type;
+  p.scale = (double)w / (q * q) / 2.0;
+  p.aspect = (double)w / h;
+  p.fov = s->persp;
+  p.intensity = s->global_ilum;
+  p.cs = c->d_ssao_tab;
+  p.sn = c->d_ssao_tab + q;
+  if (on_device) {
+    HIPCHECK(c, launch_ssao(p, pl->z, pl->opacity, pl->colour, pl->image16, st));
+    return MBR_OK;
+  }
+  // host planes: stage z, opacity, colour and image16 on the device
+  const size_t n = (size_t)w * h;
+  const size_t bytes = n * (4 + 2 + 3 + 6) + 64 * 6;
+  if (bytes > c->d_post_bytes) {
+    if (c->d_post) HIPCHECK(c, hipFree(c->d_post));
+    c->d_post = nullptr;
+    c->d_post_bytes = 0;
+    HIPCHECK(c, hipMalloc(&c->d_post, bytes));
+    c->d_post_bytes = bytes;
+  }
+  char *base = (char *)c->d_post;
+  size_t off = 0;
+  auto take = [&](size_t sz) { void *ptr = base + off; off += (sz + 63) / 64 * 64; return ptr; };
+  float *dz = (float *)take(n * 4);
+  uint16_t *dop = (uint16_t *)take(n * 2);
+  uint8_t *dcol[3] = {(uint8_t *)take(n), (uint8_t *)take(n), (uint8_t *)take(n)};
+  uint16_t *dimg = (uint16_t *)take(n * 6);
+  HIPCHECK(c, hipMemcpyAsync(dz, pl->z, n * 4, hipMemcpyHostToDevice, st));
+  HIPCHECK(c, hipMemcpyAsync(dop, pl->opacity, n * 2, hipMemcpyHostToDevice, st));
+  for (int i = 0; i < 3; i++) HIPCHECK(c, hipMemcpyAsync(dcol[i], pl->colour[i], n, hipMemcpyHostToDevice, st));
+  HIPCHECK(c, hipMemcpyAsync(dimg, pl->image16, n * 6, hipMemcpyHostToDevice, st));
+  HIPCHECK(c, launch_ssao(p, dz, dop, dcol, dimg, st));
+  HIPCHECK(c, hipMemcpyAsync(pl->image16, dimg, n * 6, hipMemcpyDeviceToHost, st));
+  HIPCHECK(c, hipStreamSynchronize(st));
   return MBR_OK;
 }
 

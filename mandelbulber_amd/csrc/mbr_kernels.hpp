@@ -117,4 +117,14 @@ hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first,
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s);
 
+// Screen-space ambient occlusion of one compiled frame (mbr_post.hip).
+// cs/sn: device tables of cos/sin(2 pi i / quality), i < quality.
+struct SsaoParams {
+  int width, height, quality, type;  // type: enum mbr_perspective
+  double scale, aspect, fov, intensity;
+  const double *cs, *sn;
+};
+hipError_t launch_ssao(const SsaoParams &p, const float *z, const uint16_t *opacity, const uint8_t *const colour[3],
+                       uint16_t *image16, hipStream_t s);
+
 }  // namespace mbr

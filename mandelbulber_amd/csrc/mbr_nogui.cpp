@@ -191,6 +191,23 @@ int main(int argc, char **argv) {
   std::vector<uint16_t> img16(3 * n);
   const float *planes[3] = {r.data(), g.data(), b.data()};
   rc = mbr_compile_image(&s, w, h, planes, img16.data());
+  // post effects in the reference's order (src/Render3D.cpp:1094-1105)
+  if (rc == MBR_OK && s.ssao_enabled) {
+    mbr_post_planes pp;
+    std::memset(&pp, 0, sizeof pp);
+    pp.width = w;
+    pp.height = h;
+    pp.z = z.data();
+    pp.opacity = opacity.data();
+    pp.colour[0] = cr.data();
+    pp.colour[1] = cg.data();
+    pp.colour[2] = cb.data();
+    pp.image16 = img16.data();
+    pp.alpha = alpha.data();
+    rc = mbr_post_ssao(ctx, &s, &pp, 0, nullptr);
+    if (rc != MBR_OK) std::printf("SSAO failed: %s\n", mbr_last_error(ctx));
+    else std::printf("SSAO rendered (quality %d)\n", s.ssao_quality);
+  }
   if (out.empty()) {
     char idx[16];
     std::snprintf(idx, sizeof idx, "%05d", start_frame);

@@ -94,7 +94,8 @@ std::vector<KeyDef> build_table() {
   I("post_fog_enabled", OFF(post_fog_enabled)); D("post_fog_visibility", OFF(fog_visibility));
   G("post_fog_visibility_front");
   RGB("post_fog_color_", OFF_RGB(post_fog_color, R), OFF_RGB(post_fog_color, G), OFF_RGB(post_fog_color, B));
-  G("post_SSAO_enabled"); G("post_SSAO_quality"); G("post_DOF_enabled"); G("post_DOF_focus"); G("post_DOF_radius");
+  I("post_SSAO_enabled", OFF(ssao_enabled)); I("post_SSAO_quality", OFF(ssao_quality));
+  I("post_DOF_enabled", OFF(dof_enabled)); D("post_DOF_focus", OFF(dof_focus)); D("post_DOF_radius", OFF(dof_radius));
   D("main_light_intensity", OFF(main_light_intensity));
   D("main_light_alfa", OFF(main_light_alpha)); D("main_light_beta", OFF(main_light_beta));
   RGB("main_light_colour_", OFF_RGB(main_light_colour, R), OFF_RGB(main_light_colour, G), OFF_RGB(main_light_colour, B));
@@ -374,6 +375,11 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->image_gamma = 1.0;
   s->hdr = 0;
   std::strncpy(s->file_destination, "images/image", sizeof(s->file_destination) - 1);
+  s->ssao_enabled = 1;               // share/defaults: post_SSAO_enabled 1, quality 20
+  s->ssao_quality = 20;
+  s->dof_enabled = 0;
+  s->dof_focus = 21.699999999999999;
+  s->dof_radius = 10.0;
   return MBR_OK;
 }
 

@@ -134,6 +134,23 @@ class Renderer:
                                 C.byref(st)), self._ctx)
         return list(out), st
 
+    def post_ssao(self, scene, z, opacity, colour, image16, stream=None):
+        """PostRendering_SSAO on one frame.  numpy planes: host call, image16
+        (H, W, 3) uint16 is updated in place and returned; torch CUDA tensors:
+        queued on `stream` (default: torch's current stream)."""
+        h, w = z.shape
+        on_device = hasattr(z, "data_ptr")
+        for a in (z, opacity, image16) + tuple(colour[i] for i in range(3)):
+            assert (a.is_contiguous() if on_device else a.flags["C_CONTIGUOUS"])
+        assert tuple(image16.shape) == (h, w, 3) and tuple(opacity.shape) == (h, w)
+        if on_device and stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        p = post_planes(w, h, z, opacity, colour, image16)
+        check(lib().mbr_post_ssao(self._ctx, C.byref(scene), C.byref(p), 1 if on_device else 0,
+                                  stream if on_device else None), self._ctx)
+        return image16
+
     def render_device(self, planes, precision=_abi.PREC_FP32_FIXUP, rows=None, stream=None):
         """Render into device planes (see device_framebuffer); returns Stats."""
         s = self.scene
@@ -142,6 +159,22 @@ class Renderer:
         check(lib().mbr_render_device(self._ctx, C.byref(self._opts(precision, rows, stream)), C.byref(fb),
                                       C.byref(st)), self._ctx)
         return st
+
+
+def post_planes(w, h, z, opacity, colour, image16, alpha=None):
+    """mbr_post_planes over numpy arrays (host) or torch tensors (device):
+    z (H, W) float32, opacity (H, W) uint16/int16, colour (3, H, W) uint8,
+    image16 (H, W, 3) uint16/int16, alpha (H, W) or None."""
+    def ptr(a):
+        if a is None:
+            return None
+        return a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+    p = _abi.PostPlanes()
+    p.width, p.height = w, h
+    p.z, p.opacity, p.image16, p.alpha = ptr(z), ptr(opacity), ptr(image16), ptr(alpha)
+    for i in range(3):
+        p.colour[i] = ptr(colour[i])
+    return p
 
 
 def compile_image(scene, rgb):

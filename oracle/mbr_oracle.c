@@ -1215,3 +1215,116 @@ int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin
   free(dirs); free(rgb);
   return MBR_OK;
 }
+
+/* ---------------------------------------------------------------- SSAO */
+
+/* Screen position of pixel (x, y) at depth z in the plane units of the
+ * SSAO horizon test, for the three projections (src/image.cpp:288-314 for
+ * the centre pixel, :337-362 for the samples).  The sample form keeps its
+ * own operation order (z2 * persp grouped; fish-eye tests the loop radius,
+ * not its own radius, before dividing). */
+static void ssao_centre(int type, double fov, double aspect, int width, int height, int x, int y, double z,
+                        double *px, double *py) {
+  double x2, y2;
+  if (type == MBR_PERSP_FISH_EYE) {
+    x2 = M_PI * ((double)x / width - 0.5) * aspect;
+    y2 = M_PI * ((double)y / height - 0.5);
+    double r = sqrt(x2 * x2 + y2 * y2);
+    if (r != 0.0) {
+      x2 = x2 / r * sin(r * fov) * z;
+      y2 = y2 / r * sin(r * fov) * z;
+    }
+  } else if (type == MBR_PERSP_EQUIRECTANGULAR) {
+    x2 = M_PI * ((double)x / width - 0.5) * aspect;
+    y2 = M_PI * ((double)y / height - 0.5);
+    x2 = sin(fov * x2) * cos(fov * y2) * z;
+    y2 = sin(fov * y2) * z;
+  } else {
+    x2 = ((double)x / width - 0.5) * aspect;
+    y2 = ((double)y / height - 0.5);
+    x2 = x2 * z * fov;
+    y2 = y2 * z * fov;
+  }
+  *px = x2;
+  *py = y2;
+}
+
+static void ssao_sample(int type, double fov, double aspect, int width, int height, double xx, double yy,
+                        double z2, double *px, double *py) {
+  double xx2, yy2;
+  if (type == MBR_PERSP_FISH_EYE) {
+    xx2 = M_PI * ((double)xx / width - 0.5) * aspect;
+    yy2 = M_PI * ((double)yy / height - 0.5);
+    double r2 = sqrt(xx2 * xx2 + yy2 * yy2);
+    xx2 = xx2 / r2 * sin(r2 * fov) * z2;  /* loop radius >= 1: the guard always passes */
+    yy2 = yy2 / r2 * sin(r2 * fov) * z2;
+  } else if (type == MBR_PERSP_EQUIRECTANGULAR) {
+    xx2 = M_PI * (xx / width - 0.5) * aspect;
+    yy2 = M_PI * (yy / height - 0.5);
+    xx2 = sin(fov * xx2) * cos(fov * yy2) * z2;
+    yy2 = sin(fov * yy2) * z2;
+  } else {
+    xx2 = (xx / width - 0.5) * aspect;
+    yy2 = (yy / height - 0.5);
+    xx2 = xx2 * (z2 * fov);
+    yy2 = yy2 * (z2 * fov);
+  }
+  *px = xx2;
+  *py = yy2;
+}
+
+int oracle_ssao(const mbr_scene *s, int width, int height, const float *zb, const uint16_t *opacity,
+                const uint8_t *const colour[3], uint16_t *image16) {
+  const int q = s->ssao_quality;
+  if (q < 1 || width <= 0 || height <= 0) return MBR_ERR_INVALID_ARG;
+  double *sn = (double *)malloc(sizeof(double) * q), *cs = (double *)malloc(sizeof(double) * q);
+  for (int i = 0; i < q; i++) {  /* image.cpp:266-271 */
+    sn[i] = sin((double)i / q * 2.0 * M_PI);
+    cs[i] = cos((double)i / q * 2.0 * M_PI);
+  }
+  const double scale = (double)width / (q * q) / 2.0;  /* image.cpp:273 */
+  const double aspect = (double)width / height;
+  const double fov = s->persp;
+  const double intensity = s->global_ilum;
+  const int type = s->perspective_type;
+  for (int y = 0; y < height; y++) {
+    for (int x = 0; x < width; x++) {
+      const size_t p = (size_t)y * width + x;
+      const double z = zb[p];
+      const double op = opacity[p] / 65535.0;
+      double total = 0;
+      if (z < 1e19) {
+        double x2, y2;
+        ssao_centre(type, fov, aspect, width, height, x, y, z, &x2, &y2);
+        double ambient = 0;
+        for (int a = 0; a < q; a++) {
+          double best = -1e50;  /* largest -dz/dr along the direction */
+          for (double r = 1.0; r < q; r += 1.0) {
+            const double rr = r * r * scale;
+            const double xx = x + rr * cs[a], yy = y + rr * sn[a];
+            if ((int)xx == x && (int)yy == y) continue;
+            if (xx < 0 || xx > width - 1 || yy < 0 || yy > height - 1) continue;
+            const double z2 = zb[(size_t)(int)yy * width + (int)xx];
+            double xx2, yy2;
+            ssao_sample(type, fov, aspect, width, height, xx, yy, z2, &xx2, &yy2);
+            const double dx = xx2 - x2, dy = yy2 - y2, dz = z2 - z;
+            const double dr = sqrt(dx * dx + dy * dy);
+            const double diff = -dz / dr;
+            if (diff > best) best = diff;
+          }
+          ambient += -atan(best) / M_PI + 0.5;
+        }
+        total = ambient / q;
+        if (total < 0) total = 0;
+      }
+      for (int c = 0; c < 3; c++) {  /* image.cpp:375-383 */
+        double v = image16[3 * p + c] + 65535.0 * colour[c][p] / 256.0 * total * intensity * (1.0 - op);
+        if (v > 65535) v = 65535;
+        image16[3 * p + c] = (uint16_t)v;
+      }
+    }
+  }
+  free(sn);
+  free(cs);
+  return MBR_OK;
+}

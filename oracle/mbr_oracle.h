@@ -101,6 +101,13 @@ int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin
                   int nthreads, int jitter, uint32_t glibc_seed, mbr_framebuffer *fb,
                   oracle_stats *st);
 
+/* Screen-space ambient occlusion (PostRendering_SSAO / ThreadSSAO,
+ * src/image.cpp:249-415) on host planes with the scene's ssao_quality,
+ * persp, perspective_type and global_ilum; image16 (interleaved RGB16) in
+ * place. */
+int oracle_ssao(const mbr_scene *s, int width, int height, const float *z, const uint16_t *opacity,
+                const uint8_t *const colour[3], uint16_t *image16);
+
 #ifdef __cplusplus
 }
 #endif

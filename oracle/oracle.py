@@ -59,6 +59,8 @@ def _load(name):
     lib.oracle_object_shader.argtypes = [S, D3, D3, C.c_double, C.c_double, C.c_double, C.c_int,
                                          P(ShadeOut)]
     lib.oracle_object_shader.restype = C.c_int
+    lib.oracle_ssao.argtypes = [S, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p * 3, C.c_void_p]
+    lib.oracle_ssao.restype = C.c_int
     return lib
 
 
@@ -200,3 +202,19 @@ def render(scene, rows=None, threads=0, jitter=None, glibc_seed=1234, fast=False
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
     return frame, st
+
+
+def ssao(scene, z, opacity, colour, image16):
+    """PostRendering_SSAO of the oracle on numpy planes: z (h, w) float32,
+    opacity (h, w) uint16, colour (3, h, w) uint8, image16 (h, w, 3) uint16.
+    Returns the new image16 (the input is not changed)."""
+    h, w = z.shape
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    op = np.ascontiguousarray(opacity, dtype=np.uint16)
+    col = [np.ascontiguousarray(colour[i], dtype=np.uint8) for i in range(3)]
+    out = np.array(image16, dtype=np.uint16, order="C", copy=True)
+    cp = (C.c_void_p * 3)(*[c.ctypes.data for c in col])
+    rc = lib().oracle_ssao(C.byref(scene), w, h, z.ctypes.data, op.ctypes.data, cp, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_ssao failed: {rc}")
+    return out

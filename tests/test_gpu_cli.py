@@ -54,9 +54,12 @@ def test_cli_renders_settings_file(tmp_path):
     try:
         rr.set_scene(s)
         frame, _ = rr.render(precision=_abi.PREC_FP32_FIXUP)
+        expect = compile_image(s, frame.rgb)
+        # post_SSAO_enabled is on in the defaults: the driver adds it after CompileImage
+        assert s.ssao_enabled == 1 and "SSAO rendered" in r.stdout
+        expect = rr.post_ssao(s, frame.z, frame.opacity, frame.colour, expect)
     finally:
         rr.close()
-    expect = compile_image(s, frame.rgb)
     assert np.array_equal(px.astype(np.uint16), expect)
     assert s.brightness == 1.3 and s.contrast == 1.2 and abs(s.image_gamma - 1.8) < 1e-15
 

@@ -1,0 +1,73 @@
+"""Post effects on the GPU against the oracle: screen-space ambient
+occlusion (PostRendering_SSAO, src/image.cpp:249-415) of a rendered,
+compiled frame, for the three projections and two qualities, through the
+host-plane and the device-plane calls of mbr_post_ssao.  The arithmetic is
+FP64 in the reference's order on both sides, so the 16-bit results must be
+equal; the only allowed difference is one unit on at most 1e-4 of the
+values (a 1-ulp difference of the device atan/sin/cos at a truncation
+boundary)."""
+import numpy as np
+import pytest
+
+from mandelbulber_amd import _abi, scene
+from mandelbulber_amd.render import compile_image
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    from mandelbulber_amd import Renderer
+    r = Renderer(0)
+    yield r
+    r.close()
+
+
+def _check(label, got, want):
+    d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+    n_diff = int(np.count_nonzero(d))
+    print(f"[ssao] {label}: max diff {int(d.max())}, {n_diff} of {d.size} values differ", flush=True)
+    assert d.max() <= 1 and n_diff <= 1e-4 * d.size, label
+
+
+@pytest.mark.parametrize("ptype,quality", [(0, 20), (0, 7), (1, 12), (2, 12)])
+def test_ssao_matches_oracle(renderer, ptype, quality):
+    import torch
+    from oracle import oracle
+    s = scene.mandelbulb_full(160, 120, seed=2)
+    s.perspective_type = ptype
+    if ptype != 0:
+        s.persp = 0.3   # narrower angle: the bulb fills about a quarter of the frame
+    s.ssao_quality = quality
+    s.global_ilum = 1.3
+    print(f"[ssao] perspective {ptype} quality {quality}: rendering", flush=True)
+    renderer.set_scene(s)
+    frame, _ = renderer.render(precision=_abi.PREC_FP64)
+    img = compile_image(s, frame.rgb)
+    hits = int(np.count_nonzero(frame.z < 1e19))
+    assert hits > 1000
+    want = oracle.ssao(s, frame.z, frame.opacity, frame.colour, img)
+    assert np.count_nonzero(want != img) > hits // 2   # the effect changed most hit pixels
+    got = renderer.post_ssao(s, frame.z, frame.opacity, frame.colour, img.copy())
+    _check(f"host planes p{ptype} q{quality}", got, want)
+    dev = torch.device("cuda:0")
+    z = torch.from_numpy(frame.z).to(dev)
+    op = torch.from_numpy(frame.opacity.view(np.int16)).to(dev)
+    col = torch.from_numpy(frame.colour).to(dev)
+    im = torch.from_numpy(img.view(np.int16)).to(dev)
+    renderer.post_ssao(s, z, op, [col[0], col[1], col[2]], im)
+    torch.cuda.synchronize()
+    _check(f"device planes p{ptype} q{quality}", im.cpu().numpy().view(np.uint16), want)
+
+
+def test_ssao_rejects_bad_quality(renderer):
+    from mandelbulber_amd._lib import MbrError
+    s = scene.mandelbulb_full(32, 24, seed=1)
+    s.ssao_quality = 0
+    z = np.ones((24, 32), np.float32)
+    with pytest.raises(MbrError):
+        renderer.post_ssao(s, z, np.zeros((24, 32), np.uint16), np.zeros((3, 24, 32), np.uint8),
+                           np.zeros((24, 32, 3), np.uint16))
