@@ -428,6 +428,19 @@ typedef struct mbr_post_planes {
 int mbr_post_ssao(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *planes,
                   int on_device, void *stream);
 
+/* Depth of field (PostRendering_DOF, src/image.cpp:57-247; called from
+ * src/Render3D.cpp:1101-1105) with focus 10^(dof_focus / 10 - 16) and
+ * radius dof_radius x width / 1000: every pixel is splatted as a disc of
+ * radius |z - focus| / z x radius (at most 100 pixels) onto image16 and
+ * alpha (may be NULL), in the reference's order (z sorted, then shuffled
+ * among pixels of similar blur with glibc-rand draws after
+ * srand(jitter_seed)).  The order is computed on the host (the device z
+ * plane is read back first); the splats run on the GPU and give the
+ * reference's result exactly.  on_device / stream as for mbr_post_ssao;
+ * the call waits for the device once (the size of the splat lists). */
+int mbr_post_dof(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *planes,
+                 int on_device, void *stream);
+
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,
  * reflections_max, jitter_seed (in that order). */

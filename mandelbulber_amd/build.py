@@ -15,7 +15,8 @@ OUT = os.path.join(HERE, "libmbr.so")
 CLI = os.path.join(HERE, "mbr_nogui")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_post.hip", "mbr_settings.cpp", "mbr_image.cpp"]
+SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_post.hip", "mbr_post_order.cpp",
+           "mbr_settings.cpp", "mbr_image.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

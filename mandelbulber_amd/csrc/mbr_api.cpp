@@ -290,6 +290,11 @@ struct mbr_ctx {
   int ssao_tab_q = 0;
   void *d_post = nullptr;
   size_t d_post_bytes = 0;
+  // DOF: per-pixel buffers and the (tile, splat) lists
+  void *d_dof = nullptr;
+  size_t d_dof_bytes = 0;
+  void *d_dof_pairs = nullptr;
+  size_t d_dof_pairs_bytes = 0;
 
   int fail(int rc, const std::string &msg) {
     err = msg;
@@ -507,7 +512,7 @@ void mbr_destroy(mbr_ctx *c) {
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
   void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad, c->d_ssao_tab,
-                c->d_post};
+                c->d_post, c->d_dof, c->d_dof_pairs};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
@@ -909,6 +914,113 @@ int mbr_post_ssao(mbr_ctx *c, const mbr_scene *s, const mbr_post_planes *pl, int
   HIPCHECK(c, launch_ssao(p, dz, dop, dcol, dimg, st));
   HIPCHECK(c, hipMemcpyAsync(pl->image16, dimg, n * 6, hipMemcpyDeviceToHost, st));
   HIPCHECK(c, hipStreamSynchronize(st));
+  return MBR_OK;
+}
+
+int mbr_post_dof(mbr_ctx *c, const mbr_scene *s, const mbr_post_planes *pl, int on_device, void *stream) {
+  if (!c || !s || !pl) return MBR_ERR_INVALID_ARG;
+  const int w = pl->width, h = pl->height;
+  if (w <= 0 || h <= 0 || !pl->z || !pl->image16) return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_dof: planes missing");
+  if ((long long)w * h > (1ll << 28)) return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_dof: image too large");
+  HIPCHECK(c, hipSetDevice(c->device));
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  const int n = w * h;
+  DofParams p;
+  p.width = w;
+  p.height = h;
+  p.tiles_x = (w + 15) / 16;
+  p.tiles_y = (h + 15) / 16;
+  p.rank_bits = 1;
+  while ((1ll << p.rank_bits) < n) p.rank_bits++;
+  int tile_bits = 1;
+  while ((1ll << tile_bits) < (long long)p.tiles_x * p.tiles_y) tile_bits++;
+  p.deep = s->dof_radius * w / 1000.0;  // src/Render3D.cpp:1103-1104
+  p.neutral = pow(10, s->dof_focus / 10.0 - 16.0);
+  const int tiles = p.tiles_x * p.tiles_y;
+
+  // splat order on the host (sequential by definition: sort + shuffle)
+  std::vector<float> hz;
+  const float *zh = pl->z;
+  if (on_device) {
+    hz.resize(n);
+    HIPCHECK(c, hipMemcpyAsync(hz.data(), pl->z, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    zh = hz.data();
+  }
+  std::vector<uint32_t> rank(n);
+  dof_order(zh, w, h, p.neutral, p.deep, s->jitter_seed, rank.data());
+
+  // per-pixel device buffers: rank, tile counts, offsets, segment starts,
+  // copies of the input image and alpha (and z, output for host planes)
+  size_t scan_tmp = 0;
+  HIPCHECK(c, dof_scan(nullptr, scan_tmp, nullptr, nullptr, n, st));
+  const size_t bytes = (size_t)n * (4 + 8 + 8 + 4 + 6 + 2 + 6 + 2) + ((size_t)tiles + 1) * 8 + scan_tmp + 64 * 12;
+  if (bytes > c->d_dof_bytes) {
+    if (c->d_dof) HIPCHECK(c, hipFree(c->d_dof));
+    c->d_dof = nullptr;
+    c->d_dof_bytes = 0;
+    HIPCHECK(c, hipMalloc(&c->d_dof, bytes));
+    c->d_dof_bytes = bytes;
+  }
+  char *base = (char *)c->d_dof;
+  size_t off = 0;
+  auto take = [&](size_t sz) { void *ptr = base + off; off += (sz + 63) / 64 * 64; return ptr; };
+  uint32_t *d_rank = (uint32_t *)take((size_t)n * 4);
+  unsigned long long *d_cnt = (unsigned long long *)take((size_t)n * 8);
+  unsigned long long *d_off = (unsigned long long *)take((size_t)n * 8);
+  float *d_z = (float *)take((size_t)n * 4);
+  uint16_t *d_img_in = (uint16_t *)take((size_t)n * 6);
+  uint16_t *d_alpha_in = (uint16_t *)take((size_t)n * 2);
+  uint16_t *d_img_out = (uint16_t *)take((size_t)n * 6);
+  uint16_t *d_alpha_out = (uint16_t *)take((size_t)n * 2);
+  unsigned long long *d_seg = (unsigned long long *)take(((size_t)tiles + 1) * 8);
+  void *d_scan_tmp = take(scan_tmp);
+  const hipMemcpyKind in_kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const float *z = on_device ? pl->z : d_z;
+  HIPCHECK(c, hipMemcpyAsync(d_rank, rank.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+  if (!on_device) HIPCHECK(c, hipMemcpyAsync(d_z, pl->z, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  HIPCHECK(c, hipMemcpyAsync(d_img_in, pl->image16, (size_t)n * 6, in_kind, st));
+  if (pl->alpha) HIPCHECK(c, hipMemcpyAsync(d_alpha_in, pl->alpha, (size_t)n * 2, in_kind, st));
+  HIPCHECK(c, dof_count(p, z, d_cnt, st));
+  HIPCHECK(c, dof_scan(d_scan_tmp, scan_tmp, d_cnt, d_off, n, st));
+  unsigned long long last[2] = {0, 0};
+  HIPCHECK(c, hipMemcpyAsync(&last[0], d_off + n - 1, 8, hipMemcpyDeviceToHost, st));
+  HIPCHECK(c, hipMemcpyAsync(&last[1], d_cnt + n - 1, 8, hipMemcpyDeviceToHost, st));
+  HIPCHECK(c, hipStreamSynchronize(st));
+  const unsigned long long pairs = last[0] + last[1];
+  if (pairs > 0x7fffffffull) return c->fail(MBR_ERR_INVALID_ARG, "mbr_post_dof: too many splat-tile pairs");
+  uint16_t *img_out = on_device ? pl->image16 : d_img_out;
+  uint16_t *alpha_out = pl->alpha ? (on_device ? pl->alpha : d_alpha_out) : nullptr;
+  const uint16_t *alpha_in = pl->alpha ? d_alpha_in : nullptr;
+  // (tile, rank) keys and source values, double-buffered for the sort
+  const int np = (int)pairs;
+  size_t sort_tmp = 0;
+  HIPCHECK(c, dof_sort(nullptr, sort_tmp, nullptr, nullptr, nullptr, nullptr, np, p.rank_bits + tile_bits, st));
+  const size_t pbytes = (size_t)np * (8 + 8 + 4 + 4) + sort_tmp + 64 * 6;
+  if (pbytes > c->d_dof_pairs_bytes) {
+    if (c->d_dof_pairs) HIPCHECK(c, hipFree(c->d_dof_pairs));
+    c->d_dof_pairs = nullptr;
+    c->d_dof_pairs_bytes = 0;
+    HIPCHECK(c, hipMalloc(&c->d_dof_pairs, pbytes));
+    c->d_dof_pairs_bytes = pbytes;
+  }
+  base = (char *)c->d_dof_pairs;
+  off = 0;
+  unsigned long long *k0 = (unsigned long long *)take((size_t)np * 8);
+  unsigned long long *k1 = (unsigned long long *)take((size_t)np * 8);
+  uint32_t *v0 = (uint32_t *)take((size_t)np * 4);
+  uint32_t *v1 = (uint32_t *)take((size_t)np * 4);
+  void *d_sort_tmp = take(sort_tmp);
+  if (np > 0) {
+    HIPCHECK(c, dof_fill(p, z, d_rank, d_off, k0, v0, st));
+    HIPCHECK(c, dof_sort(d_sort_tmp, sort_tmp, k0, k1, v0, v1, np, p.rank_bits + tile_bits, st));
+  }
+  HIPCHECK(c, dof_fold(p, z, k1, np, d_seg, v1, d_img_in, alpha_in, img_out, alpha_out, st));
+  if (!on_device) {
+    HIPCHECK(c, hipMemcpyAsync(pl->image16, d_img_out, (size_t)n * 6, hipMemcpyDeviceToHost, st));
+    if (pl->alpha) HIPCHECK(c, hipMemcpyAsync(pl->alpha, d_alpha_out, (size_t)n * 2, hipMemcpyDeviceToHost, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+  }
   return MBR_OK;
 }
 

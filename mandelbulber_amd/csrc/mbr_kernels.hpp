@@ -127,4 +127,24 @@ struct SsaoParams {
 hipError_t launch_ssao(const SsaoParams &p, const float *z, const uint16_t *opacity, const uint8_t *const colour[3],
                        uint16_t *image16, hipStream_t s);
 
+// Depth of field (mbr_post.hip, mbr_post_order.cpp).  tiles_x/tiles_y:
+// 16x16 pixel tiles; rank_bits: bits of a splat rank in the sort key.
+struct DofParams {
+  int width, height, tiles_x, tiles_y, rank_bits;
+  double neutral, deep;
+};
+// host: rank[pixel] = its position in the splat order
+void dof_order(const float *z, int width, int height, double neutral, double deep, uint32_t seed, uint32_t *rank);
+hipError_t dof_count(const DofParams &p, const float *z, unsigned long long *cnt, hipStream_t s);
+hipError_t dof_scan(void *tmp, size_t &tmp_bytes, const unsigned long long *cnt, unsigned long long *off, int n,
+                    hipStream_t s);
+hipError_t dof_fill(const DofParams &p, const float *z, const uint32_t *rank, const unsigned long long *off,
+                    unsigned long long *keys, uint32_t *vals, hipStream_t s);
+hipError_t dof_sort(void *tmp, size_t &tmp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                    const uint32_t *vin, uint32_t *vout, int n, int bits, hipStream_t s);
+// seg: tiles + 1 entries (filled here); keys/vals: the sorted lists
+hipError_t dof_fold(const DofParams &p, const float *z, const unsigned long long *keys, long long n,
+                    unsigned long long *seg, const uint32_t *vals, const uint16_t *img_in, const uint16_t *alpha_in,
+                    uint16_t *img_out, uint16_t *alpha_out, hipStream_t s);
+
 }  // namespace mbr

@@ -208,6 +208,18 @@ int main(int argc, char **argv) {
     if (rc != MBR_OK) std::printf("SSAO failed: %s\n", mbr_last_error(ctx));
     else std::printf("SSAO rendered (quality %d)\n", s.ssao_quality);
   }
+  if (rc == MBR_OK && s.dof_enabled) {
+    mbr_post_planes pp;
+    std::memset(&pp, 0, sizeof pp);
+    pp.width = w;
+    pp.height = h;
+    pp.z = z.data();
+    pp.image16 = img16.data();
+    pp.alpha = alpha.data();
+    rc = mbr_post_dof(ctx, &s, &pp, 0, nullptr);
+    if (rc != MBR_OK) std::printf("DOF failed: %s\n", mbr_last_error(ctx));
+    else std::printf("DOF rendered (focus %g, radius %g)\n", s.dof_focus, s.dof_radius);
+  }
   if (out.empty()) {
     char idx[16];
     std::snprintf(idx, sizeof idx, "%05d", start_frame);

@@ -151,6 +151,20 @@ class Renderer:
                                   stream if on_device else None), self._ctx)
         return image16
 
+    def post_dof(self, scene, z, image16, alpha=None, stream=None):
+        """PostRendering_DOF on one frame (numpy: host call, image16 and alpha
+        updated in place; torch CUDA tensors: queued on `stream`)."""
+        h, w = z.shape
+        on_device = hasattr(z, "data_ptr")
+        assert tuple(image16.shape) == (h, w, 3)
+        if on_device and stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        p = post_planes(w, h, z, None, [None, None, None], image16, alpha)
+        check(lib().mbr_post_dof(self._ctx, C.byref(scene), C.byref(p), 1 if on_device else 0,
+                                 stream if on_device else None), self._ctx)
+        return image16, alpha
+
     def render_device(self, planes, precision=_abi.PREC_FP32_FIXUP, rows=None, stream=None):
         """Render into device planes (see device_framebuffer); returns Stats."""
         s = self.scene

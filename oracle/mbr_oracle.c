@@ -1328,3 +1328,102 @@ int oracle_ssao(const mbr_scene *s, int width, int height, const float *zb, cons
   free(cs);
   return MBR_OK;
 }
+
+/* ----------------------------------------------------------------- DOF */
+
+typedef struct { float z; int i; } zsort_t;
+
+/* QuickSortZBuffer (src/common_math.h:53-84): Hoare partition around the
+ * middle element, ascending z; recursion on the two sides. */
+static void zsort(zsort_t *d, int l, int p) {
+  int i = l, j = p;
+  const zsort_t x = d[(l + p) / 2];
+  do {
+    while (d[i].z < x.z) i++;
+    while (x.z < d[j].z) j--;
+    if (i <= j) {
+      zsort_t t = d[i]; d[i] = d[j]; d[j] = t;
+      i++; j--;
+    }
+  } while (i <= j);
+  if (l < j) zsort(d, l, j);
+  if (i < p) zsort(d, i, p);
+}
+
+int oracle_dof_order(const mbr_scene *s, int width, int height, const float *zb, uint32_t seed, int *order) {
+  const int n = width * height;
+  if (n <= 0) return MBR_ERR_INVALID_ARG;
+  const double deep = s->dof_radius * width / 1000.0;               /* Render3D.cpp:1103-1104 */
+  const double neutral = pow(10, s->dof_focus / 10.0 - 16.0);
+  zsort_t *d = (zsort_t *)malloc(sizeof(zsort_t) * n);
+  for (int k = 0; k < n; k++) { d[k].z = zb[k]; d[k].i = k; }
+  if (n > 1) zsort(d, 1, n - 1);                                     /* image.cpp:86: from index 1 */
+  srand(seed);
+  for (int i = n - 1; i >= 0; i--) {                                 /* image.cpp:96-152 */
+    const zsort_t t = d[i];
+    const double z1 = t.z;
+    const double size1 = (z1 - neutral) / z1 * deep;
+    int step = i, ii, done;
+    do {
+      ii = i - (int)(rand() % (step + 1));
+      if (ii <= 0) ii = 0;
+      const double z2 = d[ii].z;
+      const double size2 = (z2 - neutral) / z2 * deep;
+      if (size1 * size2 > 0) {
+        const double cmp = size1 > 0 ? size2 / size1 : size1 / size2;
+        done = cmp > 0.7;
+      } else {
+        done = 0;  /* also when both sizes are 0: the assignment above it is overwritten */
+      }
+      step = (int)(step * 0.7 - 1);
+      if (step <= 0) done = 1;
+    } while (!done);
+    d[i] = d[ii];
+    d[ii] = t;
+  }
+  for (int k = 0; k < n; k++) order[k] = d[n - 1 - k].i;            /* processed from the end */
+  free(d);
+  return MBR_OK;
+}
+
+int oracle_dof(const mbr_scene *s, int width, int height, const float *zb, uint32_t seed, uint16_t *image16,
+               uint16_t *alpha) {
+  const int n = width * height;
+  if (n <= 0) return MBR_ERR_INVALID_ARG;
+  int *order = (int *)malloc(sizeof(int) * n);
+  int rc = oracle_dof_order(s, width, height, zb, seed, order);
+  if (rc != MBR_OK) { free(order); return rc; }
+  const double deep = s->dof_radius * width / 1000.0;
+  const double neutral = pow(10, s->dof_focus / 10.0 - 16.0);
+  uint16_t *img0 = (uint16_t *)malloc(sizeof(uint16_t) * 3 * (size_t)n);
+  uint16_t *a0 = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)n);
+  memcpy(img0, image16, sizeof(uint16_t) * 3 * (size_t)n);
+  if (alpha) memcpy(a0, alpha, sizeof(uint16_t) * (size_t)n);
+  for (int k = 0; k < n; k++) {                                      /* image.cpp:165-205 */
+    const int src = order[k];
+    const int x = src % width, y = src / width;
+    const double z = zb[src];
+    double blur = fabs(z - neutral) / z * deep;
+    if (blur > 100) blur = 100.0;
+    const int size = (int)blur;
+    const double factor = blur * blur * sqrt(blur) * M_PI / 3.0;
+    for (int yy = y - size; yy <= y + size; yy++) {
+      for (int xx = x - size; xx <= x + size; xx++) {
+        if (xx < 0 || xx >= width || yy < 0 || yy >= height) continue;
+        const int dx = xx - x, dy = yy - y;
+        const double r = sqrtf((float)dx * dx + dy * dy);           /* the float overload in C++ */
+        double op = (blur - r) / factor;
+        if (op > 1.0) op = 1.0;
+        if (op < 0.0) op = 0.0;
+        if (op > 0.0) {
+          const double opN = 1.0 - op;
+          const size_t q = (size_t)yy * width + xx;
+          for (int c = 0; c < 3; c++) image16[3 * q + c] = (uint16_t)(image16[3 * q + c] * opN + img0[3 * (size_t)src + c] * op);
+          if (alpha) alpha[q] = (uint16_t)(alpha[q] * opN + a0[src] * op);
+        }
+      }
+    }
+  }
+  free(order); free(img0); free(a0);
+  return MBR_OK;
+}

@@ -108,6 +108,16 @@ int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin
 int oracle_ssao(const mbr_scene *s, int width, int height, const float *z, const uint16_t *opacity,
                 const uint8_t *const colour[3], uint16_t *image16);
 
+/* Depth of field (PostRendering_DOF, src/image.cpp:57-247) with deep =
+ * dof_radius x width / 1000 and focus 10^(dof_focus / 10 - 16)
+ * (src/Render3D.cpp:1101-1105).  The z order and its randomisation draw
+ * glibc rand() after srand(seed).  oracle_dof_order returns the pixels in
+ * processing order; oracle_dof blurs image16 (interleaved RGB16) and alpha
+ * (may be NULL) in place. */
+int oracle_dof_order(const mbr_scene *s, int width, int height, const float *z, uint32_t seed, int *order);
+int oracle_dof(const mbr_scene *s, int width, int height, const float *z, uint32_t seed, uint16_t *image16,
+               uint16_t *alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,10 @@ def _load(name):
     lib.oracle_object_shader.restype = C.c_int
     lib.oracle_ssao.argtypes = [S, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p * 3, C.c_void_p]
     lib.oracle_ssao.restype = C.c_int
+    lib.oracle_dof_order.argtypes = [S, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
+    lib.oracle_dof_order.restype = C.c_int
+    lib.oracle_dof.argtypes = [S, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.oracle_dof.restype = C.c_int
     return lib
 
 
@@ -218,3 +222,28 @@ def ssao(scene, z, opacity, colour, image16):
     if rc != 0:
         raise RuntimeError(f"oracle_ssao failed: {rc}")
     return out
+
+
+def dof_order(scene, z, seed):
+    """Pixels in the DOF processing order (sorted z, randomised with glibc
+    rand after srand(seed))."""
+    h, w = z.shape
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    order = np.empty(h * w, np.int32)
+    rc = lib().oracle_dof_order(C.byref(scene), w, h, z.ctypes.data, seed, order.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_dof_order failed: {rc}")
+    return order
+
+
+def dof(scene, z, image16, alpha, seed):
+    """PostRendering_DOF of the oracle: returns new (image16, alpha)."""
+    h, w = z.shape
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    img = np.array(image16, dtype=np.uint16, order="C", copy=True)
+    a = None if alpha is None else np.array(alpha, dtype=np.uint16, order="C", copy=True)
+    rc = lib().oracle_dof(C.byref(scene), w, h, z.ctypes.data, seed, img.ctypes.data,
+                          None if a is None else a.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_dof failed: {rc}")
+    return img, a

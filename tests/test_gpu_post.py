@@ -29,7 +29,7 @@ def renderer():
 def _check(label, got, want):
     d = np.abs(got.astype(np.int32) - want.astype(np.int32))
     n_diff = int(np.count_nonzero(d))
-    print(f"[ssao] {label}: max diff {int(d.max())}, {n_diff} of {d.size} values differ", flush=True)
+    print(f"[post] {label}: max diff {int(d.max())}, {n_diff} of {d.size} values differ", flush=True)
     assert d.max() <= 1 and n_diff <= 1e-4 * d.size, label
 
 
@@ -71,3 +71,37 @@ def test_ssao_rejects_bad_quality(renderer):
     with pytest.raises(MbrError):
         renderer.post_ssao(s, z, np.zeros((24, 32), np.uint16), np.zeros((3, 24, 32), np.uint8),
                            np.zeros((24, 32, 3), np.uint16))
+
+
+@pytest.mark.parametrize("radius,focus_quantile", [(60.0, 0.5), (400.0, 0.2)])
+def test_dof_matches_oracle(renderer, radius, focus_quantile):
+    """Depth of field: the GPU gather (tile lists in splat order) against the
+    oracle's sequential splat loop, on a rendered frame focused inside the
+    object; the radius-400 case blurs the background by 64 pixels and the
+    near side up to the 100-pixel cap."""
+    import math
+    import torch
+    from oracle import oracle
+    s = scene.mandelbulb_full(160, 120, seed=2)
+    print(f"[dof] radius {radius}: rendering", flush=True)
+    renderer.set_scene(s)
+    frame, _ = renderer.render(precision=_abi.PREC_FP64)
+    hit_z = frame.z[frame.z < 1e19]
+    s.dof_enabled = 1
+    s.dof_focus = 10.0 * (16.0 + math.log10(float(np.quantile(hit_z, focus_quantile))))
+    s.dof_radius = radius
+    img = compile_image(s, frame.rgb)
+    alpha = frame.alpha.copy()
+    want_img, want_a = oracle.dof(s, frame.z, img, alpha, s.jitter_seed)
+    assert np.count_nonzero(np.any(want_img != img, axis=2)) > img.shape[0] * img.shape[1] // 4
+    got_img, got_a = renderer.post_dof(s, frame.z, img.copy(), alpha.copy())
+    _check(f"dof host r{radius} rgb", got_img, want_img)
+    _check(f"dof host r{radius} alpha", got_a, want_a)
+    dev = torch.device("cuda:0")
+    z = torch.from_numpy(frame.z).to(dev)
+    im = torch.from_numpy(img.view(np.int16)).to(dev)
+    a = torch.from_numpy(alpha.view(np.int16)).to(dev)
+    renderer.post_dof(s, z, im, a)
+    torch.cuda.synchronize()
+    _check(f"dof device r{radius} rgb", im.cpu().numpy().view(np.uint16), want_img)
+    _check(f"dof device r{radius} alpha", a.cpu().numpy().view(np.uint16), want_a)
