@@ -219,6 +219,65 @@ def bench_deep_zoom(torch, r, _abi, scene, compare, width, height):
     return out
 
 
+def bench_post(torch, r, _abi, sc, planes, with_cpu):
+    """Post effects on the 1080p frame: SSAO (quality from the scene, 20 by
+    default) and DOF (focus at the median hit depth, radius 10) on device
+    planes; with_cpu: the oracle's SSAO on the same frame, timed on one host
+    thread and compared with the GPU result."""
+    import math
+    import numpy as np
+    from mandelbulber_amd import scene as S
+    from mandelbulber_amd.render import compile_image
+    r.set_scene(sc)
+    r.render_device(planes, precision=_abi.PREC_FP32_FIXUP)
+    torch.cuda.synchronize()
+    h, w = planes["z"].shape
+    rgb = planes["rgb"].cpu().numpy()
+    img = compile_image(sc, rgb)
+    z_host = planes["z"].cpu().numpy()
+    col = planes["colour"]
+    img_dev = torch.from_numpy(img.view(np.int16)).to(planes["z"].device)
+    out = {"workload": f"{w}x{h} Mandelbulb frame of the main workload, compiled to 16 bits"}
+
+    def timed_call(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    work = img_dev.clone()
+    ssao_ms = timed_call(lambda: r.post_ssao(sc, planes["z"], planes["opacity"], [col[0], col[1], col[2]],
+                                             work.copy_(img_dev)), 3)
+    out["ssao"] = {"quality": sc.ssao_quality, "ms": ssao_ms,
+                   "Mpixels_per_s": w * h / ssao_ms / 1e3}
+    if with_cpu:
+        from oracle import oracle
+        op_h = planes["opacity"].cpu().numpy().view(np.uint16)
+        col_h = col.cpu().numpy()
+        t0 = time.perf_counter()
+        want = oracle.ssao(sc, z_host, op_h, col_h, img)
+        cpu_s = time.perf_counter() - t0
+        r.post_ssao(sc, planes["z"], planes["opacity"], [col[0], col[1], col[2]], work.copy_(img_dev))
+        torch.cuda.synchronize()
+        got = work.cpu().numpy().view(np.uint16)
+        out["ssao"]["cpu_oracle_ms"] = cpu_s * 1e3
+        out["ssao"]["cpu_oracle_threads"] = 1
+        out["ssao"]["equal_to_oracle"] = bool(np.array_equal(got, want))
+    hit_z = z_host[z_host < 1e19]
+    d = S.copy(sc)
+    d.dof_enabled = 1
+    d.dof_focus = 10.0 * (16.0 + math.log10(float(np.median(hit_z)))) if hit_z.size else 21.7
+    d.dof_radius = 10.0
+    alpha = planes["alpha"].clone()
+    dof_ms = timed_call(lambda: r.post_dof(d, planes["z"], work.copy_(img_dev), alpha.copy_(planes["alpha"])), 2)
+    out["dof"] = {"radius": d.dof_radius, "focus_depth": float(np.median(hit_z)) if hit_z.size else None,
+                  "ms": dof_ms, "note": "includes the host z sort and shuffle that fix the splat order"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +294,7 @@ def main():
     ap.add_argument("--flight-replays", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-deep-zoom", action="store_true")
+    ap.add_argument("--no-post", action="store_true")
     ap.add_argument("--adjoint-scenes", type=int, default=16)
     ap.add_argument("--adjoint-size", type=int, default=512)
     args = ap.parse_args()
@@ -288,6 +348,7 @@ def main():
     ao64_ms = sum(s.ms_ao for s in st64) / launches64
     ao64_flops = sum(algorithmic_flops(sc, s, "ao") for s in st64) / launches64
 
+    post = None if args.no_post else bench_post(torch, r, _abi, sc, planes, not args.no_cpu)
     flight = None if args.no_flight else bench_flight(torch, r, _abi, scene, args.flight_frames, args.flight_replays)
     adjoint = None if args.no_adjoint else bench_adjoint(torch, r, scene, args.adjoint_scenes, args.adjoint_size)
     deep = None if args.no_deep_zoom else bench_deep_zoom(torch, r, _abi, scene, compare, 3840, 2160)
@@ -329,6 +390,7 @@ def main():
         "flight": flight,
         "adjoint": adjoint,
         "deep_zoom": deep,
+        "post": post,
         "cpu_baseline": cpu,
     }
     r.close()
