@@ -44,9 +44,17 @@ enum mbr_status {
 /* ---- formula enum: values of enumFractalFormula (src/fractal.h:25-65) ---- */
 enum mbr_formula {
   MBR_FORMULA_NONE = 0,
+  MBR_FORMULA_TRIG_DE = 1,       /* Mandelbulb with the analytic derivative vector */
   MBR_FORMULA_TRIG_OPTIM = 2,    /* Mandelbulb, "trigonometric" in the GUI */
   MBR_FORMULA_MENGER_SPONGE = 7,
-  MBR_FORMULA_TGLAD = 8          /* Mandelbox */
+  MBR_FORMULA_TGLAD = 8,         /* Mandelbox */
+  MBR_FORMULA_XENODREAMBUIE = 11,
+  MBR_FORMULA_MANDELBULB2 = 13,
+  MBR_FORMULA_MANDELBULB3 = 14,
+  MBR_FORMULA_MANDELBULB4 = 15
+  /* 1, 11, 13, 14, 15 render through the generic FP64 path (delta DE for
+   * 11, 13, 14, 15, as the reference's analitycDE = false): every precision
+   * mode renders them in FP64 */
 };
 
 /* enumPerspectiveType (src/common_math.h) */
@@ -237,6 +245,7 @@ typedef struct mbr_scene {
 
   /* post effects on the compiled image (src/image.cpp; order in the
    * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
+  int linear_DE_mode;              /* linear_DE_mode (delta-DE formulas: 0.5 r / dr) */
   int ssao_enabled;                /* post_SSAO_enabled */
   int ssao_quality;                /* post_SSAO_quality */
   int dof_enabled;                 /* post_DOF_enabled */

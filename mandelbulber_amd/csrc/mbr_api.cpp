@@ -102,6 +102,7 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   f.frame_no = s.frame_no;
   f.power = (T)s.power;
   f.ipow = (s.power == std::floor(s.power) && s.power >= 2 && s.power <= 32) ? (int)s.power : 0;
+  f.linear_de = s.linear_DE_mode;
   f.bailout = (T)s.bailout;
   f.constant_factor = (T)s.constant_factor;
   for (int i = 0; i < 3; i++) f.julia[i] = (T)s.julia[i];
@@ -434,10 +435,10 @@ int band_cap(int precision, int ao_count) {
 
 int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
-  if (s.formula != MBR_FORMULA_TRIG_OPTIM && s.formula != MBR_FORMULA_MENGER_SPONGE && s.formula != MBR_FORMULA_TGLAD)
+  if (!formula_supported(s.formula))
     return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
-                                            " is not in the render core (supported: 2 trig_optim, 7 menger_sponge, "
-                                            "8 tglad)");
+                                            " is not in the render core (supported: 1 trig_DE, 2 trig_optim, "
+                                            "7 menger_sponge, 8 tglad, 11 xenodreambuie, 13-15 mandelbulb2-4)");
   if (s.ifs_folding_mode) return c->fail(MBR_ERR_UNSUPPORTED, "IFS folding mode is not in the render core");
   if (s.reflect > 0.0 && !s.raytraced_reflections)
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
@@ -783,9 +784,11 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   const mbr_scene &s = c->scene;
   if (fb->width != s.image_width || fb->height != s.image_height)
     return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
-  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
   int rc = check_precision(c, precision);
   if (rc != MBR_OK) return rc;
+  // formulas without kernels of their own (delta DE, trig_DE) render in FP64
+  if (formula_generic(s.formula)) precision = MBR_PREC_FP64;
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
   if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
   if (row0 < 0) row0 = 0;
@@ -1161,9 +1164,12 @@ extern "C" {
 int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_texture *lightmap, mbr_framebuffer *fbs,
                      const mbr_batch_opts *opts, mbr_stats *stats, mbr_batch_result *result) {
   if (!c || !scenes || !fbs || n <= 0) return MBR_ERR_INVALID_ARG;
-  const int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
   int rc = check_precision(c, precision);
   if (rc != MBR_OK) return rc;
+  // a frame of a generic formula (FP64 only) makes the whole batch FP64
+  for (int i = 0; i < n; i++)
+    if (formula_generic(scenes[i].formula)) precision = MBR_PREC_FP64;
   HIPCHECK(c, hipSetDevice(c->device));
   // derive every scene on the host
   std::vector<BatchFrame> frames(n);

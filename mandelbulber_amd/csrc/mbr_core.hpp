@@ -36,6 +36,7 @@ template <>
 struct Math<double> {
   static MBR_HD double sqrt(double x) { return ::sqrt(x); }
   static MBR_HD double asin(double x) { return ::asin(x); }
+  static MBR_HD double acos(double x) { return ::acos(x); }
   static MBR_HD double atan2(double y, double x) { return ::atan2(y, x); }
   static MBR_HD double pow(double x, double y) { return ::pow(x, y); }
   static MBR_HD double cos(double x) { return ::cos(x); }
@@ -49,6 +50,7 @@ template <>
 struct Math<float> {
   static MBR_HD float sqrt(float x) { return ::sqrtf(x); }
   static MBR_HD float asin(float x) { return ::asinf(x); }
+  static MBR_HD float acos(float x) { return ::acosf(x); }
   static MBR_HD float atan2(float y, float x) { return ::atan2f(y, x); }
   static MBR_HD float pow(float x, float y) { return ::powf(x, y); }
   static MBR_HD float cos(float x) { return ::cosf(x); }
@@ -60,6 +62,14 @@ struct Math<float> {
 
 template <class T>
 MBR_HD T length(V3<T> a) { return Math<T>::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// |a| with every product and sum rounded on its own (no fused multiply-add),
+// as the reference's CVector3::Length compiles on x86-64
+template <class T>
+MBR_HD T length_nc(V3<T> a) {
+#pragma clang fp contract(off)
+  return Math<T>::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
 
 // Hardware reciprocal / reciprocal square root / log2 of the FP32 path
 // (v_rcp_f32, v_rsq_f32, v_log_f32: about 1 ulp), in place of the

@@ -15,7 +15,14 @@
 
 namespace mbr {
 
-enum { MODE_NORMAL = 0, MODE_COLOURING = 1 };
+// the values of enumCalculationMode (src/fractal.h)
+enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
+
+// Formulas of the render core.  2, 7 and 8 have kernels of their own (F = 2,
+// 7, 8); the others run through the generic FP64 path (F = 0): one
+// CalculateDistance per DE call, analytic for trig_DE, by finite
+// differences of the orbit radius (delta DE) for the rest.
+MBR_HD bool formula_analytic(int formula) { return formula == 1 || formula == 2 || formula == 7 || formula == 8; }
 enum { OBJ_FRACTAL = 0, OBJ_PLANE = 1, OBJ_WATER = 2, OBJ_SPHERE = 3, OBJ_SPHERE_INV = 4, OBJ_BOX = 5,
        OBJ_BOX_INV = 6 };
 
@@ -27,17 +34,47 @@ MBR_HD V3<T> ld3(const T *p) { return V3<T>{p[0], p[1], p[2]}; }
 template <class T>
 MBR_HD V3<T> rot9(const T *m, V3<T> v) { return mat_apply(m, v); }
 
+// CVector3(alfa, beta) (src/algebra.hpp:33-38)
+template <class T>
+MBR_HD V3<T> angle_vector(T alfa, T beta) {
+  using M = Math<T>;
+  return V3<T>{M::cos(beta) * M::cos(alfa), M::cos(beta) * M::sin(alfa), M::sin(beta)};
+}
+
+// m = m * R_axis(angle) (CRotationMatrix::RotateX/Y/Z, src/algebra.cpp:94-143;
+// m row-major), skipped for a zero angle as the reference does.
+template <class T>
+MBR_HD void rotate_right(T *m, bool &ident, int axis, T angle) {
+#pragma clang fp contract(off)
+  if (angle == T(0)) return;
+  using M = Math<T>;
+  const T s = M::sin(angle), c = M::cos(angle);
+  T r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (axis == 0) { r[0] = 1; r[4] = c; r[8] = c; r[5] = -s; r[7] = s; }
+  else if (axis == 1) { r[4] = 1; r[8] = c; r[0] = c; r[6] = -s; r[2] = s; }
+  else { r[8] = 1; r[0] = c; r[4] = c; r[1] = -s; r[3] = s; }
+  T o[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) o[3 * i + j] = m[3 * i] * r[j] + m[3 * i + 1] * r[3 + j] + m[3 * i + 2] * r[6 + j];
+  for (int k = 0; k < 9; k++) m[k] = o[k];
+  ident = false;
+}
+
 template <class T, int Mode>
 MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
+#pragma clang fp contract(off)
   using M = Math<T>;
   int L;
   T distance = 0;
   T r_dz = 1;
+  V3<T> dz = V3<T>{T(1), T(0), T(0)};  // trig_DE's derivative vector (src/fractal.cpp:49-53)
+  T ph_dz = 0, th_dz = 0;
+  const int delta_limit = (Mode == MODE_DELTA2 && iter_count) ? *iter_count : -1;
   const T p = f.power;
   T tgladDE = 1;
   T tgladColor = 1;
   const V3<T> constant = f.julia_mode ? ld3(f.julia) : z * f.constant_factor;
-  T r = length(z);
+  T r = length_nc(z);
   T min = Mode == MODE_COLOURING ? T(3.0e38) : T(0);  // min radius of the orbit (1e200 in FP64)
   if (Mode == MODE_COLOURING && sizeof(T) == 8) min = T(1e200);
 
@@ -50,13 +87,13 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       else if (z.y < -lim) { z.y = -val - z.y; tgladColor *= T(0.9); }
       if (z.z > lim) { z.z = val - z.z; tgladColor *= T(0.9); }
       else if (z.z < -lim) { z.z = -val - z.z; tgladColor *= T(0.9); }
-      r = length(z);
+      r = length_nc(z);
     }
     if (f.sph_fold) {
       T r2 = r * r;
       if (r2 < f.sph_mR2) { z = z * f.sph_factor1; tgladDE *= f.sph_factor1; tgladColor += 1; }
       else if (r2 < f.sph_fR2) { T tf2 = f.sph_fR2 / r2; z = z * tf2; tgladDE *= tf2; tgladColor += 10; }
-      r = length(z);
+      r = length_nc(z);
     }
 
     if (f.formula == 2) {  // trig_optim
@@ -69,18 +106,18 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       r_dz = rp * r_dz * p + T(1);
       rp *= r;
       z = V3<T>{cth * M::cos(ph), cth * M::sin(ph), M::sin(th)} * rp + constant;
-      r = length(z);
+      r = length_nc(z);
     } else if (f.formula == 7) {  // menger_sponge
       T temp;
       z.x = M::fabs(z.x); z.y = M::fabs(z.y); z.z = M::fabs(z.z);
       if (z.x - z.y < 0) { temp = z.y; z.y = z.x; z.x = temp; }
       if (z.x - z.z < 0) { temp = z.z; z.z = z.x; z.x = temp; }
       if (z.y - z.z < 0) { temp = z.z; z.z = z.y; z.y = temp; }
-      if (Mode == MODE_COLOURING) { T length2 = length(z); if (length2 < min) min = length2; }
+      if (Mode == MODE_COLOURING) { T length2 = length_nc(z); if (length2 < min) min = length2; }
       z = z * T(3);
       z.x -= T(2); z.y -= T(2);
       if (z.z > T(1)) z.z -= T(2);
-      r = length(z);
+      r = length_nc(z);
       tgladDE *= T(3);
     } else if (f.formula == 8) {  // tglad (Mandelbox)
       const T lim = f.mb_limit, val = f.mb_value;
@@ -114,7 +151,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
         if (z.z > lim) { z.z = val - z.z; tgladColor += f.mb_col_Z; }
         else if (z.z < -lim) { z.z = -val - z.z; tgladColor += f.mb_col_Z; }
       }
-      r = length(z);
+      r = length_nc(z);
       T r2 = r * r;
       z = z + ld3(f.mb_offset);
       if (r2 < f.mb_mR2) { z = z * f.mb_factor1; tgladDE *= f.mb_factor1; tgladColor += f.mb_col_Sp1; }
@@ -123,20 +160,117 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
       z = z * f.mb_scale + constant;
       tgladDE = tgladDE * f.mb_abs_scale + T(1);
-      r = length(z);
+      r = length_nc(z);
+    } else if (f.formula == 1) {  // trig_DE (src/fractal.cpp:189-209)
+      const T r1 = M::pow(r, p - T(1));
+      const T r2 = r1 * r;
+      const T th = M::atan2(z.y, z.x);                                   // GetAlpha
+      const T ph = -M::atan2(z.z, M::sqrt(z.x * z.x + z.y * z.y));       // -GetBeta
+      if (Mode == MODE_NORMAL) {
+        const T p_r1_rdz = p * r1 * r_dz;
+        const T ph_phdz = (p - T(1)) * ph + ph_dz;
+        const T th_thdz = (p - T(1)) * th + th_dz;
+        dz = angle_vector(th_thdz, ph_phdz) * p_r1_rdz + V3<T>{T(1), T(0), T(0)};
+        r_dz = length_nc(dz);
+        th_dz = M::atan2(dz.y, dz.x);
+        ph_dz = -M::atan2(dz.z, M::sqrt(dz.x * dz.x + dz.y * dz.y));
+      }
+      z = angle_vector(p * th, p * ph) * r2 + constant;
+      r = length_nc(z);
+    } else if (f.formula == 13) {  // mandelbulb2 (src/fractal.cpp:226-254)
+      T temp, tempR;
+      tempR = M::sqrt(z.x * z.x + z.y * z.y);
+      z = z * (T(1) / tempR);
+      temp = z.x * z.x - z.y * z.y;
+      z.y = T(2) * z.x * z.y;
+      z.x = temp;
+      z = z * tempR;
+      tempR = M::sqrt(z.y * z.y + z.z * z.z);
+      z = z * (T(1) / tempR);
+      temp = z.y * z.y - z.z * z.z;
+      z.z = T(2) * z.y * z.z;
+      z.y = temp;
+      z = z * tempR;
+      tempR = M::sqrt(z.x * z.x + z.z * z.z);
+      z = z * (T(1) / tempR);
+      temp = z.x * z.x - z.z * z.z;
+      z.z = T(2) * z.x * z.z;
+      z.x = temp;
+      z = z * tempR;
+      z = z * r;
+      z = z + constant;
+      r = length_nc(z);
+    } else if (f.formula == 14) {  // mandelbulb3 (src/fractal.cpp:255-282)
+      T temp, tempR;
+      T sign = 1, sign2 = 1;
+      if (z.x < 0) sign2 = -1;
+      tempR = M::sqrt(z.x * z.x + z.y * z.y);
+      z = z * (T(1) / tempR);
+      temp = z.x * z.x - z.y * z.y;
+      z.y = T(2) * z.x * z.y;
+      z.x = temp;
+      z = z * tempR;
+      if (z.x < 0) sign = -1;
+      tempR = M::sqrt(z.x * z.x + z.z * z.z);
+      z = z * (T(1) / tempR);
+      temp = z.x * z.x - z.z * z.z;
+      z.z = T(2) * z.x * z.z * sign2;
+      z.x = temp * sign;
+      z = z * tempR;
+      z = z * r;
+      z = z + constant;
+      r = length_nc(z);
+    } else if (f.formula == 15) {  // mandelbulb4 (src/fractal.cpp:283-299)
+      const T rp = M::pow(r, p - T(1));
+      const T angZ = M::atan2(z.y, z.x);
+      const T angY = M::atan2(z.z, z.x);
+      const T angX = M::atan2(z.z, z.y);
+      // CRotationMatrix: identity, then RotateX, RotateY, RotateZ, each a
+      // right product; RotateVector skips a matrix no rotation touched
+      T m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      bool ident = true;
+      rotate_right<T>(m, ident, 0, angX * (p - T(1)));
+      rotate_right<T>(m, ident, 1, angY * (p - T(1)));
+      rotate_right<T>(m, ident, 2, angZ * (p - T(1)));
+      if (!ident) z = V3<T>{m[0] * z.x + m[1] * z.y + m[2] * z.z, m[3] * z.x + m[4] * z.y + m[5] * z.z,
+                            m[6] * z.x + m[7] * z.y + m[8] * z.z};
+      z = z * rp + constant;
+      r = length_nc(z);
+    } else if (f.formula == 11) {  // xenodreambuie (src/fractal.cpp:300-320)
+      const T pi = T(3.14159265358979323846);
+      const T rp = M::pow(r, p);
+      const T th = M::atan2(z.y, z.x);
+      T ph = M::acos(z.z / r);
+      if (ph > T(0.5) * pi) ph = pi - ph;
+      else if (ph < -T(0.5) * pi) ph = -pi - ph;
+      z.x = rp * M::cos(th * p) * M::sin(ph * p);
+      z.y = rp * M::sin(th * p) * M::sin(ph * p);
+      z.z = rp * M::cos(ph * p);
+      z = z + constant;
+      r = length_nc(z);
+    }
+
+    // delta-DE runs (src/fractal.cpp:1110-1119) stop on their own tests
+    if (Mode == MODE_DELTA1) {
+      if (r > T(1e10)) break;
+    } else if (Mode == MODE_DELTA2) {
+      if (L == delta_limit) break;
     }
 
     // termination; "!(a <= b)" keeps an overflowed (NaN/inf) FP32 orbit
-    // counted as escaped, and equals "a > b" for every finite value.
+    // counted as escaped, and equals "a > b" for every finite value.  The
+    // formulas of the generic FP64 path keep the reference's "a > b" (a NaN
+    // orbit never escapes).
     if (f.formula == 7) {
       if (!(r <= T(1000))) { distance = (r - T(2)) / tgladDE; break; }
     } else if (f.formula == 8) {
       if (!(r <= T(1024))) { distance = r / M::fabs(tgladDE); break; }
     } else {
       if (Mode == MODE_NORMAL) {
-        if (!(r <= f.bailout)) { distance = T(0.5) * r * M::log(r) / r_dz; break; }
-      } else {
-        distance = length(z);
+        const bool esc = f.formula == 2 ? !(r <= f.bailout) : r > f.bailout;
+        if (esc) { distance = T(0.5) * r * M::log(r) / r_dz; break; }
+      } else if (Mode == MODE_COLOURING) {
+        distance = length_nc(z);
         if (f.only_plane) {
           if (!(distance <= T(1e15))) {
             distance = (T(L) - M::log(M::log(r) / M::log(T(N))) / M::log(p)) / T(100);
@@ -155,7 +289,8 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     if (L == N) distance = 0;
     return distance;
   }
-  if (f.formula == 8) return tgladColor * T(100) + length(z) * f.mb_col_R;
+  if (Mode == MODE_DELTA1 || Mode == MODE_DELTA2) return r;
+  if (f.formula == 8) return tgladColor * T(100) + length_nc(z) * f.mb_col_R;
   if (f.formula == 7) return min * T(1000);
   return distance * T(5000);
 }
@@ -498,10 +633,78 @@ MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fr
   return distance;
 }
 
+// CalculateDistance (src/fractal.cpp:1245-1448) in one call for the formulas
+// of the generic path: analytic (trig_DE) through Compute<normal>, the rest
+// by the delta DE of :1307-1342 (four orbit radii, at the point and after the
+// reference's in-place +-1e-10 updates along x, y, z).
+// One orbit run of the generic path, kept out of line (one copy per mode).
+template <class T, int Mode>
+__attribute__((noinline)) __host__ __device__ T compute_generic(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
+  return compute<T, Mode>(f, z, N, iter_count);
+}
+
+// Not inlined: the body (five orbit runs over every generic formula) would
+// otherwise be copied into each of the shaders' distance-estimate sites.
+template <class T, bool P = false>
+__attribute__((noinline)) __host__ __device__ T calc_distance_generic(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
+                               DEInfo &info) {
+#pragma clang fp contract(off)
+  using M = Math<T>;
+  info.object = OBJ_FRACTAL;
+  info.max_iter = 0;
+  info.work = 0;
+  info.iters = 0;
+  T limitBox = 0;
+  if ((!P && f.limits)) {
+    T da = point.x - f.limit_max[0], db = point.y - f.limit_max[1], dc = point.z - f.limit_max[2];
+    T ta = -(point.x - f.limit_min[0]), tb = -(point.y - f.limit_min[1]), tc = -(point.z - f.limit_min[2]);
+    da = (da > ta) ? da : ta;
+    db = (db > tb) ? db : tb;
+    dc = (dc > tc) ? dc : tc;
+    limitBox = (da > db) ? ((da > dc) ? da : dc) : ((db > dc) ? db : dc);
+    if (limitBox > detailSize) return limitBox;
+  }
+  if ((!P && f.only_plane))
+    return de_finish<T, P>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, limitBox, info);
+  if (formula_analytic(f.formula)) {
+    int L = 0;
+    const T dist = compute_generic<T, MODE_NORMAL>(f, point, N, &L);
+    return de_finish<T, P>(f, point, dist, L, true, N, minN, normalMode, detailSize, limitBox, info);
+  }
+  const T delta = T(1e-10);
+  int L = 0;
+  const T r = compute_generic<T, MODE_DELTA1>(f, point, N, &L);
+  const int retval = L;
+  V3<T> q = point;
+  q.x += delta;
+  T r2 = compute_generic<T, MODE_DELTA2>(f, q, N, &L);
+  const T dr1 = M::fabs(r2 - r) / delta;
+  q.x -= delta;
+  q.y += delta;
+  r2 = compute_generic<T, MODE_DELTA2>(f, q, N, &L);
+  const T dr2 = M::fabs(r2 - r) / delta;
+  q.y -= delta;
+  q.z += delta;
+  r2 = compute_generic<T, MODE_DELTA2>(f, q, N, &L);
+  const T dr3 = M::fabs(r2 - r) / delta;
+  const T dr = M::sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+  T dist = f.linear_de ? T(0.5) * r / dr : T(0.5) * r * M::log(r) / dr;
+  if (dist < 0) dist = 0;
+  if (retval == N) dist = 0;
+  dist = de_finish<T, P>(f, point, dist, retval, true, N, minN, normalMode, detailSize, limitBox, info);
+  info.work *= 4;  // four orbits
+  return dist;
+}
+
 // Start one CalculateDistance; true when it finished without iterating.
 template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
 MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                           DECall<T> &call, T &dist, DEInfo &info) {
+  if constexpr (F == 0) {
+    // the generic path runs the whole distance estimate in this call
+    dist = calc_distance_generic<T, P>(f, point, detailSize, N, minN, normalMode, info);
+    return true;
+  }
   info.object = OBJ_FRACTAL;
   info.max_iter = 0;
   info.work = 0;
@@ -617,7 +820,8 @@ MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, i
                        DEInfo &info) {
   if (f.formula == 7) return calc_distance_f<T, 7>(f, point, detailSize, N, minN, normalMode, info);
   if (f.formula == 8) return calc_distance_f<T, 8>(f, point, detailSize, N, minN, normalMode, info);
-  return calc_distance_f<T, 2>(f, point, detailSize, N, minN, normalMode, info);
+  if (f.formula == 2) return calc_distance_f<T, 2>(f, point, detailSize, N, minN, normalMode, info);
+  return calc_distance_generic<T>(f, point, detailSize, N, minN, normalMode, info);
 }
 
 }  // namespace mbr

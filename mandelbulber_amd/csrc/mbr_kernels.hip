@@ -915,6 +915,7 @@ static inline dim3 grid_for(int n) { return dim3((unsigned)((n + kBlock - 1) / k
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                         int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
+  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto tag, auto rtag) {
     using T = decltype(tag);
@@ -928,6 +929,13 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
       else
         hipLaunchKernelGGL((k_march<T, RT, 2, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
     };
+    if (formula_generic(formula)) {
+      // generic formulas: FP64 only (the host promotes their renders)
+      if constexpr (std::is_same<T, double>::value && std::is_same<RT, double>::value)
+        hipLaunchKernelGGL((k_march<double, double, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first,
+                           band_rays, stats);
+      return;
+    }
     if (plain) go(std::true_type{});
     else go(std::false_type{});
   };
@@ -940,6 +948,11 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
                             int persist_blocks, unsigned long long *stats, hipStream_t s) {
   const dim3 grid((unsigned)std::max(1, persist_blocks));
+  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
+  if (formula_generic(formula)) {
+    hipLaunchKernelGGL((k_march_fix<double, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    return hipGetLastError();
+  }
   auto run = [&](auto rtag) {
     using RT = decltype(rtag);
     if (formula == 7)
@@ -996,8 +1009,15 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     sb.fast_ao = (T *)sbuf.fast_ao;
     sb.ao = (T *)sbuf.ao;
     sb.n = sbuf.n;
+    const bool generic = formula_generic(formula);
     auto surf = [&](auto deep_tag) {
       constexpr bool D = decltype(deep_tag)::value;
+      if constexpr (!D && std::is_same<T, double>::value) {
+        if (generic) {
+          hipLaunchKernelGGL((k_surface<T, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+          return;
+        }
+      }
       if (formula == 7)
         hipLaunchKernelGGL((k_surface<T, 7, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
       else if (formula == 8)
@@ -1008,6 +1028,12 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     auto aok = [&](auto deep_tag, auto plain_tag) {
       constexpr bool D = decltype(deep_tag)::value;
       constexpr bool Pl = decltype(plain_tag)::value;
+      if constexpr (!D && !Pl && std::is_same<T, double>::value) {
+        if (generic) {
+          hipLaunchKernelGGL((k_ao<T, 0, false, false>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+          return;
+        }
+      }
       if (formula == 7)
         hipLaunchKernelGGL((k_ao<T, 7, D, Pl>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
       else if (formula == 8)
@@ -1016,7 +1042,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
         hipLaunchKernelGGL((k_ao<T, 2, D, Pl>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
     };
     // deep-zoom variants only for FP32 shading of scenes that need them
-    const bool deep = sbuf.deep && sizeof(T) < 8;
+    const bool deep = sbuf.deep && sizeof(T) < 8 && !generic;
     if (deep) surf(std::true_type{});
     else surf(std::false_type{});
     if (ev_ao0) {
@@ -1028,7 +1054,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
       // threshold, pre-folds or Julia constant) take a variant without those
       // branches and the scene constants they keep live
       if (deep) aok(std::true_type{}, std::false_type{});
-      else if (sbuf.plain) aok(std::false_type{}, std::true_type{});
+      else if (sbuf.plain && !generic) aok(std::false_type{}, std::true_type{});
       else aok(std::false_type{}, std::false_type{});
     }
     if (ev_ao1) {
@@ -1038,6 +1064,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     hipLaunchKernelGGL((k_compose<T, T>), grid, dim3(kBlock), 0, s, sc, b, ao, sb);
     return hipGetLastError();
   };
+  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
   if (precision == 0) return run(double{});
   return run(float{});
 }

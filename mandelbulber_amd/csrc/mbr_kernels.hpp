@@ -26,6 +26,11 @@ enum CounterIdx {
   CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_DIRS = 3, CNT_AOQ = 4, CNT_REDO = 5, CNT_CHUNKS = 6, CNT_COUNT = 8
 };
 
+// Formulas of the render core; the generic ones (no kernels of their own)
+// run in FP64 through the F = 0 kernels.
+inline bool formula_supported(int f) { return f == 1 || f == 2 || f == 7 || f == 8 || f == 11 || (f >= 13 && f <= 15); }
+inline bool formula_generic(int f) { return formula_supported(f) && f != 2 && f != 7 && f != 8; }
+
 // Maximum step records of one ray (the loop bound of src/Render3D.cpp:95).
 constexpr int kMaxSteps = 10000;
 // Overflow records come in chunks of kChunk; a ray's directory lists the

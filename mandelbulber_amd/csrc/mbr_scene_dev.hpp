@@ -21,6 +21,7 @@ struct FractalK {
   int prim_plane, prim_box, prim_ibox, prim_sphere, prim_isphere, prim_water;
   int water_iterations, frame_no, water_rotated;
   int ipow;  // power as an integer in [2, 32] when it is one, else 0
+  int linear_de;  // linear_DE_mode: delta-DE distance 0.5 r / dr instead of 0.5 r ln r / dr
   T power, bailout, constant_factor;
   T julia[3];
   T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;

@@ -142,7 +142,7 @@ std::vector<KeyDef> build_table() {
         D(std::string("mandelbox_rotation_") + axis[a] + std::to_string(fold + 1) + "_" + comp[c],
           offsetof(mbr_scene, mb_rotation) + sizeof(double) * (fold * 9 + a * 3 + c), kDeg);
   D("view_distance_max", OFF(view_distance_max)); D("view_distance_min", OFF(view_distance_min));
-  I("interior_mode", OFF(interior_mode)); G("linear_DE_mode");
+  I("interior_mode", OFF(interior_mode)); I("linear_DE_mode", OFF(linear_DE_mode));
   I("constant_DE_threshold", OFF(constant_DE_threshold));
   G("FoldingIntPow_folding_factor"); G("FoldingIntPow_z_factor"); G("dynamic_DE_correction");
   I("penetrating_lights", OFF(penetrating_lights));

@@ -47,12 +47,13 @@ struct RGBT {
 };
 
 // Distance estimate of the shaders.  F is the formula when the caller was
-// compiled for one (a kernel per formula), 0 for a run-time choice.
+// compiled for one (a kernel per formula), 0 for the kernels of the generic
+// FP64 formulas.
 template <class T, int F = 0>
 MBR_HD T de(const DevScene &sc, V3<T> p, T detail, int N, int minN, bool normalMode, DEInfo &info,
             LaneCount &cnt) {
   T d;
-  if constexpr (F == 0) d = calc_distance<T>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
+  if constexpr (F == 0) d = calc_distance_generic<T>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
   else d = calc_distance_f<T, F>(sc.frac<T>(), p, detail, N, minN, normalMode, info);
   cnt.add(info);
   return d;

@@ -147,17 +147,25 @@ static inline unsigned short x86_d2u16(double d) { return (unsigned short)(unsig
 static inline unsigned char x86_d2u8(double d) { return (unsigned char)(unsigned)x86_d2i(d); }
 
 /* --------------------------------------------------------------- Compute */
-enum { MODE_NORMAL = 0, MODE_COLOURING = 1 };
+enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
 
-/* Compute<Mode> (src/fractal.cpp:40-1241) for trig_optim (:211-225),
- * menger_sponge (:369-408) and tglad (:409-529) with the tglad and
- * spherical pre-folds (:125-181), termination (:1129-1191) and return
- * values (:1193-1240). */
+/* CVector3(alfa, beta) (src/algebra.hpp:33-38) */
+static v3 angle_vec(double alfa, double beta) { return v3m(cos(beta) * cos(alfa), cos(beta) * sin(alfa), sin(beta)); }
+
+/* Compute<Mode> (src/fractal.cpp:40-1241) for trig_DE (:189-209),
+ * trig_optim (:211-225), mandelbulb2/3/4 (:226-299), xenodreambuie
+ * (:300-320), menger_sponge (:369-408) and tglad (:409-529) with the tglad
+ * and spherical pre-folds (:125-181), the delta-DE stops (:1110-1119),
+ * termination (:1129-1191) and return values (:1193-1240).  In
+ * MODE_DELTA2, *iter_count is read as the iteration to stop at. */
 static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_count) {
   const mbr_scene *s = f->s;
   int L;
   double distance = 0;
   double r_dz = 1;
+  v3 dz = v3m(1.0, 0.0, 0.0);
+  double ph_dz = 0, th_dz = 0;
+  const int delta_limit = (mode == MODE_DELTA2 && iter_count) ? *iter_count : -1;
   double p = s->power;
   double fixedRadius = s->mb_fixed_radius;
   double fR2 = fixedRadius * fixedRadius;
@@ -265,8 +273,92 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
         r = v3len(z);
         break;
       }
+      case MBR_FORMULA_TRIG_DE: {
+        double r1 = pow(r, p - 1);
+        double r2 = r1 * r;
+        double th = atan2(z.y, z.x);
+        double ph = -atan2(z.z, sqrt(z.x * z.x + z.y * z.y));
+        if (mode == MODE_NORMAL) {
+          double p_r1_rdz = p * r1 * r_dz;
+          double ph_phdz = (p - 1.0) * ph + ph_dz;
+          double th_thdz = (p - 1.0) * th + th_dz;
+          dz = v3add(v3mul(angle_vec(th_thdz, ph_phdz), p_r1_rdz), v3m(1.0, 0.0, 0.0));
+          r_dz = v3len(dz);
+          th_dz = atan2(dz.y, dz.x);
+          ph_dz = -atan2(dz.z, sqrt(dz.x * dz.x + dz.y * dz.y));
+        }
+        z = v3add(v3mul(angle_vec(p * th, p * ph), r2), constant);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_MANDELBULB2: {
+        double temp, tempR;
+        tempR = sqrt(z.x * z.x + z.y * z.y);
+        z = v3mul(z, 1.0 / tempR);
+        temp = z.x * z.x - z.y * z.y; z.y = 2.0 * z.x * z.y; z.x = temp;
+        z = v3mul(z, tempR);
+        tempR = sqrt(z.y * z.y + z.z * z.z);
+        z = v3mul(z, 1.0 / tempR);
+        temp = z.y * z.y - z.z * z.z; z.z = 2.0 * z.y * z.z; z.y = temp;
+        z = v3mul(z, tempR);
+        tempR = sqrt(z.x * z.x + z.z * z.z);
+        z = v3mul(z, 1.0 / tempR);
+        temp = z.x * z.x - z.z * z.z; z.z = 2.0 * z.x * z.z; z.x = temp;
+        z = v3mul(z, tempR);
+        z = v3mul(z, r);
+        z = v3add(z, constant);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_MANDELBULB3: {
+        double temp, tempR, sign = 1.0, sign2 = 1.0;
+        if (z.x < 0) sign2 = -1.0;
+        tempR = sqrt(z.x * z.x + z.y * z.y);
+        z = v3mul(z, 1.0 / tempR);
+        temp = z.x * z.x - z.y * z.y; z.y = 2.0 * z.x * z.y; z.x = temp;
+        z = v3mul(z, tempR);
+        if (z.x < 0) sign = -1.0;
+        tempR = sqrt(z.x * z.x + z.z * z.z);
+        z = v3mul(z, 1.0 / tempR);
+        temp = z.x * z.x - z.z * z.z; z.z = 2.0 * z.x * z.z * sign2; z.x = temp * sign;
+        z = v3mul(z, tempR);
+        z = v3mul(z, r);
+        z = v3add(z, constant);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_MANDELBULB4: {
+        double rp = pow(r, p - 1);
+        double angZ = atan2(z.y, z.x), angY = atan2(z.z, z.x), angX = atan2(z.z, z.y);
+        rotm rm; rot_null(&rm);
+        rot_x(&rm, angX * (p - 1));
+        rot_y(&rm, angY * (p - 1));
+        rot_z(&rm, angZ * (p - 1));
+        z = v3add(v3mul(rot_apply(&rm, z), rp), constant);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_XENODREAMBUIE: {
+        double rp = pow(r, p);
+        double th = atan2(z.y, z.x);
+        double ph = acos(z.z / r);
+        if (ph > 0.5 * M_PI) ph = M_PI - ph;
+        else if (ph < -0.5 * M_PI) ph = -M_PI - ph;
+        z.x = rp * cos(th * p) * sin(ph * p);
+        z.y = rp * sin(th * p) * sin(ph * p);
+        z.z = rp * cos(ph * p);
+        z = v3add(z, constant);
+        r = v3len(z);
+        break;
+      }
       default:
         break;
+    }
+
+    if (mode == MODE_DELTA1) {
+      if (r > 1e10) break;
+    } else if (mode == MODE_DELTA2) {
+      if (L == delta_limit) break;
     }
 
     if (formula == MBR_FORMULA_MENGER_SPONGE) {
@@ -276,7 +368,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     } else {
       if (mode == MODE_NORMAL) {
         if (r > s->bailout) { distance = 0.5 * r * log(r) / r_dz; break; }
-      } else {
+      } else if (mode == MODE_COLOURING) {
         if (s->primitive_only_plane) {
           distance = v3len(z);
           if (distance > 1e15) { distance = (L - log(log(r) / log((double)N)) / log(p)) / 100.0; break; }
@@ -294,6 +386,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     if (L == N) distance = 0;
     return distance;
   }
+  if (mode == MODE_DELTA1 || mode == MODE_DELTA2) return r;
   if (formula == MBR_FORMULA_TGLAD) return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
   if (formula == MBR_FORMULA_MENGER_SPONGE) return min * 1000.0;
   return distance * 5000.0;
@@ -371,7 +464,43 @@ static double calc_distance(const fractal_ctx *f, v3 point, double detailSize, i
     }
   }
 
-  if (!s->primitive_only_plane) {
+  const int analytic = s->formula == MBR_FORMULA_TRIG_DE || s->formula == MBR_FORMULA_TRIG_OPTIM ||
+                       s->formula == MBR_FORMULA_MENGER_SPONGE || s->formula == MBR_FORMULA_TGLAD;
+  if (!s->primitive_only_plane && !analytic) {
+    /* delta DE (src/fractal.cpp:1307-1342): orbit radius at the point and
+     * at +1e-10 along x, y, z, the point updated in place as there */
+    const double deltaDE = 1e-10;
+    double r = compute(f, MODE_DELTA1, point, N, &L);
+    const int retval = L;
+    if (iters_out) *iters_out = L;
+    point.x += deltaDE;
+    double r2 = compute(f, MODE_DELTA2, point, N, &L);
+    double dr1 = fabs(r2 - r) / deltaDE;
+    point.x -= deltaDE;
+    point.y += deltaDE;
+    r2 = compute(f, MODE_DELTA2, point, N, &L);
+    double dr2 = fabs(r2 - r) / deltaDE;
+    point.y -= deltaDE;
+    point.z += deltaDE;
+    r2 = compute(f, MODE_DELTA2, point, N, &L);
+    double dr3 = fabs(r2 - r) / deltaDE;
+    point.z -= deltaDE;  /* (the reference's point is a copy: the caller's point is unchanged) */
+    double dr = sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+    distance = s->linear_DE_mode ? 0.5 * r / dr : 0.5 * r * log(r) / dr;
+    if (cnt) { cnt->calls++; cnt->iterations += 4 * ((retval < N) ? (uint64_t)retval + 1 : (uint64_t)N); }
+    if (distance < 0) distance = 0;
+    mi = 0;
+    if (retval == N) { mi = 1; distance = 0; }
+    if (L < minN && distance < detailSize) distance = detailSize;
+    if (s->interior_mode && !normalMode) {
+      if (distance < 0.5 * detailSize || L == N) { distance = detailSize; mi = 0; }
+    } else if (s->interior_mode && normalMode) {
+      if (distance < 0.9 * detailSize) { distance = detailSize - distance; mi = 0; }
+    }
+    if (s->iter_thresh && retval < N) {
+      if (distance < detailSize) distance = detailSize * 1.01;
+    }
+  } else if (!s->primitive_only_plane) {
     distance = compute(f, MODE_NORMAL, point, N, &L);
     if (cnt) { cnt->calls++; cnt->iterations += (L < N) ? (uint64_t)L + 1 : (uint64_t)N; }
     mi = (L == N);

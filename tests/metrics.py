@@ -29,13 +29,23 @@ def compare(a, b):
         depth_p999 = float(np.quantile(rel_same, 0.999)) if rel_same.size else 0.0
     else:
         depth_max = depth_p999 = 0.0
-    d = a.rgb.astype(np.float64) - b.rgb.astype(np.float64)
+    ra, rb = a.rgb.astype(np.float64), b.rgb.astype(np.float64)
+    d = ra - rb
+    # NaN colours (the reference's NaN orbits, e.g. mandelbulb2 on an axis):
+    # equal where both renders have them; a pixel with a NaN channel in one
+    # render only is a differing pixel (counted with the flips) and is left
+    # out of the PSNR
+    na, nb = np.isnan(ra), np.isnan(rb)
+    d[na & nb] = 0.0
+    nan_px = np.any(na ^ nb, axis=0)
+    flips += int(np.count_nonzero(nan_px & ~(hit_a != hit_b)))
+    d[:, nan_px] = 0.0
     mse = float(np.mean(d * d))
     psnr = float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
     return {
         "pixels": n, "hits": int(hit_b.sum()), "flips": flips, "flip_frac": flips / n,
         "depth_rel_max": depth_max, "depth_rel_p999": depth_p999, "psnr_db": psnr,
-        "rgb_max_abs": float(np.abs(d).max()),
+        "rgb_max_abs": float(np.abs(d).max()), "nan_mismatch_px": int(np.count_nonzero(nan_px)),
     }
 
 
