@@ -46,15 +46,24 @@ enum mbr_formula {
   MBR_FORMULA_NONE = 0,
   MBR_FORMULA_TRIG_DE = 1,       /* Mandelbulb with the analytic derivative vector */
   MBR_FORMULA_TRIG_OPTIM = 2,    /* Mandelbulb, "trigonometric" in the GUI */
+  MBR_FORMULA_FAST_TRIG = 3,
+  MBR_FORMULA_HYPERCOMPLEX = 4,
+  MBR_FORMULA_QUATERNION = 5,
+  MBR_FORMULA_MINUS_FAST_TRIG = 6,
   MBR_FORMULA_MENGER_SPONGE = 7,
   MBR_FORMULA_TGLAD = 8,         /* Mandelbox */
   MBR_FORMULA_XENODREAMBUIE = 11,
   MBR_FORMULA_MANDELBULB2 = 13,
   MBR_FORMULA_MANDELBULB3 = 14,
-  MBR_FORMULA_MANDELBULB4 = 15
-  /* 1, 11, 13, 14, 15 render through the generic FP64 path (delta DE for
-   * 11, 13, 14, 15, as the reference's analitycDE = false): every precision
-   * mode renders them in FP64 */
+  MBR_FORMULA_MANDELBULB4 = 15,
+  MBR_FORMULA_FOLDING_INT_POW2 = 16,
+  MBR_FORMULA_SMOOTH_MANDELBOX = 17,
+  MBR_FORMULA_AEXION = 19,
+  MBR_FORMULA_BENESI = 20,
+  MBR_FORMULA_BRISTORBROT = 21
+  /* all but 2, 7 and 8 render through the generic FP64 path (analytic DE for
+   * 1 and 17, delta DE for the rest, as the reference's analitycDE): every
+   * precision mode renders them in FP64 */
 };
 
 /* enumPerspectiveType (src/common_math.h) */
@@ -246,6 +255,10 @@ typedef struct mbr_scene {
   /* post effects on the compiled image (src/image.cpp; order in the
    * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
   int linear_DE_mode;              /* linear_DE_mode (delta-DE formulas: 0.5 r / dr) */
+  double mb_sharpness;             /* mandelbox_sharpness (smoothMandelbox) */
+  double folding_int_pow_factor;   /* FoldingIntPow_folding_factor */
+  double folding_int_pow_z_factor; /* FoldingIntPow_z_factor */
+  double c_add;                    /* c_add (aexion) */
   int ssao_enabled;                /* post_SSAO_enabled */
   int ssao_quality;                /* post_SSAO_quality */
   int dof_enabled;                 /* post_DOF_enabled */

@@ -103,6 +103,10 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   f.power = (T)s.power;
   f.ipow = (s.power == std::floor(s.power) && s.power >= 2 && s.power <= 32) ? (int)s.power : 0;
   f.linear_de = s.linear_DE_mode;
+  f.mb_sharp = (T)s.mb_sharpness;
+  f.fip_factor = (T)s.folding_int_pow_factor;
+  f.fip_zfactor = (T)s.folding_int_pow_z_factor;
+  f.c_add = (T)s.c_add;
   f.bailout = (T)s.bailout;
   f.constant_factor = (T)s.constant_factor;
   for (int i = 0; i < 3; i++) f.julia[i] = (T)s.julia[i];
@@ -437,8 +441,7 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
   if (!formula_supported(s.formula))
     return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
-                                            " is not in the render core (supported: 1 trig_DE, 2 trig_optim, "
-                                            "7 menger_sponge, 8 tglad, 11 xenodreambuie, 13-15 mandelbulb2-4)");
+                                            " is not in the render core (supported: 1-8, 11, 13-17, 19-21)");
   if (s.ifs_folding_mode) return c->fail(MBR_ERR_UNSUPPORTED, "IFS folding mode is not in the render core");
   if (s.reflect > 0.0 && !s.raytraced_reflections)
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");

@@ -37,6 +37,7 @@ struct Math<double> {
   static MBR_HD double sqrt(double x) { return ::sqrt(x); }
   static MBR_HD double asin(double x) { return ::asin(x); }
   static MBR_HD double acos(double x) { return ::acos(x); }
+  static MBR_HD double exp(double x) { return ::exp(x); }
   static MBR_HD double atan2(double y, double x) { return ::atan2(y, x); }
   static MBR_HD double pow(double x, double y) { return ::pow(x, y); }
   static MBR_HD double cos(double x) { return ::cos(x); }
@@ -51,6 +52,7 @@ struct Math<float> {
   static MBR_HD float sqrt(float x) { return ::sqrtf(x); }
   static MBR_HD float asin(float x) { return ::asinf(x); }
   static MBR_HD float acos(float x) { return ::acosf(x); }
+  static MBR_HD float exp(float x) { return ::expf(x); }
   static MBR_HD float atan2(float y, float x) { return ::atan2f(y, x); }
   static MBR_HD float pow(float x, float y) { return ::powf(x, y); }
   static MBR_HD float cos(float x) { return ::cosf(x); }

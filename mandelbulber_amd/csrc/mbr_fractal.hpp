@@ -22,7 +22,19 @@ enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
 // 7, 8); the others run through the generic FP64 path (F = 0): one
 // CalculateDistance per DE call, analytic for trig_DE, by finite
 // differences of the orbit radius (delta DE) for the rest.
-MBR_HD bool formula_analytic(int formula) { return formula == 1 || formula == 2 || formula == 7 || formula == 8; }
+MBR_HD bool formula_analytic(int formula) {
+  return formula == 1 || formula == 2 || formula == 7 || formula == 8 || formula == 17;
+}
+// termination groups of src/fractal.cpp:1133-1149: Menger-like (r > 1000,
+// (r - 2) / DE) and Mandelbox-like (r > 1024, r / |DE|)
+MBR_HD bool formula_menger_group(int formula) { return formula == 7; }
+MBR_HD bool formula_box_group(int formula) { return formula == 8 || formula == 17; }
+
+// SmoothConditionAGreaterB / ALessB (src/common_math.h:87-88)
+template <class T>
+MBR_HD T smooth_gt(T a, T b, T sharp) { return T(1) / (T(1) + Math<T>::exp(sharp * (b - a))); }
+template <class T>
+MBR_HD T smooth_lt(T a, T b, T sharp) { return T(1) / (T(1) + Math<T>::exp(sharp * (a - b))); }
 enum { OBJ_FRACTAL = 0, OBJ_PLANE = 1, OBJ_WATER = 2, OBJ_SPHERE = 3, OBJ_SPHERE_INV = 4, OBJ_BOX = 5,
        OBJ_BOX_INV = 6 };
 
@@ -70,10 +82,11 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
   V3<T> dz = V3<T>{T(1), T(0), T(0)};  // trig_DE's derivative vector (src/fractal.cpp:49-53)
   T ph_dz = 0, th_dz = 0;
   const int delta_limit = (Mode == MODE_DELTA2 && iter_count) ? *iter_count : -1;
+  T w = 0, constantw = 0;  // fourth component of hypercomplex, quaternion, aexion
   const T p = f.power;
   T tgladDE = 1;
   T tgladColor = 1;
-  const V3<T> constant = f.julia_mode ? ld3(f.julia) : z * f.constant_factor;
+  V3<T> constant = f.julia_mode ? ld3(f.julia) : z * f.constant_factor;  // aexion rewrites it
   T r = length_nc(z);
   T min = Mode == MODE_COLOURING ? T(3.0e38) : T(0);  // min radius of the orbit (1e200 in FP64)
   if (Mode == MODE_COLOURING && sizeof(T) == 8) min = T(1e200);
@@ -161,6 +174,8 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z = z * f.mb_scale + constant;
       tgladDE = tgladDE * f.mb_abs_scale + T(1);
       r = length_nc(z);
+    } else if constexpr (sizeof(T) < 8) {
+      // the generic formulas run in FP64 only
     } else if (f.formula == 1) {  // trig_DE (src/fractal.cpp:189-209)
       const T r1 = M::pow(r, p - T(1));
       const T r2 = r1 * r;
@@ -248,6 +263,124 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.z = rp * M::cos(ph * p);
       z = z + constant;
       r = length_nc(z);
+    } else if (f.formula == 3 || f.formula == 6) {  // fast_trig, minus_fast_trig (src/fractal.cpp:321-350)
+      const T x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+      const T temp = T(1) - z2 / (x2 + y2);
+      const T newx = (x2 - y2) * temp;
+      const T newy = T(2) * z.x * z.y * temp;
+      const T newz = (f.formula == 3 ? T(-2) : T(2)) * z.z * M::sqrt(x2 + y2);
+      z.x = newx + constant.x;
+      z.y = newy + constant.y;
+      z.z = newz + constant.z;
+      r = length_nc(z);
+    } else if (f.formula == 4) {  // hypercomplex (src/fractal.cpp:351-359)
+      const V3<T> nz{z.x * z.x - z.y * z.y - z.z * z.z - w * w, T(2) * z.x * z.y - T(2) * w * z.z,
+                     T(2) * z.x * z.z - T(2) * z.y * w};
+      const T nw = T(2) * z.x * w - T(2) * z.y * z.z;
+      z = nz + constant;
+      w = nw;
+      r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+    } else if (f.formula == 5) {  // quaternion (src/fractal.cpp:360-368)
+      const V3<T> nz{z.x * z.x - z.y * z.y - z.z * z.z - w * w, T(2) * z.x * z.y, T(2) * z.x * z.z};
+      const T nw = T(2) * z.x * w;
+      z = nz + constant;
+      w = nw;
+      r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+    } else if (f.formula == 16) {  // foldingIntPow2 (src/fractal.cpp:794-836)
+      const T ff = f.fip_factor;
+      if (z.x > ff) z.x = ff * T(2) - z.x;
+      else if (z.x < -ff) z.x = -ff * T(2) - z.x;
+      if (z.y > ff) z.y = ff * T(2) - z.y;
+      else if (z.y < -ff) z.y = -ff * T(2) - z.y;
+      if (z.z > ff) z.z = ff * T(2) - z.z;
+      else if (z.z < -ff) z.z = -ff * T(2) - z.z;
+      r = length_nc(z);
+      const T r2_2 = r * r;
+      if (r2_2 < T(0.25)) z = z * (T(1) / T(0.25));
+      else if (r2_2 < T(1)) z = z * (T(1) / r2_2);
+      z = z * T(2);
+      const T x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+      const T temp = T(1) - z2 / (x2 + y2);
+      const T newx = (x2 - y2) * temp;
+      const T newy = T(2) * z.x * z.y * temp;
+      const T newz = T(-2) * z.z * M::sqrt(x2 + y2);
+      z.x = newx + constant.x;
+      z.y = newy + constant.y;
+      z.z = newz + constant.z;
+      z.z *= f.fip_zfactor;
+      r = length_nc(z);
+    } else if (f.formula == 17) {  // smoothMandelbox (src/fractal.cpp:751-793)
+      const T sm = f.mb_sharp, lim = f.mb_limit, val = f.mb_value;
+      const T zk1 = smooth_gt(z.x, lim, sm), zk2 = smooth_lt(z.x, -lim, sm);
+      z.x = z.x * (T(1) - zk1) + (val - z.x) * zk1;
+      z.x = z.x * (T(1) - zk2) + (-val - z.x) * zk2;
+      tgladColor += (zk1 + zk2) * f.mb_col_X;
+      const T zk3 = smooth_gt(z.y, lim, sm), zk4 = smooth_lt(z.y, -lim, sm);
+      z.y = z.y * (T(1) - zk3) + (val - z.y) * zk3;
+      z.y = z.y * (T(1) - zk4) + (-val - z.y) * zk4;
+      tgladColor += (zk3 + zk4) * f.mb_col_Y;
+      const T zk5 = smooth_gt(z.z, lim, sm), zk6 = smooth_lt(z.z, -lim, sm);
+      z.z = z.z * (T(1) - zk5) + (val - z.z) * zk5;
+      z.z = z.z * (T(1) - zk6) + (-val - z.z) * zk6;
+      tgladColor += (zk5 + zk6) * f.mb_col_Z;
+      r = length_nc(z);
+      const T r2 = r * r;
+      const T tglad_factor2 = f.mb_fR2 / r2;
+      const T rk1 = smooth_lt(r2, f.mb_mR2, sm);
+      const T rk2 = smooth_lt(r2, f.mb_fR2, sm);
+      const T rk21 = (T(1) - rk1) * rk2;
+      z = z * (T(1) - rk1) + z * (f.mb_factor1 * rk1);
+      z = z * (T(1) - rk21) + z * (tglad_factor2 * rk21);
+      tgladDE = tgladDE * (T(1) - rk1) + tgladDE * (f.mb_factor1 * rk1);
+      tgladDE = tgladDE * (T(1) - rk21) + tgladDE * (tglad_factor2 * rk21);
+      tgladColor += rk1 * f.mb_col_Sp1;
+      tgladColor += rk21 * f.mb_col_Sp2;
+      if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
+      z = z * f.mb_scale + constant;
+      tgladDE = tgladDE * f.mb_abs_scale + T(1);
+      r = length_nc(z);
+    } else if (f.formula == 19) {  // aexion (src/fractal.cpp:916-947)
+      if (L == 0) {
+        const T cx = M::fabs(constant.x + constant.y + constant.z) + f.c_add;
+        const T cy = M::fabs(-constant.x - constant.y + constant.z) + f.c_add;
+        const T cz = M::fabs(-constant.x + constant.y - constant.z) + f.c_add;
+        const T cw = M::fabs(constant.x - constant.y - constant.z) + f.c_add;
+        constant = V3<T>{cx, cy, cz};
+        constantw = cw;
+        const T tx = M::fabs(z.x + z.y + z.z) + f.c_add;
+        const T ty = M::fabs(-z.x - z.y + z.z) + f.c_add;
+        const T tz = M::fabs(-z.x + z.y - z.z) + f.c_add;
+        const T tw = M::fabs(z.x - z.y - z.z) + f.c_add;
+        z = V3<T>{tx, ty, tz};
+        w = tw;
+      }
+      const T tx = z.x * z.x - z.y * z.y + T(2) * w * z.z + constant.x;
+      const T ty = z.y * z.y - z.x * z.x + T(2) * w * z.z + constant.y;
+      const T tz = z.z * z.z - w * w + T(2) * z.x * z.y + constant.z;
+      const T tw = w * w - z.z * z.z + T(2) * z.x * z.y + constantw;
+      z = V3<T>{tx, ty, tz};
+      w = tw;
+      r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+    } else if (f.formula == 20) {  // benesi (src/fractal.cpp:948-970)
+      T r1 = z.y * z.y + z.z * z.z;
+      T newx;
+      if (constant.x < 0 || z.x < M::sqrt(r1)) newx = z.x * z.x - r1;
+      else newx = -z.x * z.x + r1;
+      r1 = -T(1) / M::sqrt(r1) * T(2) * M::fabs(z.x);
+      const T newy = r1 * (z.y * z.y - z.z * z.z);
+      const T newz = r1 * T(2) * z.y * z.z;
+      z.x = newx + constant.x;
+      z.y = newy + constant.y;
+      z.z = newz + constant.z;
+      r = length_nc(z);
+    } else if (f.formula == 21) {  // bristorbrot (src/fractal.cpp:971-983)
+      const T newx = z.x * z.x - z.y * z.y - z.z * z.z;
+      const T newy = z.y * (T(2) * z.x - z.z);
+      const T newz = z.z * (T(2) * z.x + z.y);
+      z.x = newx + constant.x;
+      z.y = newy + constant.y;
+      z.z = newz + constant.z;
+      r = length_nc(z);
     }
 
     // delta-DE runs (src/fractal.cpp:1110-1119) stop on their own tests
@@ -261,9 +394,9 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     // counted as escaped, and equals "a > b" for every finite value.  The
     // formulas of the generic FP64 path keep the reference's "a > b" (a NaN
     // orbit never escapes).
-    if (f.formula == 7) {
+    if (formula_menger_group(f.formula)) {
       if (!(r <= T(1000))) { distance = (r - T(2)) / tgladDE; break; }
-    } else if (f.formula == 8) {
+    } else if (formula_box_group(f.formula)) {
       if (!(r <= T(1024))) { distance = r / M::fabs(tgladDE); break; }
     } else {
       if (Mode == MODE_NORMAL) {
@@ -290,7 +423,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     return distance;
   }
   if (Mode == MODE_DELTA1 || Mode == MODE_DELTA2) return r;
-  if (f.formula == 8) return tgladColor * T(100) + length_nc(z) * f.mb_col_R;
+  if (formula_box_group(f.formula)) return tgladColor * T(100) + length_nc(z) * f.mb_col_R;
   if (f.formula == 7) return min * T(1000);
   return distance * T(5000);
 }

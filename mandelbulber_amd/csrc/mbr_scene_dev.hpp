@@ -22,6 +22,7 @@ struct FractalK {
   int water_iterations, frame_no, water_rotated;
   int ipow;  // power as an integer in [2, 32] when it is one, else 0
   int linear_de;  // linear_DE_mode: delta-DE distance 0.5 r / dr instead of 0.5 r ln r / dr
+  T mb_sharp, fip_factor, fip_zfactor, c_add;  // smoothMandelbox, foldingIntPow2, aexion
   T power, bailout, constant_factor;
   T julia[3];
   T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;

@@ -124,7 +124,7 @@ std::vector<KeyDef> build_table() {
   G("stereo_enabled"); G("stereo_eye_distance");
   D("mandelbox_scale", OFF(mb_scale)); D("mandelbox_folding_limit", OFF(mb_folding_limit));
   D("mandelbox_folding_value", OFF(mb_folding_value)); D("mandelbox_folding_min_radius", OFF(mb_min_radius));
-  D("mandelbox_folding_fixed_radius", OFF(mb_fixed_radius)); G("mandelbox_sharpness");
+  D("mandelbox_folding_fixed_radius", OFF(mb_fixed_radius)); D("mandelbox_sharpness", OFF(mb_sharpness));
   D("mandelbox_offset_X", OFF_A(mb_offset, 0, double)); D("mandelbox_offset_Y", OFF_A(mb_offset, 1, double));
   D("mandelbox_offset_Z", OFF_A(mb_offset, 2, double));
   D("mandelbox_color_R", OFF(mb_color_R)); D("mandelbox_color_X", OFF(mb_color_X));
@@ -144,11 +144,11 @@ std::vector<KeyDef> build_table() {
   D("view_distance_max", OFF(view_distance_max)); D("view_distance_min", OFF(view_distance_min));
   I("interior_mode", OFF(interior_mode)); I("linear_DE_mode", OFF(linear_DE_mode));
   I("constant_DE_threshold", OFF(constant_DE_threshold));
-  G("FoldingIntPow_folding_factor"); G("FoldingIntPow_z_factor"); G("dynamic_DE_correction");
+  D("FoldingIntPow_folding_factor", OFF(folding_int_pow_factor)); D("FoldingIntPow_z_factor", OFF(folding_int_pow_z_factor)); G("dynamic_DE_correction");
   I("penetrating_lights", OFF(penetrating_lights));
   I("raytraced_reflections", OFF(raytraced_reflections)); I("reflections_max", OFF(reflections_max));
   G("mandelbox_vary_scale_vary"); G("mandelbox_vary_fold"); G("mandelbox_vary_minr");
-  G("mandelbox_vary_rpower"); G("mandelbox_vary_wadd"); G("c_add");
+  G("mandelbox_vary_rpower"); G("mandelbox_vary_wadd"); D("c_add", OFF(c_add));
   D("volumetric_fog_density", OFF(fog_density));
   D("volumetric_fog_colour_1_distance", OFF(fog_colour1_distance));
   D("volumetric_fog_colour_2_distance", OFF(fog_colour2_distance));
@@ -375,6 +375,10 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->image_gamma = 1.0;
   s->hdr = 0;
   std::strncpy(s->file_destination, "images/image", sizeof(s->file_destination) - 1);
+  s->mb_sharpness = 3.0;             // share/defaults
+  s->folding_int_pow_factor = 2.0;
+  s->folding_int_pow_z_factor = 5.0;
+  s->c_add = -1.3;
   s->ssao_enabled = 1;               // share/defaults: post_SSAO_enabled 1, quality 20
   s->ssao_quality = 20;
   s->dof_enabled = 0;

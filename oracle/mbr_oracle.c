@@ -166,6 +166,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
   v3 dz = v3m(1.0, 0.0, 0.0);
   double ph_dz = 0, th_dz = 0;
   const int delta_limit = (mode == MODE_DELTA2 && iter_count) ? *iter_count : -1;
+  double w = 0, constantw = 0;
   double p = s->power;
   double fixedRadius = s->mb_fixed_radius;
   double fR2 = fixedRadius * fixedRadius;
@@ -338,6 +339,134 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
         r = v3len(z);
         break;
       }
+      case MBR_FORMULA_FAST_TRIG:
+      case MBR_FORMULA_MINUS_FAST_TRIG: {  /* src/fractal.cpp:321-350 */
+        double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+        double temp = 1.0 - z2 / (x2 + y2);
+        double newx = (x2 - y2) * temp;
+        double newy = 2.0 * z.x * z.y * temp;
+        double newz = (formula == MBR_FORMULA_FAST_TRIG ? -2.0 : 2.0) * z.z * sqrt(x2 + y2);
+        z.x = newx + constant.x; z.y = newy + constant.y; z.z = newz + constant.z;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_HYPERCOMPLEX: {  /* :351-359 */
+        v3 nz = v3m(z.x * z.x - z.y * z.y - z.z * z.z - w * w, 2.0 * z.x * z.y - 2.0 * w * z.z,
+                    2.0 * z.x * z.z - 2.0 * z.y * w);
+        double nw = 2.0 * z.x * w - 2.0 * z.y * z.z;
+        z = v3add(nz, constant);
+        w = nw;
+        r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+        break;
+      }
+      case MBR_FORMULA_QUATERNION: {  /* :360-368 */
+        v3 nz = v3m(z.x * z.x - z.y * z.y - z.z * z.z - w * w, 2.0 * z.x * z.y, 2.0 * z.x * z.z);
+        double nw = 2.0 * z.x * w;
+        z = v3add(nz, constant);
+        w = nw;
+        r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+        break;
+      }
+      case MBR_FORMULA_FOLDING_INT_POW2: {  /* :794-836 */
+        double ff = s->folding_int_pow_factor;
+        if (z.x > ff) z.x = ff * 2.0 - z.x; else if (z.x < -ff) z.x = -ff * 2.0 - z.x;
+        if (z.y > ff) z.y = ff * 2.0 - z.y; else if (z.y < -ff) z.y = -ff * 2.0 - z.y;
+        if (z.z > ff) z.z = ff * 2.0 - z.z; else if (z.z < -ff) z.z = -ff * 2.0 - z.z;
+        r = v3len(z);
+        double fR2_2 = 1.0, mR2_2 = 0.25, r2_2 = r * r, tf1_2 = fR2_2 / mR2_2;
+        if (r2_2 < mR2_2) z = v3mul(z, tf1_2);
+        else if (r2_2 < fR2_2) { double tf2_2 = fR2_2 / r2_2; z = v3mul(z, tf2_2); }
+        z = v3mul(z, 2.0);
+        double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+        double temp = 1.0 - z2 / (x2 + y2);
+        double newx = (x2 - y2) * temp;
+        double newy = 2.0 * z.x * z.y * temp;
+        double newz = -2.0 * z.z * sqrt(x2 + y2);
+        z.x = newx + constant.x; z.y = newy + constant.y; z.z = newz + constant.z;
+        z.z *= s->folding_int_pow_z_factor;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_SMOOTH_MANDELBOX: {  /* :751-793 */
+        double sm = s->mb_sharpness, lim = s->mb_folding_limit, val = s->mb_folding_value;
+#define SGT(a, b) (1.0 / (1.0 + exp(sm * ((b) - (a)))))
+#define SLT(a, b) (1.0 / (1.0 + exp(sm * ((a) - (b)))))
+        double zk1 = SGT(z.x, lim), zk2 = SLT(z.x, -lim);
+        z.x = z.x * (1.0 - zk1) + (val - z.x) * zk1;
+        z.x = z.x * (1.0 - zk2) + (-val - z.x) * zk2;
+        tgladColor += (zk1 + zk2) * s->mb_color_X;
+        double zk3 = SGT(z.y, lim), zk4 = SLT(z.y, -lim);
+        z.y = z.y * (1.0 - zk3) + (val - z.y) * zk3;
+        z.y = z.y * (1.0 - zk4) + (-val - z.y) * zk4;
+        tgladColor += (zk3 + zk4) * s->mb_color_Y;
+        double zk5 = SGT(z.z, lim), zk6 = SLT(z.z, -lim);
+        z.z = z.z * (1.0 - zk5) + (val - z.z) * zk5;
+        z.z = z.z * (1.0 - zk6) + (-val - z.z) * zk6;
+        tgladColor += (zk5 + zk6) * s->mb_color_Z;
+        r = v3len(z);
+        double r2 = r * r;
+        double tglad_factor2 = fR2 / r2;
+        double rk1 = SLT(r2, mR2), rk2 = SLT(r2, fR2);
+        double rk21 = (1.0 - rk1) * rk2;
+#undef SGT
+#undef SLT
+        z = v3add(v3mul(z, 1.0 - rk1), v3mul(z, tglad_factor1 * rk1));
+        z = v3add(v3mul(z, 1.0 - rk21), v3mul(z, tglad_factor2 * rk21));
+        tgladDE = tgladDE * (1.0 - rk1) + tgladDE * (tglad_factor1 * rk1);
+        tgladDE = tgladDE * (1.0 - rk21) + tgladDE * (tglad_factor2 * rk21);
+        tgladColor += rk1 * s->mb_color_Sp1;
+        tgladColor += rk21 * s->mb_color_Sp2;
+        z = rot_apply(&f->mb_main, z);
+        z = v3add(v3mul(z, scale), constant);
+        tgladDE = tgladDE * fabs(scale) + 1.0;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_AEXION: {  /* :916-947 */
+        double ca = s->c_add;
+        if (L == 0) {
+          double cx = fabs(constant.x + constant.y + constant.z) + ca;
+          double cy = fabs(-constant.x - constant.y + constant.z) + ca;
+          double cz = fabs(-constant.x + constant.y - constant.z) + ca;
+          double cw = fabs(constant.x - constant.y - constant.z) + ca;
+          constant = v3m(cx, cy, cz);
+          constantw = cw;
+          double tx = fabs(z.x + z.y + z.z) + ca;
+          double ty = fabs(-z.x - z.y + z.z) + ca;
+          double tz = fabs(-z.x + z.y - z.z) + ca;
+          double tw = fabs(z.x - z.y - z.z) + ca;
+          z = v3m(tx, ty, tz);
+          w = tw;
+        }
+        double tx = z.x * z.x - z.y * z.y + 2.0 * w * z.z + constant.x;
+        double ty = z.y * z.y - z.x * z.x + 2.0 * w * z.z + constant.y;
+        double tz = z.z * z.z - w * w + 2.0 * z.x * z.y + constant.z;
+        double tw = w * w - z.z * z.z + 2.0 * z.x * z.y + constantw;
+        z = v3m(tx, ty, tz);
+        w = tw;
+        r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+        break;
+      }
+      case MBR_FORMULA_BENESI: {  /* :948-970 */
+        double r1 = z.y * z.y + z.z * z.z;
+        double newx;
+        if (constant.x < 0 || z.x < sqrt(r1)) newx = z.x * z.x - r1;
+        else newx = -z.x * z.x + r1;
+        r1 = -1.0 / sqrt(r1) * 2.0 * fabs(z.x);
+        double newy = r1 * (z.y * z.y - z.z * z.z);
+        double newz = r1 * 2.0 * z.y * z.z;
+        z.x = newx + constant.x; z.y = newy + constant.y; z.z = newz + constant.z;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_BRISTORBROT: {  /* :971-983 */
+        double newx = z.x * z.x - z.y * z.y - z.z * z.z;
+        double newy = z.y * (2.0 * z.x - z.z);
+        double newz = z.z * (2.0 * z.x + z.y);
+        z.x = newx + constant.x; z.y = newy + constant.y; z.z = newz + constant.z;
+        r = v3len(z);
+        break;
+      }
       case MBR_FORMULA_XENODREAMBUIE: {
         double rp = pow(r, p);
         double th = atan2(z.y, z.x);
@@ -363,7 +492,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
 
     if (formula == MBR_FORMULA_MENGER_SPONGE) {
       if (r > 1000) { distance = (r - 2.0) / tgladDE; break; }
-    } else if (formula == MBR_FORMULA_TGLAD) {
+    } else if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX) {
       if (r > 1024) { distance = r / fabs(tgladDE); break; }
     } else {
       if (mode == MODE_NORMAL) {
@@ -387,7 +516,8 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     return distance;
   }
   if (mode == MODE_DELTA1 || mode == MODE_DELTA2) return r;
-  if (formula == MBR_FORMULA_TGLAD) return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
+  if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX)
+    return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
   if (formula == MBR_FORMULA_MENGER_SPONGE) return min * 1000.0;
   return distance * 5000.0;
 }
@@ -465,7 +595,8 @@ static double calc_distance(const fractal_ctx *f, v3 point, double detailSize, i
   }
 
   const int analytic = s->formula == MBR_FORMULA_TRIG_DE || s->formula == MBR_FORMULA_TRIG_OPTIM ||
-                       s->formula == MBR_FORMULA_MENGER_SPONGE || s->formula == MBR_FORMULA_TGLAD;
+                       s->formula == MBR_FORMULA_MENGER_SPONGE || s->formula == MBR_FORMULA_TGLAD ||
+                       s->formula == MBR_FORMULA_SMOOTH_MANDELBOX;
   if (!s->primitive_only_plane && !analytic) {
     /* delta DE (src/fractal.cpp:1307-1342): orbit radius at the point and
      * at +1e-10 along x, y, z, the point updated in place as there */

@@ -14,8 +14,10 @@ from metrics import assert_parity, compare
 
 pytestmark = pytest.mark.gpu
 
-FORMULAS = [(1, 8.0, "trig_DE"), (11, 5.0, "xenodreambuie"), (13, 2.0, "mandelbulb2"), (14, 2.0, "mandelbulb3"),
-            (15, 8.0, "mandelbulb4")]
+FORMULAS = [(1, 8.0, "trig_DE"), (3, 2.0, "fast_trig"), (4, 2.0, "hypercomplex"), (5, 2.0, "quaternion"),
+            (6, 2.0, "minus_fast_trig"), (11, 5.0, "xenodreambuie"), (13, 2.0, "mandelbulb2"),
+            (14, 2.0, "mandelbulb3"), (15, 8.0, "mandelbulb4"), (16, 2.0, "foldingIntPow2"),
+            (17, 8.0, "smoothMandelbox"), (19, 2.0, "aexion"), (20, 2.0, "benesi"), (21, 2.0, "bristorbrot")]
 
 
 @pytest.fixture(scope="module")
@@ -32,7 +34,10 @@ def renderer():
 def _scene(formula, power, shaded):
     # the binary search ends anywhere in its band [0.99 thr, thr], thr ~ depth / width: at 256
     # pixels a different (noisy) delta-DE sequence moves a hit by up to ~4e-5 of its depth
-    s = scene.mandelbulb_full(192, 144, seed=1) if shaded else scene.mandelbulb_plain(256, 192)
+    if formula == 17:
+        s = scene.mandelbox_view(192, 144, shaded=shaded)
+    else:
+        s = scene.mandelbulb_full(192, 144, seed=1) if shaded else scene.mandelbulb_plain(256, 192)
     s.formula = formula
     s.power = power
     s.N = 40
@@ -63,8 +68,8 @@ def test_generic_formula_matches_oracle(renderer, formula, power, name, shaded):
         assert_parity(m, depth_rel=1e-3, label=f"{name} shaded={shaded}")
     else:
         assert_parity(m, label=f"{name} shaded={shaded}")
-    if formula == 1:
-        # trig_DE is analytic: no finite differences, the FP64 path reproduces the oracle
+    if formula in (1, 17):
+        # trig_DE and smoothMandelbox are analytic: no finite differences, the FP64 path reproduces the oracle
         assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
     for prec, pname in [(_abi.PREC_FP32_FIXUP, "fp32"), (_abi.PREC_FP32_MARCH, "fp32-march")]:
         print(f"[formulas] {name} shaded={shaded} {pname} (rendered in FP64)", flush=True)

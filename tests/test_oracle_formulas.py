@@ -115,3 +115,56 @@ def test_delta_de_far_field_is_a_distance_scale(formula, power):
         d, mi, _, _ = oracle.calculate_distance(s, p)
         rp = math.sqrt(sum(v * v for v in p))
         assert not mi and 0.1 * rp < d < 2.0 * rp, (formula, p, d)
+
+
+def _mandelbrot_escape(c, n, bailout=100.0):
+    # the orbit starts at the point itself (z0 = c), as Compute's does
+    z = c
+    for k in range(n):
+        z = z * z + c
+        if abs(z) > bailout:
+            return k
+    return n
+
+
+@pytest.mark.parametrize("formula", [3, 4, 5, 6, 21])
+def test_z0_plane_is_the_mandelbrot_set(formula):
+    """fast_trig, hypercomplex, quaternion, minus_fast_trig and bristorbrot
+    keep z = 0 (and w = 0) in the z = 0 plane, where each is z -> z^2 + c
+    on x + iy: the escape iteration equals the textbook Mandelbrot one."""
+    s = _scene(formula, 2.0, n=60)
+    rng = np.random.default_rng(formula)
+    checked = 0
+    for _ in range(400):
+        x, y = rng.uniform(-2.2, 1.2), rng.uniform(-1.4, 1.4)
+        want = _mandelbrot_escape(complex(x, y), s.N, s.bailout)
+        # points whose |z| lands within 1e-6 of the bailout could round either way
+        _, it = oracle.compute(s, (x, y, 0.0), mode=0)
+        if want < s.N:
+            z = complex(x, y)
+            for k in range(want + 1):
+                z = z * z + complex(x, y)
+            if abs(abs(z) - s.bailout) < 1e-6 * s.bailout:
+                continue
+        assert it == want, (formula, x, y, it, want)
+        checked += 1
+    assert checked > 350
+
+
+def test_smooth_mandelbox_tends_to_the_mandelbox():
+    """With a very sharp smooth step the smooth folds are the box and sphere
+    folds: the distance estimate equals tglad's away from the fold edges."""
+    b = S.mandelbox_view(64, 48, shaded=False)
+    a = S.copy(b)
+    a.formula = 17
+    a.mb_sharpness = 1e6
+    rng = np.random.default_rng(5)
+    n = 0
+    for _ in range(200):
+        p = tuple(rng.uniform(-4.0, 4.0, 3))
+        da, _, ia, _ = oracle.calculate_distance(a, p)
+        db, _, ib, _ = oracle.calculate_distance(b, p)
+        assert ia == ib, p
+        assert da == pytest.approx(db, rel=1e-6, abs=1e-300), p
+        n += db > 1e-3
+    assert n > 20
