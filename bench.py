@@ -249,6 +249,9 @@ def bench_post(torch, r, _abi, sc, planes, with_cpu):
         return (time.perf_counter() - t0) / n * 1e3
 
     work = img_dev.clone()
+    compile_ms = timed_call(lambda: r.compile_image_device(sc, planes["rgb"], work), 3)
+    out["compile_image"] = {"ms": compile_ms, "equal_to_host": bool(np.array_equal(
+        work.cpu().numpy().view(np.uint16), img))}
     ssao_ms = timed_call(lambda: r.post_ssao(sc, planes["z"], planes["opacity"], [col[0], col[1], col[2]],
                                              work.copy_(img_dev)), 3)
     out["ssao"] = {"quality": sc.ssao_quality, "ms": ssao_ms,

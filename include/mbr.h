@@ -423,6 +423,14 @@ int mbr_render_batch(mbr_ctx *ctx, const mbr_scene *scenes, int n,
 int mbr_compile_image(const mbr_scene *scene, int width, int height,
                       const float *const rgb[3], uint16_t *image16);
 
+/* mbr_compile_image on the GPU: the float planes and image16 are DEVICE
+ * memory; queued on `stream` (NULL = the context's) without a host wait.
+ * The same tone map and gamma table (computed on the host with the host
+ * libm, as the reference's), so the result equals mbr_compile_image's
+ * (hdr = 1 uses the device tanhf: equal up to its last bit). */
+int mbr_compile_image_device(mbr_ctx *ctx, const mbr_scene *scene, int width, int height,
+                             const float *const rgb[3], uint16_t *image16, void *stream);
+
 /* SavePNG / SavePNG16 / SavePNG16Alpha (src/files.cpp): write image16 (and
  * the alpha plane when not NULL) as an 8- or 16-bit PNG. */
 int mbr_write_png(const char *path, int width, int height, const uint16_t *image16,

@@ -293,6 +293,8 @@ struct mbr_ctx {
   // the host-memory calls
   double *d_ssao_tab = nullptr;
   int ssao_tab_q = 0;
+  uint16_t *d_gamma = nullptr;  // gamma table of the last compile
+  double gamma_of_table = 0.0;
   void *d_post = nullptr;
   size_t d_post_bytes = 0;
   // DOF: per-pixel buffers and the (tile, splat) lists
@@ -516,7 +518,7 @@ void mbr_destroy(mbr_ctx *c) {
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
   void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad, c->d_ssao_tab,
-                c->d_post, c->d_dof, c->d_dof_pairs};
+                c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
@@ -850,6 +852,31 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->ms_ao = ms_ao;
     stats->ms_total = ms_march + ms_shade + ms_finish;
   }
+  return MBR_OK;
+}
+
+int mbr_compile_image_device(mbr_ctx *c, const mbr_scene *s, int w, int h, const float *const rgb[3], uint16_t *image16,
+                             void *stream) {
+  if (!c || !s || !rgb || !rgb[0] || !rgb[1] || !rgb[2] || !image16 || w <= 0 || h <= 0)
+    return MBR_ERR_INVALID_ARG;
+  HIPCHECK(c, hipSetDevice(c->device));
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  if (!c->d_gamma || c->gamma_of_table != s->image_gamma) {
+    // CalculateGammaTable (src/cimage.cpp:176-182) with the host libm
+    std::vector<uint16_t> gamma(65536);
+    for (int i = 0; i < 65536; i++) gamma[i] = (uint16_t)(int)(std::pow(i / 65536.0, 1.0 / s->image_gamma) * 65535);
+    if (!c->d_gamma) HIPCHECK(c, hipMalloc(&c->d_gamma, 65536 * sizeof(uint16_t)));
+    else HIPCHECK(c, hipDeviceSynchronize());  // an earlier compile may still read the old table
+    HIPCHECK(c, hipMemcpyAsync(c->d_gamma, gamma.data(), 65536 * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    HIPCHECK(c, hipStreamSynchronize(st));
+    c->gamma_of_table = s->image_gamma;
+  }
+  CompileParams p;
+  p.n = w * h;
+  p.hdr = s->hdr;
+  p.brightness = (float)s->brightness;
+  p.contrast = s->contrast;
+  HIPCHECK(c, launch_compile(p, rgb[0], rgb[1], rgb[2], c->d_gamma, image16, st));
   return MBR_OK;
 }
 

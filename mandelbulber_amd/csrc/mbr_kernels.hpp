@@ -134,6 +134,15 @@ struct SsaoParams {
 hipError_t launch_ssao(const SsaoParams &p, const float *z, const uint16_t *opacity, const uint8_t *const colour[3],
                        uint16_t *image16, hipStream_t s);
 
+// CompileImage on the device (mbr_post.hip): gamma = 65536-entry table.
+struct CompileParams {
+  int n, hdr;
+  float brightness;
+  double contrast;
+};
+hipError_t launch_compile(const CompileParams &p, const float *r, const float *g, const float *b,
+                          const uint16_t *gamma, uint16_t *image16, hipStream_t s);
+
 // Depth of field (mbr_post.hip, mbr_post_order.cpp).  tiles_x/tiles_y:
 // 16x16 pixel tiles; rank_bits: bits of a splat rank in the sort key.
 struct DofParams {

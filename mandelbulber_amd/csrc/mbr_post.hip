@@ -310,6 +310,32 @@ hipError_t dof_fold(const DofParams &p, const float *z, const unsigned long long
   return hipGetLastError();
 }
 
+// CompileImage (src/cimage.cpp:132-213), one lane per pixel, in the order of
+// mbr_compile_image: FP32 brightness, FP64 contrast, clamp, tanh, clamp,
+// 16-bit index into the gamma table.
+__global__ void __launch_bounds__(256) k_compile(CompileParams p, const float *__restrict__ r,
+                                                 const float *__restrict__ g, const float *__restrict__ b,
+                                                 const uint16_t *__restrict__ gamma, uint16_t *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.n) return;
+  const float in[3] = {r[i], g[i], b[i]};
+  for (int k = 0; k < 3; k++) {
+    float v = in[k] * p.brightness;
+    v = (float)((v - 0.5) * p.contrast + 0.5);
+    if (v < 0.0f) v = 0.0f;
+    if (p.hdr) v = tanhf(v);
+    if (v > 1.0f) v = 1.0f;
+    out[3 * (size_t)i + k] = gamma[(uint16_t)(v * 65535.0)];
+  }
+}
+
+hipError_t launch_compile(const CompileParams &p, const float *r, const float *g, const float *b,
+                          const uint16_t *gamma, uint16_t *image16, hipStream_t s) {
+  hipLaunchKernelGGL(k_compile, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s, p, r, g, b, gamma, image16);
+  return hipGetLastError();
+}
+
 hipError_t launch_ssao(const SsaoParams &p, const float *z, const uint16_t *opacity, const uint8_t *const colour[3],
                        uint16_t *image16, hipStream_t s) {
   const dim3 grid((unsigned)((p.width + kTile - 1) / kTile), (unsigned)((p.height + kTile - 1) / kTile));

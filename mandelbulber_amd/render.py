@@ -151,6 +151,19 @@ class Renderer:
                                   stream if on_device else None), self._ctx)
         return image16
 
+    def compile_image_device(self, scene, rgb, image16, stream=None):
+        """CompileImage on the GPU: rgb (3, H, W) float32 and image16 (H, W, 3)
+        int16/uint16 CUDA tensors; queued on `stream` (default: torch's)."""
+        _, h, w = rgb.shape
+        assert rgb.is_contiguous() and image16.is_contiguous() and tuple(image16.shape) == (h, w, 3)
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        planes = (C.c_void_p * 3)(*[rgb[c].data_ptr() for c in range(3)])
+        check(lib().mbr_compile_image_device(self._ctx, C.byref(scene), w, h, planes, image16.data_ptr(), stream),
+              self._ctx)
+        return image16
+
     def post_dof(self, scene, z, image16, alpha=None, stream=None):
         """PostRendering_DOF on one frame (numpy: host call, image16 and alpha
         updated in place; torch CUDA tensors: queued on `stream`)."""

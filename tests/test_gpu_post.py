@@ -105,3 +105,25 @@ def test_dof_matches_oracle(renderer, radius, focus_quantile):
     torch.cuda.synchronize()
     _check(f"dof device r{radius} rgb", im.cpu().numpy().view(np.uint16), want_img)
     _check(f"dof device r{radius} alpha", a.cpu().numpy().view(np.uint16), want_a)
+
+
+@pytest.mark.parametrize("hdr", [0, 1])
+def test_compile_image_device_equals_host(renderer, hdr):
+    """CompileImage on the GPU against the host restatement (same table);
+    with the HDR curve the device tanhf may differ in its last bit, which can
+    move a value to the neighbouring gamma-table entry."""
+    import torch
+    s = scene.mandelbulb_full(160, 120, seed=3)
+    s.brightness, s.contrast, s.image_gamma, s.hdr = 1.4, 1.2, 2.2, hdr
+    renderer.set_scene(s)
+    frame, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
+    want = compile_image(s, frame.rgb)
+    dev = torch.device("cuda:0")
+    rgb = torch.from_numpy(frame.rgb).to(dev)
+    out = torch.zeros((120, 160, 3), dtype=torch.int16, device=dev)
+    renderer.compile_image_device(s, rgb, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.uint16)
+    n_diff = int(np.count_nonzero(got != want))
+    print(f"[post] compile hdr {hdr}: {n_diff} of {got.size} values differ", flush=True)
+    assert n_diff == 0 if not hdr else n_diff <= 1e-3 * got.size
