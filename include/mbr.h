@@ -252,6 +252,13 @@ typedef struct mbr_scene {
   int hdr;                         /* hdr */
   char file_destination[512];      /* file_destination (path prefix of the images) */
 
+  /* animation (src/Render3D.cpp:1995-2310) */
+  int start_frame;                 /* start_frame */
+  int end_frame;                   /* end_frame */
+  int frames_per_keyframe;         /* frames_per_keyframe */
+  char file_animation_path[512];   /* file_animation_path (flight path: vp x y z, alfa beta gamma per line) */
+  char file_keyframes[512];        /* file_keyframes (prefix of the keyframe .fract files) */
+
   /* post effects on the compiled image (src/image.cpp; order in the
    * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
   int linear_DE_mode;              /* linear_DE_mode (delta-DE formulas: 0.5 r / dr) */
@@ -470,6 +477,15 @@ int mbr_post_ssao(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *p
  * the call waits for the device once (the size of the splat lists). */
 int mbr_post_dof(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *planes,
                  int on_device, void *stream);
+
+/* Keyframe animation (CMorph::CatmullRom, src/morph.cpp:49-109, applied as
+ * src/Render3D.cpp:2278-2295 does): every double parameter of `scene` (the
+ * double keys of the settings table, and N, which the reference keeps as a
+ * double) becomes the Catmull-Rom spline through the n_keys keyframe values
+ * at `frame` (frames_per_key frames per key; logarithmic where all four
+ * support values share a sign and vary by more than 10% of their mean);
+ * the other fields of `scene` are left as they are. */
+int mbr_keyframe_scene(const mbr_scene *keys, int n_keys, int frames_per_key, int frame, mbr_scene *scene);
 
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,

@@ -89,6 +89,8 @@ class Scene(C.Structure):
         ("jitter_mode", C.c_int), ("jitter_seed", C.c_uint32), ("hit_refine", C.c_int),
         ("brightness", C.c_double), ("contrast", C.c_double), ("image_gamma", C.c_double), ("hdr", C.c_int),
         ("file_destination", C.c_char * 512),
+        ("start_frame", C.c_int), ("end_frame", C.c_int), ("frames_per_keyframe", C.c_int),
+        ("file_animation_path", C.c_char * 512), ("file_keyframes", C.c_char * 512),
         ("linear_DE_mode", C.c_int), ("mb_sharpness", C.c_double), ("folding_int_pow_factor", C.c_double),
         ("folding_int_pow_z_factor", C.c_double), ("c_add", C.c_double), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
         ("dof_focus", C.c_double), ("dof_radius", C.c_double),

@@ -1,7 +1,8 @@
 // mbr_nogui.cpp — command-line renderer over the C ABI, the -nogui driver of
-// the reference (ReadComandlineParams, src/interface.cpp:3334-3561; the
-// still-image branch of MainRender, src/Render3D.cpp:2305-2470): load a
-// settings file with overrides, render it, compile the image, write it.
+// the reference (ReadComandlineParams, src/interface.cpp:3334-3561; the frame
+// loop of MainRender, src/Render3D.cpp:1995-2470): load a settings file with
+// overrides, render one frame or an animation (flight path or keyframes),
+// compile each image, apply the post effects, write it.
 //
 //   mbr_nogui [options...] settings_file
 //     -nogui              accepted (this driver has no GUI)
@@ -10,15 +11,18 @@
 //     -fpk N              override frames_per_keyframe
 //     -format FORMAT      jpg | png | png16 | png16alpha
 //     -start N, -end N    frame range (animation modes)
+//     -flight             play the flight path file_animation_path (one line
+//                         "vp_x vp_y vp_z alfa beta gamma" per frame, radians)
+//     -keyframe           interpolate the keyframes file_keyframes%05d.fract
+//                         (Catmull-Rom, frames_per_keyframe per key)
 //     -precision P        fp32 (default) | fp64 | fp32march   (this driver)
 //     -out PATH           write the image to PATH instead of
 //                         file_destination + 00000 + extension (this driver)
 //
 // The image goes to file_destination + frame number (%05d) + ".png", as
 // IndexFilename (src/files.cpp:30-35) names it.  There is no JPEG encoder in
-// this build: -format jpg writes an 8-bit PNG.  Flight and keyframe
-// animation (-flight, -keyframe) and network rendering (-ip, -port) are not
-// part of the render core and are rejected.
+// this build: -format jpg writes an 8-bit PNG.  Network rendering (-ip,
+// -port) is not part of the render core and is rejected.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -41,6 +45,8 @@ void usage() {
       "  -fpk N            - override frames per key parameter\n"
       "  -start N          - start rendering from frame number N\n"
       "  -end N            - rendering will end on frame number N\n"
+      "  -flight           - render the flight path (file_animation_path)\n"
+      "  -keyframe         - render the keyframe animation (file_keyframes)\n"
       "  -format FORMAT    - image output format: jpg (written as 8-bit PNG), png, png16, png16alpha\n"
       "  -precision P      - fp32 (FP64 march, FP32 shading; default), fp64, fp32march\n"
       "  -out PATH         - output file (default: file_destination + frame + .png)\n");
@@ -52,104 +58,25 @@ void make_parent(const std::string &path) {
   }
 }
 
-}  // namespace
+// IndexFilename (src/files.cpp:30-35)
+std::string index_filename(const char *prefix, const char *ext, int number) {
+  char idx[16];
+  std::snprintf(idx, sizeof idx, "%05d", number);
+  return std::string(prefix) + idx + "." + ext;
+}
 
-int main(int argc, char **argv) {
-  std::vector<std::string> overrides;
-  std::string settings, format = "jpg", out;
-  int precision = MBR_PREC_FP32_FIXUP;
-  int start_frame = 0;
-  for (int i = 1; i < argc; i++) {
-    const std::string a = argv[i];
-    auto need = [&](const char *what) -> const char * {
-      if (i + 1 >= argc) {
-        std::printf("commandline: %s needs an argument\n", what);
-        usage();
-        std::exit(2);
-      }
-      return argv[++i];
-    };
-    if (a == "-nogui") continue;
-    if (a == "-help" || a == "--help") {
-      usage();
-      return 0;
-    }
-    if (a == "-flight" || a == "-keyframe" || a == "-ip" || a == "-port") {
-      std::printf("commandline: %s (animation / network rendering) is not part of this render core\n", a.c_str());
-      return 2;
-    }
-    if (a == "-o") { overrides.push_back(need("-o")); continue; }
-    if (a == "-res") {
-      int w = 0, h = 0;
-      if (std::sscanf(need("-res"), "%dx%d", &w, &h) != 2 || w <= 0 || h <= 0) {
-        std::printf("commandline: Argument must be -res WWWWxHHHH\n");
-        return 2;
-      }
-      overrides.push_back("image_width=" + std::to_string(w));
-      overrides.push_back("image_height=" + std::to_string(h));
-      continue;
-    }
-    if (a == "-fpk") {
-      const int fpk = std::atoi(need("-fpk"));
-      if (fpk <= 0) {
-        std::printf("commandline: argument must be -fpk NNN\n");
-        return 2;
-      }
-      overrides.push_back("frames_per_keyframe=" + std::to_string(fpk));
-      continue;
-    }
-    if (a == "-start") { start_frame = std::atoi(need("-start")); continue; }
-    if (a == "-end") { (void)need("-end"); continue; }
-    if (a == "-format") {
-      format = need("-format");
-      if (format != "jpg" && format != "png" && format != "png16" && format != "png16alpha") {
-        std::printf("commandline: wrong image format\n");
-        return 2;
-      }
-      continue;
-    }
-    if (a == "-precision") {
-      const std::string p = need("-precision");
-      if (p == "fp64") precision = MBR_PREC_FP64;
-      else if (p == "fp32") precision = MBR_PREC_FP32_FIXUP;
-      else if (p == "fp32march") precision = MBR_PREC_FP32_MARCH;
-      else {
-        std::printf("commandline: precision must be fp32, fp64 or fp32march\n");
-        return 2;
-      }
-      continue;
-    }
-    if (a == "-out") { out = need("-out"); continue; }
-    if (!a.empty() && a[0] == '-') {
-      std::printf("commandline: wrong parameters\n");
-      usage();
-      return 2;
-    }
-    settings = a;
-  }
-  if (settings.empty()) {
-    usage();
-    return 2;
-  }
-  std::vector<const char *> ov;
-  for (const std::string &o : overrides) ov.push_back(o.c_str());
-  mbr_scene s;
-  int rc = mbr_load_settings(settings.c_str(), ov.empty() ? nullptr : ov.data(), (int)ov.size(), &s);
-  if (rc != MBR_OK) {
-    std::printf("cannot load settings %s (status %d)\n", settings.c_str(), rc);
-    return 1;
-  }
-  s.frame_no = start_frame;
-  mbr_ctx *ctx = nullptr;
-  rc = mbr_init(nullptr, &ctx);
-  if (rc != MBR_OK) {
-    std::printf("mbr_init failed (status %d): no usable GPU\n", rc);
-    return 1;
-  }
-  rc = mbr_set_scene(ctx, &s, nullptr);
+bool file_exists(const std::string &path) {
+  struct stat st;
+  return stat(path.c_str(), &st) == 0;
+}
+
+// Render one frame of scene s, compile it, apply the post effects and write
+// it to `out` (or file_destination + index).  0 on success.
+int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::string &format, std::string out,
+                 int index) {
+  int rc = mbr_set_scene(ctx, &s, nullptr);
   if (rc != MBR_OK) {
     std::printf("scene rejected: %s\n", mbr_last_error(ctx));
-    mbr_destroy(ctx);
     return 1;
   }
   const int w = s.image_width, h = s.image_height;
@@ -178,7 +105,6 @@ int main(int argc, char **argv) {
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (rc != MBR_OK) {
     std::printf("render failed: %s\n", mbr_last_error(ctx));
-    mbr_destroy(ctx);
     return 1;
   }
   // the reference's end-of-render report (src/Render3D.cpp:638-652)
@@ -220,20 +146,184 @@ int main(int argc, char **argv) {
     if (rc != MBR_OK) std::printf("DOF failed: %s\n", mbr_last_error(ctx));
     else std::printf("DOF rendered (focus %g, radius %g)\n", s.dof_focus, s.dof_radius);
   }
-  if (out.empty()) {
-    char idx[16];
-    std::snprintf(idx, sizeof idx, "%05d", start_frame);
-    out = std::string(s.file_destination) + idx + ".png";
-  }
+  if (out.empty()) out = index_filename(s.file_destination, "png", index);
   make_parent(out);
   const int bits = (format == "png16" || format == "png16alpha") ? 16 : 8;
-  if (rc == MBR_OK) rc = mbr_write_png(out.c_str(), w, h, img16.data(), format == "png16alpha" ? alpha.data() : nullptr,
-                                       bits);
-  mbr_destroy(ctx);
+  if (rc == MBR_OK)
+    rc = mbr_write_png(out.c_str(), w, h, img16.data(), format == "png16alpha" ? alpha.data() : nullptr, bits);
   if (rc != MBR_OK) {
     std::printf("cannot write %s (status %d)\n", out.c_str(), rc);
     return 1;
   }
   std::printf("image saved: %s%s\n", out.c_str(), format == "jpg" ? " (8-bit PNG: no JPEG encoder in this build)" : "");
   return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::vector<std::string> overrides;
+  std::string settings, format = "jpg", out;
+  int precision = MBR_PREC_FP32_FIXUP;
+  int start_frame = -1, end_frame = -1;
+  bool flight = false, keyframe = false;
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto need = [&](const char *what) -> const char * {
+      if (i + 1 >= argc) {
+        std::printf("commandline: %s needs an argument\n", what);
+        usage();
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (a == "-nogui") continue;
+    if (a == "-help" || a == "--help") {
+      usage();
+      return 0;
+    }
+    if (a == "-flight") { flight = true; std::printf("commandline: flight mode\n"); continue; }
+    if (a == "-keyframe") { keyframe = true; std::printf("commandline: keyframe mode\n"); continue; }
+    if (a == "-ip" || a == "-port") {
+      std::printf("commandline: %s (network rendering) is not part of this render core\n", a.c_str());
+      return 2;
+    }
+    if (a == "-o") { overrides.push_back(need("-o")); continue; }
+    if (a == "-res") {
+      int w = 0, h = 0;
+      if (std::sscanf(need("-res"), "%dx%d", &w, &h) != 2 || w <= 0 || h <= 0) {
+        std::printf("commandline: Argument must be -res WWWWxHHHH\n");
+        return 2;
+      }
+      overrides.push_back("image_width=" + std::to_string(w));
+      overrides.push_back("image_height=" + std::to_string(h));
+      continue;
+    }
+    if (a == "-fpk") {
+      const int fpk = std::atoi(need("-fpk"));
+      if (fpk <= 0) {
+        std::printf("commandline: argument must be -fpk NNN\n");
+        return 2;
+      }
+      overrides.push_back("frames_per_keyframe=" + std::to_string(fpk));
+      continue;
+    }
+    if (a == "-start") { start_frame = std::atoi(need("-start")); continue; }
+    if (a == "-end") { end_frame = std::atoi(need("-end")); continue; }
+    if (a == "-format") {
+      format = need("-format");
+      if (format != "jpg" && format != "png" && format != "png16" && format != "png16alpha") {
+        std::printf("commandline: wrong image format\n");
+        return 2;
+      }
+      continue;
+    }
+    if (a == "-precision") {
+      const std::string p = need("-precision");
+      if (p == "fp64") precision = MBR_PREC_FP64;
+      else if (p == "fp32") precision = MBR_PREC_FP32_FIXUP;
+      else if (p == "fp32march") precision = MBR_PREC_FP32_MARCH;
+      else {
+        std::printf("commandline: precision must be fp32, fp64 or fp32march\n");
+        return 2;
+      }
+      continue;
+    }
+    if (a == "-out") { out = need("-out"); continue; }
+    if (!a.empty() && a[0] == '-') {
+      std::printf("commandline: wrong parameters\n");
+      usage();
+      return 2;
+    }
+    settings = a;
+  }
+  if (settings.empty() || (flight && keyframe)) {
+    usage();
+    return 2;
+  }
+  std::vector<const char *> ov;
+  for (const std::string &o : overrides) ov.push_back(o.c_str());
+  mbr_scene s;
+  int rc = mbr_load_settings(settings.c_str(), ov.empty() ? nullptr : ov.data(), (int)ov.size(), &s);
+  if (rc != MBR_OK) {
+    std::printf("cannot load settings %s (status %d)\n", settings.c_str(), rc);
+    return 1;
+  }
+  mbr_ctx *ctx = nullptr;
+  rc = mbr_init(nullptr, &ctx);
+  if (rc != MBR_OK) {
+    std::printf("mbr_init failed (status %d): no usable GPU\n", rc);
+    return 1;
+  }
+  int status = 0;
+  if (!flight && !keyframe) {
+    // a still image: frame number -start (0 by default)
+    const int index = start_frame > 0 ? start_frame : 0;
+    s.frame_no = index;
+    status = render_frame(ctx, s, precision, format, out, index);
+    mbr_destroy(ctx);
+    return status;
+  }
+  // animation frame range (src/Render3D.cpp:2088-2099)
+  int first = s.start_frame, last = s.end_frame;
+  if (start_frame > 0) first = start_frame;
+  if (end_frame >= 0 && end_frame < 99999) last = end_frame;
+  if (keyframe) {
+    // keyframes: file_keyframes + %05d + .fract while they exist (:2020-2068)
+    std::vector<mbr_scene> keys;
+    for (int k = 0;; k++) {
+      const std::string path = index_filename(s.file_keyframes, "fract", k);
+      if (!file_exists(path)) break;
+      mbr_scene ks;
+      rc = mbr_load_settings(path.c_str(), nullptr, 0, &ks);
+      if (rc != MBR_OK) {
+        std::printf("cannot load keyframe %s (status %d)\n", path.c_str(), rc);
+        mbr_destroy(ctx);
+        return 1;
+      }
+      keys.push_back(ks);
+    }
+    std::printf("Found %d keyframes\n", (int)keys.size());
+    if (keys.empty()) {
+      mbr_destroy(ctx);
+      return 1;
+    }
+    const int fpk = s.frames_per_keyframe > 0 ? s.frames_per_keyframe : 1;
+    if (last > (int)keys.size() * fpk) last = (int)keys.size() * fpk;
+    for (int index = first; index < last && status == 0; index++) {
+      mbr_scene fs = s;
+      rc = mbr_keyframe_scene(keys.data(), (int)keys.size(), fpk, index, &fs);
+      if (rc != MBR_OK) {
+        status = 1;
+        break;
+      }
+      fs.frame_no = index;
+      std::printf("Frame: %d, Keyframe %f\n", index, (double)index / fpk);
+      status = render_frame(ctx, fs, precision, format, "", index);
+    }
+  } else {
+    // flight path: one camera per line (:2262-2274), skipping to the first frame
+    FILE *path = std::fopen(s.file_animation_path, "r");
+    if (!path) {
+      std::printf("cannot open flight path %s\n", s.file_animation_path);
+      mbr_destroy(ctx);
+      return 1;
+    }
+    mbr_scene fs = s;
+    auto read_camera = [&](mbr_scene &t) {
+      return std::fscanf(path, "%lf %lf %lf %lf %lf %lf", &t.vp[0], &t.vp[1], &t.vp[2], &t.alpha, &t.beta,
+                         &t.gamma) == 6;
+    };
+    bool ok = true;
+    for (int i = 0; i < first && ok; i++) ok = read_camera(fs);
+    for (int index = first; index < last && ok && status == 0; index++) {
+      if (!read_camera(fs)) break;  // the path ended: the animation ends
+      fs.frame_no = index;
+      std::printf("Frame: %d\n", index);
+      status = render_frame(ctx, fs, precision, format, "", index);
+    }
+    std::fclose(path);
+  }
+  mbr_destroy(ctx);
+  return status;
 }

@@ -115,7 +115,8 @@ std::vector<KeyDef> build_table() {
     std::string p = "IFS_" + std::to_string(i) + "_";
     for (const char *c : {"x", "y", "z", "alfa", "beta", "gamma", "distance", "intensity", "enabled"}) G(p + c);
   }
-  G("start_frame"); G("end_frame"); G("frames_per_keyframe"); G("hybrid_cyclic");
+  I("start_frame", OFF(start_frame)); I("end_frame", OFF(end_frame));
+  I("frames_per_keyframe", OFF(frames_per_keyframe)); G("hybrid_cyclic");
   for (int i = 1; i <= 5; i++) {
     G("hybrid_formula_" + std::to_string(i)); G("hybrid_iterations_" + std::to_string(i));
     G("hybrid_power_" + std::to_string(i));
@@ -207,8 +208,9 @@ std::vector<KeyDef> build_table() {
   G("ocl_custom_DE_mode"); G("ocl_custom_formula_name"); G("ocl_delta_DE_step"); G("ocl_DOF_method");
   for (int i = 0; i < 15; i++) G("ocl_custom_par_" + std::to_string(i));
   t.push_back({"file_destination", KT_STRING, OFF(file_destination), 1.0});
-  for (const char *k : {"file_background", "file_envmap", "file_lightmap", "file_animation_path", "file_keyframes"})
-    G(k);
+  t.push_back({"file_animation_path", KT_STRING, OFF(file_animation_path), 1.0});
+  t.push_back({"file_keyframes", KT_STRING, OFF(file_keyframes), 1.0});
+  for (const char *k : {"file_background", "file_envmap", "file_lightmap"}) G(k);
   t.push_back({"palette", KT_PALETTE, 0, 1.0});
   // render-core settings with no reference key
   D("bailout", OFF(bailout));
@@ -375,7 +377,12 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->image_gamma = 1.0;
   s->hdr = 0;
   std::strncpy(s->file_destination, "images/image", sizeof(s->file_destination) - 1);
-  s->mb_sharpness = 3.0;             // share/defaults
+  s->start_frame = 0;                // share/defaults
+  s->end_frame = 1000;
+  s->frames_per_keyframe = 100;
+  std::strncpy(s->file_animation_path, "paths/path.txt", sizeof(s->file_animation_path) - 1);
+  std::strncpy(s->file_keyframes, "keyframes/keyframe", sizeof(s->file_keyframes) - 1);
+  s->mb_sharpness = 3.0;
   s->folding_int_pow_factor = 2.0;
   s->folding_int_pow_z_factor = 5.0;
   s->c_add = -1.3;
@@ -384,6 +391,61 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->dof_enabled = 0;
   s->dof_focus = 21.699999999999999;
   s->dof_radius = 10.0;
+  return MBR_OK;
+}
+
+namespace {
+
+// CMorph::CatmullRom for one value (src/morph.cpp:49-109)
+double catmull_rom(double v1, double v2, double v3, double v4, double factor) {
+  const double factor2 = factor * factor, factor3 = factor2 * factor;
+  bool logarithmic = false, negative = false;
+  if ((v1 > 0 && v2 > 0 && v3 > 0 && v4 > 0) || (v1 < 0 && v2 < 0 && v3 < 0 && v4 < 0)) {
+    if (v1 < 0) negative = true;
+    const double average = (v1 + v2 + v3 + v4) / 4.0;
+    if (average > 0) {
+      const double deviation = (std::fabs(v2 - v1) + std::fabs(v3 - v2) + std::fabs(v4 - v3)) / average;
+      if (deviation > 0.1) {
+        v1 = std::log(std::fabs(v1));
+        v2 = std::log(std::fabs(v2));
+        v3 = std::log(std::fabs(v3));
+        v4 = std::log(std::fabs(v4));
+        logarithmic = true;
+      }
+    }
+  }
+  double value = 0.5 * ((2 * v2) + (-v1 + v3) * factor + (2 * v1 - 5 * v2 + 4 * v3 - v4) * factor2 +
+                        (-v1 + 3 * v2 - 3 * v3 + v4) * factor3);
+  if (logarithmic) value = negative ? -std::exp(value) : std::exp(value);
+  if (value > 1e20) value = 1e20;
+  if (value < -1e20) value = 1e20;  // (the reference clamps negative overflow to +1e20 too)
+  if (std::fabs(value) < 1e-20) value = 0.0;
+  return value;
+}
+
+}  // namespace
+
+extern "C" int mbr_keyframe_scene(const mbr_scene *keys, int n_keys, int frames_per_key, int frame, mbr_scene *s) {
+  if (!keys || !s || n_keys < 1 || frames_per_key < 1 || frame < 0) return MBR_ERR_INVALID_ARG;
+  const int key = frame / frames_per_key;
+  const double factor = (frame % frames_per_key) / (double)frames_per_key;
+  auto at = [&](int k) -> const mbr_scene & { return keys[k]; };
+  // support values as CMorph picks them (clamped at both ends)
+  const int k1 = key >= 1 ? key - 1 : key;
+  const int k2 = key < n_keys ? key : n_keys - 1;
+  const int k3 = key < n_keys - 1 ? key + 1 : n_keys - 1;
+  const int k4 = key < n_keys - 2 ? key + 2 : n_keys - 1;
+  if (k1 >= n_keys) return MBR_ERR_INVALID_ARG;
+  auto spline = [&](size_t offset) {
+    auto v = [&](int k) { return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(&at(k)) + offset); };
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(s) + offset) = catmull_rom(v(k1), v(k2), v(k3), v(k4), factor);
+  };
+  for (const KeyDef &k : table())
+    if (k.type == KT_DOUBLE) spline(k.offset);
+  // N is a double of sFractalD in the reference, read back as int
+  const double n = catmull_rom(at(k1).N, at(k2).N, at(k3).N, at(k4).N, factor);
+  s->N = (int)n;
+  if (s->zoom < 1e-15) s->zoom = 1e-15;  // src/Render3D.cpp:2287
   return MBR_OK;
 }
 

@@ -41,6 +41,16 @@ def apply(scene, keys):
     return scene
 
 
+def keyframe(keys, frames_per_key, frame, base):
+    """The scene of animation frame `frame`: base with every double parameter
+    (and N) interpolated through the keyframe scenes `keys` (Catmull-Rom,
+    src/morph.cpp:49-109)."""
+    arr = (_abi.Scene * len(keys))(*keys)
+    out = copy(base)
+    check(lib().mbr_keyframe_scene(arr, len(keys), frames_per_key, frame, C.byref(out)))
+    return out
+
+
 def copy(scene):
     s = _abi.Scene()
     C.memmove(C.byref(s), C.byref(scene), C.sizeof(_abi.Scene))

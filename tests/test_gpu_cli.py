@@ -71,3 +71,68 @@ def test_cli_default_output_name(tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert (tmp_path / "out" / "image00007.png").exists()
+
+
+def _library_png(s, precision=_abi.PREC_FP32_FIXUP):
+    """What the driver writes for scene s: render, CompileImage, SSAO."""
+    from mandelbulber_amd import Renderer
+    rr = Renderer(0)
+    try:
+        rr.set_scene(s)
+        frame, _ = rr.render(precision=precision)
+        img = compile_image(s, frame.rgb)
+        if s.ssao_enabled:
+            img = rr.post_ssao(s, frame.z, frame.opacity, frame.colour, img)
+    finally:
+        rr.close()
+    return img
+
+
+def test_cli_keyframe_animation(tmp_path):
+    from test_image import _read_png
+    kdir = tmp_path / "kf"
+    kdir.mkdir()
+    for k, (power, vy) in enumerate([(8, -3.5), (7, -3.2), (6, -3.0)]):
+        (kdir / f"key{k:05d}.fract").write_text(SETTINGS + f"power {power};\nview_point_y {vy};\n")
+    main = tmp_path / "main.fract"
+    main.write_text(SETTINGS + f"file_keyframes {kdir}/key;\nfile_destination {tmp_path}/anim/frame;\n"
+                               "frames_per_keyframe 2;\nend_frame 1000;\n")
+    print("[cli] keyframe animation", flush=True)
+    r = subprocess.run([CLI, "-keyframe", "-res", "48x36", "-format", "png16", str(main)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Found 3 keyframes" in r.stdout
+    frames = sorted(p.name for p in (tmp_path / "anim").iterdir())
+    assert frames == [f"frame{i:05d}.png" for i in range(6)]   # end clamps to 3 keys x 2 frames
+    keys = [scene.load(str(kdir / f"key{k:05d}.fract")) for k in range(3)]
+    base = scene.load(str(main), ["image_width=48", "image_height=36"])
+    for index in (2, 3):
+        fs = scene.keyframe(keys, 2, index, base)
+        fs.frame_no = index
+        px, _ = _read_png(str(tmp_path / "anim" / f"frame{index:05d}.png"))
+        assert np.array_equal(px.astype(np.uint16), _library_png(fs)), index
+    assert scene.keyframe(keys, 2, 2, base).power == pytest.approx(7.0, rel=1e-12)   # log space: exp(log 7)
+
+
+def test_cli_flight_path(tmp_path):
+    from test_image import _read_png
+    path = tmp_path / "path.txt"
+    cams = [(0.0, -3.5, 0.0, 0.0, 0.0, 0.0), (0.1, -3.4, 0.05, 0.05, -0.02, 0.01),
+            (0.2, -3.3, 0.1, 0.1, -0.04, 0.02)]
+    path.write_text("".join("%.11f %.11f %.11f %f %f %f\n" % c for c in cams))
+    main = tmp_path / "main.fract"
+    main.write_text(SETTINGS + f"file_animation_path {path};\nfile_destination {tmp_path}/fl/frame;\n")
+    print("[cli] flight path", flush=True)
+    r = subprocess.run([CLI, "-flight", "-res", "48x36", "-format", "png16", "-end", "10", str(main)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    frames = sorted(p.name for p in (tmp_path / "fl").iterdir())
+    assert frames == [f"frame{i:05d}.png" for i in range(3)]   # the path has three cameras
+    base = scene.load(str(main), ["image_width=48", "image_height=36"])
+    fs = scene.copy(base)
+    fs.vp[0], fs.vp[1], fs.vp[2], fs.alpha, fs.beta, fs.gamma = cams[1]
+    fs.frame_no = 1
+    px, _ = _read_png(str(tmp_path / "fl" / "frame00001.png"))
+    assert np.array_equal(px.astype(np.uint16), _library_png(fs))

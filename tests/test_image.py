@@ -103,7 +103,9 @@ def test_cli_help_and_argument_errors():
     assert r.returncode == 2 and "-res" in r.stdout
     r = subprocess.run([CLI, "-format", "tiff", "x.fract"], capture_output=True, text=True)
     assert r.returncode == 2 and "wrong image format" in r.stdout
-    r = subprocess.run([CLI, "-flight", "x.fract"], capture_output=True, text=True)
+    r = subprocess.run([CLI, "-ip", "10.0.0.1", "x.fract"], capture_output=True, text=True)
+    assert r.returncode == 2 and "network rendering" in r.stdout
+    r = subprocess.run([CLI, "-flight", "-keyframe", "x.fract"], capture_output=True, text=True)
     assert r.returncode == 2
     r = subprocess.run([CLI, "/nonexistent.fract"], capture_output=True, text=True)
     assert r.returncode == 1 and "cannot load settings" in r.stdout
