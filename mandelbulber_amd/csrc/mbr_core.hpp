@@ -95,7 +95,20 @@ MBR_HD float fast_rsq(float x) {
 // FP64 forms of the same operations for templates shared by both
 // precisions: the correctly rounded division, square root and log.
 MBR_HD double fast_rcp(double x) { return 1.0 / x; }
-MBR_HD double fast_rsq(double x) { return 1.0 / Math<double>::sqrt(x); }
+// FP64 1/sqrt: the hardware v_rsq_f64 estimate refined by two Newton steps
+// (y <- y (3/2 - x/2 y^2)), about 1 ulp; zero and infinity keep the
+// estimate's inf / 0.
+MBR_HD double fast_rsq(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y0 = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double y = y0 * (1.5 - h * y0 * y0);
+  y = y * (1.5 - h * y * y);
+  return (x > 0.0 && x < 1.7976931348623157e308) ? y : y0;
+#else
+  return 1.0 / Math<double>::sqrt(x);
+#endif
+}
 MBR_HD double fast_ln(double x) { return Math<double>::log(x); }
 // a / b: exact in FP64, hardware reciprocal in FP32
 MBR_HD double fdiv(double a, double b) { return a / b; }
