@@ -642,8 +642,8 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
 template <class T, int F, bool Deep, bool Plain>
-__global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
-                                               SurfBufs<T> sb, unsigned long long *stats) {
+__device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
+                                        unsigned long long *stats) {
   const DevScene &sc = *scp;
   const long long n_hits = (long long)b.counters[CNT_HITS];
   unsigned long long *queue = &b.counters[CNT_AOQ];
@@ -793,6 +793,20 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   v[ST_AO_ITERS] = cnt.iterations;
   v[ST_AO_STEPS] = ao_steps;
   flush_stats(stats, v);
+}
+
+template <class T, int F, bool Deep, bool Plain>
+__global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
+                                               SurfBufs<T> sb, unsigned long long *stats) {
+  ao_body<T, F, Deep, Plain>(scp, b, ao, sb, stats);
+}
+
+// The FP32 plain variant (the benchmarked AO pass) held to 6 waves per SIMD.
+template <class T, int F>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+    k_ao_f32_plain(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
+                   unsigned long long *stats) {
+  ao_body<T, F, false, true>(scp, b, ao, sb, stats);
 }
 
 template <class T, class RT>
@@ -1028,6 +1042,15 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     auto aok = [&](auto deep_tag, auto plain_tag) {
       constexpr bool D = decltype(deep_tag)::value;
       constexpr bool Pl = decltype(plain_tag)::value;
+      if constexpr (!D && Pl && std::is_same<T, float>::value) {
+        if (formula == 7)
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        else if (formula == 8)
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        else
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        return;
+      }
       if constexpr (!D && !Pl && std::is_same<T, double>::value) {
         if (generic) {
           hipLaunchKernelGGL((k_ao<T, 0, false, false>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
