@@ -52,6 +52,7 @@ enum mbr_formula {
   MBR_FORMULA_MINUS_FAST_TRIG = 6,
   MBR_FORMULA_MENGER_SPONGE = 7,
   MBR_FORMULA_TGLAD = 8,         /* Mandelbox */
+  MBR_FORMULA_KALEIDOSCOPIC = 10, /* kaleidoscopic IFS */
   MBR_FORMULA_XENODREAMBUIE = 11,
   MBR_FORMULA_MANDELBULB2 = 13,
   MBR_FORMULA_MANDELBULB3 = 14,
@@ -266,6 +267,20 @@ typedef struct mbr_scene {
   double folding_int_pow_factor;   /* FoldingIntPow_folding_factor */
   double folding_int_pow_z_factor; /* FoldingIntPow_z_factor */
   double c_add;                    /* c_add (aexion) */
+
+  /* kaleidoscopic IFS (sFractalIFS, src/fractal.h; angles in radians, as
+   * the settings files store them) */
+  double ifs_scale;                /* IFS_scale */
+  double ifs_rotation[3];          /* IFS_rot_alfa / beta / gamma */
+  double ifs_offset[3];            /* IFS_offsetX/Y/Z */
+  double ifs_edge[3];              /* IFS_edgeX/Y/Z */
+  int ifs_abs[3];                  /* IFS_absX/Y/Z */
+  int ifs_menger_sponge_mode;      /* IFS_menger_sponge_mode */
+  double ifs_direction[9][3];      /* IFS_%d_x/y/z (normalised by mbr_set_scene) */
+  double ifs_angles[9][3];         /* IFS_%d_alfa/beta/gamma */
+  double ifs_distance[9];          /* IFS_%d_distance */
+  double ifs_intensity[9];         /* IFS_%d_intensity */
+  int ifs_enabled[9];              /* IFS_%d_enabled */
   int ssao_enabled;                /* post_SSAO_enabled */
   int ssao_quality;                /* post_SSAO_quality */
   int dof_enabled;                 /* post_DOF_enabled */

@@ -92,7 +92,11 @@ class Scene(C.Structure):
         ("start_frame", C.c_int), ("end_frame", C.c_int), ("frames_per_keyframe", C.c_int),
         ("file_animation_path", C.c_char * 512), ("file_keyframes", C.c_char * 512),
         ("linear_DE_mode", C.c_int), ("mb_sharpness", C.c_double), ("folding_int_pow_factor", C.c_double),
-        ("folding_int_pow_z_factor", C.c_double), ("c_add", C.c_double), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
+        ("folding_int_pow_z_factor", C.c_double), ("c_add", C.c_double),
+        ("ifs_scale", C.c_double), ("ifs_rotation", C.c_double * 3), ("ifs_offset", C.c_double * 3),
+        ("ifs_edge", C.c_double * 3), ("ifs_abs", C.c_int * 3), ("ifs_menger_sponge_mode", C.c_int),
+        ("ifs_direction", (C.c_double * 3) * 9), ("ifs_angles", (C.c_double * 3) * 9),
+        ("ifs_distance", C.c_double * 9), ("ifs_intensity", C.c_double * 9), ("ifs_enabled", C.c_int * 9), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
         ("dof_focus", C.c_double), ("dof_radius", C.c_double),
     ]
 

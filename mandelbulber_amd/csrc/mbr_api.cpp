@@ -107,6 +107,37 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   f.fip_factor = (T)s.folding_int_pow_factor;
   f.fip_zfactor = (T)s.folding_int_pow_z_factor;
   f.c_add = (T)s.c_add;
+  {
+    // RecalculateIFSParams: SetRotation(alfa, beta, gamma) = Z(alfa) Y(beta)
+    // X(gamma) (src/algebra.cpp:153-159); CVector3::Normalize (x 1/length)
+    Rot mr;
+    mr.rz(s.ifs_rotation[0]);
+    mr.ry(s.ifs_rotation[1]);
+    mr.rx(s.ifs_rotation[2]);
+    f.ifs_main_on = !mr.identity;
+    for (int k = 0; k < 9; k++) f.ifs_main[k] = (T)mr.m[k];
+    for (int i = 0; i < 9; i++) {
+      Rot r;
+      r.rz(s.ifs_angles[i][0]);
+      r.ry(s.ifs_angles[i][1]);
+      r.rx(s.ifs_angles[i][2]);
+      f.ifs_rot_on[i] = !r.identity;
+      for (int k = 0; k < 9; k++) f.ifs_rot[i][k] = (T)r.m[k];
+      const double *dv = s.ifs_direction[i];
+      const double n = 1.0 / std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      for (int c = 0; c < 3; c++) f.ifs_dir[i][c] = (T)(dv[c] * n);
+      f.ifs_dist[i] = (T)s.ifs_distance[i];
+      f.ifs_int[i] = (T)s.ifs_intensity[i];
+      f.ifs_en[i] = s.ifs_enabled[i];
+    }
+    for (int c = 0; c < 3; c++) {
+      f.ifs_abs[c] = s.ifs_abs[c];
+      f.ifs_offset[c] = (T)s.ifs_offset[c];
+      f.ifs_edge[c] = (T)s.ifs_edge[c];
+    }
+    f.ifs_menger = s.ifs_menger_sponge_mode;
+    f.ifs_scale = (T)s.ifs_scale;
+  }
   f.bailout = (T)s.bailout;
   f.constant_factor = (T)s.constant_factor;
   for (int i = 0; i < 3; i++) f.julia[i] = (T)s.julia[i];

@@ -23,11 +23,11 @@ enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
 // CalculateDistance per DE call, analytic for trig_DE, by finite
 // differences of the orbit radius (delta DE) for the rest.
 MBR_HD bool formula_analytic(int formula) {
-  return formula == 1 || formula == 2 || formula == 7 || formula == 8 || formula == 17;
+  return formula == 1 || formula == 2 || formula == 7 || formula == 8 || formula == 10 || formula == 17;
 }
 // termination groups of src/fractal.cpp:1133-1149: Menger-like (r > 1000,
 // (r - 2) / DE) and Mandelbox-like (r > 1024, r / |DE|)
-MBR_HD bool formula_menger_group(int formula) { return formula == 7; }
+MBR_HD bool formula_menger_group(int formula) { return formula == 7 || formula == 10; }
 MBR_HD bool formula_box_group(int formula) { return formula == 8 || formula == 17; }
 
 // SmoothConditionAGreaterB / ALessB (src/common_math.h:87-88)
@@ -373,6 +373,47 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.y = newy + constant.y;
       z.z = newz + constant.z;
       r = length_nc(z);
+    } else if (f.formula == 10) {  // kaleidoscopic IFS (src/fractal.cpp:837-886)
+      if (f.ifs_abs[0]) z.x = M::fabs(z.x);
+      if (f.ifs_abs[1]) z.y = M::fabs(z.y);
+      if (f.ifs_abs[2]) z.z = M::fabs(z.z);
+      for (int i = 0; i < 9; i++) {
+        if (!f.ifs_en[i]) continue;
+        if (f.ifs_rot_on[i]) {
+          const T *m = f.ifs_rot[i];
+          z = V3<T>{m[0] * z.x + m[1] * z.y + m[2] * z.z, m[3] * z.x + m[4] * z.y + m[5] * z.z,
+                    m[6] * z.x + m[7] * z.y + m[8] * z.z};
+        }
+        const V3<T> dirv = ld3(f.ifs_dir[i]);
+        const T len = z.x * dirv.x + z.y * dirv.y + z.z * dirv.z;
+        if (len < f.ifs_dist[i]) z = z - dirv * (T(2) * (len - f.ifs_dist[i]) * f.ifs_int[i]);
+      }
+      const V3<T> off = ld3(f.ifs_offset);
+      V3<T> zo = z - off;
+      if (f.ifs_main_on) {
+        const T *m = f.ifs_main;
+        zo = V3<T>{m[0] * zo.x + m[1] * zo.y + m[2] * zo.z, m[3] * zo.x + m[4] * zo.y + m[5] * zo.z,
+                   m[6] * zo.x + m[7] * zo.y + m[8] * zo.z};
+      }
+      z = zo + off;
+      if (f.ifs_edge[0] > 0) z.x = f.ifs_edge[0] - M::fabs(f.ifs_edge[0] - z.x);
+      if (f.ifs_edge[1] > 0) z.y = f.ifs_edge[1] - M::fabs(f.ifs_edge[1] - z.y);
+      if (f.ifs_edge[2] > 0) z.z = f.ifs_edge[2] - M::fabs(f.ifs_edge[2] - z.z);
+      if (Mode == MODE_COLOURING) {
+        const T length2 = length_nc(z);
+        if (length2 < min) min = length2;
+      }
+      z = z * f.ifs_scale;
+      const T sm1 = f.ifs_scale - T(1);
+      if (f.ifs_menger) {
+        z.x -= off.x * sm1;
+        z.y -= off.y * sm1;
+        if (z.z > T(0.5) * off.z * sm1) z.z -= off.z * sm1;
+      } else {
+        z = z - off * sm1;
+      }
+      tgladDE *= f.ifs_scale;
+      r = length_nc(z);
     } else if (f.formula == 21) {  // bristorbrot (src/fractal.cpp:971-983)
       const T newx = z.x * z.x - z.y * z.y - z.z * z.z;
       const T newy = z.y * (T(2) * z.x - z.z);
@@ -424,7 +465,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
   }
   if (Mode == MODE_DELTA1 || Mode == MODE_DELTA2) return r;
   if (formula_box_group(f.formula)) return tgladColor * T(100) + length_nc(z) * f.mb_col_R;
-  if (f.formula == 7) return min * T(1000);
+  if (formula_menger_group(f.formula)) return min * T(1000);
   return distance * T(5000);
 }
 

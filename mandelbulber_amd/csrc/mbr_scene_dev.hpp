@@ -23,6 +23,12 @@ struct FractalK {
   int ipow;  // power as an integer in [2, 32] when it is one, else 0
   int linear_de;  // linear_DE_mode: delta-DE distance 0.5 r / dr instead of 0.5 r ln r / dr
   T mb_sharp, fip_factor, fip_zfactor, c_add;  // smoothMandelbox, foldingIntPow2, aexion
+  // kaleidoscopic IFS (RecalculateIFSParams, src/callbacks.cpp:1538-1546):
+  // per enabled vector its rotation (identity flag), unit direction,
+  // distance and intensity; the main rotation about the offset
+  int ifs_abs[3], ifs_menger, ifs_en[9], ifs_rot_on[9], ifs_main_on;
+  T ifs_scale, ifs_offset[3], ifs_edge[3];
+  T ifs_rot[9][9], ifs_dir[9][3], ifs_dist[9], ifs_int[9], ifs_main[9];
   T power, bailout, constant_factor;
   T julia[3];
   T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;

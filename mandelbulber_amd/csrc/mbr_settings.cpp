@@ -108,12 +108,24 @@ std::vector<KeyDef> build_table() {
     G(p + "colour_R"); G(p + "colour_G"); G(p + "colour_B");
     I(p + "enabled", OFF_A(aux_light_pre_enabled, i - 1, int));
   }
-  G("IFS_scale"); G("IFS_rot_alfa"); G("IFS_rot_beta"); G("IFS_rot_gamma");
-  G("IFS_offsetX"); G("IFS_offsetY"); G("IFS_offsetZ"); G("IFS_edgeX"); G("IFS_edgeY"); G("IFS_edgeZ");
-  G("IFS_absX"); G("IFS_absY"); G("IFS_absZ"); G("IFS_menger_sponge_mode");
+  D("IFS_scale", OFF(ifs_scale));
+  D("IFS_rot_alfa", OFF_A(ifs_rotation, 0, double)); D("IFS_rot_beta", OFF_A(ifs_rotation, 1, double));
+  D("IFS_rot_gamma", OFF_A(ifs_rotation, 2, double));
+  D("IFS_offsetX", OFF_A(ifs_offset, 0, double)); D("IFS_offsetY", OFF_A(ifs_offset, 1, double));
+  D("IFS_offsetZ", OFF_A(ifs_offset, 2, double));
+  D("IFS_edgeX", OFF_A(ifs_edge, 0, double)); D("IFS_edgeY", OFF_A(ifs_edge, 1, double));
+  D("IFS_edgeZ", OFF_A(ifs_edge, 2, double));
+  I("IFS_absX", OFF_A(ifs_abs, 0, int)); I("IFS_absY", OFF_A(ifs_abs, 1, int)); I("IFS_absZ", OFF_A(ifs_abs, 2, int));
+  I("IFS_menger_sponge_mode", OFF(ifs_menger_sponge_mode));
   for (int i = 0; i < 9; i++) {
-    std::string p = "IFS_" + std::to_string(i) + "_";
-    for (const char *c : {"x", "y", "z", "alfa", "beta", "gamma", "distance", "intensity", "enabled"}) G(p + c);
+    const std::string p = "IFS_" + std::to_string(i) + "_";
+    for (int c = 0; c < 3; c++) {
+      D(p + "xyz"[c], OFF(ifs_direction) + sizeof(double) * (3 * i + c));
+      D(p + (c == 0 ? "alfa" : c == 1 ? "beta" : "gamma"), OFF(ifs_angles) + sizeof(double) * (3 * i + c));
+    }
+    D(p + "distance", OFF_A(ifs_distance, i, double));
+    D(p + "intensity", OFF_A(ifs_intensity, i, double));
+    I(p + "enabled", OFF_A(ifs_enabled, i, int));
   }
   I("start_frame", OFF(start_frame)); I("end_frame", OFF(end_frame));
   I("frames_per_keyframe", OFF(frames_per_keyframe)); G("hybrid_cyclic");
@@ -382,6 +394,12 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->frames_per_keyframe = 100;
   std::strncpy(s->file_animation_path, "paths/path.txt", sizeof(s->file_animation_path) - 1);
   std::strncpy(s->file_keyframes, "keyframes/keyframe", sizeof(s->file_keyframes) - 1);
+  s->ifs_scale = 2.0;                // share/defaults: IFS_scale 2, IFS_offsetX 1, IFS_%d_x 1, intensity 1
+  s->ifs_offset[0] = 1.0;
+  for (int i = 0; i < 9; i++) {
+    s->ifs_direction[i][0] = 1.0;
+    s->ifs_intensity[i] = 1.0;
+  }
   s->mb_sharpness = 3.0;
   s->folding_int_pow_factor = 2.0;
   s->folding_int_pow_z_factor = 5.0;

@@ -108,10 +108,22 @@ static void rot_transpose(const rotm *r, rotm *out) {
 typedef struct {
   const mbr_scene *s;
   rotm mb_main, mb_rot[2][3], mb_rotinv[2][3];
+  rotm ifs_main, ifs_rot[9];
+  v3 ifs_dir[9];
 } fractal_ctx;
 
 static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
   f->s = s;
+  /* RecalculateIFSParams, IFS part (src/callbacks.cpp:1540-1546):
+   * SetRotation(alfa, beta, gamma) = Z(alfa) Y(beta) X(gamma) (src/algebra.cpp:153-159) */
+  rot_null(&f->ifs_main);
+  rot_z(&f->ifs_main, s->ifs_rotation[0]); rot_y(&f->ifs_main, s->ifs_rotation[1]); rot_x(&f->ifs_main, s->ifs_rotation[2]);
+  for (int i = 0; i < 9; i++) {
+    rot_null(&f->ifs_rot[i]);
+    rot_z(&f->ifs_rot[i], s->ifs_angles[i][0]); rot_y(&f->ifs_rot[i], s->ifs_angles[i][1]);
+    rot_x(&f->ifs_rot[i], s->ifs_angles[i][2]);
+    f->ifs_dir[i] = v3norm(v3from(s->ifs_direction[i]));
+  }
   /* RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555) */
   rot_set3(&f->mb_main, s->mb_rotation_main);
   for (int fold = 0; fold < 2; fold++)
@@ -459,6 +471,36 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
         r = v3len(z);
         break;
       }
+      case MBR_FORMULA_KALEIDOSCOPIC: {  /* :837-886 */
+        if (s->ifs_abs[0]) z.x = fabs(z.x);
+        if (s->ifs_abs[1]) z.y = fabs(z.y);
+        if (s->ifs_abs[2]) z.z = fabs(z.z);
+        for (int i = 0; i < 9; i++) {
+          if (!s->ifs_enabled[i]) continue;
+          z = rot_apply(&f->ifs_rot[i], z);
+          double length = v3dot(z, f->ifs_dir[i]);
+          if (length < s->ifs_distance[i])
+            z = v3sub(z, v3mul(f->ifs_dir[i], 2.0 * (length - s->ifs_distance[i]) * s->ifs_intensity[i]));
+        }
+        v3 off = v3from(s->ifs_offset);
+        z = v3add(rot_apply(&f->ifs_main, v3sub(z, off)), off);
+        if (s->ifs_edge[0] > 0) z.x = s->ifs_edge[0] - fabs(s->ifs_edge[0] - z.x);
+        if (s->ifs_edge[1] > 0) z.y = s->ifs_edge[1] - fabs(s->ifs_edge[1] - z.y);
+        if (s->ifs_edge[2] > 0) z.z = s->ifs_edge[2] - fabs(s->ifs_edge[2] - z.z);
+        if (mode == MODE_COLOURING) { double length2 = v3len(z); if (length2 < min) min = length2; }
+        double sc = s->ifs_scale;
+        z = v3mul(z, sc);
+        if (s->ifs_menger_sponge_mode) {
+          z.x -= off.x * (sc - 1.0);
+          z.y -= off.y * (sc - 1.0);
+          if (z.z > 0.5 * off.z * (sc - 1.0)) z.z -= off.z * (sc - 1.0);
+        } else {
+          z = v3sub(z, v3mul(off, sc - 1.0));
+        }
+        tgladDE *= sc;
+        r = v3len(z);
+        break;
+      }
       case MBR_FORMULA_BRISTORBROT: {  /* :971-983 */
         double newx = z.x * z.x - z.y * z.y - z.z * z.z;
         double newy = z.y * (2.0 * z.x - z.z);
@@ -490,7 +532,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
       if (L == delta_limit) break;
     }
 
-    if (formula == MBR_FORMULA_MENGER_SPONGE) {
+    if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) {
       if (r > 1000) { distance = (r - 2.0) / tgladDE; break; }
     } else if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX) {
       if (r > 1024) { distance = r / fabs(tgladDE); break; }
@@ -518,7 +560,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
   if (mode == MODE_DELTA1 || mode == MODE_DELTA2) return r;
   if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX)
     return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
-  if (formula == MBR_FORMULA_MENGER_SPONGE) return min * 1000.0;
+  if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) return min * 1000.0;
   return distance * 5000.0;
 }
 
@@ -596,7 +638,7 @@ static double calc_distance(const fractal_ctx *f, v3 point, double detailSize, i
 
   const int analytic = s->formula == MBR_FORMULA_TRIG_DE || s->formula == MBR_FORMULA_TRIG_OPTIM ||
                        s->formula == MBR_FORMULA_MENGER_SPONGE || s->formula == MBR_FORMULA_TGLAD ||
-                       s->formula == MBR_FORMULA_SMOOTH_MANDELBOX;
+                       s->formula == MBR_FORMULA_SMOOTH_MANDELBOX || s->formula == MBR_FORMULA_KALEIDOSCOPIC;
   if (!s->primitive_only_plane && !analytic) {
     /* delta DE (src/fractal.cpp:1307-1342): orbit radius at the point and
      * at +1e-10 along x, y, z, the point updated in place as there */

@@ -75,3 +75,42 @@ def test_generic_formula_matches_oracle(renderer, formula, power, name, shaded):
         print(f"[formulas] {name} shaded={shaded} {pname} (rendered in FP64)", flush=True)
         f, _ = renderer.render(precision=prec)
         assert np.array_equal(f.z, f64.z) and np.array_equal(f.rgb, f64.rgb, equal_nan=True)
+
+
+def _ifs_scene(variant, shaded):
+    s = scene.menger_view(192, 144, shaded=shaded)
+    s.formula = 10
+    s.ifs_abs[0] = s.ifs_abs[1] = s.ifs_abs[2] = 1
+    s.ifs_scale = 3.0
+    for c in range(3):
+        s.ifs_offset[c] = 1.0
+    s.ifs_menger_sponge_mode = 1
+    for i, d in enumerate([(1, -1, 0), (1, 0, -1), (0, 1, -1)]):
+        for c in range(3):
+            s.ifs_direction[i][c] = d[c]
+        s.ifs_enabled[i] = 1
+    if variant == "rotated":
+        # main rotation about the offset, a rotated fold and an edge fold
+        s.ifs_rotation[0], s.ifs_rotation[1] = 0.3, -0.2
+        s.ifs_angles[1][2] = 0.25
+        s.ifs_edge[0] = 0.9
+        s.ifs_menger_sponge_mode = 0
+        s.ifs_scale = 2.6
+    return s
+
+
+@pytest.mark.parametrize("shaded", [False, True])
+@pytest.mark.parametrize("variant", ["menger", "rotated"])
+def test_kaleidoscopic_ifs_matches_oracle(renderer, variant, shaded):
+    from oracle import oracle
+    sc = _ifs_scene(variant, shaded)
+    ref, _ = oracle.render(sc)
+    renderer.set_scene(sc)
+    print(f"[formulas] kaleidoscopic IFS {variant} shaded={shaded} fp64", flush=True)
+    f64, _ = renderer.render(precision=_abi.PREC_FP64)
+    m = compare(f64, ref)
+    print(f"  {m}", flush=True)
+    assert m["hits"] > 200
+    assert_parity(m, label=f"ifs {variant} shaded={shaded}")
+    f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
+    assert np.array_equal(f32.z, f64.z)

@@ -168,3 +168,42 @@ def test_smooth_mandelbox_tends_to_the_mandelbox():
         assert da == pytest.approx(db, rel=1e-6, abs=1e-300), p
         n += db > 1e-3
     assert n > 20
+
+
+def _menger_ifs(s):
+    """The Menger sponge as a kaleidoscopic IFS: |x|, |y|, |z|, the folds
+    through the planes x = y, x = z, y = z, scale 3 about (1, 1, 1) in Menger
+    mode."""
+    s.formula = 10
+    s.ifs_abs[0] = s.ifs_abs[1] = s.ifs_abs[2] = 1
+    s.ifs_scale = 3.0
+    for c in range(3):
+        s.ifs_offset[c] = 1.0
+    s.ifs_menger_sponge_mode = 1
+    for i, d in enumerate([(1, -1, 0), (1, 0, -1), (0, 1, -1)]):
+        for c in range(3):
+            s.ifs_direction[i][c] = d[c]
+        s.ifs_distance[i] = 0.0
+        s.ifs_intensity[i] = 1.0
+        s.ifs_enabled[i] = 1
+    return s
+
+
+def test_kaleidoscopic_ifs_reproduces_the_menger_sponge():
+    """A fold through a plane is a reflection: for x < y the fold through
+    x = y swaps x and y, which is menger_sponge's sort step; the rest of the
+    IFS step (scale 3, offset 2 in x and y, z shifted above 1/3) is its
+    affine step.  The distance estimates agree up to the rounding of the
+    reflections (the orbit grows by 3 per iteration)."""
+    m = S.menger_view(64, 48, shaded=False)
+    ifs = _menger_ifs(S.copy(m))
+    rng = np.random.default_rng(9)
+    n = 0
+    for _ in range(300):
+        p = tuple(rng.uniform(-1.6, 1.6, 3))
+        d7, _, i7, _ = oracle.calculate_distance(m, p)
+        d10, _, i10, _ = oracle.calculate_distance(ifs, p)
+        assert i7 == i10, p
+        assert d10 == pytest.approx(d7, rel=1e-6, abs=1e-12), p
+        n += d7 > 0
+    assert n > 100
