@@ -53,6 +53,7 @@ enum mbr_formula {
   MBR_FORMULA_MENGER_SPONGE = 7,
   MBR_FORMULA_TGLAD = 8,         /* Mandelbox */
   MBR_FORMULA_KALEIDOSCOPIC = 10, /* kaleidoscopic IFS */
+  MBR_FORMULA_HYBRID = 12,       /* hybrid_formula_1..5 sequence */
   MBR_FORMULA_XENODREAMBUIE = 11,
   MBR_FORMULA_MANDELBULB2 = 13,
   MBR_FORMULA_MANDELBULB3 = 14,
@@ -61,7 +62,13 @@ enum mbr_formula {
   MBR_FORMULA_SMOOTH_MANDELBOX = 17,
   MBR_FORMULA_AEXION = 19,
   MBR_FORMULA_BENESI = 20,
-  MBR_FORMULA_BRISTORBROT = 21
+  MBR_FORMULA_BRISTORBROT = 21,
+  /* hybrid building blocks (src/fractal.cpp:984-1103) */
+  MBR_FORMULA_INVERT_X = 22, MBR_FORMULA_INVERT_Y = 23, MBR_FORMULA_INVERT_Z = 24, MBR_FORMULA_INVERT_R = 25,
+  MBR_FORMULA_SPHERICAL_FOLD = 26, MBR_FORMULA_POW_XYZ = 27,
+  MBR_FORMULA_SCALE_X = 28, MBR_FORMULA_SCALE_Y = 29, MBR_FORMULA_SCALE_Z = 30,
+  MBR_FORMULA_OFFSET_X = 31, MBR_FORMULA_OFFSET_Y = 32, MBR_FORMULA_OFFSET_Z = 33,
+  MBR_FORMULA_ANGLE_MULTIPLY_X = 34, MBR_FORMULA_ANGLE_MULTIPLY_Y = 35, MBR_FORMULA_ANGLE_MULTIPLY_Z = 36
   /* all but 2, 7 and 8 render through the generic FP64 path (analytic DE for
    * 1 and 17, delta DE for the rest, as the reference's analitycDE): every
    * precision mode renders them in FP64 */
@@ -281,6 +288,12 @@ typedef struct mbr_scene {
   double ifs_distance[9];          /* IFS_%d_distance */
   double ifs_intensity[9];         /* IFS_%d_intensity */
   int ifs_enabled[9];              /* IFS_%d_enabled */
+
+  /* hybrid sequence (CreateFormulaSequence, src/callbacks.cpp:1666-1703) */
+  int hybrid_formula[5];           /* hybrid_formula_1..5 (0 = none) */
+  int hybrid_iterations[5];        /* hybrid_iterations_1..5 */
+  double hybrid_power[5];          /* hybrid_power_1..5 */
+  int hybrid_cyclic;               /* hybrid_cyclic */
   int ssao_enabled;                /* post_SSAO_enabled */
   int ssao_quality;                /* post_SSAO_quality */
   int dof_enabled;                 /* post_DOF_enabled */

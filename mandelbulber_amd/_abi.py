@@ -96,7 +96,9 @@ class Scene(C.Structure):
         ("ifs_scale", C.c_double), ("ifs_rotation", C.c_double * 3), ("ifs_offset", C.c_double * 3),
         ("ifs_edge", C.c_double * 3), ("ifs_abs", C.c_int * 3), ("ifs_menger_sponge_mode", C.c_int),
         ("ifs_direction", (C.c_double * 3) * 9), ("ifs_angles", (C.c_double * 3) * 9),
-        ("ifs_distance", C.c_double * 9), ("ifs_intensity", C.c_double * 9), ("ifs_enabled", C.c_int * 9), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
+        ("ifs_distance", C.c_double * 9), ("ifs_intensity", C.c_double * 9), ("ifs_enabled", C.c_int * 9),
+        ("hybrid_formula", C.c_int * 5), ("hybrid_iterations", C.c_int * 5), ("hybrid_power", C.c_double * 5),
+        ("hybrid_cyclic", C.c_int), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
         ("dof_focus", C.c_double), ("dof_radius", C.c_double),
     ]
 

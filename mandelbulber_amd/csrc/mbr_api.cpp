@@ -136,6 +136,15 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
       f.ifs_edge[c] = (T)s.ifs_edge[c];
     }
     f.ifs_menger = s.ifs_menger_sponge_mode;
+    int period = 0;
+    for (int k = 0; k < 5; k++) {
+      f.hyb_f[k] = s.hybrid_formula[k];
+      f.hyb_it[k] = s.hybrid_iterations[k];
+      f.hyb_p[k] = (T)s.hybrid_power[k];
+      if (s.hybrid_formula[k] != 0 && (k < 4 || s.hybrid_cyclic)) period += s.hybrid_iterations[k];
+    }
+    f.hyb_cyclic = s.hybrid_cyclic;
+    f.hyb_period = period;
     f.ifs_scale = (T)s.ifs_scale;
   }
   f.bailout = (T)s.bailout;
@@ -472,6 +481,19 @@ int band_cap(int precision, int ao_count) {
 
 int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
+  if (s.formula == MBR_FORMULA_HYBRID) {
+    int period = 0;
+    for (int k = 0; k < 5; k++) {
+      const int hf = s.hybrid_formula[k];
+      if (hf != 0 && (hf == MBR_FORMULA_HYBRID || !formula_supported(hf)))
+        return c->fail(MBR_ERR_UNSUPPORTED, "hybrid slot " + std::to_string(k + 1) + ": formula " + std::to_string(hf) +
+                                                " is not in the render core");
+      if (hf != 0 && s.hybrid_iterations[k] < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative hybrid iterations");
+      if (hf != 0 && (k < 4 || s.hybrid_cyclic)) period += s.hybrid_iterations[k];
+    }
+    // a cyclic sequence of no iterations never ends in the reference
+    if (s.hybrid_cyclic && period == 0) return c->fail(MBR_ERR_INVALID_ARG, "cyclic hybrid of zero iterations");
+  }
   if (!formula_supported(s.formula))
     return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
                                             " is not in the render core (supported: 1-8, 11, 13-17, 19-21)");

@@ -46,6 +46,26 @@ MBR_HD V3<T> ld3(const T *p) { return V3<T>{p[0], p[1], p[2]}; }
 template <class T>
 MBR_HD V3<T> rot9(const T *m, V3<T> v) { return mat_apply(m, v); }
 
+// Slot of the hybrid sequence at iteration L (CreateFormulaSequence,
+// src/callbacks.cpp:1666-1703): slots 1-4 in turn for their iteration
+// counts, then slot 5 for the rest (or, cyclic, for its count and the cycle
+// repeats); -1 where the sequence holds no formula.
+template <class T>
+MBR_HD int hybrid_slot(const FractalK<T> &f, int L) {
+  int pos = L;
+  if (f.hyb_cyclic) {
+    if (f.hyb_period <= 0) return -1;
+    pos = L % f.hyb_period;
+  }
+  for (int k = 0; k < 4; k++) {
+    if (f.hyb_f[k] == 0) continue;
+    if (pos < f.hyb_it[k]) return k;
+    pos -= f.hyb_it[k];
+  }
+  if (f.hyb_f[4] != 0 && (!f.hyb_cyclic || pos < f.hyb_it[4])) return 4;
+  return -1;
+}
+
 // CVector3(alfa, beta) (src/algebra.hpp:33-38)
 template <class T>
 MBR_HD V3<T> angle_vector(T alfa, T beta) {
@@ -83,7 +103,8 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
   T ph_dz = 0, th_dz = 0;
   const int delta_limit = (Mode == MODE_DELTA2 && iter_count) ? *iter_count : -1;
   T w = 0, constantw = 0;  // fourth component of hypercomplex, quaternion, aexion
-  const T p = f.power;
+  T p = f.power;
+  T mscale = f.mb_scale, mabs = f.mb_abs_scale;  // the hybrid sets scale = p
   T tgladDE = 1;
   T tgladColor = 1;
   V3<T> constant = f.julia_mode ? ld3(f.julia) : z * f.constant_factor;  // aexion rewrites it
@@ -92,6 +113,18 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
   if (Mode == MODE_COLOURING && sizeof(T) == 8) min = T(1e200);
 
   for (L = 0; L < N; L++) {
+    int af = f.formula;  // the formula of this iteration (the hybrid sequence's, src/fractal.cpp:88-96)
+    if constexpr (sizeof(T) == 8) {
+      if (f.formula == 12) {
+        // the sequence has N entries; longer runs (normals at 5N, colouring
+        // at 10N) stay on its last entry (src/fractal.cpp:92-93)
+        const int k = hybrid_slot(f, L < f.N ? L : f.N - 1);
+        af = k >= 0 ? f.hyb_f[k] : 0;
+        p = k >= 0 ? f.hyb_p[k] : T(0);
+        mscale = p;
+        mabs = M::fabs(p);
+      }
+    }
     if (f.tglad_fold) {
       const T lim = f.fold_limit, val = f.fold_value;
       if (z.x > lim) { z.x = val - z.x; tgladColor *= T(0.9); }
@@ -109,7 +142,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       r = length_nc(z);
     }
 
-    if (f.formula == 2) {  // trig_optim
+    if (af == 2) {  // trig_optim
       T th0 = M::asin(z.z / r);
       T ph0 = M::atan2(z.y, z.x);
       T rp = M::pow(r, p - T(1));
@@ -120,7 +153,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       rp *= r;
       z = V3<T>{cth * M::cos(ph), cth * M::sin(ph), M::sin(th)} * rp + constant;
       r = length_nc(z);
-    } else if (f.formula == 7) {  // menger_sponge
+    } else if (af == 7) {  // menger_sponge
       T temp;
       z.x = M::fabs(z.x); z.y = M::fabs(z.y); z.z = M::fabs(z.z);
       if (z.x - z.y < 0) { temp = z.y; z.y = z.x; z.x = temp; }
@@ -132,7 +165,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       if (z.z > T(1)) z.z -= T(2);
       r = length_nc(z);
       tgladDE *= T(3);
-    } else if (f.formula == 8) {  // tglad (Mandelbox)
+    } else if (af == 8) {  // tglad (Mandelbox)
       const T lim = f.mb_limit, val = f.mb_value;
       if (f.mb_rot_on) {
         bool lockout = false;
@@ -171,12 +204,12 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       else if (r2 < f.mb_fR2) { T tf2 = f.mb_fR2 / r2; z = z * tf2; tgladDE *= tf2; tgladColor += f.mb_col_Sp2; }
       z = z - ld3(f.mb_offset);
       if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
-      z = z * f.mb_scale + constant;
-      tgladDE = tgladDE * f.mb_abs_scale + T(1);
+      z = z * mscale + constant;
+      tgladDE = tgladDE * mabs + T(1);
       r = length_nc(z);
     } else if constexpr (sizeof(T) < 8) {
       // the generic formulas run in FP64 only
-    } else if (f.formula == 1) {  // trig_DE (src/fractal.cpp:189-209)
+    } else if (af == 1) {  // trig_DE (src/fractal.cpp:189-209)
       const T r1 = M::pow(r, p - T(1));
       const T r2 = r1 * r;
       const T th = M::atan2(z.y, z.x);                                   // GetAlpha
@@ -192,7 +225,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       }
       z = angle_vector(p * th, p * ph) * r2 + constant;
       r = length_nc(z);
-    } else if (f.formula == 13) {  // mandelbulb2 (src/fractal.cpp:226-254)
+    } else if (af == 13) {  // mandelbulb2 (src/fractal.cpp:226-254)
       T temp, tempR;
       tempR = M::sqrt(z.x * z.x + z.y * z.y);
       z = z * (T(1) / tempR);
@@ -215,7 +248,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z = z * r;
       z = z + constant;
       r = length_nc(z);
-    } else if (f.formula == 14) {  // mandelbulb3 (src/fractal.cpp:255-282)
+    } else if (af == 14) {  // mandelbulb3 (src/fractal.cpp:255-282)
       T temp, tempR;
       T sign = 1, sign2 = 1;
       if (z.x < 0) sign2 = -1;
@@ -235,7 +268,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z = z * r;
       z = z + constant;
       r = length_nc(z);
-    } else if (f.formula == 15) {  // mandelbulb4 (src/fractal.cpp:283-299)
+    } else if (af == 15) {  // mandelbulb4 (src/fractal.cpp:283-299)
       const T rp = M::pow(r, p - T(1));
       const T angZ = M::atan2(z.y, z.x);
       const T angY = M::atan2(z.z, z.x);
@@ -251,7 +284,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
                             m[6] * z.x + m[7] * z.y + m[8] * z.z};
       z = z * rp + constant;
       r = length_nc(z);
-    } else if (f.formula == 11) {  // xenodreambuie (src/fractal.cpp:300-320)
+    } else if (af == 11) {  // xenodreambuie (src/fractal.cpp:300-320)
       const T pi = T(3.14159265358979323846);
       const T rp = M::pow(r, p);
       const T th = M::atan2(z.y, z.x);
@@ -263,30 +296,30 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.z = rp * M::cos(ph * p);
       z = z + constant;
       r = length_nc(z);
-    } else if (f.formula == 3 || f.formula == 6) {  // fast_trig, minus_fast_trig (src/fractal.cpp:321-350)
+    } else if (af == 3 || af == 6) {  // fast_trig, minus_fast_trig (src/fractal.cpp:321-350)
       const T x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
       const T temp = T(1) - z2 / (x2 + y2);
       const T newx = (x2 - y2) * temp;
       const T newy = T(2) * z.x * z.y * temp;
-      const T newz = (f.formula == 3 ? T(-2) : T(2)) * z.z * M::sqrt(x2 + y2);
+      const T newz = (af == 3 ? T(-2) : T(2)) * z.z * M::sqrt(x2 + y2);
       z.x = newx + constant.x;
       z.y = newy + constant.y;
       z.z = newz + constant.z;
       r = length_nc(z);
-    } else if (f.formula == 4) {  // hypercomplex (src/fractal.cpp:351-359)
+    } else if (af == 4) {  // hypercomplex (src/fractal.cpp:351-359)
       const V3<T> nz{z.x * z.x - z.y * z.y - z.z * z.z - w * w, T(2) * z.x * z.y - T(2) * w * z.z,
                      T(2) * z.x * z.z - T(2) * z.y * w};
       const T nw = T(2) * z.x * w - T(2) * z.y * z.z;
       z = nz + constant;
       w = nw;
       r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
-    } else if (f.formula == 5) {  // quaternion (src/fractal.cpp:360-368)
+    } else if (af == 5) {  // quaternion (src/fractal.cpp:360-368)
       const V3<T> nz{z.x * z.x - z.y * z.y - z.z * z.z - w * w, T(2) * z.x * z.y, T(2) * z.x * z.z};
       const T nw = T(2) * z.x * w;
       z = nz + constant;
       w = nw;
       r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
-    } else if (f.formula == 16) {  // foldingIntPow2 (src/fractal.cpp:794-836)
+    } else if (af == 16) {  // foldingIntPow2 (src/fractal.cpp:794-836)
       const T ff = f.fip_factor;
       if (z.x > ff) z.x = ff * T(2) - z.x;
       else if (z.x < -ff) z.x = -ff * T(2) - z.x;
@@ -309,7 +342,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.z = newz + constant.z;
       z.z *= f.fip_zfactor;
       r = length_nc(z);
-    } else if (f.formula == 17) {  // smoothMandelbox (src/fractal.cpp:751-793)
+    } else if (af == 17) {  // smoothMandelbox (src/fractal.cpp:751-793)
       const T sm = f.mb_sharp, lim = f.mb_limit, val = f.mb_value;
       const T zk1 = smooth_gt(z.x, lim, sm), zk2 = smooth_lt(z.x, -lim, sm);
       z.x = z.x * (T(1) - zk1) + (val - z.x) * zk1;
@@ -336,10 +369,10 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       tgladColor += rk1 * f.mb_col_Sp1;
       tgladColor += rk21 * f.mb_col_Sp2;
       if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
-      z = z * f.mb_scale + constant;
-      tgladDE = tgladDE * f.mb_abs_scale + T(1);
+      z = z * mscale + constant;
+      tgladDE = tgladDE * mabs + T(1);
       r = length_nc(z);
-    } else if (f.formula == 19) {  // aexion (src/fractal.cpp:916-947)
+    } else if (af == 19) {  // aexion (src/fractal.cpp:916-947)
       if (L == 0) {
         const T cx = M::fabs(constant.x + constant.y + constant.z) + f.c_add;
         const T cy = M::fabs(-constant.x - constant.y + constant.z) + f.c_add;
@@ -361,7 +394,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z = V3<T>{tx, ty, tz};
       w = tw;
       r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
-    } else if (f.formula == 20) {  // benesi (src/fractal.cpp:948-970)
+    } else if (af == 20) {  // benesi (src/fractal.cpp:948-970)
       T r1 = z.y * z.y + z.z * z.z;
       T newx;
       if (constant.x < 0 || z.x < M::sqrt(r1)) newx = z.x * z.x - r1;
@@ -373,7 +406,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.y = newy + constant.y;
       z.z = newz + constant.z;
       r = length_nc(z);
-    } else if (f.formula == 10) {  // kaleidoscopic IFS (src/fractal.cpp:837-886)
+    } else if (af == 10) {  // kaleidoscopic IFS (src/fractal.cpp:837-886)
       if (f.ifs_abs[0]) z.x = M::fabs(z.x);
       if (f.ifs_abs[1]) z.y = M::fabs(z.y);
       if (f.ifs_abs[2]) z.z = M::fabs(z.z);
@@ -414,7 +447,47 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       }
       tgladDE *= f.ifs_scale;
       r = length_nc(z);
-    } else if (f.formula == 21) {  // bristorbrot (src/fractal.cpp:971-983)
+    } else if (af >= 22 && af <= 36) {  // hybrid building blocks (src/fractal.cpp:984-1103)
+      if (af == 22) z.x = z.x >= T(0) ? z.x * z.x / (M::fabs(z.x) + p) : -z.x * z.x / (M::fabs(z.x) + p);
+      else if (af == 23) z.y = z.y >= T(0) ? z.y * z.y / (M::fabs(z.y) + p) : -z.y * z.y / (M::fabs(z.y) + p);
+      else if (af == 24) z.z = z.z >= T(0) ? z.z * z.z / (M::fabs(z.z) + p) : -z.z * z.z / (M::fabs(z.z) + p);
+      else if (af == 25) {  // invertR
+        const T rInv = r * r / (r + p);
+        z.x = z.x / r * rInv;
+        z.y = z.y / r * rInv;
+        z.z = z.z / r * rInv;
+      } else if (af == 26) {  // sphericalFold (sets every component, as the reference)
+        const T rr = r * r, pp = p * p;
+        if (rr < pp) z = V3<T>{T(1) / pp, T(1) / pp, T(1) / pp};
+        else if (rr < pp * T(4)) z = V3<T>{T(1) / rr, T(1) / rr, T(1) / rr};
+      } else if (af == 27) {  // powXYZ
+        z.x = z.x >= 0 ? M::pow(z.x, p) : -M::pow(-z.x, p);
+        z.y = z.y >= 0 ? M::pow(z.y, p) : -M::pow(-z.y, p);
+        z.z = z.z >= 0 ? M::pow(z.z, p) : -M::pow(-z.z, p);
+      } else if (af == 28) z.x = z.x * p;
+      else if (af == 29) z.y = z.y * p;
+      else if (af == 30) z.z = z.z * p;
+      else if (af == 31) z.x = z.x + p;
+      else if (af == 32) z.y = z.y + p;
+      else if (af == 33) z.z = z.z + p;
+      else if (af == 34) {
+        const T angle = M::atan2(z.z, z.y) * p;
+        const T tempR = M::sqrt(z.z * z.z + z.y * z.y);
+        z.y = tempR * M::cos(angle);
+        z.z = tempR * M::sin(angle);
+      } else if (af == 35) {
+        const T angle = M::atan2(z.z, z.x) * p;
+        const T tempR = M::sqrt(z.z * z.z + z.x * z.x);
+        z.x = tempR * M::cos(angle);
+        z.z = tempR * M::sin(angle);
+      } else {
+        const T angle = M::atan2(z.y, z.x) * p;
+        const T tempR = M::sqrt(z.x * z.x + z.y * z.y);
+        z.x = tempR * M::cos(angle);
+        z.y = tempR * M::sin(angle);
+      }
+      r = length_nc(z);
+    } else if (af == 21) {  // bristorbrot (src/fractal.cpp:971-983)
       const T newx = z.x * z.x - z.y * z.y - z.z * z.z;
       const T newy = z.y * (T(2) * z.x - z.z);
       const T newz = z.z * (T(2) * z.x + z.y);
@@ -435,13 +508,13 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     // counted as escaped, and equals "a > b" for every finite value.  The
     // formulas of the generic FP64 path keep the reference's "a > b" (a NaN
     // orbit never escapes).
-    if (formula_menger_group(f.formula)) {
+    if (formula_menger_group(af)) {
       if (!(r <= T(1000))) { distance = (r - T(2)) / tgladDE; break; }
-    } else if (formula_box_group(f.formula)) {
+    } else if (formula_box_group(af)) {
       if (!(r <= T(1024))) { distance = r / M::fabs(tgladDE); break; }
     } else {
       if (Mode == MODE_NORMAL) {
-        const bool esc = f.formula == 2 ? !(r <= f.bailout) : r > f.bailout;
+        const bool esc = af == 2 ? !(r <= f.bailout) : r > f.bailout;
         if (esc) { distance = T(0.5) * r * M::log(r) / r_dz; break; }
       } else if (Mode == MODE_COLOURING) {
         distance = length_nc(z);
@@ -464,6 +537,12 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     return distance;
   }
   if (Mode == MODE_DELTA1 || Mode == MODE_DELTA2) return r;
+  if (f.formula == 12) {  // hybrid colouring (src/fractal.cpp:1218-1229)
+    if (min > T(100)) min = T(100);
+    if (distance > T(20)) distance = T(20);
+    if (tgladColor > T(1000)) tgladColor = T(1000);
+    return distance * T(5000) + tgladColor * T(100) + min * T(1000);
+  }
   if (formula_box_group(f.formula)) return tgladColor * T(100) + length_nc(z) * f.mb_col_R;
   if (formula_menger_group(f.formula)) return min * T(1000);
   return distance * T(5000);

@@ -29,7 +29,7 @@ enum CounterIdx {
 // Formulas of the render core; the generic ones (no kernels of their own)
 // run in FP64 through the F = 0 kernels.
 inline bool formula_supported(int f) {
-  return (f >= 1 && f <= 8) || f == 10 || f == 11 || (f >= 13 && f <= 17) || (f >= 19 && f <= 21);
+  return (f >= 1 && f <= 8) || (f >= 10 && f <= 17) || (f >= 19 && f <= 36);
 }
 inline bool formula_generic(int f) { return formula_supported(f) && f != 2 && f != 7 && f != 8; }
 

@@ -29,6 +29,10 @@ struct FractalK {
   int ifs_abs[3], ifs_menger, ifs_en[9], ifs_rot_on[9], ifs_main_on;
   T ifs_scale, ifs_offset[3], ifs_edge[3];
   T ifs_rot[9][9], ifs_dir[9][3], ifs_dist[9], ifs_int[9], ifs_main[9];
+  // hybrid sequence: formulas, iterations and powers of the five slots,
+  // cyclic flag and the length of one cycle
+  int hyb_f[5], hyb_it[5], hyb_cyclic, hyb_period;
+  T hyb_p[5];
   T power, bailout, constant_factor;
   T julia[3];
   T fold_limit, fold_value, sph_fR2, sph_mR2, sph_factor1;

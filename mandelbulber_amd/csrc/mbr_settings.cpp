@@ -128,10 +128,11 @@ std::vector<KeyDef> build_table() {
     I(p + "enabled", OFF_A(ifs_enabled, i, int));
   }
   I("start_frame", OFF(start_frame)); I("end_frame", OFF(end_frame));
-  I("frames_per_keyframe", OFF(frames_per_keyframe)); G("hybrid_cyclic");
+  I("frames_per_keyframe", OFF(frames_per_keyframe)); I("hybrid_cyclic", OFF(hybrid_cyclic));
   for (int i = 1; i <= 5; i++) {
-    G("hybrid_formula_" + std::to_string(i)); G("hybrid_iterations_" + std::to_string(i));
-    G("hybrid_power_" + std::to_string(i));
+    I("hybrid_formula_" + std::to_string(i), OFF_A(hybrid_formula, i - 1, int));
+    I("hybrid_iterations_" + std::to_string(i), OFF_A(hybrid_iterations, i - 1, int));
+    D("hybrid_power_" + std::to_string(i), OFF_A(hybrid_power, i - 1, double));
   }
   I("fish_eye", OFF(perspective_type)); I("fish_eye_180cut", OFF(fish_eye_cut));
   G("stereo_enabled"); G("stereo_eye_distance");
@@ -394,6 +395,11 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->frames_per_keyframe = 100;
   std::strncpy(s->file_animation_path, "paths/path.txt", sizeof(s->file_animation_path) - 1);
   std::strncpy(s->file_keyframes, "keyframes/keyframe", sizeof(s->file_keyframes) - 1);
+  for (int i = 0; i < 5; i++) {       // share/defaults: formula 0 0 0 0 2, iterations 1, power 2
+    s->hybrid_formula[i] = i == 4 ? 2 : 0;
+    s->hybrid_iterations[i] = 1;
+    s->hybrid_power[i] = 2.0;
+  }
   s->ifs_scale = 2.0;                // share/defaults: IFS_scale 2, IFS_offsetX 1, IFS_%d_x 1, intensity 1
   s->ifs_offset[0] = 1.0;
   for (int i = 0; i < 9; i++) {

@@ -110,6 +110,8 @@ typedef struct {
   rotm mb_main, mb_rot[2][3], mb_rotinv[2][3];
   rotm ifs_main, ifs_rot[9];
   v3 ifs_dir[9];
+  int *seq;        /* hybrid formula sequence, N entries (formulaSequence) */
+  double *seq_pow; /* hybridPowerSequence */
 } fractal_ctx;
 
 static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
@@ -123,6 +125,32 @@ static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
     rot_z(&f->ifs_rot[i], s->ifs_angles[i][0]); rot_y(&f->ifs_rot[i], s->ifs_angles[i][1]);
     rot_x(&f->ifs_rot[i], s->ifs_angles[i][2]);
     f->ifs_dir[i] = v3norm(v3from(s->ifs_direction[i]));
+  }
+  /* CreateFormulaSequence (src/callbacks.cpp:1666-1703), literally */
+  f->seq = NULL;
+  f->seq_pow = NULL;
+  if (s->formula == MBR_FORMULA_HYBRID && s->N > 0) {
+    const int N = s->N;
+    f->seq = (int *)calloc((size_t)N, sizeof(int));
+    f->seq_pow = (double *)calloc((size_t)N, sizeof(double));
+    int number = 0;
+    while (number < N) {
+      for (int h = 0; h < 4; h++) {
+        if (s->hybrid_formula[h] == 0) continue;
+        for (int i = 0; i < s->hybrid_iterations[h]; i++) {
+          if (number < N) { f->seq[number] = s->hybrid_formula[h]; f->seq_pow[number] = s->hybrid_power[h]; }
+          number++;
+        }
+      }
+      int temp_end = N;
+      if (s->hybrid_cyclic) temp_end = s->hybrid_iterations[4];
+      if (s->hybrid_formula[4] != 0)
+        for (int i = 0; i < temp_end; i++) {
+          if (number < N) { f->seq[number] = s->hybrid_formula[4]; f->seq_pow[number] = s->hybrid_power[4]; }
+          number++;
+        }
+      if (!s->hybrid_cyclic) break;
+    }
   }
   /* RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555) */
   rot_set3(&f->mb_main, s->mb_rotation_main);
@@ -188,12 +216,19 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
   double tgladDE = 1.0;
   double scale = s->mb_scale;
   double tgladColor = 1.0;
-  const int formula = s->formula;
+  int formula = s->formula;
   v3 constant = s->julia_mode ? v3from(s->julia) : v3mul(z, s->constant_factor);
   double r = v3len(z);
   double min = 1e200;
 
   for (L = 0; L < N; L++) {
+    if (f->seq) {  /* hybrid: this iteration's formula and power (src/fractal.cpp:88-96) */
+      int tempL = L;
+      if (tempL > s->N - 1) tempL = s->N - 1;
+      formula = f->seq[tempL];
+      p = f->seq_pow[tempL];
+      scale = p;
+    }
     if (s->tglad_folding_mode) {
       double lim = s->folding_limit, val = s->folding_value;
       if (z.x > lim) { z.x = val - z.x; tgladColor *= 0.9; }
@@ -501,6 +536,52 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
         r = v3len(z);
         break;
       }
+      case MBR_FORMULA_INVERT_X: z.x = z.x >= 0.0 ? z.x * z.x / (fabs(z.x) + p) : -z.x * z.x / (fabs(z.x) + p); r = v3len(z); break;
+      case MBR_FORMULA_INVERT_Y: z.y = z.y >= 0.0 ? z.y * z.y / (fabs(z.y) + p) : -z.y * z.y / (fabs(z.y) + p); r = v3len(z); break;
+      case MBR_FORMULA_INVERT_Z: z.z = z.z >= 0.0 ? z.z * z.z / (fabs(z.z) + p) : -z.z * z.z / (fabs(z.z) + p); r = v3len(z); break;
+      case MBR_FORMULA_INVERT_R: {  /* :1002-1010 */
+        double rInv = r * r / (r + p);
+        z.x = z.x / r * rInv; z.y = z.y / r * rInv; z.z = z.z / r * rInv;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_SPHERICAL_FOLD: {  /* :1011-1029 */
+        double rr = r * r, pp = p * p;
+        if (rr < pp) { z.x = 1.0 / pp; z.y = 1.0 / pp; z.z = 1.0 / pp; }
+        else if (rr < pp * 4.0) { z.x = 1.0 / rr; z.y = 1.0 / rr; z.z = 1.0 / rr; }
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_POW_XYZ:  /* :1030-1037 */
+        z.x = z.x >= 0 ? pow(z.x, p) : -pow(-z.x, p);
+        z.y = z.y >= 0 ? pow(z.y, p) : -pow(-z.y, p);
+        z.z = z.z >= 0 ? pow(z.z, p) : -pow(-z.z, p);
+        r = v3len(z);
+        break;
+      case MBR_FORMULA_SCALE_X: z.x = z.x * p; r = v3len(z); break;
+      case MBR_FORMULA_SCALE_Y: z.y = z.y * p; r = v3len(z); break;
+      case MBR_FORMULA_SCALE_Z: z.z = z.z * p; r = v3len(z); break;
+      case MBR_FORMULA_OFFSET_X: z.x = z.x + p; r = v3len(z); break;
+      case MBR_FORMULA_OFFSET_Y: z.y = z.y + p; r = v3len(z); break;
+      case MBR_FORMULA_OFFSET_Z: z.z = z.z + p; r = v3len(z); break;
+      case MBR_FORMULA_ANGLE_MULTIPLY_X: {  /* :1074-1082 */
+        double angle = atan2(z.z, z.y) * p, tempR = sqrt(z.z * z.z + z.y * z.y);
+        z.y = tempR * cos(angle); z.z = tempR * sin(angle);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_ANGLE_MULTIPLY_Y: {
+        double angle = atan2(z.z, z.x) * p, tempR = sqrt(z.z * z.z + z.x * z.x);
+        z.x = tempR * cos(angle); z.z = tempR * sin(angle);
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_ANGLE_MULTIPLY_Z: {
+        double angle = atan2(z.y, z.x) * p, tempR = sqrt(z.x * z.x + z.y * z.y);
+        z.x = tempR * cos(angle); z.y = tempR * sin(angle);
+        r = v3len(z);
+        break;
+      }
       case MBR_FORMULA_BRISTORBROT: {  /* :971-983 */
         double newx = z.x * z.x - z.y * z.y - z.z * z.z;
         double newy = z.y * (2.0 * z.x - z.z);
@@ -558,6 +639,12 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     return distance;
   }
   if (mode == MODE_DELTA1 || mode == MODE_DELTA2) return r;
+  if (s->formula == MBR_FORMULA_HYBRID) {  /* src/fractal.cpp:1218-1229 */
+    if (min > 100) min = 100;
+    if (distance > 20) distance = 20;
+    if (tgladColor > 1000) tgladColor = 1000;
+    return distance * 5000.0 + tgladColor * 100.0 + min * 1000.0;
+  }
   if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX)
     return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
   if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) return min * 1000.0;

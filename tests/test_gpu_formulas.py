@@ -114,3 +114,48 @@ def test_kaleidoscopic_ifs_matches_oracle(renderer, variant, shaded):
     assert_parity(m, label=f"ifs {variant} shaded={shaded}")
     f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
     assert np.array_equal(f32.z, f64.z)
+
+
+HYBRIDS = {
+    # a Mandelbox-Mandelbulb hybrid: two box iterations (scale 2), one bulb, then bulbs
+    "box+bulb": ({0: (8, 2, 2.0), 1: (2, 1, 8.0), 4: (2, 1, 8.0)}, 0, "box"),
+    # cyclic building blocks: angleMultiplyZ, invertR, then one trig_optim
+    "blocks": ({0: (36, 1, 2.0), 1: (25, 1, 0.5), 4: (2, 1, 8.0)}, 1, "bulb"),
+}
+
+
+@pytest.mark.parametrize("shaded", [False, True])
+@pytest.mark.parametrize("name", sorted(HYBRIDS))
+def test_hybrid_matches_oracle(renderer, name, shaded):
+    """Hybrid sequences: the device picks each iteration's slot in closed
+    form (hybrid_slot), the oracle from the explicit sequence the reference
+    builds (CreateFormulaSequence); shading DEs run past N (normals at 5N,
+    colouring at 10N) on the sequence's last entry."""
+    from oracle import oracle
+    slots, cyclic, cam = HYBRIDS[name]
+    sc = scene.mandelbox_view(192, 144, shaded=shaded) if cam == "box" else (
+        scene.mandelbulb_full(192, 144, seed=1) if shaded else scene.mandelbulb_plain(256, 192))
+    if shaded:
+        sc.ao_quality = 2
+    sc.formula = 12
+    sc.N = 30
+    for k in range(5):
+        f, it, pw = slots.get(k, (0, 1, 2.0))
+        sc.hybrid_formula[k], sc.hybrid_iterations[k], sc.hybrid_power[k] = f, it, pw
+    sc.hybrid_cyclic = cyclic
+    ref, _ = oracle.render(sc)
+    renderer.set_scene(sc)
+    print(f"[formulas] hybrid {name} shaded={shaded} fp64", flush=True)
+    f64, _ = renderer.render(precision=_abi.PREC_FP64)
+    m = compare(f64, ref)
+    print(f"  {m}", flush=True)
+    assert m["hits"] > 200
+    # trig_optim slots bring pow/atan2/sin/cos into every cycle, and the Mandelbox slots fold the
+    # orbit's last-bit differences across fold edges: a few grazing hits move further within the
+    # binary search's band.  Such pixels count as differing pixels: with the flips, at most 0.1%.
+    both = (f64.z < 1e19) & (ref.z < 1e19)
+    rel = np.abs(f64.z[both].astype(np.float64) - ref.z[both]) / ref.z[both]
+    outliers = int(np.count_nonzero(rel > 1e-4))
+    print(f"  depth outliers (> 1e-4): {outliers}", flush=True)
+    assert m["flips"] + outliers <= 1e-3 * m["pixels"], m
+    assert m["psnr_db"] >= 45.0, m

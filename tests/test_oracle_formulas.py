@@ -207,3 +207,33 @@ def test_kaleidoscopic_ifs_reproduces_the_menger_sponge():
         assert d10 == pytest.approx(d7, rel=1e-6, abs=1e-12), p
         n += d7 > 0
     assert n > 100
+
+
+def _hybrid(slots, cyclic, n, julia=None):
+    """slots: {index 0..4: (formula, iterations, power)}; the rest none."""
+    s = _scene(12, 2.0, julia=julia, n=n)
+    for k in range(5):
+        f, it, pw = slots.get(k, (0, 1, 2.0))
+        s.hybrid_formula[k], s.hybrid_iterations[k], s.hybrid_power[k] = f, it, pw
+    s.hybrid_cyclic = 1 if cyclic else 0
+    return s
+
+
+def test_hybrid_with_one_slot_is_that_formula():
+    """The default hybrid (slot 5 = trig_optim) iterates exactly as
+    trig_optim with slot 5's power."""
+    h = _hybrid({4: (2, 1, 8.0)}, cyclic=False, n=40)
+    b = _scene(2, 8.0, n=40)
+    for p in POINTS:
+        assert oracle.compute(h, p, mode=MODE_DELTA1) == oracle.compute(b, p, mode=MODE_DELTA1)
+
+
+@pytest.mark.parametrize("cyclic,want", [(True, lambda x: 4 * x + 1.5), (False, lambda x: 2 * x + 1.5)])
+def test_hybrid_sequence_by_hand(cyclic, want):
+    """Slots: scaleX (power 2) once, then offsetX (power 0.5).  On the x axis
+    with a zero Julia constant, four iterations give x -> 2x -> 2x + 0.5 ->
+    (cyclic) 4x + 1 -> 4x + 1.5, or (slot 5 for the rest) 2x + 1 -> 2x + 1.5."""
+    s = _hybrid({0: (28, 1, 2.0), 4: (31, 1, 0.5)}, cyclic=cyclic, n=4, julia=(0.0, 0.0, 0.0))
+    for x in (0.3, -0.7, 1.9):
+        r, it = oracle.compute(s, (x, 0.0, 0.0), mode=MODE_DELTA1)
+        assert it == 4 and r == pytest.approx(abs(want(x)), rel=1e-15)
