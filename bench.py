@@ -32,8 +32,8 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # HBM bytes per k_ao launch on this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1,
 # the factors that make a 1 GiB device copy read 1 GiB and write 1 GiB),
 # from the rocprofv3 --pmc runs summarised in the file named here.
-AO_TRAFFIC_BYTES = 776e6
-AO_TRAFFIC_SOURCE = "profiles/render_1080p_mandelbulb_pmc.txt (FETCH_SIZE 79.4 MiB x 2 + WRITE_SIZE 581.2 MiB)"
+AO_TRAFFIC_BYTES = 811e6
+AO_TRAFFIC_SOURCE = "profiles/render_1080p_mandelbulb_pmc.txt (FETCH_SIZE 79.5 MiB x 2 + WRITE_SIZE 614.6 MiB)"
 
 
 def planes_on_gpu(torch, w, h):
@@ -343,7 +343,7 @@ def main():
     default_workload = (args.width, args.height, args.seed) == (1920, 1080, 1)
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS["f32"], "traffic": AO_TRAFFIC_BYTES if default_workload else None,
-            "traffic_source": AO_TRAFFIC_SOURCE if default_workload else None, "kernel": "k_ao<float, 2, false, true>",
+            "traffic_source": AO_TRAFFIC_SOURCE if default_workload else None, "kernel": "k_ao_f32_plain<float, 2>",
             "flops_per_launch": ao_flops, "ms_per_launch": ao_ms,
             "ao_de_calls": st32[-1].ao_de_calls, "ao_iterations": st32[-1].ao_iterations,
             "ao_steps": st32[-1].ao_steps}

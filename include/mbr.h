@@ -232,12 +232,15 @@ typedef struct mbr_scene {
   double fog_distance_factor;      /* volumetric_fog_distance_factor */
   mbr_rgb fog_colour1, fog_colour2, fog_colour3; /* fog_colour_{1,2,3}_* */
 
-  /* features outside the render core: a scene that enables any of them is
-   * rejected with MBR_ERR_UNSUPPORTED */
+  /* auxiliary lights: random ones, predefined ones (parameters below) */
   int aux_light_number;            /* aux_light_number */
   int aux_light_pre_enabled[4];    /* aux_light_predefined_N_enabled */
+  /* volumetric lights (main light, aux lights 1-4) and iteration fog:
+   * intensities and opacities below */
   int volumetric_light_enabled[5]; /* volumetric_light_enabled_N */
   int iter_fog_enabled;            /* iteration_fog_enable */
+  /* features outside the render core: a scene that enables any of them is
+   * rejected with MBR_ERR_UNSUPPORTED */
   int fake_lights_enabled;         /* fake_lights_enabled */
   int raytraced_reflections;       /* raytraced_reflections */
   int reflections_max;             /* reflections_max */
@@ -267,8 +270,7 @@ typedef struct mbr_scene {
   char file_animation_path[512];   /* file_animation_path (flight path: vp x y z, alfa beta gamma per line) */
   char file_keyframes[512];        /* file_keyframes (prefix of the keyframe .fract files) */
 
-  /* post effects on the compiled image (src/image.cpp; order in the
-   * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
+  /* formula parameters of the delta-DE and fold formulas */
   int linear_DE_mode;              /* linear_DE_mode (delta-DE formulas: 0.5 r / dr) */
   double mb_sharpness;             /* mandelbox_sharpness (smoothMandelbox) */
   double folding_int_pow_factor;   /* FoldingIntPow_folding_factor */
@@ -294,11 +296,35 @@ typedef struct mbr_scene {
   int hybrid_iterations[5];        /* hybrid_iterations_1..5 */
   double hybrid_power[5];          /* hybrid_power_1..5 */
   int hybrid_cyclic;               /* hybrid_cyclic */
+
+  /* post effects on the compiled image (src/image.cpp; order in the
+   * -nogui render: CompileImage, SSAO, DOF, src/Render3D.cpp:1094-1105) */
   int ssao_enabled;                /* post_SSAO_enabled */
   int ssao_quality;                /* post_SSAO_quality */
   int dof_enabled;                 /* post_DOF_enabled */
   double dof_focus;                /* post_DOF_focus */
   double dof_radius;               /* post_DOF_radius */
+
+  /* auxiliary lights: placement (PlaceRandomLights, src/shaders.cpp:1134-1212),
+   * surface shading (AuxLightsShader / LightShading / AuxShadow, :997-1132)
+   * and their glow in VolumetricShader (:241-300).  aux_light_number and
+   * aux_light_pre_enabled sit with the switches above. */
+  double aux_light_intensity;           /* aux_light_intensity */
+  int aux_light_random_seed;            /* aux_light_random_seed (srand seed of the placement) */
+  double aux_light_max_dist;            /* aux_light_max_dist */
+  double aux_light_distribution_radius; /* aux_light_distribution_radius */
+  double aux_light_visibility;          /* aux_light_visibility */
+  double aux_light_random_center[3];    /* aux_light_random_center_X/Y/Z */
+  double aux_light_pre[4][3];           /* aux_light_predefined_N_x/y/z */
+  double aux_light_pre_intensity[4];    /* aux_light_predefined_N_intensity */
+  mbr_rgb aux_light_pre_colour[4];      /* aux_light_predefined_N_colour_R/G/B */
+
+  /* volumetric lights and iteration fog (VolumetricShader, src/shaders.cpp:
+   * 314-341, 385-465; their effect on MainShadow / AuxShadow, :494-548,
+   * :1084-1117); the switches sit with volumetric_light_enabled above */
+  double volumetric_light_intensity[5]; /* volumetric_light_intensity_0..4 */
+  double iter_fog_opacity;              /* iteration_fog_opacity */
+  double iter_fog_opacity_trim;         /* iteration_fog_opacity_trim */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
@@ -514,6 +540,38 @@ int mbr_post_dof(mbr_ctx *ctx, const mbr_scene *scene, const mbr_post_planes *pl
  * support values share a sign and vary by more than 10% of their mean);
  * the other fields of `scene` are left as they are. */
 int mbr_keyframe_scene(const mbr_scene *keys, int n_keys, int frames_per_key, int frame, mbr_scene *scene);
+
+/* One auxiliary light (sLight, src/shaders.h; the Lights array of
+ * src/shaders.cpp:17). */
+typedef struct mbr_light {
+  double position[3];
+  double intensity;
+  int colour[3];                   /* sRGB: 16-bit channel values */
+  int enabled;
+} mbr_light;
+
+/* Replaces PlaceRandomLights(params, false) (src/shaders.cpp:1134-1212), on
+ * the host, in FP64: aux_light_number random lights drawn with the C
+ * library's rand() sequence of srand(aux_light_random_seed) (glibc's
+ * generator, kept private: the process's own rand() state is untouched) and
+ * accepted where the distance estimate lies in (0, aux_light_max_dist), then
+ * the predefined lights written over slots 0-3.  Writes aux_light_number + 4
+ * records (`capacity` must allow that; *count gets the number).  Returns
+ * MBR_ERR_INVALID_ARG when 20 million draws place no further light (the
+ * reference would loop forever). */
+int mbr_place_lights(const mbr_scene *scene, mbr_light *lights, int capacity, int *count);
+
+/* The context's light table, which mbr_set_scene uses.  Without a call,
+ * mbr_set_scene places the lights itself (mbr_place_lights) on its first
+ * scene with aux lights and again whenever the placement inputs (seed,
+ * number, radius, centre, maximum distance) change; the random lights then
+ * stay where they are while fractal parameters change, as in the
+ * reference's animation loop (src/Render3D.cpp:1990-1993, :2305).  Either
+ * way each mbr_set_scene writes the scene's predefined lights over slots 0-3
+ * (PlaceRandomLights(params, true)).  lights == NULL (or count 0) returns
+ * to automatic placement. */
+int mbr_set_lights(mbr_ctx *ctx, const mbr_light *lights, int count);
+int mbr_get_lights(mbr_ctx *ctx, mbr_light *lights, int capacity, int *count);
 
 /* Layout probe for bindings: sizeof(mbr_scene) and the offsets of
  * abi_version, vp, formula, mb_rotation, palette, fog_colour3,

@@ -100,7 +100,20 @@ class Scene(C.Structure):
         ("hybrid_formula", C.c_int * 5), ("hybrid_iterations", C.c_int * 5), ("hybrid_power", C.c_double * 5),
         ("hybrid_cyclic", C.c_int), ("ssao_enabled", C.c_int), ("ssao_quality", C.c_int), ("dof_enabled", C.c_int),
         ("dof_focus", C.c_double), ("dof_radius", C.c_double),
+        ("aux_light_intensity", C.c_double), ("aux_light_random_seed", C.c_int),
+        ("aux_light_max_dist", C.c_double), ("aux_light_distribution_radius", C.c_double),
+        ("aux_light_visibility", C.c_double), ("aux_light_random_center", C.c_double * 3),
+        ("aux_light_pre", (C.c_double * 3) * 4), ("aux_light_pre_intensity", C.c_double * 4),
+        ("aux_light_pre_colour", RGB * 4),
+        ("volumetric_light_intensity", C.c_double * 5), ("iter_fog_opacity", C.c_double),
+        ("iter_fog_opacity_trim", C.c_double),
     ]
+
+
+class Light(C.Structure):
+    """One auxiliary light (mbr_light)."""
+    _fields_ = [("position", C.c_double * 3), ("intensity", C.c_double), ("colour", C.c_int * 3),
+                ("enabled", C.c_int)]
 
 
 LAYOUT_FIELDS = ("abi_version", "vp", "formula", "mb_rotation", "palette", "fog_colour3",

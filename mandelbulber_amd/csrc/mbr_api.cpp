@@ -23,6 +23,7 @@
 #include "mbr_core.hpp"
 #include "mbr_fractal.hpp"
 #include "mbr_kernels.hpp"
+#include "mbr_rand.hpp"
 #include "mbr_scene_dev.hpp"
 
 using namespace mbr;
@@ -246,6 +247,8 @@ void fill_shade(ShadeK<T> &k, const mbr_scene &s, const double light[3], double 
   k.fog_c2_dist = (T)s.fog_colour2_distance;
   k.fog_reduce = (T)s.fog_distance_factor;
   k.glow_intensity = (T)s.glow_intensity;
+  k.aux_intensity = (T)s.aux_light_intensity;
+  k.aux_visibility = (T)s.aux_light_visibility;
 }
 
 void rgb3(int *o, const mbr_rgb &c) { o[0] = c.R; o[1] = c.G; o[2] = c.B; }
@@ -295,6 +298,7 @@ struct Slot {
   int prec = -1;
   // shading buffers (per hit; element type of the band's precision)
   void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
+  void *sh_aux = nullptr;
   size_t sh_ao_bytes = 0;
 };
 
@@ -342,6 +346,15 @@ struct mbr_ctx {
   size_t d_dof_bytes = 0;
   void *d_dof_pairs = nullptr;
   size_t d_dof_pairs_bytes = 0;
+  // aux lights: the Lights array (placed by mbr_place_lights or given by
+  // mbr_set_lights), the inputs of the automatic placement, and the device
+  // list of the lights of the current scene
+  std::vector<mbr_light> lights;
+  bool lights_user = false, lights_placed = false;
+  double lights_key[7] = {};
+  double *d_lights_d = nullptr;
+  float *d_lights_f = nullptr;
+  int d_lights_cap = 0;
 
   int fail(int rc, const std::string &msg) {
     err = msg;
@@ -371,10 +384,10 @@ void free_band(Slot &sl) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   b = BandBufs{};
-  void *sp[] = {sl.sh_base, sl.sh_spec, sl.sh_col, sl.sh_fast, sl.sh_ao};
+  void *sp[] = {sl.sh_base, sl.sh_spec, sl.sh_col, sl.sh_fast, sl.sh_ao, sl.sh_aux};
   for (void *p : sp)
     if (p) (void)hipFree(p);
-  sl.sh_base = sl.sh_spec = sl.sh_col = sl.sh_fast = sl.sh_ao = nullptr;
+  sl.sh_base = sl.sh_spec = sl.sh_col = sl.sh_fast = sl.sh_ao = sl.sh_aux = nullptr;
   sl.sh_ao_bytes = 0;
   sl.prec = -1;
 }
@@ -454,6 +467,7 @@ int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count) {
     HIPCHECK(c, hipMalloc(&sl.sh_spec, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_col, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_fast, n * b.rec_bytes));
+    HIPCHECK(c, hipMalloc(&sl.sh_aux, 6 * n * b.rec_bytes));
     // one directory and one chunk per 64 rays to begin with; grown when a
     // frame needs more
     int rc = alloc_pool(c, sl, std::max<long long>(256, (long long)n / 64), std::max<long long>(256, (long long)n / 64));
@@ -502,12 +516,7 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
   if (s.raytraced_reflections && s.reflections_max > 0)
     return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections are not in the render core");
-  if (s.aux_light_number > 0) return c->fail(MBR_ERR_UNSUPPORTED, "random aux lights are not in the render core");
-  for (int i = 0; i < 4; i++)
-    if (s.aux_light_pre_enabled[i]) return c->fail(MBR_ERR_UNSUPPORTED, "predefined aux lights are not in the render core");
-  for (int i = 0; i < 5; i++)
-    if (s.volumetric_light_enabled[i])
-      return c->fail(MBR_ERR_UNSUPPORTED, "volumetric lights are not in the render core");
+  if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
   if (s.iter_fog_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "iteration fog is not in the render core");
   if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
   if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
@@ -571,7 +580,7 @@ void mbr_destroy(mbr_ctx *c) {
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
   void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad, c->d_ssao_tab,
-                c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma};
+                c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma, c->d_lights_d, c->d_lights_f};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
@@ -667,7 +676,17 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   d.ao_quality = s.ao_quality;
   d.slow_shading = s.slow_shading;
   d.coloring = s.coloring_enabled;
-  d.bsoft = (!s.limits_enabled && !s.iter_thresh) && std::tan(s.shadow_cone_angle / 180.0 * kPi) > 0.0;
+  d.bsoft = (!s.iter_fog_enabled && !s.limits_enabled && !s.iter_thresh) &&
+            std::tan(s.shadow_cone_angle / 180.0 * kPi) > 0.0;
+  // volumetric lights (src/interface.cpp:697-702) and iteration fog change
+  // the shadow marches; the aux lights' volumetric switches are resolved
+  // against the light table in setup_lights
+  d.vol_main = s.volumetric_light_enabled[0];
+  d.shadow_fx = s.iter_fog_enabled;
+  for (int i = 0; i < 5; i++) {
+    if (s.volumetric_light_enabled[i]) d.shadow_fx = 1;
+    d.vol_intensity[i] = s.volumetric_light_intensity[i];
+  }
   d.post_fog = s.post_fog_enabled;
   d.N = s.N;
   d.minN = s.minN;
@@ -714,15 +733,201 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   return MBR_OK;
 }
 
+bool scene_has_lights(const mbr_scene &s) {
+  if (s.aux_light_number > 0) return true;
+  for (int i = 0; i < 4; i++)
+    if (s.aux_light_pre_enabled[i]) return true;
+  return false;
+}
+
+constexpr long long kMaxLightDraws = 20000000;  // placement trials before giving up
+
+// PlaceRandomLights(params, false) (src/shaders.cpp:1134-1212) on the host,
+// in FP64, with the generic distance estimator (the reference's operation
+// order, as in the oracle) and a private copy of the
+// C library's rand() sequence.  The reference draws the two terms of each
+// coordinate left to right and the three colour channels in order; an
+// accepted light gets its colour normalised to a maximum channel of 65536.
+int place_lights(const mbr_scene &s, std::vector<mbr_light> &out) {
+  const int n = std::max(0, s.aux_light_number);
+  out.assign((size_t)n + 4, mbr_light{});
+  FractalK<double> fd;
+  std::memset(&fd, 0, sizeof fd);
+  fill_fractal(fd, s);
+  GlibcRand rnd((uint32_t)s.aux_light_random_seed);
+  auto random = [&](int max) { return rnd.next() % (max + 1); };  // Random, src/common_math.cpp:25-32
+  int trial_number = 0;
+  double radius_multiplier = 1.0;
+  long long draws = 0;
+  for (int i = 0; i < n; i++) {
+    trial_number++;
+    if (++draws > kMaxLightDraws) return MBR_ERR_INVALID_ARG;
+    double pos[3];
+    for (int k = 0; k < 3; k++) {
+      const double a = random(2000000) / 1000000.0 - 1.0;
+      const double b = random(1000000) / 1.0e12;
+      pos[k] = s.aux_light_random_center[k] + (a + b) * s.aux_light_distribution_radius * radius_multiplier;
+    }
+    DEInfo info;
+    const double distance =
+        calc_distance_generic<double>(fd, v3<double>(pos[0], pos[1], pos[2]), 0.0, s.N, s.minN, false, info);
+    if (trial_number > 1000) {
+      radius_multiplier *= 1.1;
+      trial_number = 0;
+    }
+    if (distance > 0 && !info.max_iter && distance < s.aux_light_max_dist * radius_multiplier) {
+      radius_multiplier = 1.0;
+      mbr_light &L = out[(size_t)i];
+      for (int k = 0; k < 3; k++) L.position[k] = pos[k];
+      int col[3];
+      for (int k = 0; k < 3; k++) col[k] = 20000 + random(80000);
+      const double maxColour = std::max(std::max((double)col[0], (double)col[1]), (double)col[2]);
+      for (int k = 0; k < 3; k++) L.colour[k] = (int)(col[k] * 65536.0 / maxColour);
+      L.intensity = distance * distance / s.aux_light_max_dist;
+      L.enabled = 1;
+    } else {
+      i--;
+    }
+  }
+  return MBR_OK;
+}
+
+// PlaceRandomLights(params, true): the scene's predefined lights over slots
+// 0-3, the others of those slots marked disabled.
+void apply_predefined(const mbr_scene &s, std::vector<mbr_light> &t) {
+  for (int i = 0; i < 4 && i < (int)t.size(); i++) {
+    mbr_light &L = t[(size_t)i];
+    if (s.aux_light_pre_enabled[i]) {
+      for (int k = 0; k < 3; k++) L.position[k] = s.aux_light_pre[i][k];
+      L.intensity = s.aux_light_pre_intensity[i];
+      L.colour[0] = s.aux_light_pre_colour[i].R;
+      L.colour[1] = s.aux_light_pre_colour[i].G;
+      L.colour[2] = s.aux_light_pre_colour[i].B;
+      L.enabled = 1;
+    } else {
+      L.enabled = 0;
+    }
+  }
+}
+
+// The lights of scene s in the device scene block d: place them when the
+// context has no placement for these inputs (MainRender places them once,
+// src/Render3D.cpp:1990-1993), write the predefined ones over slots 0-3
+// (:2305), and upload the list of the lights that take part.
+int setup_lights(mbr_ctx *c, const mbr_scene &s, DevScene &d) {
+  d.n_lights = 0;
+  d.light_div = 0;
+  d.lights_d = nullptr;
+  d.lights_f = nullptr;
+  if (!scene_has_lights(s)) return MBR_OK;
+  const double key[7] = {(double)s.aux_light_random_seed, (double)s.aux_light_number, s.aux_light_distribution_radius,
+                         s.aux_light_max_dist, s.aux_light_random_center[0], s.aux_light_random_center[1],
+                         s.aux_light_random_center[2]};
+  if (!c->lights_user && (!c->lights_placed || std::memcmp(key, c->lights_key, sizeof key) != 0)) {
+    std::vector<mbr_light> t;
+    if (place_lights(s, t) != MBR_OK)
+      return c->fail(MBR_ERR_INVALID_ARG, "aux light placement: no position accepted in 20 million draws");
+    c->lights.swap(t);
+    std::memcpy(c->lights_key, key, sizeof key);
+    c->lights_placed = true;
+  }
+  const int number = s.aux_light_number;
+  const int count = std::max(number, 4);  // lightsPlaced
+  if ((int)c->lights.size() < count)
+    return c->fail(MBR_ERR_INVALID_ARG, "light table has " + std::to_string(c->lights.size()) +
+                                            " entries, the scene needs " + std::to_string(count));
+  apply_predefined(s, c->lights);
+  // volumetric aux lights: slots 0-3 when enabled (src/shaders.cpp:326-328)
+  for (int k = 0; k < 4; k++) {
+    const mbr_light &L = c->lights[(size_t)k];
+    d.vol_aux_on[k] = s.volumetric_light_enabled[k + 1] && L.enabled;
+    const double v[8] = {L.position[0], L.position[1], L.position[2], L.intensity,
+                         (double)L.colour[0], (double)L.colour[1], (double)L.colour[2], 0.0};
+    std::memcpy(d.vol_aux[k], v, sizeof v);
+  }
+  std::vector<double> ld;
+  for (int i = 0; i < count; i++) {
+    const mbr_light &L = c->lights[(size_t)i];
+    if (!(i < number || L.enabled)) continue;
+    const double v[8] = {L.position[0], L.position[1], L.position[2], L.intensity,
+                         (double)L.colour[0], (double)L.colour[1], (double)L.colour[2], 0.0};
+    ld.insert(ld.end(), v, v + 8);
+  }
+  const int n_used = (int)(ld.size() / 8);
+  d.light_div = count;
+  if (n_used == 0) return MBR_OK;
+  // renders of the previous scene may still read the table
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (n_used > c->d_lights_cap) {
+    if (c->d_lights_d) (void)hipFree(c->d_lights_d);
+    if (c->d_lights_f) (void)hipFree(c->d_lights_f);
+    c->d_lights_d = nullptr;
+    c->d_lights_f = nullptr;
+    c->d_lights_cap = 0;
+    HIPCHECK(c, hipMalloc((void **)&c->d_lights_d, (size_t)n_used * 8 * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&c->d_lights_f, (size_t)n_used * 8 * sizeof(float)));
+    c->d_lights_cap = n_used;
+  }
+  std::vector<float> lf(ld.begin(), ld.end());
+  HIPCHECK(c, hipMemcpy(c->d_lights_d, ld.data(), ld.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->d_lights_f, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
+  d.lights_d = c->d_lights_d;
+  d.lights_f = c->d_lights_f;
+  d.n_lights = n_used;
+  return MBR_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mbr_place_lights(const mbr_scene *scene, mbr_light *lights, int capacity, int *count) {
+  if (!scene || !count) return MBR_ERR_INVALID_ARG;
+  mbr_ctx tmp;
+  int rc = check_supported(&tmp, *scene);
+  if (rc != MBR_OK) return rc;
+  const int n = std::max(0, scene->aux_light_number) + 4;
+  *count = n;
+  if (!lights || capacity < n) return MBR_ERR_INVALID_ARG;
+  std::vector<mbr_light> t;
+  rc = place_lights(*scene, t);
+  if (rc != MBR_OK) return rc;
+  apply_predefined(*scene, t);
+  std::memcpy(lights, t.data(), t.size() * sizeof(mbr_light));
+  return MBR_OK;
+}
+
+int mbr_set_lights(mbr_ctx *c, const mbr_light *lights, int count) {
+  if (!c || count < 0 || (count > 0 && !lights)) return MBR_ERR_INVALID_ARG;
+  if (!lights || count == 0) {
+    c->lights.clear();
+    c->lights_user = false;
+    c->lights_placed = false;
+    return MBR_OK;
+  }
+  c->lights.assign(lights, lights + count);
+  c->lights_user = true;
+  return MBR_OK;
+}
+
+int mbr_get_lights(mbr_ctx *c, mbr_light *lights, int capacity, int *count) {
+  if (!c || !count) return MBR_ERR_INVALID_ARG;
+  *count = (int)c->lights.size();
+  if (!lights) return MBR_OK;
+  if (capacity < *count) return c->fail(MBR_ERR_INVALID_ARG, "capacity below the light count");
+  std::memcpy(lights, c->lights.data(), c->lights.size() * sizeof(mbr_light));
+  return MBR_OK;
+}
 
 int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap) {
   if (!c || !scene_in) return MBR_ERR_INVALID_ARG;
   Prepared p;
   int rc = prepare_scene(c, scene_in, lightmap, p);
   if (rc != MBR_OK) return rc;
+  HIPCHECK(c, hipSetDevice(c->device));
+  rc = setup_lights(c, p.s, p.d);
+  if (rc != MBR_OK) return rc;
+  p.d.vol_any = p.d.vol_main || p.d.vol_aux_on[0] || p.d.vol_aux_on[1] || p.d.vol_aux_on[2] || p.d.vol_aux_on[3];
   const DevScene &d = p.d;
   const std::vector<double> &dirs = p.dirs;
   const std::vector<int> &rgb = p.rgb;
@@ -769,11 +974,13 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
   sbuf.col = sl.sh_col;
   sbuf.fast_ao = sl.sh_fast;
   sbuf.ao = sl.sh_ao;
+  sbuf.aux = sl.sh_aux;
   sbuf.n = b.cap;
   sbuf.slow_ao = host.ao && !host.fast_ao;
   sbuf.deep = host.deep;
   sbuf.plain = host.plain;
   sbuf.formula = formula;
+  sbuf.lights = host.n_lights > 0;
   int band = 0;
   for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
     if (cancel && *cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
@@ -788,7 +995,8 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, c->d_stats, st,
                              e ? e[4] : nullptr, e ? e[5] : nullptr));
     if (e) HIPCHECK(c, hipEventRecord(e[2], st));
-    HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st));
+    HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula, host.vol_any,
+                              host.n_lights > 0 && host.sd.aux_visibility > 0, persist, c->d_stats));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
   return MBR_OK;
@@ -1261,6 +1469,10 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   for (int i = 0; i < n; i++) {
     rc = prepare_scene(c, &scenes[i], lightmap, frames[i].p);
     if (rc != MBR_OK) return c->fail(rc, "scene " + std::to_string(i) + ": " + c->err);
+    if (scene_has_lights(frames[i].p.s) || frames[i].p.d.shadow_fx)
+      return c->fail(MBR_ERR_UNSUPPORTED, "scene " + std::to_string(i) +
+                                              ": aux lights and volumetric lights render through mbr_set_scene, "
+                                              "not in a batch");
     const mbr_scene &s = frames[i].p.s;
     if (fbs[i].width != s.image_width || fbs[i].height != s.image_height)
       return c->fail(MBR_ERR_INVALID_ARG, "framebuffer " + std::to_string(i) + " differs from its scene's size");
@@ -1401,12 +1613,13 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
   const mbr_scene &s = c->scene;
   const DevScene &h = c->host_sc;
   if (s.formula != MBR_FORMULA_TRIG_OPTIM) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: formula must be 2");
+  if (h.n_lights > 0) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: aux lights have no gradient here");
   if (!s.hit_refine) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: the scene needs hit_refine = 1");
-  if (h.shadow || h.ao || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
+  if (h.shadow || h.ao || h.shadow_fx || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
       s.perspective_type != MBR_PERSP_THREE_POINT || s.tiles != 1 || s.constant_DE_threshold || s.interior_mode ||
       s.limits_enabled || s.iter_thresh || h.fd.prim_plane || h.fd.prim_box || h.fd.prim_ibox ||
       h.fd.prim_sphere || h.fd.prim_isphere || h.fd.prim_water || h.fd.tglad_fold || h.fd.sph_fold)
-    return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: shadows, AO, glow, fog, post fog, fish-eye, tiles, "
+    return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: shadows, AO, volumetric lights, glow, fog, post fog, fish-eye, tiles, "
                                         "constant threshold, interior mode, limits, iteration threshold, "
                                         "primitives and pre-folds have no gradient here");
   hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;

@@ -552,6 +552,7 @@ struct SurfBufs {
   T *col;     // [3][n] surface colour
   T *fast_ao; // [n]
   T *ao;      // [n * dirs] AO intensity per (hit, direction)
+  T *aux;     // [6][n] aux lights: diffuse R, G, B, specular R, G, B
   long long n;
 };
 
@@ -608,9 +609,54 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     sb.col[1 * n + tid] = colour.G;
     sb.col[2 * n + tid] = colour.B;
     if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T, F, Deep>(sc, in, cnt);
+    if (sc.n_lights > 0) {
+      // k_aux shades the aux lights from the normal and the detail size
+      // MainShadow left (the aux shadows inherit it)
+      sb.aux[0 * n + tid] = in.normal.x;
+      sb.aux[1 * n + tid] = in.normal.y;
+      sb.aux[2 * n + tid] = in.normal.z;
+      sb.aux[3 * n + tid] = in.detail;
+    }
     b.col_r[r] = x86_d2u8((double)colour.R * 255);
     b.col_g[r] = x86_d2u8((double)colour.G * 255);
     b.col_b[r] = x86_d2u8((double)colour.B * 255);
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_SHADE_DE] = cnt.de_calls;
+  v[ST_SHADE_ITERS] = cnt.iterations;
+  flush_stats(stats, v);
+}
+
+// AuxLightsShader (src/shaders.cpp:1051-1073) per hit, after k_surface: the
+// normal and detail size come from k_surface's entries in sb.aux, which are
+// replaced by the diffuse and specular sums.  A kernel of its own, so the
+// surface kernel of scenes without aux lights keeps its registers.
+template <class T, int F, bool Deep>
+__global__ void __launch_bounds__(kBlock) k_aux(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                SurfBufs<T> sb, unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const long long n_hits = (long long)b.counters[CNT_HITS];
+  const long long n = sb.n;
+  LaneCount cnt = {0, 0};
+  for (long long tid = (long long)blockIdx.x * kBlock + threadIdx.x; tid < n_hits;
+       tid += (long long)gridDim.x * kBlock) {
+    const int r = b.hit_list[tid];
+    const int p = band_first + r;
+    V3<double> dir;
+    view_vector(sc, p % sc.width, p / sc.width, dir);
+    ShaderIn<T> in;
+    load_hit(sc, b, r, in);
+    in.view = cvt<T>(dir);
+    in.normal = v3<T>(sb.aux[0 * n + tid], sb.aux[1 * n + tid], sb.aux[2 * n + tid]);
+    in.detail = sb.aux[3 * n + tid];
+    RGBT<T> al, as;
+    aux_lights<T, F, Deep>(sc, in, al, as, cnt);
+    sb.aux[0 * n + tid] = al.R;
+    sb.aux[1 * n + tid] = al.G;
+    sb.aux[2 * n + tid] = al.B;
+    sb.aux[3 * n + tid] = as.R;
+    sb.aux[4 * n + tid] = as.G;
+    sb.aux[5 * n + tid] = as.B;
   }
   unsigned long long v[ST_COUNT] = {0};
   v[ST_SHADE_DE] = cnt.de_calls;
@@ -851,20 +897,31 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
     }
     if (lane == 0) {
       const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
-      ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
-      ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
-      ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
+      if (sc.n_lights > 0) {
+        // (ambient + main light) x colour + aux x colour, then aux specular
+        // + main specular (ObjectShader, src/shaders.cpp:96-108)
+        ((RT *)b.sh_r)[r] = (RT)(((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.aux[0 * n + tid] * cR) +
+                                 (sb.aux[3 * n + tid] + sb.spec[0 * n + tid]));
+        ((RT *)b.sh_g)[r] = (RT)(((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.aux[1 * n + tid] * cG) +
+                                 (sb.aux[4 * n + tid] + sb.spec[1 * n + tid]));
+        ((RT *)b.sh_b)[r] = (RT)(((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.aux[2 * n + tid] * cB) +
+                                 (sb.aux[5 * n + tid] + sb.spec[2 * n + tid]));
+      } else {
+        ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
+        ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
+        ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------- finish
-template <class T, class RT>
-__global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                   int band_rays, FrameOut out) {
-  const DevScene &sc = *scp;
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= band_rays) return;
+// One pixel: background or shaded surface, then VolumetricShader over the
+// ray's step records.  FX: the volumetric lights' shadow marches run here
+// (distance estimates of formula F).
+template <class T, class RT, int F, bool FX, bool VIS>
+__device__ __forceinline__ void finish_body(const DevScene &sc, const BandBufs &b, int band_first, int r,
+                                            const FrameOut &out, LaneCount &cnt) {
   const int p = band_first + r;
   const bool found = b.found[r];
   RGBT<T> px;
@@ -888,18 +945,73 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
   const RT *st = (const RT *)b.rec_step, *di = (const RT *)b.rec_dist, *th = (const RT *)b.rec_thr;
   const long long cap = b.cap;
   const int slab = b.slab;
+  // visible aux lights glow along the ray: its direction and the scan of
+  // the last record (the sum of all recorded steps)
+  VisRay vray;
+  const VisRay *vr = nullptr;
+  const bool vis = VIS && sc.n_lights > 0 && sc.sd.aux_visibility > 0;
+  if (vis || FX) {
+    const int x = p % sc.width, y = p / sc.width;
+    view_vector(sc, x, y, vray.dir);
+    vray.start = v3<double>(sc.start[0], sc.start[1], sc.start[2]);
+    vr = &vray;
+  }
+  // the volumetric lights of a step group (src/shaders.cpp:314-341), from
+  // the shader input of the pixel with the group's point and threshold
+  ShaderIn<T> pin;
+  if constexpr (FX) {
+    load_hit(sc, b, r, pin);
+    pin.view = cvt<T>(vray.dir);
+  }
+  auto fx = [&](V3<double> p64, T thr, T step, VolOut<T> &o) {
+    if constexpr (FX) {
+      pin.point64 = p64;
+      pin.point = cvt<T>(p64);
+      pin.dist_thresh = thr;
+      if (sc.vol_main) {
+        const T sh = main_shadow<T, F, false>(sc, pin, cnt);
+        const T I0 = (T)sc.vol_intensity[0];
+        o.rgb.R += sh * step * I0 * T(sc.main_light_colour[0]) / T(65536);
+        o.rgb.G += sh * step * I0 * T(sc.main_light_colour[1]) / T(65536);
+        o.rgb.B += sh * step * I0 * T(sc.main_light_colour[2]) / T(65536);
+        o.alpha += (sh + sh + sh) / T(3) * step * I0;
+      }
+      for (int k = 0; k < 4; k++) {
+        if (!sc.vol_aux_on[k]) continue;
+        const double *L = sc.vol_aux[k];
+        const V3<T> d = v3<T>((T)L[0], (T)L[1], (T)L[2]) - pin.point;
+        const T dl = length(d);
+        const T d2 = dl * dl;
+        const V3<T> lv = normalize(d);
+        const T light = aux_shadow<T, F, false>(sc, pin, dl, lv, cnt);
+        const T Ik = (T)sc.vol_intensity[k + 1];
+        o.rgb.R += light * T(L[4]) / T(65536) * Ik * step / d2;
+        o.rgb.G += light * T(L[5]) / T(65536) * Ik * step / d2;
+        o.rgb.B += light * T(L[6]) / T(65536) * Ik * step / d2;
+        o.alpha += light * Ik * step / d2;
+      }
+    } else {
+      (void)p64; (void)thr; (void)step; (void)o;
+    }
+  };
   if (n <= slab) {
-    v = volumetric<T>(sc, depth, n, px, alpha, [&](int i, T &s, T &d, T &t) {
+    auto rec = [&](int i, T &s, T &d, T &t) {
       const long long o = (long long)i * cap + r;
       s = (T)st[o]; d = (T)di[o]; t = (T)th[o];
-    });
+    };
+    if (vr) {
+      double sum = 0.0;
+      for (int i = 0; i < n; i++) sum += (double)st[(long long)i * cap + r];
+      vray.scan_end = sum;
+    }
+    v = volumetric<T>(sc, depth, n, px, alpha, rec, vr, vis, fx);
   } else {
     // records from the slab on sit in the chunks the ray's directory lists
     // (the host renders a band again when a record found the pool full)
     const RT *ost = (const RT *)b.ovf_step, *odi = (const RT *)b.ovf_dist, *oth = (const RT *)b.ovf_thr;
     const long long dj = b.ovf_off[r];
     const int *dir = b.ovf_dir + (dj >= 0 ? dj : 0) * kDirSlots;
-    v = volumetric<T>(sc, depth, n, px, alpha, [&](int i, T &s, T &d, T &t) {
+    auto rec = [&](int i, T &s, T &d, T &t) {
       const int ch = (i >= slab && dj >= 0) ? dir[(i - slab) / kChunk] : -1;
       if (i < slab) {
         const long long o = (long long)i * cap + r;
@@ -910,7 +1022,17 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
       } else {
         s = 0; d = 0; t = 0;
       }
-    });
+    };
+    if (vr) {
+      double sum = 0.0;
+      for (int i = 0; i < n; i++) {
+        T s, d, t;
+        rec(i, s, d, t);
+        sum += (double)s;
+      }
+      vray.scan_end = sum;
+    }
+    v = volumetric<T>(sc, depth, n, px, alpha, rec, vr, vis, fx);
   }
   if (out.rgb[0]) out.rgb[0][p] = (float)v.rgb.R;
   if (out.rgb[1]) out.rgb[1][p] = (float)v.rgb.G;
@@ -921,6 +1043,30 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
   if (out.colour[2]) out.colour[2][p] = cb;
   if (out.opacity) out.opacity[p] = x86_d2u16((double)v.opacity * 65535.0);
   if (out.alpha) out.alpha[p] = x86_d2u16((double)v.alpha * 65535.0);
+}
+
+// VIS: the variant with the glow of visible aux lights
+template <class T, class RT, bool VIS>
+__global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                   int band_rays, FrameOut out) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= band_rays) return;
+  LaneCount cnt = {0, 0};
+  finish_body<T, RT, 0, false, VIS>(*scp, b, band_first, r, out, cnt);
+}
+
+// With volumetric lights: a grid-stride loop (the shadow marches make the
+// per-pixel cost vary widely), distance-estimate counts into the stats.
+template <class T, class RT, int F>
+__global__ void __launch_bounds__(kBlock) k_finish_fx(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                      int band_rays, FrameOut out, unsigned long long *stats) {
+  LaneCount cnt = {0, 0};
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < band_rays; r += gridDim.x * kBlock)
+    finish_body<T, RT, F, true, true>(*scp, b, band_first, r, out, cnt);
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_SHADE_DE] = cnt.de_calls;
+  v[ST_SHADE_ITERS] = cnt.iterations;
+  flush_stats(stats, v);
 }
 
 // ---------------------------------------------------------------- launch
@@ -1022,6 +1168,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     sb.col = (T *)sbuf.col;
     sb.fast_ao = (T *)sbuf.fast_ao;
     sb.ao = (T *)sbuf.ao;
+    sb.aux = (T *)sbuf.aux;
     sb.n = sbuf.n;
     const bool generic = formula_generic(formula);
     auto surf = [&](auto deep_tag) {
@@ -1068,6 +1215,24 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     const bool deep = sbuf.deep && sizeof(T) < 8 && !generic;
     if (deep) surf(std::true_type{});
     else surf(std::false_type{});
+    if (sbuf.lights) {
+      if (generic) {
+        if constexpr (std::is_same<T, double>::value)
+          hipLaunchKernelGGL((k_aux<T, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+      } else {
+        auto aux = [&](auto deep_tag) {
+          constexpr bool D = decltype(deep_tag)::value;
+          if (formula == 7)
+            hipLaunchKernelGGL((k_aux<T, 7, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+          else if (formula == 8)
+            hipLaunchKernelGGL((k_aux<T, 8, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+          else
+            hipLaunchKernelGGL((k_aux<T, 2, D>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
+        };
+        if (deep) aux(std::true_type{});
+        else aux(std::false_type{});
+      }
+    }
     if (ev_ao0) {
       hipError_t e = hipEventRecord(ev_ao0, s);
       if (e != hipSuccess) return e;
@@ -1093,14 +1258,47 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
 }
 
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                         FrameOut out, hipStream_t s) {
+                         FrameOut out, hipStream_t s, int formula, int fx, int vis, int persist_blocks,
+                         unsigned long long *stats) {
   if (band_rays <= 0) return hipSuccess;
-  if (precision == 0)
-    hipLaunchKernelGGL((k_finish<double, double>), grid_for(band_rays), dim3(kBlock), 0, s, sc, b, band_first,
-                       band_rays, out);
-  else
-    hipLaunchKernelGGL((k_finish<float, float>), grid_for(band_rays), dim3(kBlock), 0, s, sc, b, band_first,
-                       band_rays, out);
+  if (fx) {
+    if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)std::max(1, std::min(persist_blocks, (band_rays + kBlock - 1) / kBlock)));
+    if (formula_generic(formula)) {
+      hipLaunchKernelGGL((k_finish_fx<double, double, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays,
+                         out, stats);
+      return hipGetLastError();
+    }
+    auto run = [&](auto tag, auto rtag) {
+      using T = decltype(tag);
+      using RT = decltype(rtag);
+      if (formula == 7)
+        hipLaunchKernelGGL((k_finish_fx<T, RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
+                           stats);
+      else if (formula == 8)
+        hipLaunchKernelGGL((k_finish_fx<T, RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
+                           stats);
+      else
+        hipLaunchKernelGGL((k_finish_fx<T, RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
+                           stats);
+    };
+    if (precision == 0) run(double{}, double{});
+    else run(float{}, float{});
+    return hipGetLastError();
+  }
+  auto plain = [&](auto tag, auto vis_tag) {
+    using T = decltype(tag);
+    constexpr bool V = decltype(vis_tag)::value;
+    hipLaunchKernelGGL((k_finish<T, T, V>), grid_for(band_rays), dim3(kBlock), 0, s, sc, b, band_first, band_rays,
+                       out);
+  };
+  if (precision == 0) {
+    if (vis) plain(double{}, std::true_type{});
+    else plain(double{}, std::false_type{});
+  } else {
+    if (vis) plain(float{}, std::true_type{});
+    else plain(float{}, std::false_type{});
+  }
   return hipGetLastError();
 }
 

@@ -99,11 +99,13 @@ hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
+  void *aux;                    // [6][n]: aux light diffuse and specular terms
   long long n;                  // hit capacity (plane stride)
   int slow_ao;                  // 1: run the AmbientOcclusion rays
   int deep;                     // 1: FP32 shading starts distance estimates in FP64 where needed
   int plain;                    // 1: no primitives, limits, interior, iteration threshold, pre-folds, Julia
   int formula;
+  int lights;                   // 1: aux lights take part (k_aux runs after k_surface)
 };
 // Shading in three launches: surface (normals, shadow, specular, colour,
 // fast AO), AO rays, composition; ev_ao (when not null) is recorded around
@@ -121,8 +123,13 @@ struct AdjParams {
 hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
                           const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
                           int blocks, hipStream_t s);
+// Background, VolumetricShader and framebuffer planes.  fx: the scene has
+// volumetric lights, whose shadow marches run in a grid-stride kernel of
+// persist_blocks workgroups templated on the formula; vis: visible aux
+// lights glow along the rays.
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                         FrameOut out, hipStream_t s);
+                         FrameOut out, hipStream_t s, int formula, int fx, int vis, int persist_blocks,
+                         unsigned long long *stats);
 
 // Screen-space ambient occlusion of one compiled frame (mbr_post.hip).
 // cs/sn: device tables of cos/sin(2 pi i / quality), i < quality.

@@ -255,6 +255,21 @@ int main(int argc, char **argv) {
     std::printf("mbr_init failed (status %d): no usable GPU\n", rc);
     return 1;
   }
+  // the random aux lights are placed once, from the loaded settings, and
+  // keep their places over an animation; each frame rewrites the predefined
+  // ones (src/Render3D.cpp:1990-1993, :2305)
+  if (s.aux_light_number > 0) {
+    std::vector<mbr_light> tab((size_t)s.aux_light_number + 4);
+    int count = 0;
+    rc = mbr_place_lights(&s, tab.data(), (int)tab.size(), &count);
+    if (rc == MBR_OK) rc = mbr_set_lights(ctx, tab.data(), count);
+    if (rc != MBR_OK) {
+      std::printf("aux light placement failed (status %d)\n", rc);
+      mbr_destroy(ctx);
+      return 1;
+    }
+    std::printf("%d aux lights placed\n", s.aux_light_number);
+  }
   int status = 0;
   if (!flight && !keyframe) {
     // a still image: frame number -start (0 by default)

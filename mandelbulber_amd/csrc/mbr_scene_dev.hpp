@@ -61,6 +61,7 @@ struct ShadeK {
   T fog_visibility;     // pow(10, post_fog_visibility/10 - 16)
   T fog_density, fog_c1_dist, fog_c2_dist, fog_reduce;
   T glow_intensity;
+  T aux_intensity, aux_visibility;  // auxLightIntensity, auxLightVisibility
 };
 
 struct DevScene {
@@ -82,6 +83,25 @@ struct DevScene {
   int slow_shading, coloring, bsoft, post_fog;
   int N, minN;
 
+  // auxiliary lights that take part, in the order of the reference's Lights
+  // array: 8 values per light (x, y, z, intensity, colour R, G, B, unused),
+  // in FP64 and FP32.  light_div is lightsPlaced (at least 4), the divisor
+  // of every light's intensity (src/shaders.cpp:1053-1062).
+  const double *lights_d;
+  const float *lights_f;
+  int n_lights, light_div;
+
+  // volumetric lights (VolumetricShader, src/shaders.cpp:314-341): vol_main
+  // for the main light; vol_aux_on[k] when aux light slot k is enabled and
+  // its volumetric light switched on, vol_aux[k] its position, intensity and
+  // colour as in the light table; vol_intensity[0..4] their intensities.
+  // shadow_fx: volumetric lights or iteration fog are on, so the shadow
+  // marches take unit DE steps and a threshold from the distance to the eye.
+  int shadow_fx, vol_main, vol_any;
+  int vol_aux_on[4];
+  double vol_aux[4][8];
+  double vol_intensity[5];
+
   // integer colours (sRGB, 16-bit channel values)
   int main_light_colour[3];
   int glow1[3], glow2[3];
@@ -100,6 +120,8 @@ struct DevScene {
   __host__ __device__ const FractalK<T> &frac() const;
   template <class T>
   __host__ __device__ const ShadeK<T> &shade() const;
+  template <class T>
+  __host__ __device__ const T *lights() const;
 };
 
 template <>
@@ -110,5 +132,9 @@ template <>
 __host__ __device__ inline const ShadeK<double> &DevScene::shade<double>() const { return sd; }
 template <>
 __host__ __device__ inline const ShadeK<float> &DevScene::shade<float>() const { return sf; }
+template <>
+__host__ __device__ inline const double *DevScene::lights<double>() const { return lights_d; }
+template <>
+__host__ __device__ inline const float *DevScene::lights<float>() const { return lights_f; }
 
 }  // namespace mbr

@@ -99,13 +99,21 @@ std::vector<KeyDef> build_table() {
   D("main_light_intensity", OFF(main_light_intensity));
   D("main_light_alfa", OFF(main_light_alpha)); D("main_light_beta", OFF(main_light_beta));
   RGB("main_light_colour_", OFF_RGB(main_light_colour, R), OFF_RGB(main_light_colour, G), OFF_RGB(main_light_colour, B));
-  G("aux_light_intensity"); G("aux_light_random_seed"); I("aux_light_number", OFF(aux_light_number));
-  G("aux_light_max_dist"); G("aux_light_distribution_radius"); G("aux_light_visibility");
-  G("aux_light_random_center_X"); G("aux_light_random_center_Y"); G("aux_light_random_center_Z");
+  D("aux_light_intensity", OFF(aux_light_intensity)); I("aux_light_random_seed", OFF(aux_light_random_seed));
+  I("aux_light_number", OFF(aux_light_number));
+  D("aux_light_max_dist", OFF(aux_light_max_dist));
+  D("aux_light_distribution_radius", OFF(aux_light_distribution_radius));
+  D("aux_light_visibility", OFF(aux_light_visibility));
+  D("aux_light_random_center_X", OFF_A(aux_light_random_center, 0, double));
+  D("aux_light_random_center_Y", OFF_A(aux_light_random_center, 1, double));
+  D("aux_light_random_center_Z", OFF_A(aux_light_random_center, 2, double));
   for (int i = 1; i <= 4; i++) {
     std::string p = "aux_light_predefined_" + std::to_string(i) + "_";
-    G(p + "x"); G(p + "y"); G(p + "z"); G(p + "intensity");
-    G(p + "colour_R"); G(p + "colour_G"); G(p + "colour_B");
+    const size_t pos = OFF(aux_light_pre) + sizeof(double) * 3 * (i - 1);
+    D(p + "x", pos); D(p + "y", pos + sizeof(double)); D(p + "z", pos + 2 * sizeof(double));
+    D(p + "intensity", OFF_A(aux_light_pre_intensity, i - 1, double));
+    const size_t col = OFF(aux_light_pre_colour) + sizeof(mbr_rgb) * (i - 1);
+    RGB(p + "colour_", col + offsetof(mbr_rgb, R), col + offsetof(mbr_rgb, G), col + offsetof(mbr_rgb, B));
     I(p + "enabled", OFF_A(aux_light_pre_enabled, i - 1, int));
   }
   D("IFS_scale", OFF(ifs_scale));
@@ -167,11 +175,12 @@ std::vector<KeyDef> build_table() {
   D("volumetric_fog_colour_1_distance", OFF(fog_colour1_distance));
   D("volumetric_fog_colour_2_distance", OFF(fog_colour2_distance));
   D("volumetric_fog_distance_factor", OFF(fog_distance_factor));
-  I("iteration_fog_enable", OFF(iter_fog_enabled)); G("iteration_fog_opacity"); G("iteration_fog_opacity_trim");
+  I("iteration_fog_enable", OFF(iter_fog_enabled)); D("iteration_fog_opacity", OFF(iter_fog_opacity));
+  D("iteration_fog_opacity_trim", OFF(iter_fog_opacity_trim));
   G("volumetric_light_intensity"); G("volumetric_light_quality");
   for (int i = 0; i < 5; i++) {
     I("volumetric_light_enabled_" + std::to_string(i), OFF_A(volumetric_light_enabled, i, int));
-    G("volumetric_light_intensity_" + std::to_string(i));
+    D("volumetric_light_intensity_" + std::to_string(i), OFF_A(volumetric_light_intensity, i, double));
   }
   I("primitive_only_plane", OFF(primitive_only_plane));
   I("primitive_plane_enabled", OFF(plane_enabled));
@@ -415,6 +424,23 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->dof_enabled = 0;
   s->dof_focus = 21.699999999999999;
   s->dof_radius = 10.0;
+  // share/defaults: volumetric lights and iteration fog
+  for (int i = 0; i < 5; i++) s->volumetric_light_intensity[i] = 100.0;
+  s->iter_fog_opacity = 1000.0;
+  s->iter_fog_opacity_trim = 4.0;
+  // share/defaults: aux light block
+  s->aux_light_intensity = 1.0;
+  s->aux_light_random_seed = 1234;
+  s->aux_light_max_dist = 0.10000000000000001;
+  s->aux_light_distribution_radius = 3.0;
+  s->aux_light_visibility = 1.0;
+  const double pre[4][4] = {{3, -3, -3, 1.3}, {-3, -3, 0, 1}, {-3, 3, -1, 3}, {0, -1, 3, 2}};
+  const int pre_col[4][3] = {{45761, 53633, 59498}, {62875, 55818, 50083}, {64884, 64928, 48848}, {52704, 62492, 45654}};
+  for (int i = 0; i < 4; i++) {
+    for (int k = 0; k < 3; k++) s->aux_light_pre[i][k] = pre[i][k];
+    s->aux_light_pre_intensity[i] = pre[i][3];
+    s->aux_light_pre_colour[i] = mbr_rgb{pre_col[i][0], pre_col[i][1], pre_col[i][2]};
+  }
   return MBR_OK;
 }
 

@@ -9,8 +9,11 @@
 //   object_shader<T>      ObjectShader           src/shaders.cpp:21-112
 //   background<T>         BackgroundShader       src/shaders.cpp:114-183
 //   volumetric<T>         VolumetricShader       src/shaders.cpp:185-478
+//   aux_shadow<T>         AuxShadow              src/shaders.cpp:1075-1132
+//   aux_lights<T>         AuxLightsShader        src/shaders.cpp:1051-1073
+//                         + LightShading         src/shaders.cpp:997-1049
 //
-// Features outside the render core (env mapping, aux/fake/volumetric lights,
+// Features outside the render core (env mapping, fake/volumetric lights,
 // iteration fog) are rejected at mbr_set_scene, so their terms are zero here.
 #pragma once
 
@@ -107,6 +110,18 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   return n;
 }
 
+// Threshold of a shadow-march point when volumetric lights or iteration fog
+// are on (src/shaders.cpp:511-519, :1102-1110): from its distance to the eye.
+template <class T>
+MBR_HD float fx_thresh(const DevScene &sc, V3<T> point2) {
+  const ShadeK<T> &k = sc.shade<T>();
+  if (!sc.constant_thresh) {
+    const V3<T> eye = v3<T>((T)sc.start[0], (T)sc.start[1], (T)sc.start[2]);
+    return (float)(length(point2 - eye) * k.resolution * k.persp / k.quality);
+  }
+  return (float)k.quality;
+}
+
 template <class T, int F = 0, bool Deep = false>
 MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
@@ -114,7 +129,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   if (!sc.penetrating) factor = k.view_distance_max;
   T dist = in.dist_thresh;
   in.detail = in.dist_thresh;
-  const T DE_factor = k.DE_factor;
+  const T DE_factor = sc.shadow_fx ? T(1) : k.DE_factor;
   T start = in.dist_thresh;
   if (sc.interior) start = in.dist_thresh * DE_factor;
   T shadowTemp = 1;
@@ -123,7 +138,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   int guard = 0;
   for (T i = start; i < factor && guard < kMaxShadeSteps; i += dist * DE_factor, guard++) {
 
-    const float dist_thresh = (float)in.dist_thresh;
+    const float dist_thresh = sc.shadow_fx ? fx_thresh<T>(sc, in.point + in.light * i) : (float)in.dist_thresh;
     in.detail = (T)dist_thresh;
     dist = de_at<T, F, Deep>(sc, in, in.light * i, in.detail, sc.N, sc.minN, false, info, cnt);
     if (sc.bsoft) {
@@ -201,6 +216,73 @@ MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTa
   ao.G /= den;
   ao.B /= den;
   return ao;
+}
+
+// AuxShadow without iteration fog (its opacity term is 0): march from the
+// hit towards the light until the light's distance.
+template <class T, int F = 0, bool Deep = false>
+MBR_HD T aux_shadow(const DevScene &sc, ShaderIn<T> &in, T distance, V3<T> lightVector, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  T dist = in.delta;
+  T shadowTemp = 1;
+  const T DE_factor = sc.shadow_fx ? T(1) : k.DE_factor;
+  DEInfo info;
+  int guard = 0;
+  for (T i = in.delta; i < distance && guard < kMaxShadeSteps; i += dist * DE_factor, guard++) {
+    dist = de_at<T, F, Deep>(sc, in, lightVector * i, in.detail, sc.N, sc.minN, false, info, cnt);
+    const float dist_thresh = sc.shadow_fx ? fx_thresh<T>(sc, in.point + lightVector * i) : (float)in.dist_thresh;
+    if (dist < (T)dist_thresh || info.max_iter || shadowTemp < T(0)) {
+      if (sc.penetrating) {
+        shadowTemp -= (distance - i) / distance;
+        if (shadowTemp < T(0)) shadowTemp = 0;
+      } else {
+        shadowTemp = 0;
+      }
+      break;
+    }
+  }
+  return shadowTemp;
+}
+
+// AuxLightsShader: the sum over the lights that take part of LightShading's
+// diffuse (`shade`) and specular (`spec`) terms, in the lights' order.
+template <class T, int F = 0, bool Deep = false>
+MBR_HD void aux_lights(const DevScene &sc, ShaderIn<T> &in, RGBT<T> &shade, RGBT<T> &spec, LaneCount &cnt) {
+  const ShadeK<T> &k = sc.shade<T>();
+  const T *tab = sc.lights<T>();
+  shade = RGBT<T>{0, 0, 0};
+  spec = RGBT<T>{0, 0, 0};
+  for (int l = 0; l < sc.n_lights; l++) {
+    const T *L = tab + 8 * l;
+    const V3<T> d = v3<T>(L[0], L[1], L[2]) - in.point;
+    const T distance = length(d);
+    const V3<T> lightVector = normalize(d);
+    const T intensity = k.aux_intensity * T(100) * L[3] / (distance * distance) / T(sc.light_div);
+    T sh = dot(in.normal, lightVector);
+    if (sh < T(0)) sh = 0;
+    sh = sh * intensity;
+    if (sh > T(500)) sh = 500;
+    const V3<T> half = normalize(lightVector - in.view);
+    T sh2 = dot(in.normal, half);
+    if (sh2 < T(0)) sh2 = 0;
+    sh2 = Math<T>::pow(sh2, T(30));
+    sh2 *= intensity * k.specular;
+    if (sh2 > T(15)) sh2 = 15;
+    if ((sh > T(0.01) || sh2 > T(0.01)) && sc.shadow) {
+      const T light = aux_shadow<T, F, Deep>(sc, in, distance, lightVector, cnt);
+      sh *= light;
+      sh2 *= light;
+    } else if (sc.shadow) {
+      sh = 0;
+      sh2 = 0;
+    }
+    shade.R += sh * L[4] / T(65536);
+    shade.G += sh * L[5] / T(65536);
+    shade.B += sh * L[6] / T(65536);
+    spec.R += sh2 * L[4] / T(65536);
+    spec.G += sh2 * L[5] / T(65536);
+    spec.B += sh2 * L[6] / T(65536);
+  }
 }
 
 MBR_HD void index_to_colour(int index, const int (*palette)[3], int out[3]) {
@@ -325,8 +407,73 @@ struct VolOut {
   T opacity;
 };
 
-template <class T, class RecFn>
-MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> px, T alpha, RecFn rec) {
+// The ray of a pixel, for the glow of visible aux lights: a march point is
+// start + dir * scan, scan being the sum of the recorded steps up to it
+// (src/Render3D.cpp:96-139: step i is recorded before scan advances).
+struct VisRay {
+  V3<double> start, dir;
+  double scan_end;  // scan at the last record
+};
+
+// Glow of the visible aux lights over one group of march steps ending at
+// `point` (src/shaders.cpp:241-300), in FP64.
+template <class T>
+MBR_HD void visible_lights(const DevScene &sc, V3<double> point, V3<double> view, double step, VolOut<T> &o) {
+  const ShadeK<double> &k = sc.sd;
+  const double *tab = sc.lights_d;
+  double lowestLightSize = 1e10, lowestLightDist = 1e10;
+  for (int l = 0; l < sc.n_lights; l++) {
+    const double *L = tab + 8 * l;
+    const double lightDist = length(point - v3<double>(L[0], L[1], L[2]));
+    const double r2 = lightDist / (L[3] * k.aux_intensity * k.aux_visibility);
+    if (r2 < lowestLightSize) {
+      lowestLightSize = r2;
+      lowestLightDist = lightDist;
+    }
+  }
+  int smallSteps = x86_d2i(10.0 * step / (lowestLightDist + 1.0e-20));
+  if (smallSteps > 50) smallSteps = 50;
+  int s0 = 0, s1 = 1;
+  double step2 = step;
+  if (smallSteps > 0) {
+    s0 = 1;
+    s1 = smallSteps + 1;
+    step2 = step / (smallSteps + 1.0);
+  }
+  double R = (double)o.rgb.R, G = (double)o.rgb.G, B = (double)o.rgb.B, A = (double)o.alpha;
+  for (int ss = s0; ss < s1; ss++) {
+    const V3<double> point2 = point - (view * step2) * (double)ss;
+    for (int l = 0; l < sc.n_lights; l++) {
+      const double *L = tab + 8 * l;
+      const double lightDist = length(point2 - v3<double>(L[0], L[1], L[2]));
+      const double lightSize = L[3] * k.aux_intensity * k.aux_visibility;
+      double r2 = lightDist / lightSize;
+      if (r2 > 1.0) r2 = 1.0;
+      const double bell = (cos(r2 * 3.14159265358979323846) + 1.0) / (r2 * r2 + 0.02) * 0.3;
+      const double lightDensity = step2 * bell / lightSize;
+      R += lightDensity * L[4] / 65536.0;
+      G += lightDensity * L[5] / 65536.0;
+      B += lightDensity * L[6] / 65536.0;
+      A += lightDensity;
+    }
+  }
+  o.rgb.R = (T)R;
+  o.rgb.G = (T)G;
+  o.rgb.B = (T)B;
+  o.alpha = (T)A;
+}
+
+struct NoFx {
+  template <class T>
+  MBR_HD void operator()(V3<double>, T, T, VolOut<T> &) const {}
+};
+
+// VolumetricShader over the step records.  With a ray (vr), a step group's
+// point is known: the glow of visible aux lights is added when `vis`, and
+// `fx(point, threshold, step, out)` adds the volumetric lights.
+template <class T, class RecFn, class FxFn = NoFx>
+MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> px, T alpha, RecFn rec,
+                            const VisRay *vr = nullptr, bool vis = false, FxFn fx = NoFx{}) {
   const ShadeK<T> &k = sc.shade<T>();
   VolOut<T> o;
   o.rgb = px;
@@ -340,9 +487,12 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
   const T glowG = (T(sc.glow1[1]) * glowN + T(sc.glow2[1]) * glow) / T(65536);
   const T glowB = (T(sc.glow1[2]) * glowN + T(sc.glow2[2]) * glow) / T(65536);
   T totalStep = 0;
+  double scan = vr ? vr->scan_end : 0.0;
   for (int index = stepCount - 1; index > 0; index--) {
     T step, distance, dist_thresh;
     rec(index, step, distance, dist_thresh);
+    const double scan_i = scan;  // scan of this record's point
+    scan -= (double)step;
     totalStep += step;
     if (totalStep < dist_thresh * k.quality) continue;
     step = totalStep;
@@ -353,6 +503,12 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
     o.rgb.G = glowOpacity * glowG + (T(1) - glowOpacity) * o.rgb.G;
     o.rgb.B = glowOpacity * glowB + (T(1) - glowOpacity) * o.rgb.B;
     o.alpha += glowOpacity;
+
+    if (vr) {
+      const V3<double> point = vr->start + vr->dir * scan_i;
+      if (vis) visible_lights<T>(sc, point, vr->dir, (double)step, o);
+      fx(point, dist_thresh, step, o);
+    }
 
     if (sc.post_fog) {
       T fd = step / k.fog_visibility;

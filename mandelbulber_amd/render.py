@@ -74,6 +74,22 @@ class Renderer:
               self._ctx)
         self.scene = scene
 
+    def lights(self):
+        """The context's aux light table (list of mbr_light records)."""
+        n = C.c_int()
+        check(lib().mbr_get_lights(self._ctx, None, 0, C.byref(n)), self._ctx)
+        tab = (_abi.Light * max(n.value, 1))()
+        check(lib().mbr_get_lights(self._ctx, tab, n.value, C.byref(n)), self._ctx)
+        return list(tab)[:n.value]
+
+    def set_lights(self, lights):
+        """Use `lights` (mbr_light records, or None for automatic placement)."""
+        if not lights:
+            check(lib().mbr_set_lights(self._ctx, None, 0), self._ctx)
+            return
+        tab = (_abi.Light * len(lights))(*lights)
+        check(lib().mbr_set_lights(self._ctx, tab, len(lights)), self._ctx)
+
     def _opts(self, precision, rows, stream):
         o = _abi.RenderOpts()
         o.precision = precision

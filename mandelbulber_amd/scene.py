@@ -289,3 +289,16 @@ def deep_zoom(width=3840, height=2160, seed=0, frame_width=1e-6):
     apply(s, {"shadows_enabled": 1, "shadows_cone_angle": 0, "penetrating_lights": 0,
               "main_light_alfa": a, "main_light_beta": b})
     return s
+
+
+def place_lights(scene):
+    """PlaceRandomLights: aux_light_number + 4 mbr_light records (host, FP64)."""
+    from . import _abi
+    import ctypes as C
+    n = max(scene.aux_light_number, 0) + 4
+    tab = (_abi.Light * n)()
+    cnt = C.c_int()
+    rc = lib().mbr_place_lights(C.byref(scene), tab, n, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(f"mbr_place_lights failed: {rc}")
+    return list(tab)

@@ -981,8 +981,20 @@ typedef struct {
   double detail;          /* calcParam->doubles.detailSize as the reference mutates it */
   int objectType;
   const double *ao_dirs; const int *ao_rgb; int ao_count;
+  const struct light_set *lights;
   de_counter *cnt;
 } shader_in;
+
+/* The Lights array and lightsPlaced of src/shaders.cpp:17-18: light i takes
+ * part when i < auxLightNumber or it is enabled (:1059, :248). */
+typedef struct light_set {
+  const mbr_light *tab;
+  int number;   /* auxLightNumber */
+  int count;    /* lightsPlaced, at least 4 */
+  int any;      /* some light takes part */
+} light_set;
+
+static int light_used(const light_set *L, int i) { return i < L->number || L->tab[i].enabled; }
 
 /* CalculateNormals (src/shaders.cpp:675-757) */
 static v3 calculate_normals(shader_in *in) {
@@ -1016,27 +1028,54 @@ static v3 calculate_normals(shader_in *in) {
   return normal;
 }
 
-/* MainShadow (src/shaders.cpp:480-570) without iteration fog and
- * volumetric lights (rejected scene features). */
+/* imageSwitches.volumetricLightEnabled (src/interface.cpp:697-702) or
+ * iteration fog: the shadow marches then take unit DE steps and a threshold
+ * from the distance to the eye (src/shaders.cpp:494, :511-519, :1084, :1102-1110) */
+static int shadow_fx(const mbr_scene *s) {
+  int v = s->iter_fog_enabled;
+  for (int i = 0; i < 5; i++)
+    if (s->volumetric_light_enabled[i]) v = 1;
+  return v;
+}
+
+static float fx_thresh(const shader_in *in, v3 point2) {
+  const mbr_scene *s = in->f->s;
+  if (!s->constant_DE_threshold)
+    return (float)(v3len(v3sub(point2, in->c->start)) * in->c->resolution * s->persp / s->quality);
+  return (float)s->quality;
+}
+
+/* IterOpacity (src/shaders.cpp:1214-1222) */
+static double iter_opacity(double step, double iters, double maxN, double trim, double opacitySp) {
+  double opacity = (iters - trim) / maxN;
+  if (opacity < 0.0) opacity = 0.0;
+  opacity *= opacity;
+  opacity *= step * opacitySp;
+  if (opacity > 1.0) opacity = 1.0;
+  return opacity;
+}
+
+/* MainShadow (src/shaders.cpp:480-570) */
 static double main_shadow(shader_in *in) {
   const mbr_scene *s = in->f->s;
+  const int fx = shadow_fx(s);
   double factor = in->delta / in->c->resolution;
   if (!s->penetrating_lights) factor = s->view_distance_max;
   double dist = in->dist_thresh;
   in->detail = in->dist_thresh;
-  double DE_factor = s->DE_factor;
+  double DE_factor = fx ? 1.0 : s->DE_factor;
   double start = in->dist_thresh;
   if (s->interior_mode) start = in->dist_thresh * DE_factor;
   double shadowTemp = 1.0;
   double softRange = tan(s->shadow_cone_angle / 180.0 * M_PI);
   double maxSoft = 0.0;
-  int bSoft = (!s->limits_enabled && !s->iter_thresh) && softRange > 0.0;
-  int mi;
+  int bSoft = (!s->iter_fog_enabled && !s->limits_enabled && !s->iter_thresh) && softRange > 0.0;
+  int mi, it;
   for (double i = start; i < factor; i += dist * DE_factor) {
     v3 point2 = v3add(in->point, v3mul(in->lightVect, i));
-    float dist_thresh = (float)in->dist_thresh;
+    float dist_thresh = fx ? fx_thresh(in, point2) : (float)in->dist_thresh;
     in->detail = dist_thresh;
-    dist = calc_distance(in->f, point2, in->detail, s->N, s->minN, 0, &mi, NULL, NULL, in->cnt);
+    dist = calc_distance(in->f, point2, in->detail, s->N, s->minN, 0, &mi, &it, NULL, in->cnt);
     if (bSoft) {
       double angle = (dist - dist_thresh) / i;
       if (angle < 0) angle = 0;
@@ -1046,6 +1085,10 @@ static double main_shadow(shader_in *in) {
       if (softShadow < 0) softShadow = 0;
       if (softShadow > maxSoft) maxSoft = softShadow;
     }
+    double opacity = s->iter_fog_enabled
+                         ? iter_opacity(dist * DE_factor, it, s->N, s->iter_fog_opacity_trim, s->iter_fog_opacity)
+                         : 0.0;
+    shadowTemp -= opacity * (factor - i) / factor;
     if (dist < dist_thresh || shadowTemp < 0.0) {
       shadowTemp -= (factor - i) / factor;
       if (!s->penetrating_lights) shadowTemp = 0.0;
@@ -1156,7 +1199,170 @@ static shade_t surface_colour(shader_in *in) {
   return out;
 }
 
-/* ObjectShader (src/shaders.cpp:21-112) without env mapping, aux and fake
+/* AuxShadow (src/shaders.cpp:1075-1132) */
+static double aux_shadow(shader_in *in, double distance, v3 lightVector) {
+  const mbr_scene *s = in->f->s;
+  const int fx = shadow_fx(s);
+  double dist = in->delta;
+  double shadowTemp = 1.0;
+  double DE_factor = fx ? 1.0 : s->DE_factor;
+  int mi, it;
+  for (double i = in->delta; i < distance; i += dist * DE_factor) {
+    v3 point2 = v3add(in->point, v3mul(lightVector, i));
+    dist = calc_distance(in->f, point2, in->detail, s->N, s->minN, 0, &mi, &it, NULL, in->cnt);
+    double opacity = s->iter_fog_enabled
+                         ? iter_opacity(dist * DE_factor, it, s->N, s->iter_fog_opacity_trim, s->iter_fog_opacity)
+                         : 0.0;
+    shadowTemp -= opacity * (distance - i) / distance;
+    float dist_thresh = fx ? fx_thresh(in, point2) : (float)in->dist_thresh;
+    if (dist < dist_thresh || mi || shadowTemp < 0.0) {
+      if (s->penetrating_lights) {
+        shadowTemp -= (distance - i) / distance;
+        if (shadowTemp < 0.0) shadowTemp = 0.0;
+      } else {
+        shadowTemp = 0.0;
+      }
+      break;
+    }
+  }
+  return shadowTemp;
+}
+
+/* LightShading (src/shaders.cpp:997-1049) */
+static shade_t light_shading(shader_in *in, const mbr_light *light, int number, shade_t *outSpecular) {
+  const mbr_scene *s = in->f->s;
+  v3 d = v3sub(v3from(light->position), in->point);
+  double distance = v3len(d);
+  v3 lightVector = v3norm(d);
+  double intensity = s->aux_light_intensity * 100.0 * light->intensity / (distance * distance) / number;
+  double shade = v3dot(in->normal, lightVector);
+  if (shade < 0) shade = 0;
+  shade = shade * intensity;
+  if (shade > 500.0) shade = 500.0;
+  v3 half = v3norm(v3sub(lightVector, in->viewVector));
+  double shade2 = v3dot(in->normal, half);
+  if (shade2 < 0.0) shade2 = 0.0;
+  shade2 = pow(shade2, 30.0) * 1.0;
+  shade2 *= intensity * s->specular;
+  if (shade2 > 15.0) shade2 = 15.0;
+  if ((shade > 0.01 || shade2 > 0.01) && s->shadow) {
+    double l = aux_shadow(in, distance, lightVector);
+    shade *= l;
+    shade2 *= l;
+  } else if (s->shadow) {
+    shade = 0;
+    shade2 = 0;
+  }
+  shade_t out = {shade * light->colour[0] / 65536.0, shade * light->colour[1] / 65536.0,
+                 shade * light->colour[2] / 65536.0, 0.0};
+  outSpecular->R = shade2 * light->colour[0] / 65536.0;
+  outSpecular->G = shade2 * light->colour[1] / 65536.0;
+  outSpecular->B = shade2 * light->colour[2] / 65536.0;
+  return out;
+}
+
+/* AuxLightsShader (src/shaders.cpp:1051-1073) */
+static shade_t aux_lights_shader(shader_in *in, shade_t *specularOut) {
+  shade_t sum = {0, 0, 0, 0}, spec = {0, 0, 0, 0};
+  const light_set *L = in->lights;
+  if (L && L->any) {
+    for (int i = 0; i < L->count; i++) {
+      if (!light_used(L, i)) continue;
+      shade_t sp;
+      shade_t sh = light_shading(in, &L->tab[i], L->count, &sp);
+      sum.R += sh.R; sum.G += sh.G; sum.B += sh.B;
+      spec.R += sp.R; spec.G += sp.G; spec.B += sp.B;
+    }
+  }
+  *specularOut = spec;
+  return sum;
+}
+
+/* Random (src/common_math.cpp:25-32) as built on Linux */
+static int ref_random(int max) { return rand() % (max + 1); }
+
+static double dmax3(double a, double b, double c) {
+  if (a > b) return a > c ? a : c;
+  return b > c ? b : c;
+}
+
+/* PlaceRandomLights(params, false) (src/shaders.cpp:1134-1212).  The two
+ * draws of one coordinate are taken left to right (the order GCC gives the
+ * reference's expression).  Returns aux_light_number + 4 records, -1 when
+ * `capacity` is short, -2 after 20 million draws without a placement. */
+int oracle_place_lights(const mbr_scene *s, mbr_light *lights, int capacity) {
+  int n = s->aux_light_number > 0 ? s->aux_light_number : 0;
+  if (capacity < n + 4) return -1;
+  memset(lights, 0, sizeof(mbr_light) * (size_t)(n + 4));
+  fractal_ctx f; fractal_ctx_init(&f, s);
+  srand((unsigned)s->aux_light_random_seed);
+  int trial_number = 0;
+  double radius_multiplier = 1.0;
+  long long draws = 0;
+  v3 centre = v3from(s->aux_light_random_center);
+  for (int i = 0; i < n; i++) {
+    trial_number++;
+    if (++draws > 20000000) return -2;
+    double c[3];
+    for (int k = 0; k < 3; k++) {
+      double a = ref_random(2000000) / 1000000.0 - 1.0;
+      double b = ref_random(1000000) / 1.0e12;
+      c[k] = a + b;
+    }
+    v3 position = v3add(centre, v3mul(v3mul(v3from(c), s->aux_light_distribution_radius), radius_multiplier));
+    int mi = 0;
+    double distance = calc_distance(&f, position, 0.0, s->N, s->minN, 0, &mi, NULL, NULL, NULL);
+    if (trial_number > 1000) {
+      radius_multiplier *= 1.1;
+      trial_number = 0;
+    }
+    if (distance > 0 && !mi && distance < s->aux_light_max_dist * radius_multiplier) {
+      radius_multiplier = 1.0;
+      lights[i].position[0] = position.x; lights[i].position[1] = position.y; lights[i].position[2] = position.z;
+      int col[3];
+      col[0] = 20000 + ref_random(80000);
+      col[1] = 20000 + ref_random(80000);
+      col[2] = 20000 + ref_random(80000);
+      double maxColour = dmax3(col[0], col[1], col[2]);
+      for (int k = 0; k < 3; k++) lights[i].colour[k] = (int)(col[k] * 65536.0 / maxColour);
+      lights[i].intensity = distance * distance / s->aux_light_max_dist;
+      lights[i].enabled = 1;
+    } else {
+      i--;
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    if (s->aux_light_pre_enabled[i]) {
+      for (int k = 0; k < 3; k++) lights[i].position[k] = s->aux_light_pre[i][k];
+      lights[i].intensity = s->aux_light_pre_intensity[i];
+      lights[i].colour[0] = s->aux_light_pre_colour[i].R;
+      lights[i].colour[1] = s->aux_light_pre_colour[i].G;
+      lights[i].colour[2] = s->aux_light_pre_colour[i].B;
+      lights[i].enabled = 1;
+    } else {
+      lights[i].enabled = 0;
+    }
+  }
+  return n + 4;
+}
+
+int oracle_scene_has_lights(const mbr_scene *s) {
+  if (s->aux_light_number > 0) return 1;
+  for (int i = 0; i < 4; i++)
+    if (s->aux_light_pre_enabled[i]) return 1;
+  return 0;
+}
+
+static void light_set_init(light_set *L, const mbr_scene *s, const mbr_light *tab) {
+  L->tab = tab;
+  L->number = s->aux_light_number > 0 ? s->aux_light_number : 0;
+  L->count = L->number < 4 ? 4 : L->number;
+  L->any = 0;
+  for (int i = 0; i < L->count; i++)
+    if (light_used(L, i)) L->any = 1;
+}
+
+/* ObjectShader (src/shaders.cpp:21-112) without env mapping and fake
  * lights (rejected scene features; their terms are zero). */
 static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *specularOut) {
   const mbr_scene *s = in->f->s;
@@ -1198,13 +1404,15 @@ static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *spe
   output.R = 0.0 * ambient.R + (ambient2.R + mainLight.R * shade * shadow) * colour.R;
   output.G = 0.0 * ambient.G + (ambient2.G + mainLight.G * shade * shadow) * colour.G;
   output.B = 0.0 * ambient.B + (ambient2.B + mainLight.B * shade * shadow) * colour.B;
-  output.R += (0.0 + 0.0) * colour.R;
-  output.G += (0.0 + 0.0) * colour.G;
-  output.B += (0.0 + 0.0) * colour.B;
+  shade_t auxSpecular;
+  shade_t auxLights = aux_lights_shader(in, &auxSpecular);
+  output.R += (auxLights.R + 0.0) * colour.R;
+  output.G += (auxLights.G + 0.0) * colour.G;
+  output.B += (auxLights.B + 0.0) * colour.B;
   output.alpha = 1.0;
-  specularOut->R = 0.0 + 0.0 + mainLight.R * specular * shadow;
-  specularOut->G = 0.0 + 0.0 + mainLight.G * specular * shadow;
-  specularOut->B = 0.0 + 0.0 + mainLight.B * specular * shadow;
+  specularOut->R = auxSpecular.R + 0.0 + mainLight.R * specular * shadow;
+  specularOut->G = auxSpecular.G + 0.0 + mainLight.G * specular * shadow;
+  specularOut->B = auxSpecular.B + 0.0 + mainLight.B * specular * shadow;
   specularOut->alpha = output.alpha;
   return output;
 }
@@ -1239,8 +1447,17 @@ static shade_t background_shader(const mbr_scene *s, v3 viewVector) {
  * scene features).  *opacity_written is 0 when no step group was emitted
  * (the reference then leaves opacityOut uninitialised; we report 0). */
 static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_rec *buf, int stepCount,
-                                 shade_t oldPixel, double *opacityOut) {
+                                 shade_t oldPixel, const light_set *L, v3 viewVector, const shader_in *pin,
+                                 double *opacityOut) {
   shade_t output = oldPixel;
+  /* the shader's own copy of the pixel's shader input (the reference passes
+   * it by value); point and threshold follow the step records */
+  shader_in in;
+  if (pin) in = *pin;
+  const int volLights = pin && (s->volumetric_light_enabled[0] || s->volumetric_light_enabled[1] ||
+                                s->volumetric_light_enabled[2] || s->volumetric_light_enabled[3] ||
+                                s->volumetric_light_enabled[4]);
+  const int iterFog = pin && s->iter_fog_enabled;
   float totalOpacity = 0.0f;
   double fogVisibility = pow(10.0, s->fog_visibility / 10 - 16.0);
   double colourThresh = s->fog_colour1_distance;
@@ -1260,6 +1477,8 @@ static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_re
     double distance = buf[index].distance;
     totalStep += step;
     double dist_thresh = buf[index].distThresh;
+    in.point = buf[index].point;
+    in.dist_thresh = dist_thresh;
     if (totalStep < dist_thresh * s->quality) continue;
     step = totalStep;
     totalStep = 0.0;
@@ -1269,6 +1488,75 @@ static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_re
     output.G = glowOpacity * glowG + (1.0 - glowOpacity) * output.G;
     output.B = glowOpacity * glowB + (1.0 - glowOpacity) * output.B;
     output.alpha += glowOpacity;
+
+    /* visible aux lights (:241-300); without a light taking part the block
+     * changes nothing and is skipped */
+    if (s->aux_light_visibility > 0 && L && L->any) {
+      v3 point = buf[index].point;
+      double lowestLightSize = 1e10;
+      double lowestLightDist = 1e10;
+      for (int i = 0; i < L->count; ++i) {
+        if (!light_used(L, i)) continue;
+        double lightDist = v3len(v3sub(point, v3from(L->tab[i].position)));
+        double lightSize = L->tab[i].intensity * s->aux_light_intensity * s->aux_light_visibility;
+        double r2 = lightDist / lightSize;
+        if (r2 < lowestLightSize) {
+          lowestLightSize = r2;
+          lowestLightDist = lightDist;
+        }
+      }
+      int smallSteps = x86_d2i(10.0 * step / (lowestLightDist + 1.0e-20));
+      if (smallSteps > 50) smallSteps = 50;
+      int smallStep_start = 0, smallStep_end = 1;
+      double step2 = step;
+      if (smallSteps > 0) {
+        smallStep_start = 1;
+        smallStep_end = smallSteps + 1;
+        step2 = step / (smallSteps + 1.0);
+      }
+      for (int smallStep = smallStep_start; smallStep < smallStep_end; smallStep++) {
+        v3 point2 = v3sub(point, v3mul(v3mul(viewVector, step2), (double)smallStep));
+        for (int i = 0; i < L->count; ++i) {
+          if (!light_used(L, i)) continue;
+          double lightDist = v3len(v3sub(point2, v3from(L->tab[i].position)));
+          double lightSize = L->tab[i].intensity * s->aux_light_intensity * s->aux_light_visibility;
+          double r2 = lightDist / lightSize;
+          if (r2 > 1.0) r2 = 1.0;
+          double bellFunction = (cos(r2 * M_PI) + 1.0) / (r2 * r2 + 0.02) * 0.3;
+          double lightDensity = step2 * bellFunction / lightSize;
+          output.R += lightDensity * L->tab[i].colour[0] / 65536.0;
+          output.G += lightDensity * L->tab[i].colour[1] / 65536.0;
+          output.B += lightDensity * L->tab[i].colour[2] / 65536.0;
+          output.alpha += lightDensity;
+        }
+      }
+    }
+
+    /* volumetric lights with shadows in fog (:314-341): the main light
+     * (slot 0) and aux lights 1-4 (Lights[0..3], when enabled) */
+    if (volLights) {
+      for (int i = 0; i < 5; i++) {
+        if (i == 0 && s->volumetric_light_enabled[0]) {
+          double sh = main_shadow(&in);
+          output.R += sh * step * s->volumetric_light_intensity[0] * s->main_light_colour.R / 65536.0;
+          output.G += sh * step * s->volumetric_light_intensity[0] * s->main_light_colour.G / 65536.0;
+          output.B += sh * step * s->volumetric_light_intensity[0] * s->main_light_colour.B / 65536.0;
+          output.alpha += (sh + sh + sh) / 3.0 * step * s->volumetric_light_intensity[0];
+        }
+        if (i > 0 && L && L->tab[i - 1].enabled && s->volumetric_light_enabled[i]) {
+          const mbr_light *lt = &L->tab[i - 1];
+          v3 d = v3sub(v3from(lt->position), in.point);
+          double dl = v3len(d);
+          double distance2 = dl * dl;
+          v3 lightVectorTemp = v3norm(d);
+          double light = aux_shadow(&in, dl, lightVectorTemp);
+          output.R += light * lt->colour[0] / 65536.0 * s->volumetric_light_intensity[i] * step / distance2;
+          output.G += light * lt->colour[1] / 65536.0 * s->volumetric_light_intensity[i] * step / distance2;
+          output.B += light * lt->colour[2] / 65536.0 * s->volumetric_light_intensity[i] * step / distance2;
+          output.alpha += light * s->volumetric_light_intensity[i] * step / distance2;
+        }
+      }
+    }
 
     if (s->post_fog_enabled) {
       double fogDensity = step / fogVisibility;
@@ -1301,6 +1589,64 @@ static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_re
       output.B = fogDensity * fogBtemp / 65536.0 + (1.0 - fogDensity) * output.B;
       totalOpacity = fogDensity + (1.0 - fogDensity) * totalOpacity;
       output.alpha = fogDensity + (1.0 - fogDensity) * output.alpha;
+    }
+
+    /* iteration fog (:385-465) */
+    if (iterFog) {
+      int Lit = buf[index].iters;
+      double opacity = iter_opacity(step, Lit, s->N, s->iter_fog_opacity_trim, s->iter_fog_opacity);
+      shade_t newColour = {0.0, 0.0, 0.0, 0.0};
+      if (opacity > 0) {
+        double iterFactor = (double)2.0 * (Lit - s->iter_fog_opacity_trim) / (s->N - s->iter_fog_opacity_trim);
+        double k = iterFactor;
+        if (k > 1.0) k = 1.0;
+        if (k < 0.0) k = 0.0;
+        double kn = 1.0 - k;
+        double fogColR = (s->fog_colour1.R * kn + s->fog_colour2.R * k);
+        double fogColG = (s->fog_colour1.G * kn + s->fog_colour2.G * k);
+        double fogColB = (s->fog_colour1.B * kn + s->fog_colour2.B * k);
+        double k2 = iterFactor - 1.0;
+        if (k2 < 0.0) k2 = 0.0;
+        if (k2 > 1.0) k2 = 1.0;
+        kn = 1.0 - k2;
+        fogColR = (fogColR * kn + s->fog_colour3.R * k2);
+        fogColG = (fogColG * kn + s->fog_colour3.G * k2);
+        fogColB = (fogColB * kn + s->fog_colour3.B * k2);
+        for (int i = 0; i < 5; i++) {
+          if (i == 0) {
+            if (s->main_light_intensity * s->direct_light > 0.0) {
+              double sh = main_shadow(&in);
+              newColour.R += sh * s->main_light_colour.R / 65536.0 * s->main_light_intensity * s->direct_light;
+              newColour.G += sh * s->main_light_colour.G / 65536.0 * s->main_light_intensity * s->direct_light;
+              newColour.B += sh * s->main_light_colour.B / 65536.0 * s->main_light_intensity * s->direct_light;
+            }
+          }
+          if (i > 0 && L && L->tab[i - 1].enabled) {
+            const mbr_light *lt = &L->tab[i - 1];
+            v3 d = v3sub(v3from(lt->position), in.point);
+            double dl = v3len(d);
+            double distance2 = dl * dl;
+            v3 lightVectorTemp = v3norm(d);
+            double light = aux_shadow(&in, dl, lightVectorTemp);
+            double intensity = lt->intensity * 100.0;
+            newColour.R += light * lt->colour[0] / 65536.0 / distance2 * intensity;
+            newColour.G += light * lt->colour[1] / 65536.0 / distance2 * intensity;
+            newColour.B += light * lt->colour[2] / 65536.0 / distance2 * intensity;
+          }
+        }
+        if (s->global_illumination && !s->fast_global_illumination) {
+          shade_t AO = ambient_occlusion(&in);
+          newColour.R += AO.R * s->global_ilum;
+          newColour.G += AO.G * s->global_ilum;
+          newColour.B += AO.B * s->global_ilum;
+        }
+        if (opacity > 1.0) opacity = 1.0;
+        output.R = output.R * (1.0 - opacity) + newColour.R * opacity * fogColR / 65536.0;
+        output.G = output.G * (1.0 - opacity) + newColour.G * opacity * fogColG / 65536.0;
+        output.B = output.B * (1.0 - opacity) + newColour.B * opacity * fogColB / 65536.0;
+        totalOpacity = opacity + (1.0 - opacity) * totalOpacity;
+        output.alpha = opacity + (1.0 - opacity) * output.alpha;
+      }
     }
 
     if (totalOpacity > 1.0) totalOpacity = 1.0;
@@ -1410,7 +1756,7 @@ void oracle_volumetric(const mbr_scene *s, double depth, const double *steps, in
     buf[i].distThresh = steps[3 * i + 2];
   }
   shade_t px = {pixel[0], pixel[1], pixel[2], pixel[3]};
-  shade_t o = volumetric_shader(s, depth, buf, n, px, opacity);
+  shade_t o = volumetric_shader(s, depth, buf, n, px, NULL, v3m(0, 0, 0), NULL, opacity);
   out[0] = o.R; out[1] = o.G; out[2] = o.B; out[3] = o.alpha;
   free(buf);
 }
@@ -1447,6 +1793,18 @@ int oracle_object_shader(const mbr_scene *s, const double point[3], const double
   in.depth = depth;
   in.objectType = object;
   in.ao_dirs = dirs; in.ao_rgb = rgb; in.ao_count = count;
+  mbr_light *tab = NULL;
+  light_set L;
+  if (oracle_scene_has_lights(s)) {
+    int n = (s->aux_light_number > 0 ? s->aux_light_number : 0) + 4;
+    tab = (mbr_light *)calloc((size_t)n, sizeof(mbr_light));
+    if (!tab || oracle_place_lights(s, tab, n) != n) {
+      free(tab); free(dirs); free(rgb);
+      return -1;
+    }
+    light_set_init(&L, s, tab);
+    in.lights = &L;
+  }
   shade_t colour, spec;
   shade_t o = object_shader(&in, &colour, &spec);
   out->normal[0] = in.normal.x; out->normal[1] = in.normal.y; out->normal[2] = in.normal.z;
@@ -1471,7 +1829,7 @@ int oracle_object_shader(const mbr_scene *s, const double point[3], const double
   out->colour[0] = colour.R; out->colour[1] = colour.G; out->colour[2] = colour.B;
   out->rgb[0] = o.R; out->rgb[1] = o.G; out->rgb[2] = o.B;
   out->spec_rgb[0] = spec.R; out->spec_rgb[1] = spec.G; out->spec_rgb[2] = spec.B;
-  free(dirs); free(rgb);
+  free(dirs); free(rgb); free(tab);
   return 0;
 }
 
@@ -1484,6 +1842,7 @@ typedef struct {
   mbr_framebuffer *fb;
   int row_begin, row_end;
   int jitter; uint32_t glibc_seed;
+  const light_set *lights;
   volatile int next_row;
   pthread_mutex_t lock;
   oracle_stats st;
@@ -1520,6 +1879,7 @@ static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_st
   in.point = m.point;
   in.viewVector = viewVector;
   in.ao_dirs = job->ao_dirs; in.ao_rgb = job->ao_rgb; in.ao_count = job->ao_count;
+  in.lights = job->lights;
   in.lastDist = m.lastDist;
   in.depth = m.depth;
   in.objectType = m.object;
@@ -1536,7 +1896,7 @@ static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_st
   pixel_sh.B = objectShader.B + backgroundShader.B + specular.B;
   pixel_sh.alpha = objectShader.alpha + backgroundShader.alpha;
   double opacity;
-  shade_t vol = volumetric_shader(s, m.depth, buf, m.buffCount, pixel_sh, &opacity);
+  shade_t vol = volumetric_shader(s, m.depth, buf, m.buffCount, pixel_sh, job->lights, viewVector, &in, &opacity);
 
   size_t idx = (size_t)x + (size_t)y * W;
   mbr_framebuffer *fb = job->fb;
@@ -1588,6 +1948,21 @@ int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin
   job.s = s; job.f = &f; job.c = &c; job.ao_dirs = dirs; job.ao_rgb = rgb; job.ao_count = count;
   job.fb = fb; job.row_begin = row_begin; job.row_end = row_end; job.next_row = row_begin;
   job.jitter = jitter; job.glibc_seed = glibc_seed;
+  /* the lights are placed before the frame, as MainRender does
+   * (src/Render3D.cpp:1990-1993) */
+  mbr_light *tab = NULL;
+  light_set L;
+  memset(&L, 0, sizeof L);
+  if (oracle_scene_has_lights(s)) {
+    int n = (s->aux_light_number > 0 ? s->aux_light_number : 0) + 4;
+    tab = (mbr_light *)calloc((size_t)n, sizeof(mbr_light));
+    if (!tab || oracle_place_lights(s, tab, n) != n) {
+      free(tab); free(dirs); free(rgb);
+      return MBR_ERR_INVALID_ARG;
+    }
+    light_set_init(&L, s, tab);
+    job.lights = &L;
+  }
   pthread_mutex_init(&job.lock, NULL);
   if (jitter == ORACLE_JITTER_GLIBC) { nthreads = 1; srand(glibc_seed); }
   if (nthreads <= 0) { long n = sysconf(_SC_NPROCESSORS_ONLN); nthreads = n > 0 ? (int)n : 1; }
@@ -1601,7 +1976,7 @@ int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin
   }
   pthread_mutex_destroy(&job.lock);
   if (st) *st = job.st;
-  free(dirs); free(rgb);
+  free(dirs); free(rgb); free(tab);
   return MBR_OK;
 }
 

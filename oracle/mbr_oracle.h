@@ -97,6 +97,13 @@ int oracle_object_shader(const mbr_scene *s, const double point[3], const double
 
 /* Render rows [row_begin,row_end) of the scene into host planes (planes
  * sized for the whole image).  nthreads <= 0 uses the host core count. */
+/* PlaceRandomLights(params, false) (src/shaders.cpp:1134-1212) with the C
+ * library's srand/rand: aux_light_number + 4 records; -1 when capacity is
+ * short, -2 after 20 million draws without a placement.  Not thread safe
+ * (rand). */
+int oracle_place_lights(const mbr_scene *s, mbr_light *lights, int capacity);
+int oracle_scene_has_lights(const mbr_scene *s);
+
 int oracle_render(const mbr_scene *s, const mbr_texture *lightmap, int row_begin, int row_end,
                   int nthreads, int jitter, uint32_t glibc_seed, mbr_framebuffer *fb,
                   oracle_stats *st);

@@ -65,6 +65,8 @@ def _load(name):
     lib.oracle_dof_order.restype = C.c_int
     lib.oracle_dof.argtypes = [S, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
     lib.oracle_dof.restype = C.c_int
+    lib.oracle_place_lights.argtypes = [S, P(_abi.Light), C.c_int]
+    lib.oracle_place_lights.restype = C.c_int
     return lib
 
 
@@ -247,3 +249,14 @@ def dof(scene, z, image16, alpha, seed):
     if rc != 0:
         raise RuntimeError(f"oracle_dof failed: {rc}")
     return img, a
+
+
+def place_lights(scene):
+    """PlaceRandomLights of the oracle: aux_light_number + 4 light records
+    (list of (position, intensity, colour, enabled))."""
+    n = max(scene.aux_light_number, 0) + 4
+    tab = (_abi.Light * n)()
+    rc = lib().oracle_place_lights(C.byref(scene), tab, n)
+    if rc != n:
+        raise RuntimeError(f"oracle_place_lights failed: {rc}")
+    return [(tuple(l.position), l.intensity, tuple(l.colour), l.enabled) for l in tab]
