@@ -377,8 +377,8 @@ namespace {
 void free_band(Slot &sl) {
   BandBufs &b = sl.band;
   void *ptrs[] = {b.found, b.object, b.count, b.depth, b.thr, b.last, b.px, b.py, b.pz, b.rec_step, b.rec_dist,
-                  b.rec_thr, b.ovf_off, b.hit_list, b.redo_list, b.counters, b.ovf_dir, b.ovf_step, b.ovf_dist,
-                  b.ovf_thr,
+                  b.rec_thr, b.rec_it, b.ovf_it, b.ovf_off, b.hit_list, b.redo_list, b.counters, b.ovf_dir,
+                  b.ovf_step, b.ovf_dist, b.ovf_thr,
                   b.hand_ray, b.hand_i, b.hand_scan, b.hand_step, b.hand_drift, b.hand_dist,
                   b.sh_r, b.sh_g, b.sh_b, b.col_r, b.col_g, b.col_b};
   for (void *p : ptrs)
@@ -396,17 +396,19 @@ void free_band(Slot &sl) {
 // previous one).
 int alloc_pool(mbr_ctx *c, Slot &sl, long long dirs, long long chunks) {
   BandBufs &b = sl.band;
-  void *old[] = {b.ovf_dir, b.ovf_step, b.ovf_dist, b.ovf_thr};
+  void *old[] = {b.ovf_dir, b.ovf_step, b.ovf_dist, b.ovf_thr, b.ovf_it};
   for (void *p : old)
     if (p) (void)hipFree(p);
   b.ovf_dir = nullptr;
   b.ovf_step = b.ovf_dist = b.ovf_thr = nullptr;
+  b.ovf_it = nullptr;
   b.ovf_dirs = b.ovf_chunks = 0;
   const size_t bytes = (size_t)chunks * kChunk * b.rec_bytes;
   HIPCHECK(c, hipMalloc((void **)&b.ovf_dir, (size_t)dirs * kDirSlots * sizeof(int)));
   HIPCHECK(c, hipMalloc(&b.ovf_step, bytes));
   HIPCHECK(c, hipMalloc(&b.ovf_dist, bytes));
   HIPCHECK(c, hipMalloc(&b.ovf_thr, bytes));
+  if (b.rec_it) HIPCHECK(c, hipMalloc((void **)&b.ovf_it, (size_t)chunks * kChunk * sizeof(int)));
   b.ovf_dirs = dirs;
   b.ovf_chunks = chunks;
   return MBR_OK;
@@ -426,7 +428,7 @@ constexpr int kPoolAttempts = 8;  // renders of a frame while its overflow pool 
 
 // Band buffers for `cap` rays and AO intensities for `ao_count` directions
 // per hit; kept while precision and cap stay the same.
-int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count) {
+int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count, bool iters = false) {
   if (sl.prec != precision || sl.band.cap != cap) {
     free_band(sl);
     BandBufs &b = sl.band;
@@ -482,6 +484,12 @@ int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count) {
     HIPCHECK(c, hipMalloc(&sl.sh_ao, need));
     sl.sh_ao_bytes = need;
   }
+  // iteration counts of the step records (iteration fog), kept once made
+  BandBufs &b = sl.band;
+  if (iters && !b.rec_it) {
+    HIPCHECK(c, hipMalloc((void **)&b.rec_it, (size_t)b.cap * (size_t)b.slab * sizeof(int)));
+    HIPCHECK(c, hipMalloc((void **)&b.ovf_it, (size_t)b.ovf_chunks * kChunk * sizeof(int)));
+  }
   return MBR_OK;
 }
 
@@ -517,7 +525,6 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.raytraced_reflections && s.reflections_max > 0)
     return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections are not in the render core");
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
-  if (s.iter_fog_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "iteration fog is not in the render core");
   if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
   if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
   if (s.image_width <= 0 || s.image_height <= 0 || s.N <= 0 || s.N > 100000)
@@ -682,6 +689,10 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   // the shadow marches; the aux lights' volumetric switches are resolved
   // against the light table in setup_lights
   d.vol_main = s.volumetric_light_enabled[0];
+  d.iter_fog = s.iter_fog_enabled;
+  d.iter_fog_opacity = s.iter_fog_opacity;
+  d.iter_fog_trim = s.iter_fog_opacity_trim;
+  d.main_light_intensity = s.main_light_intensity;
   d.shadow_fx = s.iter_fog_enabled;
   for (int i = 0; i < 5; i++) {
     if (s.volumetric_light_enabled[i]) d.shadow_fx = 1;
@@ -841,6 +852,7 @@ int setup_lights(mbr_ctx *c, const mbr_scene &s, DevScene &d) {
   for (int k = 0; k < 4; k++) {
     const mbr_light &L = c->lights[(size_t)k];
     d.vol_aux_on[k] = s.volumetric_light_enabled[k + 1] && L.enabled;
+    d.aux_slot_on[k] = L.enabled;
     const double v[8] = {L.position[0], L.position[1], L.position[2], L.intensity,
                          (double)L.colour[0], (double)L.colour[1], (double)L.colour[2], 0.0};
     std::memcpy(d.vol_aux[k], v, sizeof v);
@@ -995,8 +1007,9 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, c->d_stats, st,
                              e ? e[4] : nullptr, e ? e[5] : nullptr));
     if (e) HIPCHECK(c, hipEventRecord(e[2], st));
-    HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula, host.vol_any,
-                              host.n_lights > 0 && host.sd.aux_visibility > 0, persist, c->d_stats));
+    HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula,
+                              host.vol_any || host.iter_fog, host.n_lights > 0 && host.sd.aux_visibility > 0,
+                              persist, c->d_stats, ao));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
   return MBR_OK;
@@ -1064,7 +1077,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   const bool slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
   const int ao_count = slow_ao ? c->host_sc.ao_count : 0;
   Slot &sl = c->slots[0];
-  rc = alloc_slot(c, sl, precision, band_cap(precision, ao_count), ao_count);
+  rc = alloc_slot(c, sl, precision, band_cap(precision, ao_count), ao_count, c->host_sc.iter_fog != 0);
   if (rc != MBR_OK) return rc;
   const FrameOut out = frame_out(fb);
   const AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
@@ -1471,8 +1484,8 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
     if (rc != MBR_OK) return c->fail(rc, "scene " + std::to_string(i) + ": " + c->err);
     if (scene_has_lights(frames[i].p.s) || frames[i].p.d.shadow_fx)
       return c->fail(MBR_ERR_UNSUPPORTED, "scene " + std::to_string(i) +
-                                              ": aux lights and volumetric lights render through mbr_set_scene, "
-                                              "not in a batch");
+                                              ": aux lights, volumetric lights and iteration fog render through "
+                                              "mbr_set_scene, not in a batch");
     const mbr_scene &s = frames[i].p.s;
     if (fbs[i].width != s.image_width || fbs[i].height != s.image_height)
       return c->fail(MBR_ERR_INVALID_ARG, "framebuffer " + std::to_string(i) + " differs from its scene's size");

@@ -33,17 +33,19 @@ template <class RT>
 struct RecPlanes {
   RT *step, *dist, *thr;     // slab planes
   RT *ostep, *odist, *othr;  // overflow pool planes
+  int *it, *oit;             // iteration counts (null unless iteration fog is on)
   long long *ovf_off;
   int *dir;
   unsigned long long *dirs, *chunks, *stats;
   long long cap, max_dirs, max_chunks;
   int slab, r;
-  __device__ __forceinline__ void put(int i, double step_v, double dist_v, double thr_v) const {
+  __device__ __forceinline__ void put(int i, double step_v, double dist_v, double thr_v, int iters) const {
     long long o;
     RT *st, *di, *th;
+    int *itp;
     if (i < slab) {
       o = (long long)i * cap + r;
-      st = step; di = dist; th = thr;
+      st = step; di = dist; th = thr; itp = it;
     } else {
       long long dj = ovf_off[r];
       if (dj == -1) {
@@ -72,11 +74,12 @@ struct RecPlanes {
         return;
       }
       o = (long long)ch * kChunk + rel % kChunk;
-      st = ostep; di = odist; th = othr;
+      st = ostep; di = odist; th = othr; itp = oit;
     }
     st[o] = (RT)step_v;
     di[o] = (RT)dist_v;
     th[o] = (RT)thr_v;
+    if (itp) itp[o] = iters;
   }
 };
 
@@ -151,11 +154,13 @@ __device__ __forceinline__ void march_before_de(const DevScene &sc, V3<double> s
   if (sc.persp_type == 1 || sc.persp_type == 2) s.thr *= kPi;
 }
 
+// obj_it: the estimate's object type in the low 3 bits, its iteration count
+// (CalculateDistance's itersOut) above them.
 template <class T, class RT>
-__device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, double d, int object, uint32_t pixel,
+__device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, double d, int obj_it, uint32_t pixel,
                                               const RecPlanes<RT> &rec) {
   const ShadeK<double> &k = sc.sd;
-  s.object = object;
+  s.object = obj_it & 7;
   const double dist = d > 3.0 ? 3.0 : d;
   double err = 0.0;
   if constexpr (sizeof(T) < 8) {
@@ -169,7 +174,7 @@ __device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, d
   const double jf = sc.jitter_mode ? 1.0 - jitter_draw(sc.jitter_seed, sc.frame_no, pixel, (uint32_t)s.i) / 10000.0 : 1.0;
   const double old_step = s.step;
   s.step = (dist - (sc.interior ? 0.8 : 0.5) * s.thr) * k.DE_factor * jf;
-  rec.put(s.i, old_step, d, s.thr);
+  rec.put(s.i, old_step, d, s.thr, obj_it >> 3);
   s.scan += s.step;
   s.drift += (err * k.DE_factor) * (err * k.DE_factor);
   s.dist = dist;
@@ -214,7 +219,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
     const double jf = sc.jitter_mode ? 1.0 - jitter_draw(sc.jitter_seed, sc.frame_no, pixel, (uint32_t)s.i) / 10000.0 : 1.0;
     const double old_step = s.step;
     s.step = (dist - (sc.interior ? 0.8 : 0.5) * s.thr) * k.DE_factor * jf;
-    rec.put(s.i, old_step, dist_raw, s.thr);
+    rec.put(s.i, old_step, dist_raw, s.thr, info.iters);
     prev_dist = dist;
     prev_scan = s.scan;
     have_prev = true;
@@ -325,6 +330,8 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
   rec.ostep = (RT *)b.ovf_step;
   rec.odist = (RT *)b.ovf_dist;
   rec.othr = (RT *)b.ovf_thr;
+  rec.it = b.rec_it;
+  rec.oit = b.ovf_it;
   rec.ovf_off = b.ovf_off;
   rec.dir = b.ovf_dir;
   rec.dirs = &b.counters[CNT_DIRS];
@@ -372,7 +379,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
     state = MS_ITER;
     if (de_call_begin<T, F, Plain, kAlg>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
       state = MS_DONE;
-      object = info.object;
+      object = info.object | (info.iters << 3);
       cnt.de_calls += 1;
     }
   };
@@ -417,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
       DEInfo info;
       if (de_call_step<T, F, Plain, kAlg>(f, call, (T)s.thr, N, minN, false, dist, info)) {
         state = MS_DONE;
-        object = info.object;
+        object = info.object | (info.iters << 3);
         cnt.de_calls += 1;
         cnt.iterations += (uint32_t)info.work;
       }
@@ -921,7 +928,7 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
 // (distance estimates of formula F).
 template <class T, class RT, int F, bool FX, bool VIS>
 __device__ __forceinline__ void finish_body(const DevScene &sc, const BandBufs &b, int band_first, int r,
-                                            const FrameOut &out, LaneCount &cnt) {
+                                            const FrameOut &out, LaneCount &cnt, const AODev &ao) {
   const int p = band_first + r;
   const bool found = b.found[r];
   RGBT<T> px;
@@ -963,11 +970,89 @@ __device__ __forceinline__ void finish_body(const DevScene &sc, const BandBufs &
     load_hit(sc, b, r, pin);
     pin.view = cvt<T>(vray.dir);
   }
-  auto fx = [&](V3<double> p64, T thr, T step, VolOut<T> &o) {
+  // iteration fog of a step group (src/shaders.cpp:385-465), at pin's point:
+  // the record's iteration count sets the opacity and the fog colour, the
+  // group is lit by the main light and the enabled aux slots through their
+  // shadows, plus AO
+  auto iter_fog_stage = [&](int index, T step, VolOut<T> &o, float &totalOpacity) {
+    if constexpr (FX) {
+      int L = 0;
+      if (index < b.slab) {
+        L = b.rec_it[(long long)index * b.cap + r];
+      } else {
+        const long long dj = b.ovf_off[r];
+        const int ch = dj >= 0 ? b.ovf_dir[dj * kDirSlots + (index - b.slab) / kChunk] : -1;
+        if (ch >= 0) L = b.ovf_it[(long long)ch * kChunk + (index - b.slab) % kChunk];
+      }
+      T opacity = iter_opacity<T>(step, T(L), T(sc.N), (T)sc.iter_fog_trim, (T)sc.iter_fog_opacity);
+      if (!(opacity > T(0))) return;
+      const T trim = (T)sc.iter_fog_trim;
+      const T iterFactor = T(2) * (T(L) - trim) / (T(sc.N) - trim);
+      T kk = iterFactor;
+      if (kk > T(1)) kk = 1;
+      if (kk < T(0)) kk = 0;
+      T kn = T(1) - kk;
+      T fR = T(sc.fog1[0]) * kn + T(sc.fog2[0]) * kk;
+      T fG = T(sc.fog1[1]) * kn + T(sc.fog2[1]) * kk;
+      T fB = T(sc.fog1[2]) * kn + T(sc.fog2[2]) * kk;
+      T k2 = iterFactor - T(1);
+      if (k2 < T(0)) k2 = 0;
+      if (k2 > T(1)) k2 = 1;
+      kn = T(1) - k2;
+      fR = fR * kn + T(sc.fog3[0]) * k2;
+      fG = fG * kn + T(sc.fog3[1]) * k2;
+      fB = fB * kn + T(sc.fog3[2]) * k2;
+      const ShadeK<T> &k = sc.shade<T>();
+      RGBT<T> nc = {0, 0, 0};
+      const T mi = (T)sc.main_light_intensity, dl = k.direct_light;
+      if (mi * dl > T(0)) {
+        const T sh = main_shadow<T, F, false>(sc, pin, cnt);
+        nc.R += sh * T(sc.main_light_colour[0]) / T(65536) * mi * dl;
+        nc.G += sh * T(sc.main_light_colour[1]) / T(65536) * mi * dl;
+        nc.B += sh * T(sc.main_light_colour[2]) / T(65536) * mi * dl;
+      }
+      for (int s4 = 0; s4 < 4; s4++) {
+        if (!sc.aux_slot_on[s4]) continue;
+        const double *Lt = sc.vol_aux[s4];
+        const V3<T> d = v3<T>((T)Lt[0], (T)Lt[1], (T)Lt[2]) - pin.point;
+        const T dist = length(d);
+        const T d2 = dist * dist;
+        const V3<T> lv = normalize(d);
+        const T light = aux_shadow<T, F, false>(sc, pin, dist, lv, cnt);
+        const T intensity = (T)Lt[3] * T(100);
+        nc.R += light * T(Lt[4]) / T(65536) / d2 * intensity;
+        nc.G += light * T(Lt[5]) / T(65536) / d2 * intensity;
+        nc.B += light * T(Lt[6]) / T(65536) / d2 * intensity;
+      }
+      if (sc.ao && !sc.fast_ao) {
+        AOTable<T> tab;
+        if constexpr (sizeof(T) == 8) tab.dir = ao.dir_d; else tab.dir = ao.dir_f;
+        tab.rgb = ao.rgb;
+        tab.count = ao.count;
+        const RGBT<T> a = ambient_occlusion<T, F, false>(sc, pin, tab, cnt);
+        nc.R += a.R * k.global_ilum;
+        nc.G += a.G * k.global_ilum;
+        nc.B += a.B * k.global_ilum;
+      }
+      if (opacity > T(1)) opacity = 1;
+      o.rgb.R = o.rgb.R * (T(1) - opacity) + nc.R * opacity * fR / T(65536);
+      o.rgb.G = o.rgb.G * (T(1) - opacity) + nc.G * opacity * fG / T(65536);
+      o.rgb.B = o.rgb.B * (T(1) - opacity) + nc.B * opacity * fB / T(65536);
+      totalOpacity = (float)(opacity + (T(1) - opacity) * T(totalOpacity));
+      o.alpha = opacity + (T(1) - opacity) * o.alpha;
+    } else {
+      (void)index; (void)step; (void)o; (void)totalOpacity;
+    }
+  };
+  auto fx = [&](int stage, int index, V3<double> p64, T thr, T step, VolOut<T> &o, float &totalOpacity) {
     if constexpr (FX) {
       pin.point64 = p64;
       pin.point = cvt<T>(p64);
       pin.dist_thresh = thr;
+      if (stage == 1) {
+        if (sc.iter_fog) iter_fog_stage(index, step, o, totalOpacity);
+        return;
+      }
       if (sc.vol_main) {
         const T sh = main_shadow<T, F, false>(sc, pin, cnt);
         const T I0 = (T)sc.vol_intensity[0];
@@ -991,7 +1076,7 @@ __device__ __forceinline__ void finish_body(const DevScene &sc, const BandBufs &
         o.alpha += light * Ik * step / d2;
       }
     } else {
-      (void)p64; (void)thr; (void)step; (void)o;
+      (void)stage; (void)index; (void)p64; (void)thr; (void)step; (void)o; (void)totalOpacity;
     }
   };
   if (n <= slab) {
@@ -1052,17 +1137,18 @@ __global__ void __launch_bounds__(kBlock) k_finish(const DevScene *__restrict__ 
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= band_rays) return;
   LaneCount cnt = {0, 0};
-  finish_body<T, RT, 0, false, VIS>(*scp, b, band_first, r, out, cnt);
+  finish_body<T, RT, 0, false, VIS>(*scp, b, band_first, r, out, cnt, AODev{});
 }
 
 // With volumetric lights: a grid-stride loop (the shadow marches make the
 // per-pixel cost vary widely), distance-estimate counts into the stats.
 template <class T, class RT, int F>
 __global__ void __launch_bounds__(kBlock) k_finish_fx(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                      int band_rays, FrameOut out, unsigned long long *stats) {
+                                                      int band_rays, FrameOut out, unsigned long long *stats,
+                                                      AODev ao) {
   LaneCount cnt = {0, 0};
   for (int r = blockIdx.x * kBlock + threadIdx.x; r < band_rays; r += gridDim.x * kBlock)
-    finish_body<T, RT, F, true, true>(*scp, b, band_first, r, out, cnt);
+    finish_body<T, RT, F, true, true>(*scp, b, band_first, r, out, cnt, ao);
   unsigned long long v[ST_COUNT] = {0};
   v[ST_SHADE_DE] = cnt.de_calls;
   v[ST_SHADE_ITERS] = cnt.iterations;
@@ -1259,14 +1345,14 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
 
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s, int formula, int fx, int vis, int persist_blocks,
-                         unsigned long long *stats) {
+                         unsigned long long *stats, AODev ao) {
   if (band_rays <= 0) return hipSuccess;
   if (fx) {
     if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)std::max(1, std::min(persist_blocks, (band_rays + kBlock - 1) / kBlock)));
     if (formula_generic(formula)) {
       hipLaunchKernelGGL((k_finish_fx<double, double, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays,
-                         out, stats);
+                         out, stats, ao);
       return hipGetLastError();
     }
     auto run = [&](auto tag, auto rtag) {
@@ -1274,13 +1360,13 @@ hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, i
       using RT = decltype(rtag);
       if (formula == 7)
         hipLaunchKernelGGL((k_finish_fx<T, RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
-                           stats);
+                           stats, ao);
       else if (formula == 8)
         hipLaunchKernelGGL((k_finish_fx<T, RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
-                           stats);
+                           stats, ao);
       else
         hipLaunchKernelGGL((k_finish_fx<T, RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, out,
-                           stats);
+                           stats, ao);
     };
     if (precision == 0) run(double{}, double{});
     else run(float{}, float{});

@@ -54,6 +54,7 @@ struct BandBufs {
   int *count;       // step record length (buffCount)
   double *depth, *thr, *last, *px, *py, *pz;
   void *rec_step, *rec_dist, *rec_thr;   // slab planes [slab][cap]
+  int *rec_it;                           // iteration counts [slab][cap] (iteration fog only, else null)
   long long *ovf_off;                    // per ray: its directory, -1 none, -2 pool full
   // compaction lists: hits, rays to re-march in FP64
   int *hit_list, *redo_list;
@@ -65,6 +66,7 @@ struct BandBufs {
   // ovf_chunks chunks of kChunk records
   int *ovf_dir;
   void *ovf_step, *ovf_dist, *ovf_thr;
+  int *ovf_it;                           // with rec_it
   long long ovf_dirs, ovf_chunks;
   // shade output per ray
   void *sh_r, *sh_g, *sh_b;
@@ -124,12 +126,12 @@ hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first,
                           const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
                           int blocks, hipStream_t s);
 // Background, VolumetricShader and framebuffer planes.  fx: the scene has
-// volumetric lights, whose shadow marches run in a grid-stride kernel of
-// persist_blocks workgroups templated on the formula; vis: visible aux
-// lights glow along the rays.
+// volumetric lights or iteration fog, whose shadow marches (and AO rays)
+// run in a grid-stride kernel of persist_blocks workgroups templated on the
+// formula; vis: visible aux lights glow along the rays.
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s, int formula, int fx, int vis, int persist_blocks,
-                         unsigned long long *stats);
+                         unsigned long long *stats, AODev ao);
 
 // Screen-space ambient occlusion of one compiled frame (mbr_post.hip).
 // cs/sn: device tables of cos/sin(2 pi i / quality), i < quality.

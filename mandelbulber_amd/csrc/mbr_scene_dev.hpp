@@ -101,6 +101,13 @@ struct DevScene {
   int vol_aux_on[4];
   double vol_aux[4][8];
   double vol_intensity[5];
+  // iteration fog (IterOpacity, src/shaders.cpp:1214-1222, and its
+  // VolumetricShader block :385-465); aux_slot_on[k]: aux light slot k is
+  // enabled (the fog lights every enabled slot 0-3)
+  int iter_fog;
+  int aux_slot_on[4];
+  double iter_fog_opacity, iter_fog_trim;
+  double main_light_intensity;  // mainLightIntensity (the fog's light)
 
   // integer colours (sRGB, 16-bit channel values)
   int main_light_colour[3];

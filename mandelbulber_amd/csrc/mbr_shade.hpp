@@ -13,8 +13,10 @@
 //   aux_lights<T>         AuxLightsShader        src/shaders.cpp:1051-1073
 //                         + LightShading         src/shaders.cpp:997-1049
 //
-// Features outside the render core (env mapping, fake/volumetric lights,
-// iteration fog) are rejected at mbr_set_scene, so their terms are zero here.
+// The volumetric lights' per-step shadow marches run in k_finish_fx through
+// the `fx` hook of volumetric<T>.  Features outside the render core (env
+// mapping, fake lights, iteration fog) are rejected at mbr_set_scene, so
+// their terms are zero here.
 #pragma once
 
 #include "mbr_fractal.hpp"
@@ -122,6 +124,25 @@ MBR_HD float fx_thresh(const DevScene &sc, V3<T> point2) {
   return (float)k.quality;
 }
 
+// IterOpacity (src/shaders.cpp:1214-1222)
+template <class T>
+MBR_HD T iter_opacity(T step, T iters, T maxN, T trim, T opacitySp) {
+  T opacity = (iters - trim) / maxN;
+  if (opacity < T(0)) opacity = 0;
+  opacity *= opacity;
+  opacity *= step * opacitySp;
+  if (opacity > T(1)) opacity = 1;
+  return opacity;
+}
+
+// the fog opacity a shadow march collects at an estimate (0 without
+// iteration fog)
+template <class T>
+MBR_HD T fog_opacity(const DevScene &sc, T dist_step, int iters) {
+  if (!sc.iter_fog) return T(0);
+  return iter_opacity<T>(dist_step, T(iters), T(sc.N), (T)sc.iter_fog_trim, (T)sc.iter_fog_opacity);
+}
+
 template <class T, int F = 0, bool Deep = false>
 MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
@@ -150,6 +171,7 @@ MBR_HD T main_shadow(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
       if (softShadow < 0) softShadow = 0;
       if (softShadow > maxSoft) maxSoft = softShadow;
     }
+    if (sc.iter_fog) shadowTemp -= fog_opacity<T>(sc, dist * DE_factor, info.iters) * (factor - i) / factor;
     if (dist < (T)dist_thresh || shadowTemp < T(0)) {
       shadowTemp -= (factor - i) / factor;
       if (!sc.penetrating) shadowTemp = 0;
@@ -218,8 +240,8 @@ MBR_HD RGBT<T> ambient_occlusion(const DevScene &sc, ShaderIn<T> &in, const AOTa
   return ao;
 }
 
-// AuxShadow without iteration fog (its opacity term is 0): march from the
-// hit towards the light until the light's distance.
+// AuxShadow: march from the point towards the light until the light's
+// distance.
 template <class T, int F = 0, bool Deep = false>
 MBR_HD T aux_shadow(const DevScene &sc, ShaderIn<T> &in, T distance, V3<T> lightVector, LaneCount &cnt) {
   const ShadeK<T> &k = sc.shade<T>();
@@ -230,6 +252,7 @@ MBR_HD T aux_shadow(const DevScene &sc, ShaderIn<T> &in, T distance, V3<T> light
   int guard = 0;
   for (T i = in.delta; i < distance && guard < kMaxShadeSteps; i += dist * DE_factor, guard++) {
     dist = de_at<T, F, Deep>(sc, in, lightVector * i, in.detail, sc.N, sc.minN, false, info, cnt);
+    if (sc.iter_fog) shadowTemp -= fog_opacity<T>(sc, dist * DE_factor, info.iters) * (distance - i) / distance;
     const float dist_thresh = sc.shadow_fx ? fx_thresh<T>(sc, in.point + lightVector * i) : (float)in.dist_thresh;
     if (dist < (T)dist_thresh || info.max_iter || shadowTemp < T(0)) {
       if (sc.penetrating) {
@@ -465,12 +488,14 @@ MBR_HD void visible_lights(const DevScene &sc, V3<double> point, V3<double> view
 
 struct NoFx {
   template <class T>
-  MBR_HD void operator()(V3<double>, T, T, VolOut<T> &) const {}
+  MBR_HD void operator()(int, int, V3<double>, T, T, VolOut<T> &, float &) const {}
 };
 
 // VolumetricShader over the step records.  With a ray (vr), a step group's
 // point is known: the glow of visible aux lights is added when `vis`, and
-// `fx(point, threshold, step, out)` adds the volumetric lights.
+// `fx(stage, index, point, threshold, step, out, totalOpacity)` adds the
+// volumetric lights (stage 0, before the fogs) and the iteration fog
+// (stage 1, after them).
 template <class T, class RecFn, class FxFn = NoFx>
 MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> px, T alpha, RecFn rec,
                             const VisRay *vr = nullptr, bool vis = false, FxFn fx = NoFx{}) {
@@ -504,10 +529,11 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
     o.rgb.B = glowOpacity * glowB + (T(1) - glowOpacity) * o.rgb.B;
     o.alpha += glowOpacity;
 
+    V3<double> point = v3<double>(0, 0, 0);
     if (vr) {
-      const V3<double> point = vr->start + vr->dir * scan_i;
+      point = vr->start + vr->dir * scan_i;
       if (vis) visible_lights<T>(sc, point, vr->dir, (double)step, o);
-      fx(point, dist_thresh, step, o);
+      fx(0, index, point, dist_thresh, step, o, totalOpacity);
     }
 
     if (sc.post_fog) {
@@ -543,6 +569,8 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
       totalOpacity = (float)(fogDensity + (T(1) - fogDensity) * T(totalOpacity));
       o.alpha = fogDensity + (T(1) - fogDensity) * o.alpha;
     }
+
+    if (vr) fx(1, index, point, dist_thresh, step, o, totalOpacity);
 
     if (totalOpacity > 1.0f) totalOpacity = 1.0f;
     if (o.alpha > T(1)) o.alpha = 1;

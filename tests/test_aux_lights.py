@@ -245,3 +245,51 @@ def test_volumetric_main_light_is_shadowed_by_the_bulb():
     assert vals.max() > 1e-3
     # the gathered light varies over the misses: some pass through the shadow
     assert vals.min() < 0.9 * vals.max()
+
+
+def _fog_scene(w=32, h=24):
+    sc = scene.mandelbulb_plain(w, h)
+    sc.N = 20
+    sc.view_distance_max = 8.0
+    sc.aux_light_number = 0
+    return sc
+
+
+def test_iteration_fog_at_zero_opacity_changes_only_the_shadows():
+    """Iteration fog switches the shadow marches to unit DE steps and the
+    eye-distance threshold, as a volumetric light does; at opacity 0 its
+    own terms vanish, so the frame equals that of a volumetric light of
+    intensity 0."""
+    sc = _fog_scene()
+    sc.shadow = 1
+    sc.iter_fog_enabled = 1
+    sc.iter_fog_opacity = 0.0
+    fog0, _ = oracle.render(sc)
+    sc.iter_fog_enabled = 0
+    sc.volumetric_light_enabled[0] = 1
+    sc.volumetric_light_intensity[0] = 0.0
+    vol0, _ = oracle.render(sc)
+    assert np.array_equal(fog0.rgb, vol0.rgb) and np.array_equal(fog0.z, vol0.z)
+
+
+def test_iteration_fog_takes_its_colour():
+    """Fog colours all pure red: where the fog is thick the green and blue
+    channels only lose (output x (1 - opacity)), red gains."""
+    sc = _fog_scene()
+    sc.shadow = 0
+    # the bulb's orbits escape within a few iterations: no trim
+    sc.iter_fog_opacity = 1000.0
+    sc.iter_fog_opacity_trim = 0.0
+    for c in (sc.fog_colour1, sc.fog_colour2, sc.fog_colour3):
+        c.R, c.G, c.B = 65536, 0, 0
+    sc.volumetric_light_enabled[0] = 1  # the same shadow-march rules without fog
+    sc.volumetric_light_intensity[0] = 0.0
+    clear, _ = oracle.render(sc)
+    sc.volumetric_light_enabled[0] = 0
+    sc.iter_fog_enabled = 1
+    fog, _ = oracle.render(sc)
+    d = fog.rgb.astype(np.float64) - clear.rgb.astype(np.float64)
+    assert np.abs(d).max() > 1e-3
+    assert np.all(d[1] <= 1e-7) and np.all(d[2] <= 1e-7)
+    assert d[0].max() > 1e-3
+    assert np.all(fog.alpha >= clear.alpha)

@@ -1,8 +1,9 @@
 """Auxiliary lights on the GPU against the oracle: predefined and random
 lights with their shadows (LightShading / AuxShadow, src/shaders.cpp:
-997-1132) on the shaded Mandelbulb, and their glow along the rays
-(VolumetricShader, :241-300), in FP64 and FP32_FIXUP; the context's light
-table (placement, explicit tables, animation semantics) through the C ABI."""
+997-1132) on the shaded Mandelbulb, their glow along the rays
+(VolumetricShader, :241-300), volumetric lights (:314-341) and iteration
+fog (:385-465), in FP64 and FP32_FIXUP; the context's light table
+(placement, explicit tables, animation semantics) through the C ABI."""
 import numpy as np
 import pytest
 
@@ -103,6 +104,51 @@ def test_volumetric_lights_match_oracle(renderer, kind):
         assert_parity(m, label=f"volumetric {kind} {name}")
         if prec == _abi.PREC_FP64:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+
+
+def _fog_scene(kind):
+    if kind == "box":
+        s = scene.mandelbox_view(96, 72, shaded=True)
+        s.N = 18
+        s.iter_fog_opacity = 200.0
+        s.iter_fog_opacity_trim = 10.0
+        s.global_illumination = 0
+    else:
+        s = scene.mandelbulb_full(96 if kind == "bulb" else 32, 72 if kind == "bulb" else 24, seed=1)
+        s.N = 20
+        s.view_distance_max = 8.0
+        s.iter_fog_opacity = 300.0
+        s.iter_fog_opacity_trim = 0.0
+        s.global_illumination = 1 if kind == "bulb_ao" else 0
+        s.ao_quality = 2
+        s.aux_light_pre_enabled[1] = 1
+        s.aux_light_intensity = 0.2
+    s.fog_density = 0.0
+    s.iter_fog_enabled = 1
+    return s
+
+
+@pytest.mark.parametrize("kind", ["bulb", "bulb_ao", "box"])
+def test_iteration_fog_matches_oracle(renderer, kind):
+    from oracle import oracle
+    sc = _fog_scene(kind)
+    ref, _ = oracle.render(sc)
+    renderer.set_scene(sc)
+    for prec, name in [(_abi.PREC_FP64, "fp64"), (_abi.PREC_FP32_FIXUP, "fp32")]:
+        print(f"[iteration fog] {kind} {name}", flush=True)
+        f, st = renderer.render(precision=prec)
+        m = compare(f, ref)
+        print(f"  {m}", flush=True)
+        assert m["hits"] > 100
+        assert_parity(m, label=f"iteration fog {kind} {name}")
+        if prec == _abi.PREC_FP64:
+            assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+    # the fog shows
+    clear = scene.copy(sc)
+    clear.iter_fog_opacity = 0.0
+    renderer.set_scene(clear)
+    f0, _ = renderer.render(precision=_abi.PREC_FP64)
+    assert np.abs(f0.rgb - f.rgb).max() > 1e-3
 
 
 def _tuples(tab):
