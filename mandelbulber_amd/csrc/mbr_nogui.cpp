@@ -20,7 +20,9 @@
 //                         file_destination + 00000 + extension (this driver)
 //
 // The image goes to file_destination + frame number (%05d) + ".png", as
-// IndexFilename (src/files.cpp:30-35) names it.  There is no JPEG encoder in
+// IndexFilename (src/files.cpp:30-35) names it; with the settings' tiles > 1
+// the frame is rendered tile by tile and joined into file_destination +
+// "_fromTiles.png" (16-bit).  There is no JPEG encoder in
 // this build: -format jpg writes an 8-bit PNG.  Network rendering (-ip,
 // -port) is not part of the render core and is rejected.
 #include <sys/stat.h>
@@ -70,10 +72,12 @@ bool file_exists(const std::string &path) {
   return stat(path.c_str(), &st) == 0;
 }
 
-// Render one frame of scene s, compile it, apply the post effects and write
-// it to `out` (or file_destination + index).  0 on success.
-int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::string &format, std::string out,
-                 int index) {
+// Render one image of scene s, compile it, apply the post effects and write
+// it to `out` (or file_destination + index).  With tile_out set the compiled
+// 16-bit image is handed back instead of written (the tile loop stores it).
+// 0 on success.
+int render_image(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::string &format, std::string out,
+                 int index, std::vector<uint16_t> *tile_out) {
   int rc = mbr_set_scene(ctx, &s, nullptr);
   if (rc != MBR_OK) {
     std::printf("scene rejected: %s\n", mbr_last_error(ctx));
@@ -146,6 +150,10 @@ int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::str
     if (rc != MBR_OK) std::printf("DOF failed: %s\n", mbr_last_error(ctx));
     else std::printf("DOF rendered (focus %g, radius %g)\n", s.dof_focus, s.dof_radius);
   }
+  if (tile_out) {
+    if (rc == MBR_OK) tile_out->swap(img16);
+    return rc == MBR_OK ? 0 : 1;
+  }
   if (out.empty()) out = index_filename(s.file_destination, "png", index);
   make_parent(out);
   const int bits = (format == "png16" || format == "png16alpha") ? 16 : 8;
@@ -156,6 +164,66 @@ int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::str
     return 1;
   }
   std::printf("image saved: %s%s\n", out.c_str(), format == "jpg" ? " (8-bit PNG: no JPEG encoder in this build)" : "");
+  return 0;
+}
+
+// One frame (src/Render3D.cpp:2314-2496).  With tiles > 1 the frame is
+// rendered as tiles x tiles images of the frame's size, each a tile of the
+// full view; each tile's 16-bit image goes to file_destination + "tile" +
+// index (raw sRGB16, as the reference saves them), and when the last tile is
+// done the tiles are joined into file_destination + "_fromTiles.png", a
+// 16-bit PNG of (width x tiles) x (height x tiles), and removed
+// (SaveFromTilesPNG16, src/files.cpp:269-373).
+int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::string &format, std::string out,
+                 int index) {
+  const int tiles = s.tiles < 1 ? 1 : (s.tiles > 100 ? 100 : s.tiles);
+  if (tiles == 1) return render_image(ctx, s, precision, format, out, index, nullptr);
+  const int w = s.image_width, h = s.image_height;
+  for (int tile = 0; tile < tiles * tiles; tile++) {
+    const int index2 = index * tiles * tiles + tile;
+    std::printf("---------- Tile: %d -------------\n", index2);
+    mbr_scene ts = s;
+    ts.tiles = tiles;
+    ts.tile_index = tile;
+    std::vector<uint16_t> img;
+    if (render_image(ctx, ts, precision, format, "", index2, &img) != 0) return 1;
+    const std::string path = index_filename(s.file_destination, "tile", index2);
+    make_parent(path);
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f || std::fwrite(img.data(), sizeof(uint16_t), img.size(), f) != img.size()) {
+      if (f) std::fclose(f);
+      std::printf("cannot write %s\n", path.c_str());
+      return 1;
+    }
+    std::fclose(f);
+  }
+  // join the tiles, row of tiles by row of tiles
+  const int W = w * tiles, H = h * tiles;
+  std::vector<uint16_t> full((size_t)3 * W * H);
+  for (int tile = 0; tile < tiles * tiles; tile++) {
+    const std::string path = index_filename(s.file_destination, "tile", index * tiles * tiles + tile);
+    FILE *f = std::fopen(path.c_str(), "rb");
+    std::vector<uint16_t> img((size_t)3 * w * h);
+    const bool ok = f && std::fread(img.data(), sizeof(uint16_t), img.size(), f) == img.size();
+    if (f) std::fclose(f);
+    if (!ok) {
+      std::printf("Reading error of image tile files\n");
+      return 1;
+    }
+    const int tx = tile % tiles, ty = tile / tiles;
+    for (int y = 0; y < h; y++)
+      std::memcpy(&full[3 * ((size_t)(ty * h + y) * W + (size_t)tx * w)], &img[3 * (size_t)y * w],
+                  sizeof(uint16_t) * 3 * (size_t)w);
+    std::remove(path.c_str());
+  }
+  if (out.empty()) out = std::string(s.file_destination) + "_fromTiles.png";
+  make_parent(out);
+  const int rc = mbr_write_png(out.c_str(), W, H, full.data(), nullptr, 16);
+  if (rc != MBR_OK) {
+    std::printf("cannot write %s (status %d)\n", out.c_str(), rc);
+    return 1;
+  }
+  std::printf("image saved: %s (%d x %d from %d tiles)\n", out.c_str(), W, H, tiles * tiles);
   return 0;
 }
 

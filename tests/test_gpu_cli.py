@@ -136,3 +136,28 @@ def test_cli_flight_path(tmp_path):
     fs.frame_no = 1
     px, _ = _read_png(str(tmp_path / "fl" / "frame00001.png"))
     assert np.array_equal(px.astype(np.uint16), _library_png(fs))
+
+
+def test_cli_tiles_join_into_the_full_frame(tmp_path):
+    """tiles 2: four 48x36 tiles rendered one by one, stored, joined into
+    file_destination_fromTiles.png (96x72, 16-bit) and removed; without SSAO
+    (a screen-space effect sees each tile alone) and without step jitter (its
+    hash is keyed by pixel index) the joined image equals the 96x72 frame."""
+    from test_image import _read_png
+    f = tmp_path / "scene.fract"
+    dest = tmp_path / "out" / "tiled"
+    f.write_text(SETTINGS + f"file_destination {dest};\npost_SSAO_enabled 0;\ntiles 2;\njitter_mode 0;\n")
+    print("[cli] tiles 2", flush=True)
+    r = subprocess.run([CLI, "-res", "48x36", "-format", "png16", "-precision", "fp64", str(f)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = tmp_path / "out" / "tiled_fromTiles.png"
+    assert out.exists()
+    assert not list((tmp_path / "out").glob("*.tile"))
+    px, bits = _read_png(str(out))
+    assert bits == 16 and px.shape == (72, 96, 3)
+    s = scene.load(str(f), ["image_width=96", "image_height=72", "tiles=1"])
+    assert s.jitter_mode == 0
+    expect = _library_png(s, precision=_abi.PREC_FP64)
+    assert np.array_equal(px.astype(np.uint16), expect)
