@@ -27,6 +27,9 @@ extern "C" {
 
 #define MBR_ABI_VERSION 1
 
+/* largest reflections_max the render core accepts (mirrored rays per pixel) */
+#define MBR_MAX_REFLECTIONS 16
+
 /* ---- status codes (reference analogue: checkErr -> bool + CPU fallback,
  *      src/cl_support.cpp:109-135, src/Render3D.cpp:719) ---- */
 enum mbr_status {
@@ -209,7 +212,8 @@ typedef struct mbr_scene {
   double specular;                 /* specular */
   double global_ilum;              /* ambient_occlusion */
   double fast_ao_tune;             /* ambient_occlusion_fast_tune */
-  double reflect;                  /* reflect (env mapping: must be 0) */
+  double reflect;                  /* reflect (with raytraced_reflections: the fractal's reflection
+                                      coefficient; without: environment mapping, not supported) */
   double glow_intensity;           /* glow_intensity */
   mbr_rgb main_light_colour;       /* main_light_colour_R/G/B */
   mbr_rgb glow_color1, glow_color2;/* glow_color_1_*, glow_color_2_* */
@@ -325,6 +329,17 @@ typedef struct mbr_scene {
   double volumetric_light_intensity[5]; /* volumetric_light_intensity_0..4 */
   double iter_fog_opacity;              /* iteration_fog_opacity */
   double iter_fog_opacity_trim;         /* iteration_fog_opacity_trim */
+
+  /* raytraced reflections (MainThread's reflection loop, src/Render3D.cpp:
+   * 451-560): reflections_max bounces when raytraced_reflections is on
+   * (both with the switches above); the fractal reflects `reflect`, each
+   * primitive its own coefficient */
+  double plane_reflect;                 /* primitive_plane_reflect */
+  double box_reflect;                   /* primitive_box_reflect */
+  double inv_box_reflect;               /* primitive_invertedBox_reflect */
+  double sphere_reflect;                /* primitive_sphere_reflect */
+  double inv_sphere_reflect;            /* primitive_invertedSphere_reflect */
+  double water_reflect;                 /* primitive_water_reflect */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and

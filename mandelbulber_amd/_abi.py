@@ -107,6 +107,8 @@ class Scene(C.Structure):
         ("aux_light_pre_colour", RGB * 4),
         ("volumetric_light_intensity", C.c_double * 5), ("iter_fog_opacity", C.c_double),
         ("iter_fog_opacity_trim", C.c_double),
+        ("plane_reflect", C.c_double), ("box_reflect", C.c_double), ("inv_box_reflect", C.c_double),
+        ("sphere_reflect", C.c_double), ("inv_sphere_reflect", C.c_double), ("water_reflect", C.c_double),
     ]
 
 

@@ -300,6 +300,11 @@ struct Slot {
   void *sh_base = nullptr, *sh_spec = nullptr, *sh_col = nullptr, *sh_fast = nullptr, *sh_ao = nullptr;
   void *sh_aux = nullptr;
   size_t sh_ao_bytes = 0;
+  // raytraced reflections: specular planes (every level), mirrored rays
+  // (levels after the first)
+  void *rf_sp = nullptr;
+  double *rf_o = nullptr, *rf_d = nullptr, *rf_t0 = nullptr;
+  uint8_t *rf_active = nullptr;
 };
 
 constexpr int kMaxSlots = 4;
@@ -355,6 +360,11 @@ struct mbr_ctx {
   double *d_lights_d = nullptr;
   float *d_lights_f = nullptr;
   int d_lights_cap = 0;
+  // raytraced reflections: the slots of the levels after the first, and the
+  // device copy of every level's band buffers
+  std::vector<Slot> refl_slots;
+  BandBufs *d_levels = nullptr;
+  std::vector<BandBufs> levels_host;
 
   int fail(int rc, const std::string &msg) {
     err = msg;
@@ -388,6 +398,12 @@ void free_band(Slot &sl) {
   for (void *p : sp)
     if (p) (void)hipFree(p);
   sl.sh_base = sl.sh_spec = sl.sh_col = sl.sh_fast = sl.sh_ao = sl.sh_aux = nullptr;
+  void *rp[] = {sl.rf_sp, sl.rf_o, sl.rf_d, sl.rf_t0, sl.rf_active};
+  for (void *p : rp)
+    if (p) (void)hipFree(p);
+  sl.rf_sp = nullptr;
+  sl.rf_o = sl.rf_d = sl.rf_t0 = nullptr;
+  sl.rf_active = nullptr;
   sl.sh_ao_bytes = 0;
   sl.prec = -1;
 }
@@ -522,8 +538,15 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.ifs_folding_mode) return c->fail(MBR_ERR_UNSUPPORTED, "IFS folding mode is not in the render core");
   if (s.reflect > 0.0 && !s.raytraced_reflections)
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
-  if (s.raytraced_reflections && s.reflections_max > 0)
-    return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections are not in the render core");
+  if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
+    return c->fail(MBR_ERR_INVALID_ARG, "reflections_max above " + std::to_string(MBR_MAX_REFLECTIONS));
+  if (s.raytraced_reflections && s.reflections_max > 0) {
+    bool fx = s.iter_fog_enabled;
+    for (int i = 0; i < 5; i++) fx = fx || s.volumetric_light_enabled[i];
+    if (fx)
+      return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections with volumetric lights or iteration fog are not "
+                                          "in the render core");
+  }
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
   if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
   if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
@@ -590,6 +613,8 @@ void mbr_destroy(mbr_ctx *c) {
                 c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma, c->d_lights_d, c->d_lights_f};
   for (void *p : bp)
     if (p) (void)hipFree(p);
+  for (Slot &sl : c->refl_slots) free_band(sl);
+  if (c->d_levels) (void)hipFree(c->d_levels);
   void *ptrs[] = {c->d_sc, c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->d_stats, c->d_planes};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -693,6 +718,12 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   d.iter_fog_opacity = s.iter_fog_opacity;
   d.iter_fog_trim = s.iter_fog_opacity_trim;
   d.main_light_intensity = s.main_light_intensity;
+  d.refl_max = s.raytraced_reflections ? std::max(0, s.reflections_max) : 0;
+  {
+    const double rf[7] = {s.reflect, s.plane_reflect, s.water_reflect, s.sphere_reflect, s.inv_sphere_reflect,
+                          s.box_reflect, s.inv_box_reflect};  // OBJ_* order
+    std::memcpy(d.refl, rf, sizeof rf);
+  }
   d.shadow_fx = s.iter_fog_enabled;
   for (int i = 0; i < 5; i++) {
     if (s.volumetric_light_enabled[i]) d.shadow_fx = 1;
@@ -980,6 +1011,10 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
                   AODev ao, long long first, long long last, const FrameOut &out, int persist, hipStream_t st,
                   hipEvent_t *ev, volatile const int *cancel) {
   BandBufs &b = sl.band;
+  // no reflection levels: the shade keeps specular and object parts together
+  b.sp_r = b.sp_g = b.sp_b = nullptr;
+  b.ray_o = b.ray_d = b.ray_t0 = nullptr;
+  b.active = nullptr;
   ShadeBufs sbuf;
   sbuf.base = sl.sh_base;
   sbuf.spec = sl.sh_spec;
@@ -1010,6 +1045,95 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula,
                               host.vol_any || host.iter_fog, host.n_lights > 0 && host.sd.aux_visibility > 0,
                               persist, c->d_stats, ao));
+    if (e) HIPCHECK(c, hipEventRecord(e[3], st));
+  }
+  return MBR_OK;
+}
+
+ShadeBufs shade_bufs(Slot &sl, const DevScene &host, int formula) {
+  ShadeBufs sbuf;
+  sbuf.base = sl.sh_base;
+  sbuf.spec = sl.sh_spec;
+  sbuf.col = sl.sh_col;
+  sbuf.fast_ao = sl.sh_fast;
+  sbuf.ao = sl.sh_ao;
+  sbuf.aux = sl.sh_aux;
+  sbuf.n = sl.band.cap;
+  sbuf.slow_ao = host.ao && !host.fast_ao;
+  sbuf.deep = 0;
+  sbuf.plain = host.plain;
+  sbuf.formula = formula;
+  sbuf.lights = host.n_lights > 0;
+  return sbuf;
+}
+
+// The reflection planes of a level's slot (level 0: the specular planes
+// only), made once and kept with the slot.
+int alloc_refl(mbr_ctx *c, Slot &sl, bool mirrored) {
+  const size_t n = (size_t)sl.band.cap;
+  if (!sl.rf_sp) HIPCHECK(c, hipMalloc(&sl.rf_sp, 3 * n * sizeof(double)));
+  if (mirrored && !sl.rf_o) {
+    HIPCHECK(c, hipMalloc((void **)&sl.rf_o, 3 * n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&sl.rf_d, 3 * n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&sl.rf_t0, n * sizeof(double)));
+    HIPCHECK(c, hipMalloc((void **)&sl.rf_active, n));
+  }
+  BandBufs &b = sl.band;
+  b.sp_r = sl.rf_sp;
+  b.sp_g = (double *)sl.rf_sp + n;
+  b.sp_b = (double *)sl.rf_sp + 2 * n;
+  b.ray_o = mirrored ? sl.rf_o : nullptr;
+  b.ray_d = mirrored ? sl.rf_d : nullptr;
+  b.ray_t0 = mirrored ? sl.rf_t0 : nullptr;
+  b.active = mirrored ? sl.rf_active : nullptr;
+  return MBR_OK;
+}
+
+// One frame with raytraced reflections (FP64): per band the primary march,
+// then per further level the mirrored rays of the previous level's
+// reflecting hits and their march, the shading of every level, and the
+// back-to-front composition (MainThread, src/Render3D.cpp:440-589).
+int enqueue_frame_refl(mbr_ctx *c, std::vector<Slot *> &lv, const DevScene *d_sc, const DevScene &host, int formula,
+                       AODev ao, long long first, long long last, const FrameOut &out, int persist, hipStream_t st,
+                       hipEvent_t *ev, volatile const int *cancel) {
+  const int n_levels = (int)lv.size();
+  // the device copy of the levels' band buffers (the previous frame's
+  // finish may still read it)
+  HIPCHECK(c, hipStreamSynchronize(st));
+  c->levels_host.resize((size_t)n_levels);
+  for (int k = 0; k < n_levels; k++) c->levels_host[(size_t)k] = lv[(size_t)k]->band;
+  if (!c->d_levels) HIPCHECK(c, hipMalloc((void **)&c->d_levels, (MBR_MAX_REFLECTIONS + 1) * sizeof(BandBufs)));
+  HIPCHECK(c, hipMemcpy(c->d_levels, c->levels_host.data(), (size_t)n_levels * sizeof(BandBufs),
+                        hipMemcpyHostToDevice));
+  const long long cap = lv[0]->band.cap;
+  int band = 0;
+  for (long long band_first = first; band_first < last; band_first += cap, band++) {
+    if (cancel && *cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
+    const int band_rays = (int)std::min<long long>(cap, last - band_first);
+    hipEvent_t *e = ev ? ev + 6 * band : nullptr;
+    BandBufs &b0 = lv[0]->band;
+    HIPCHECK(c, launch_band_reset(b0, st));
+    if (e) HIPCHECK(c, hipEventRecord(e[0], st));
+    HIPCHECK(c, launch_march(MBR_PREC_FP64, d_sc, b0, (int)band_first, band_rays, formula, host.plain, persist,
+                             c->d_stats, st));
+    HIPCHECK(c, launch_march_fix(MBR_PREC_FP64, d_sc, b0, (int)band_first, formula, persist / 2, c->d_stats, st));
+    for (int k = 1; k < n_levels; k++) {
+      BandBufs &bk = lv[(size_t)k]->band;
+      HIPCHECK(c, launch_band_reset(bk, st));
+      HIPCHECK(c, launch_refl_clear(bk, band_rays, st));
+      HIPCHECK(c, launch_refl_setup(d_sc, lv[(size_t)k - 1]->band, bk, (int)band_first, formula, persist, c->d_stats,
+                                    st));
+      HIPCHECK(c, launch_refl_march(d_sc, bk, (int)band_first, formula, persist, c->d_stats, st));
+    }
+    if (e) HIPCHECK(c, hipEventRecord(e[1], st));
+    for (int k = 0; k < n_levels; k++) {
+      const ShadeBufs sbuf = shade_bufs(*lv[(size_t)k], host, formula);
+      HIPCHECK(c, launch_shade(MBR_PREC_FP64, d_sc, lv[(size_t)k]->band, (int)band_first, ao, sbuf, persist,
+                               c->d_stats, st, (e && k == 0) ? e[4] : nullptr, (e && k == 0) ? e[5] : nullptr));
+    }
+    if (e) HIPCHECK(c, hipEventRecord(e[2], st));
+    HIPCHECK(c, launch_refl_finish(d_sc, c->d_levels, n_levels, (int)band_first, band_rays, out,
+                                   host.n_lights > 0 && host.sd.aux_visibility > 0, st));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
   return MBR_OK;
@@ -1066,8 +1190,9 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
   int rc = check_precision(c, precision);
   if (rc != MBR_OK) return rc;
-  // formulas without kernels of their own (delta DE, trig_DE) render in FP64
-  if (formula_generic(s.formula)) precision = MBR_PREC_FP64;
+  // formulas without kernels of their own (delta DE, trig_DE) and
+  // reflections render in FP64
+  if (formula_generic(s.formula) || c->host_sc.refl_max > 0) precision = MBR_PREC_FP64;
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
   if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
   if (row0 < 0) row0 = 0;
@@ -1079,6 +1204,22 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   Slot &sl = c->slots[0];
   rc = alloc_slot(c, sl, precision, band_cap(precision, ao_count), ao_count, c->host_sc.iter_fog != 0);
   if (rc != MBR_OK) return rc;
+  // reflection levels: a slot each after the first
+  const int refl = c->host_sc.refl_max;
+  std::vector<Slot *> levels;
+  if (refl > 0) {
+    if ((int)c->refl_slots.size() < refl) c->refl_slots.resize((size_t)refl);
+    levels.push_back(&sl);
+    rc = alloc_refl(c, sl, false);
+    if (rc != MBR_OK) return rc;
+    for (int k = 0; k < refl; k++) {
+      Slot &lk = c->refl_slots[(size_t)k];
+      rc = alloc_slot(c, lk, precision, sl.band.cap, ao_count, false);
+      if (rc == MBR_OK) rc = alloc_refl(c, lk, true);
+      if (rc != MBR_OK) return rc;
+      levels.push_back(&lk);
+    }
+  }
   const FrameOut out = frame_out(fb);
   const AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
   const long long first = (long long)row0 * s.image_width;
@@ -1093,8 +1234,12 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   unsigned long long v[ST_COUNT];
   for (int attempt = 0;; attempt++) {
     HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, st));
-    rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
-                       c->ev.data(), opts ? opts->cancel : nullptr);
+    if (refl > 0)
+      rc = enqueue_frame_refl(c, levels, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
+                              c->ev.data(), opts ? opts->cancel : nullptr);
+    else
+      rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
+                         c->ev.data(), opts ? opts->cancel : nullptr);
     if (rc != MBR_OK) return rc;
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
@@ -1104,6 +1249,10 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
     rc = grow_pool(c, sl, v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
     if (rc != MBR_OK) return rc;
+    for (int k = 0; k < refl; k++) {
+      rc = grow_pool(c, c->refl_slots[(size_t)k], v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
+      if (rc != MBR_OK) return rc;
+    }
   }
   double ms_march = 0, ms_shade = 0, ms_finish = 0, ms_ao = 0;
   for (int band = 0; band < n_bands; band++) {
@@ -1482,10 +1631,10 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   for (int i = 0; i < n; i++) {
     rc = prepare_scene(c, &scenes[i], lightmap, frames[i].p);
     if (rc != MBR_OK) return c->fail(rc, "scene " + std::to_string(i) + ": " + c->err);
-    if (scene_has_lights(frames[i].p.s) || frames[i].p.d.shadow_fx)
+    if (scene_has_lights(frames[i].p.s) || frames[i].p.d.shadow_fx || frames[i].p.d.refl_max > 0)
       return c->fail(MBR_ERR_UNSUPPORTED, "scene " + std::to_string(i) +
-                                              ": aux lights, volumetric lights and iteration fog render through "
-                                              "mbr_set_scene, not in a batch");
+                                              ": aux lights, volumetric lights, iteration fog and reflections "
+                                              "render through mbr_set_scene, not in a batch");
     const mbr_scene &s = frames[i].p.s;
     if (fbs[i].width != s.image_width || fbs[i].height != s.image_height)
       return c->fail(MBR_ERR_INVALID_ARG, "framebuffer " + std::to_string(i) + " differs from its scene's size");
@@ -1628,7 +1777,7 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
   if (s.formula != MBR_FORMULA_TRIG_OPTIM) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: formula must be 2");
   if (h.n_lights > 0) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: aux lights have no gradient here");
   if (!s.hit_refine) return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: the scene needs hit_refine = 1");
-  if (h.shadow || h.ao || h.shadow_fx || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
+  if (h.shadow || h.ao || h.shadow_fx || h.refl_max > 0 || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
       s.perspective_type != MBR_PERSP_THREE_POINT || s.tiles != 1 || s.constant_DE_threshold || s.interior_mode ||
       s.limits_enabled || s.iter_thresh || h.fd.prim_plane || h.fd.prim_box || h.fd.prim_ibox ||
       h.fd.prim_sphere || h.fd.prim_isphere || h.fd.prim_water || h.fd.tglad_fold || h.fd.sph_fold)

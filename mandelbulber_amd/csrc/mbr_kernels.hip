@@ -142,15 +142,19 @@ struct MarchOut {
 struct RayState {
   V3<double> point;
   double scan, step, thr, drift, dist;
+  double thr0;  // initial threshold: 0, or the previous ray's final one for a reflected ray
   int i, object;
 };
 
 enum : int { SO_CONTINUE = 0, SO_MISS = 1, SO_HANDOFF = 2 };
 
+// Refl: a reflected ray, whose thresholds start from s.thr0
+// (distThreshInit, src/Render3D.cpp:92, :102-109); primary rays start from 0.
+template <bool Refl = false>
 __device__ __forceinline__ void march_before_de(const DevScene &sc, V3<double> start, V3<double> dir, RayState &s) {
   const ShadeK<double> &k = sc.sd;
   s.point = start + dir * s.scan;
-  s.thr = sc.constant_thresh ? k.quality : s.scan * k.resolution * k.persp / k.quality + 0.0;
+  s.thr = sc.constant_thresh ? k.quality : s.scan * k.resolution * k.persp / k.quality + (Refl ? s.thr0 : 0.0);
   if (sc.persp_type == 1 || sc.persp_type == 2) s.thr *= kPi;
 }
 
@@ -186,7 +190,7 @@ __device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, d
 // The reference loop from step s.i on, in FP64, then the binary search
 // (src/Render3D.cpp:145-183).  Sets `ambiguous` when a decision lies within
 // the drift margin of its threshold (see the hand-over rule above).
-template <class RT, int F>
+template <class RT, int F, bool Refl = false>
 __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
                                    const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
                                    bool &ambiguous) {
@@ -202,7 +206,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
   bool have_prev = s.i > 0;
   ambiguous = false;
   for (; s.i < 10000; s.i++) {
-    march_before_de(sc, start, dir, s);
+    march_before_de<Refl>(sc, start, dir, s);
     detail = s.thr;
     DEInfo info;
     dist = calc_distance_f<double, F, kAlg>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
@@ -241,7 +245,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       have_prev = true;
       if (dist > distThresh) { scan += step; point = start + dir * scan; }
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
-      distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + 0.0;
+      distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + (Refl ? s.thr0 : 0.0);
       DEInfo bi;
       dist = calc_distance_f<double, F, kAlg>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
       cnt.add(bi);
@@ -297,6 +301,7 @@ __device__ __forceinline__ RayState ray_start_state(const DevScene &sc) {
   s.scan = sc.view_distance_min;
   s.step = sc.view_distance_min;
   s.thr = 0.0;
+  s.thr0 = 0.0;
   s.drift = 0.0;
   s.dist = 0.0;
   s.i = 0;
@@ -577,6 +582,19 @@ __device__ __forceinline__ void load_hit(const DevScene &sc, const BandBufs &b, 
   in.object = b.object[r];
 }
 
+// The direction of ray r: the camera's view vector, or the mirrored
+// direction of a reflection level.
+__device__ __forceinline__ V3<double> ray_dir(const DevScene &sc, const BandBufs &b, int band_first, int r) {
+  V3<double> dir;
+  if (b.ray_d) {
+    dir = v3<double>(b.ray_d[3 * (long long)r + 0], b.ray_d[3 * (long long)r + 1], b.ray_d[3 * (long long)r + 2]);
+  } else {
+    const int p = band_first + r;
+    view_vector(sc, p % sc.width, p / sc.width, dir);
+  }
+  return dir;
+}
+
 template <class T, int F, bool Deep>
 __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                     SurfBufs<T> sb, unsigned long long *stats) {
@@ -586,10 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
   for (long long tid = (long long)blockIdx.x * kBlock + threadIdx.x; tid < n_hits;
        tid += (long long)gridDim.x * kBlock) {
     const int r = b.hit_list[tid];
-    const int p = band_first + r;
-    const int x = p % sc.width, y = p / sc.width;
-    V3<double> dir;
-    view_vector(sc, x, y, dir);
+    const V3<double> dir = ray_dir(sc, b, band_first, r);
     const ShadeK<T> &k = sc.shade<T>();
     ShaderIn<T> in;
     load_hit(sc, b, r, in);
@@ -648,9 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_aux(const DevScene *__restrict__ scp
   for (long long tid = (long long)blockIdx.x * kBlock + threadIdx.x; tid < n_hits;
        tid += (long long)gridDim.x * kBlock) {
     const int r = b.hit_list[tid];
-    const int p = band_first + r;
-    V3<double> dir;
-    view_vector(sc, p % sc.width, p / sc.width, dir);
+    const V3<double> dir = ray_dir(sc, b, band_first, r);
     ShaderIn<T> in;
     load_hit(sc, b, r, in);
     in.view = cvt<T>(dir);
@@ -902,7 +915,28 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
         amb.B /= den;
       }
     }
-    if (lane == 0) {
+    if (lane == 0 && b.sp_r) {
+      // reflections: the object part and the specular part apart
+      const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
+      T oR = (amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR;
+      T oG = (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG;
+      T oB = (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB;
+      T sR = sb.spec[0 * n + tid], sG = sb.spec[1 * n + tid], sB = sb.spec[2 * n + tid];
+      if (sc.n_lights > 0) {
+        oR += sb.aux[0 * n + tid] * cR;
+        oG += sb.aux[1 * n + tid] * cG;
+        oB += sb.aux[2 * n + tid] * cB;
+        sR = sb.aux[3 * n + tid] + sR;
+        sG = sb.aux[4 * n + tid] + sG;
+        sB = sb.aux[5 * n + tid] + sB;
+      }
+      ((RT *)b.sh_r)[r] = (RT)oR;
+      ((RT *)b.sh_g)[r] = (RT)oG;
+      ((RT *)b.sh_b)[r] = (RT)oB;
+      ((RT *)b.sp_r)[r] = (RT)sR;
+      ((RT *)b.sp_g)[r] = (RT)sG;
+      ((RT *)b.sp_b)[r] = (RT)sB;
+    } else if (lane == 0) {
       const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
       if (sc.n_lights > 0) {
         // (ambient + main light) x colour + aux x colour, then aux specular
@@ -1155,8 +1189,232 @@ __global__ void __launch_bounds__(kBlock) k_finish_fx(const DevScene *__restrict
   flush_stats(stats, v);
 }
 
+// ----------------------------------------------------------- reflections
+__global__ void k_refl_clear(uint8_t *active, int n) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) active[r] = 0;
+}
+
+// The mirrored rays of the reflecting hits of `prev` (src/Render3D.cpp:
+// 478-498): normal at the hit (CalculateNormals with the hit's threshold as
+// detail size), view vector mirrored about it, start one threshold along it.
+template <int F>
+__global__ void __launch_bounds__(kBlock) k_refl_setup(const DevScene *__restrict__ scp, BandBufs prev, BandBufs next,
+                                                       int band_first, unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const long long n_hits = (long long)prev.counters[CNT_HITS];
+  LaneCount cnt = {0, 0};
+  for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < n_hits; t += (long long)gridDim.x * kBlock) {
+    const int r = prev.hit_list[t];
+    if (sc.refl[prev.object[r] % 7] == 0.0) continue;
+    const V3<double> dir = ray_dir(sc, prev, band_first, r);
+    ShaderIn<double> in;
+    load_hit(sc, prev, r, in);
+    in.view = dir;
+    const V3<double> n = normals<double, F, false>(sc, in, cnt);
+    const V3<double> d2 = dir - (n * dot(dir, n)) * 2.0;
+    const double thr = prev.thr[r];
+    const V3<double> o = in.point64 + d2 * thr;
+    next.ray_o[3 * (long long)r + 0] = o.x;
+    next.ray_o[3 * (long long)r + 1] = o.y;
+    next.ray_o[3 * (long long)r + 2] = o.z;
+    next.ray_d[3 * (long long)r + 0] = d2.x;
+    next.ray_d[3 * (long long)r + 1] = d2.y;
+    next.ray_d[3 * (long long)r + 2] = d2.z;
+    next.ray_t0[r] = thr;
+    next.active[r] = 1;
+    next.hand_ray[atomicAdd(&next.counters[CNT_HAND], 1ull)] = r;
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_SHADE_DE] = cnt.de_calls;
+  v[ST_SHADE_ITERS] = cnt.iterations;
+  flush_stats(stats, v);
+}
+
+// The FP64 march (RayMarching with its binary search) of a reflection
+// level's rays, with the same step jitter as the pixel's primary ray.
+template <int F>
+__global__ void __launch_bounds__(kBlock) k_refl_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                       unsigned long long *stats) {
+  const DevScene &sc = *scp;
+  const long long n_rays = (long long)b.counters[CNT_HAND];
+  LaneCount cnt = {0, 0};
+  unsigned long long hits = 0, missed = 0;
+  for (long long h = (long long)blockIdx.x * kBlock + threadIdx.x; h < n_rays; h += (long long)gridDim.x * kBlock) {
+    const int r = b.hand_ray[h];
+    b.ovf_off[r] = -1;
+    const V3<double> start = v3<double>(b.ray_o[3 * (long long)r], b.ray_o[3 * (long long)r + 1],
+                                        b.ray_o[3 * (long long)r + 2]);
+    const V3<double> dir = v3<double>(b.ray_d[3 * (long long)r], b.ray_d[3 * (long long)r + 1],
+                                      b.ray_d[3 * (long long)r + 2]);
+    const uint32_t pixel = sc.pixel_base + (uint32_t)(band_first + r);
+    RayState s = ray_start_state(sc);
+    s.thr0 = b.ray_t0[r];
+    MarchOut o;
+    bool ambiguous = false;
+    march_continue_f64<double, F, true>(sc, start, dir, pixel, ray_records<double>(b, r, stats), s, o, cnt,
+                                        ambiguous);
+    write_ray(b, r, o);
+    hits += o.found;
+    missed += o.missed;
+  }
+  unsigned long long v[ST_COUNT] = {0};
+  v[ST_MARCH_DE] = cnt.de_calls;
+  v[ST_MARCH_ITERS] = cnt.iterations;
+  v[ST_HITS] = hits;
+  v[ST_MISSED] = missed;
+  flush_stats(stats, v);
+}
+
+// Per pixel: the levels from the last one reached down to the primary ray
+// (src/Render3D.cpp:501-571): each shaded surface (or background) blended
+// with the next level's result by its reflection coefficient, then that
+// level's VolumetricShader over its own step records.
+template <bool VIS>
+__global__ void __launch_bounds__(kBlock) k_refl_finish(const DevScene *__restrict__ scp,
+                                                        const BandBufs *__restrict__ levels, int n_levels,
+                                                        int band_first, int band_rays, FrameOut out) {
+  const DevScene &sc = *scp;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= band_rays) return;
+  const int p = band_first + r;
+  int rayEnd = 0;
+  for (int k = 1; k < n_levels; k++) {
+    if (!levels[k].active[r]) break;
+    rayEnd = k;
+  }
+  const int maxRay = n_levels - 1;
+  RGBT<double> result = {0, 0, 0};
+  double opacity0 = 0, alpha0 = 0;
+  for (int ray = rayEnd; ray >= 0; ray--) {
+    const BandBufs &b = levels[ray];
+    const bool found = b.found[r];
+    const V3<double> dir = ray_dir(sc, b, band_first, r);
+    RGBT<double> obj = {0, 0, 0}, bg = {0, 0, 0}, spec = {0, 0, 0};
+    if (found) {
+      obj = RGBT<double>{((const double *)b.sh_r)[r], ((const double *)b.sh_g)[r], ((const double *)b.sh_b)[r]};
+      spec = RGBT<double>{((const double *)b.sp_r)[r], ((const double *)b.sp_g)[r], ((const double *)b.sp_b)[r]};
+    } else {
+      bg = background<double>(sc, dir);
+    }
+    const double reflect = sc.refl[b.object[r] % 7];
+    RGBT<double> px;
+    if (maxRay > 0 && rayEnd > 0 && ray != rayEnd) {
+      px.R = result.R * reflect + (1.0 - reflect) * (obj.R + bg.R) + spec.R;
+      px.G = result.G * reflect + (1.0 - reflect) * (obj.G + bg.G) + spec.G;
+      px.B = result.B * reflect + (1.0 - reflect) * (obj.B + bg.B) + spec.B;
+    } else {
+      px.R = obj.R + bg.R + spec.R;
+      px.G = obj.G + bg.G + spec.G;
+      px.B = obj.B + bg.B + spec.B;
+    }
+    const double alpha = found ? 1.0 : 0.0;
+    const int n = b.count[r];
+    const double depth = b.depth[r];
+    const double *st = (const double *)b.rec_step, *di = (const double *)b.rec_dist, *th = (const double *)b.rec_thr;
+    const double *ost = (const double *)b.ovf_step, *odi = (const double *)b.ovf_dist, *oth = (const double *)b.ovf_thr;
+    const long long cap = b.cap;
+    const int slab = b.slab;
+    const long long dj = b.ovf_off[r];
+    auto rec = [&](int i, double &s, double &d, double &t) {
+      if (i < slab) {
+        const long long o = (long long)i * cap + r;
+        s = st[o]; d = di[o]; t = th[o];
+        return;
+      }
+      const int ch = dj >= 0 ? b.ovf_dir[dj * kDirSlots + (i - slab) / kChunk] : -1;
+      if (ch >= 0) {
+        const long long o = (long long)ch * kChunk + (i - slab) % kChunk;
+        s = ost[o]; d = odi[o]; t = oth[o];
+      } else {
+        s = 0; d = 0; t = 0;
+      }
+    };
+    VisRay vray;
+    const VisRay *vr = nullptr;
+    const bool vis = VIS && sc.n_lights > 0 && sc.sd.aux_visibility > 0;
+    if (vis) {
+      vray.dir = dir;
+      vray.start = ray == 0 ? v3<double>(sc.start[0], sc.start[1], sc.start[2])
+                            : v3<double>(b.ray_o[3 * (long long)r], b.ray_o[3 * (long long)r + 1],
+                                         b.ray_o[3 * (long long)r + 2]);
+      double sum = 0.0;
+      for (int i = 0; i < n; i++) {
+        double s_, d_, t_;
+        rec(i, s_, d_, t_);
+        sum += s_;
+      }
+      vray.scan_end = sum;
+      vr = &vray;
+    }
+    const VolOut<double> v = volumetric<double>(sc, depth, n, px, alpha, rec, vr, vis);
+    result = v.rgb;
+    if (ray == 0) {
+      opacity0 = v.opacity;
+      alpha0 = v.alpha;
+    }
+  }
+  const BandBufs &b0 = levels[0];
+  const bool found0 = b0.found[r];
+  if (out.rgb[0]) out.rgb[0][p] = (float)result.R;
+  if (out.rgb[1]) out.rgb[1][p] = (float)result.G;
+  if (out.rgb[2]) out.rgb[2][p] = (float)result.B;
+  if (out.z) out.z[p] = found0 ? (float)b0.depth[r] : 1e20f;
+  if (out.colour[0]) out.colour[0][p] = found0 ? b0.col_r[r] : 0;
+  if (out.colour[1]) out.colour[1][p] = found0 ? b0.col_g[r] : 0;
+  if (out.colour[2]) out.colour[2][p] = found0 ? b0.col_b[r] : 0;
+  if (out.opacity) out.opacity[p] = x86_d2u16(opacity0 * 65535.0);
+  if (out.alpha) out.alpha[p] = x86_d2u16(alpha0 * 65535.0);
+}
+
 // ---------------------------------------------------------------- launch
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+hipError_t launch_refl_clear(const BandBufs &next, int band_rays, hipStream_t s) {
+  if (band_rays <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_refl_clear, grid_for(band_rays), dim3(kBlock), 0, s, next.active, band_rays);
+  return hipGetLastError();
+}
+
+hipError_t launch_refl_setup(const DevScene *sc, const BandBufs &prev, const BandBufs &next, int band_first,
+                             int formula, int blocks, unsigned long long *stats, hipStream_t s) {
+  const dim3 grid((unsigned)std::max(1, blocks));
+  if (formula_generic(formula))
+    hipLaunchKernelGGL((k_refl_setup<0>), grid, dim3(kBlock), 0, s, sc, prev, next, band_first, stats);
+  else if (formula == 7)
+    hipLaunchKernelGGL((k_refl_setup<7>), grid, dim3(kBlock), 0, s, sc, prev, next, band_first, stats);
+  else if (formula == 8)
+    hipLaunchKernelGGL((k_refl_setup<8>), grid, dim3(kBlock), 0, s, sc, prev, next, band_first, stats);
+  else
+    hipLaunchKernelGGL((k_refl_setup<2>), grid, dim3(kBlock), 0, s, sc, prev, next, band_first, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_refl_march(const DevScene *sc, const BandBufs &next, int band_first, int formula, int blocks,
+                             unsigned long long *stats, hipStream_t s) {
+  const dim3 grid((unsigned)std::max(1, blocks));
+  if (formula_generic(formula))
+    hipLaunchKernelGGL((k_refl_march<0>), grid, dim3(kBlock), 0, s, sc, next, band_first, stats);
+  else if (formula == 7)
+    hipLaunchKernelGGL((k_refl_march<7>), grid, dim3(kBlock), 0, s, sc, next, band_first, stats);
+  else if (formula == 8)
+    hipLaunchKernelGGL((k_refl_march<8>), grid, dim3(kBlock), 0, s, sc, next, band_first, stats);
+  else
+    hipLaunchKernelGGL((k_refl_march<2>), grid, dim3(kBlock), 0, s, sc, next, band_first, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_refl_finish(const DevScene *sc, const BandBufs *levels, int n_levels, int band_first,
+                              int band_rays, FrameOut out, int vis, hipStream_t s) {
+  if (band_rays <= 0) return hipSuccess;
+  if (vis)
+    hipLaunchKernelGGL((k_refl_finish<true>), grid_for(band_rays), dim3(kBlock), 0, s, sc, levels, n_levels,
+                       band_first, band_rays, out);
+  else
+    hipLaunchKernelGGL((k_refl_finish<false>), grid_for(band_rays), dim3(kBlock), 0, s, sc, levels, n_levels,
+                       band_first, band_rays, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                         int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s) {

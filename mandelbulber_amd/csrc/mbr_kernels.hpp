@@ -71,6 +71,12 @@ struct BandBufs {
   // shade output per ray
   void *sh_r, *sh_g, *sh_b;
   uint8_t *col_r, *col_g, *col_b;
+  // raytraced reflections: the specular part of the shade, kept apart
+  // (sh_* then holds the object part); per ray of a reflection level its
+  // origin, direction and initial threshold, and whether it exists
+  void *sp_r, *sp_g, *sp_b;
+  double *ray_o, *ray_d, *ray_t0;
+  uint8_t *active;
 };
 
 struct FrameOut {
@@ -132,6 +138,20 @@ hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first,
 hipError_t launch_finish(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
                          FrameOut out, hipStream_t s, int formula, int fx, int vis, int persist_blocks,
                          unsigned long long *stats, AODev ao);
+
+// Raytraced reflections (MainThread, src/Render3D.cpp:451-560), FP64.
+// launch_refl_setup: for the hits of level `prev` whose surface reflects,
+// the mirrored ray of level `next` (origin, direction, initial threshold)
+// and its entry in next's ray list; launch_refl_march: the FP64 march of
+// next's rays; launch_refl_finish: per pixel, the levels shaded from the
+// last to the first (levels: device array of n_levels BandBufs).
+hipError_t launch_refl_clear(const BandBufs &next, int band_rays, hipStream_t s);
+hipError_t launch_refl_setup(const DevScene *sc, const BandBufs &prev, const BandBufs &next, int band_first,
+                             int formula, int blocks, unsigned long long *stats, hipStream_t s);
+hipError_t launch_refl_march(const DevScene *sc, const BandBufs &next, int band_first, int formula, int blocks,
+                             unsigned long long *stats, hipStream_t s);
+hipError_t launch_refl_finish(const DevScene *sc, const BandBufs *levels, int n_levels, int band_first,
+                              int band_rays, FrameOut out, int vis, hipStream_t s);
 
 // Screen-space ambient occlusion of one compiled frame (mbr_post.hip).
 // cs/sn: device tables of cos/sin(2 pi i / quality), i < quality.

@@ -108,6 +108,11 @@ struct DevScene {
   int aux_slot_on[4];
   double iter_fog_opacity, iter_fog_trim;
   double main_light_intensity;  // mainLightIntensity (the fog's light)
+  // raytraced reflections: bounces (0 = off) and the reflection coefficient
+  // of each object type (OBJ_*: fractal, plane, water, sphere, inverted
+  // sphere, box, inverted box)
+  int refl_max;
+  double refl[7];
 
   // integer colours (sRGB, 16-bit channel values)
   int main_light_colour[3];

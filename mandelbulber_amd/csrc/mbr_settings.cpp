@@ -220,8 +220,9 @@ std::vector<KeyDef> build_table() {
   D("primitive_water_length", OFF(water_length)); I("primitive_water_iterations", OFF(water_iterations));
   D("primitive_water_rotation", OFF(water_rotation)); D("primitive_water_anim_speed", OFF(water_anim_speed));
   RGB("primitive_water_colour_", OFF_RGB(water_colour, R), OFF_RGB(water_colour, G), OFF_RGB(water_colour, B));
-  for (const char *p : {"plane", "box", "invertedBox", "sphere", "invertedSphere", "water"})
-    G(std::string("primitive_") + p + "_reflect");
+  D("primitive_plane_reflect", OFF(plane_reflect)); D("primitive_box_reflect", OFF(box_reflect));
+  D("primitive_invertedBox_reflect", OFF(inv_box_reflect)); D("primitive_sphere_reflect", OFF(sphere_reflect));
+  D("primitive_invertedSphere_reflect", OFF(inv_sphere_reflect)); D("primitive_water_reflect", OFF(water_reflect));
   I("fake_lights_enabled", OFF(fake_lights_enabled));
   for (const char *k : {"fake_lights_min_iter", "fake_lights_max_iter", "fake_lights_visibility",
                         "fake_lights_visibility_size", "fake_lights_intensity", "fake_lights_orbit_trap_X",
@@ -424,6 +425,9 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->dof_enabled = 0;
   s->dof_focus = 21.699999999999999;
   s->dof_radius = 10.0;
+  // primitive reflection coefficients: the reference writes them only when
+  // they differ from 0 (water: 0.7), so a file without them means these
+  s->water_reflect = 0.7;
   // share/defaults: volumetric lights and iteration fog
   for (int i = 0; i < 5; i++) s->volumetric_light_intensity[i] = 100.0;
   s->iter_fog_opacity = 1000.0;

@@ -899,7 +899,8 @@ typedef struct {
 /* RayMarching (src/Render3D.cpp:79-199) with distThreshInit = 0 (primary
  * rays, :444) and binary search enabled. */
 static void ray_march(const fractal_ctx *f, const camera_t *c, v3 start, v3 direction, double maxScan,
-                      const jitter_t *jit, step_rec *stepBuff, march_result *out, de_counter *cnt) {
+                      double thr_init, const jitter_t *jit, step_rec *stepBuff, march_result *out,
+                      de_counter *cnt) {
   const mbr_scene *s = f->s;
   v3 point = v3m(0, 0, 0);
   int found = 0;
@@ -911,7 +912,7 @@ static void ray_march(const fractal_ctx *f, const camera_t *c, v3 start, v3 dire
   int counter = 0;
   double step = s->view_distance_min;
   int buffCount = 0;
-  double distThreshInit = 0.0;
+  double distThreshInit = thr_init;  /* *distThreshOut on entry: 0, or the previous ray's (reflections) */
   double detail = 0.0;
   int object = MBR_OBJ_FRACTAL;
   int fisheye = (s->perspective_type == MBR_PERSP_FISH_EYE || s->perspective_type == MBR_PERSP_EQUIRECTANGULAR);
@@ -1733,7 +1734,7 @@ int oracle_march(const mbr_scene *s, const double start[3], const double dir[3],
   jitter_t j = {s->jitter_mode == MBR_JITTER_HASH ? ORACLE_JITTER_HASH : ORACLE_JITTER_NONE,
                 s->jitter_seed, (uint32_t)s->frame_no, pixel};
   march_result m;
-  ray_march(&f, &c, v3from(start), v3from(dir), s->view_distance_max, &j, buf, &m, NULL);
+  ray_march(&f, &c, v3from(start), v3from(dir), s->view_distance_max, 0.0, &j, buf, &m, NULL);
   out->point[0] = m.point.x; out->point[1] = m.point.y; out->point[2] = m.point.z;
   out->depth = m.depth; out->dist_thresh = m.distThresh; out->last_dist = m.lastDist;
   out->found = m.found; out->step_count = m.buffCount; out->object = m.object; out->de_calls = m.de_calls;
@@ -1849,6 +1850,154 @@ typedef struct {
 } render_job;
 
 /* per-pixel body of MainThread (src/Render3D.cpp:387-590) for ray 0 */
+/* reflection coefficient of the surface a ray ended on (src/Render3D.cpp:471-477) */
+static double object_reflect(const mbr_scene *s, int object) {
+  switch (object) {
+    case MBR_OBJ_FRACTAL: return s->reflect;
+    case MBR_OBJ_WATER: return s->water_reflect;
+    case MBR_OBJ_PLANE: return s->plane_reflect;
+    case MBR_OBJ_BOX: return s->box_reflect;
+    case MBR_OBJ_BOX_INV: return s->inv_box_reflect;
+    case MBR_OBJ_SPHERE: return s->sphere_reflect;
+    case MBR_OBJ_SPHERE_INV: return s->inv_sphere_reflect;
+  }
+  return 0.0;
+}
+
+typedef struct {
+  v3 start, viewVector, point;
+  double distThresh, lastDist, depth, reflect;
+  int found, buffCount, objectType;
+  step_rec *buf;
+} refl_ray;
+
+/* MainThread's pixel with raytraced reflections (src/Render3D.cpp:440-589):
+ * march the primary ray and up to reflections_max mirrored rays (each from
+ * its hit point along the view vector mirrored about the normal, with the
+ * previous threshold as its initial one), then shade them from the last to
+ * the first, each blending the shaded result of the next by its surface's
+ * reflection coefficient, each through its own VolumetricShader. */
+static void render_pixel_reflections(render_job *job, int x, int y, v3 viewVector, int cut, const jitter_t *j,
+                                     step_rec *buf, oracle_stats *st, de_counter *mc, de_counter *sc) {
+  const mbr_scene *s = job->s;
+  const camera_t *c = job->c;
+  const int W = s->image_width;
+  const int maxRay = s->reflections_max;
+  refl_ray rb[MBR_MAX_REFLECTIONS + 1];
+  double distThresh = 0.0;
+  v3 startRay = c->start;
+  v3 point = v3m(0, 0, 0);
+  int rayEnd = 0;
+  for (int ray = 0; ray <= maxRay; ray++) {
+    refl_ray *R = &rb[ray];
+    R->buf = buf + (size_t)ray * 10002;
+    R->start = startRay;
+    R->viewVector = viewVector;
+    R->found = 0;
+    R->buffCount = 0;
+    R->lastDist = 0;
+    R->depth = 0;
+    march_result m;
+    memset(&m, 0, sizeof m);
+    if (!cut) {
+      ray_march(job->f, c, startRay, viewVector, s->view_distance_max, distThresh, j, R->buf, &m, mc);
+      point = m.point;
+      distThresh = m.distThresh;
+    }
+    st->rays++;
+    if (m.found) st->hits++;
+    st->missed_de += m.missed;
+    if ((uint64_t)m.buffCount > st->max_steps) st->max_steps = m.buffCount;
+    st->total_steps += m.buffCount;
+    R->found = m.found;
+    R->buffCount = m.buffCount;
+    R->lastDist = m.lastDist;
+    R->depth = m.depth;
+    R->point = point;
+    R->distThresh = distThresh;
+    R->objectType = m.object;
+    R->reflect = object_reflect(s, m.object);
+    rayEnd = ray;
+    if (!R->found) break;
+    if (R->reflect == 0) break;
+    /* the mirrored ray */
+    shader_in in;
+    memset(&in, 0, sizeof in);
+    in.f = job->f; in.c = c;
+    in.dist_thresh = R->distThresh;
+    in.detail = R->distThresh;
+    in.lightVect = c->light;
+    in.point = point;
+    in.viewVector = viewVector;
+    in.delta = s->constant_DE_threshold ? R->depth * c->resolution * s->persp : R->distThresh * s->quality;
+    in.cnt = sc;
+    v3 vn = calculate_normals(&in);
+    viewVector = v3sub(viewVector, v3mul(v3mul(vn, v3dot(viewVector, vn)), 2.0));
+    startRay = v3add(point, v3mul(viewVector, R->distThresh));
+  }
+
+  shade_t resultShader = {0, 0, 0, 0};
+  shade_t objectColour = {0, 0, 0, 0};
+  double opacity0 = 0.0, alpha0 = 0.0;
+  for (int ray = rayEnd; ray >= 0; ray--) {
+    refl_ray *R = &rb[ray];
+    shade_t objectShader = {0, 0, 0, 0}, backgroundShader = {0, 0, 0, 0}, specular = {0, 0, 0, 0};
+    shader_in in;
+    memset(&in, 0, sizeof in);
+    in.f = job->f; in.c = c;
+    in.dist_thresh = R->distThresh;
+    in.delta = s->constant_DE_threshold ? R->depth * c->resolution * s->persp : R->distThresh * s->quality;
+    in.detail = R->distThresh;
+    in.lightVect = c->light;
+    in.point = R->point;
+    in.viewVector = R->viewVector;
+    in.ao_dirs = job->ao_dirs; in.ao_rgb = job->ao_rgb; in.ao_count = job->ao_count;
+    in.lights = job->lights;
+    in.lastDist = R->lastDist;
+    in.depth = R->depth;
+    in.objectType = R->objectType;
+    in.cnt = sc;
+    const double depth = R->depth;
+    if (R->found) objectShader = object_shader(&in, &objectColour, &specular);
+    else {
+      backgroundShader = background_shader(s, R->viewVector);
+      R->depth = 1e20;
+    }
+    const double reflect = R->reflect;
+    shade_t pixel;
+    if (maxRay > 0 && rayEnd > 0 && ray != rayEnd) {
+      pixel.R = resultShader.R * reflect + (1.0 - reflect) * (objectShader.R + backgroundShader.R) + specular.R;
+      pixel.G = resultShader.G * reflect + (1.0 - reflect) * (objectShader.G + backgroundShader.G) + specular.G;
+      pixel.B = resultShader.B * reflect + (1.0 - reflect) * (objectShader.B + backgroundShader.B) + specular.B;
+    } else {
+      pixel.R = objectShader.R + backgroundShader.R + specular.R;
+      pixel.G = objectShader.G + backgroundShader.G + specular.G;
+      pixel.B = objectShader.B + backgroundShader.B + specular.B;
+    }
+    pixel.alpha = objectShader.alpha + backgroundShader.alpha;
+    double opacity;
+    shade_t vol = volumetric_shader(s, depth, R->buf, R->buffCount, pixel, job->lights, R->viewVector, &in, &opacity);
+    resultShader.R = vol.R;
+    resultShader.G = vol.G;
+    resultShader.B = vol.B;
+    if (ray == 0) {
+      opacity0 = opacity;
+      alpha0 = vol.alpha;
+    }
+  }
+  size_t idx = (size_t)x + (size_t)y * W;
+  mbr_framebuffer *fb = job->fb;
+  if (fb->rgb[0]) fb->rgb[0][idx] = (float)resultShader.R;
+  if (fb->rgb[1]) fb->rgb[1][idx] = (float)resultShader.G;
+  if (fb->rgb[2]) fb->rgb[2][idx] = (float)resultShader.B;
+  if (fb->z) fb->z[idx] = (float)rb[0].depth;
+  if (fb->colour[0]) fb->colour[0][idx] = x86_d2u8(objectColour.R * 255);
+  if (fb->colour[1]) fb->colour[1][idx] = x86_d2u8(objectColour.G * 255);
+  if (fb->colour[2]) fb->colour[2][idx] = x86_d2u8(objectColour.B * 255);
+  if (fb->opacity) fb->opacity[idx] = x86_d2u16(opacity0 * 65535.0);
+  if (fb->alpha) fb->alpha[idx] = x86_d2u16(alpha0 * 65535.0);
+}
+
 static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_stats *st, de_counter *mc, de_counter *sc) {
   const mbr_scene *s = job->s;
   const camera_t *c = job->c;
@@ -1859,7 +2008,11 @@ static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_st
   jitter_t j = {job->jitter, s->jitter_seed, (uint32_t)s->frame_no, pixel};
   march_result m;
   memset(&m, 0, sizeof m);
-  if (!cut) ray_march(job->f, c, c->start, viewVector, s->view_distance_max, &j, buf, &m, mc);
+  if (s->raytraced_reflections && s->reflections_max > 0) {
+    render_pixel_reflections(job, x, y, viewVector, cut, &j, buf, st, mc, sc);
+    return;
+  }
+  if (!cut) ray_march(job->f, c, c->start, viewVector, s->view_distance_max, 0.0, &j, buf, &m, mc);
   st->rays++;
   if (m.found) st->hits++;
   st->missed_de += m.missed;
@@ -1913,7 +2066,8 @@ static void render_pixel(render_job *job, int x, int y, step_rec *buf, oracle_st
 
 static void *render_worker(void *arg) {
   render_job *job = (render_job *)arg;
-  step_rec *buf = (step_rec *)malloc(sizeof(step_rec) * 10002);
+  const int levels = (job->s->raytraced_reflections && job->s->reflections_max > 0) ? job->s->reflections_max + 1 : 1;
+  step_rec *buf = (step_rec *)malloc(sizeof(step_rec) * 10002 * (size_t)levels);
   oracle_stats st; memset(&st, 0, sizeof st);
   de_counter mc = {0, 0}, sc = {0, 0};
   for (;;) {
