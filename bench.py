@@ -33,7 +33,7 @@ PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # the factors that make a 1 GiB device copy read 1 GiB and write 1 GiB),
 # from the rocprofv3 --pmc runs summarised in the file named here.
 AO_TRAFFIC_BYTES = 811e6
-AO_TRAFFIC_SOURCE = "profiles/render_1080p_mandelbulb_pmc.txt (FETCH_SIZE 79.5 MiB x 2 + WRITE_SIZE 614.6 MiB)"
+AO_TRAFFIC_SOURCE = "profiles/render_1080p_mandelbulb_pmc.txt (FETCH_SIZE 79.6 MiB x 2 + WRITE_SIZE 614.5 MiB)"
 
 
 def planes_on_gpu(torch, w, h):
