@@ -867,8 +867,7 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
   ao_body<T, F, Deep, Plain>(scp, b, ao, sb, stats);
 }
 
-// The FP32 plain variant of the Mandelbulb (the benchmarked AO pass) held to
-// 6 waves per SIMD (80 VGPRs, no spills).
+// The FP32 plain variant (the benchmarked AO pass) held to 6 waves per SIMD.
 template <class T, int F>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
     k_ao_f32_plain(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
@@ -1535,12 +1534,13 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
       constexpr bool D = decltype(deep_tag)::value;
       constexpr bool Pl = decltype(plain_tag)::value;
       if constexpr (!D && Pl && std::is_same<T, float>::value) {
-        // the Mandelbulb fits 6 waves per SIMD without spills; the Menger and
-        // Mandelbox iterations need more registers and keep the general variant
-        if (formula != 7 && formula != 8) {
+        if (formula == 7)
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        else if (formula == 8)
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        else
           hipLaunchKernelGGL((k_ao_f32_plain<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
-          return;
-        }
+        return;
       }
       if constexpr (!D && !Pl && std::is_same<T, double>::value) {
         if (generic) {
