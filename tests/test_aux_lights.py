@@ -293,3 +293,40 @@ def test_iteration_fog_takes_its_colour():
     assert np.all(d[1] <= 1e-7) and np.all(d[2] <= 1e-7)
     assert d[0].max() > 1e-3
     assert np.all(fog.alpha >= clear.alpha)
+
+
+def test_penetrating_aux_light_keeps_the_path_fraction_before_the_hit():
+    """With penetrating lights an aux shadow ray that hits an occluder after
+    a length i of its path of length d keeps 1 - (d - i)/d = i/d of the light
+    (src/shaders.cpp:1116-1120).  A point beside the bulb (its normal points
+    away from the bulb, towards the light) lights from a light whose path
+    crosses a small sphere primitive: the aux term with the shadow on, over
+    the term with shadows off, equals the distance to the sphere along the
+    ray over the distance to the light."""
+    s = scene.mandelbulb_plain(16, 16)
+    s.N = 20
+    s.main_light_intensity = 0.0
+    s.specular = 0.0
+    s.aux_light_number = 0
+    s.aux_light_intensity = 1.0  # the shade must pass the 0.01 that sends it to the shadow march
+    s.aux_light_pre_enabled[0] = 1
+    s.aux_light_pre_intensity[0] = 1.0
+    s.penetrating_lights = 1
+    p = np.array([2.5, 0.0, 0.0])
+    u = np.array([0.6, 0.8, 0.0])
+    light = p + 5.0 * u
+    s.aux_light_pre[0][:] = tuple(light)
+    s.sphere_enabled = 1
+    s.sphere_centre[:] = tuple(p + 2.5 * u)
+    s.sphere_radius = 0.3
+    view = (-1.0, 0.2, 0.1)
+    thr = 1e-3
+    s.shadow = 1
+    on = _shade(s, tuple(p), view, thr, 2.5)
+    s.shadow = 0
+    off = _shade(s, tuple(p), view, thr, 2.5)
+    assert off[0] > 1e-6
+    fraction = on[0] / off[0]
+    # the shadow march stops within its last DE step (unit steps here are the
+    # full distance estimate) of the sphere's surface at i = 2.5 - 0.3
+    assert fraction == pytest.approx(2.2 / 5.0, rel=0.01)
