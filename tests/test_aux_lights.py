@@ -330,3 +330,30 @@ def test_penetrating_aux_light_keeps_the_path_fraction_before_the_hit():
     # the shadow march stops within its last DE step (unit steps here are the
     # full distance estimate) of the sphere's surface at i = 2.5 - 0.3
     assert fraction == pytest.approx(2.2 / 5.0, rel=0.01)
+
+
+def test_light_shading_caps_diffuse_at_500_and_specular_at_15():
+    """LightShading clamps the diffuse term at 500 and the specular one at
+    15 (src/shaders.cpp:1013, :1022): a light straight along the normal,
+    seen head-on, at an enormous intensity gives a specular of exactly
+    15 x colour / 65536 and a diffuse of 500 x colour / 65536 x the
+    surface colour."""
+    s = scene.mandelbulb_plain(16, 16)
+    s.N = 20
+    s.shadow = 0
+    s.global_illumination = 0
+    s.main_light_intensity = 0.0
+    s.specular = 1.0
+    s.aux_light_number = 0
+    s.aux_light_pre_enabled[0] = 1
+    s.aux_light_pre_intensity[0] = 1.0
+    s.aux_light_intensity = 1e9
+    s.aux_light_pre_colour[0].R, s.aux_light_pre_colour[0].G, s.aux_light_pre_colour[0].B = 65536, 32768, 16384
+    point, thr, depth = _surface_point(s)
+    o = oracle.object_shader(s, point, (0.0, 1.0, 0.0), thr, depth)
+    n = np.array(o.normal)
+    s.aux_light_pre[0][:] = tuple(np.array(point) + 0.5 * n)
+    o = oracle.object_shader(s, point, tuple(-n), thr, depth)
+    col = np.array([65536, 32768, 16384]) / 65536.0
+    np.testing.assert_allclose(np.array(o.spec_rgb), 15.0 * col, rtol=1e-12)
+    np.testing.assert_allclose(np.array(o.rgb), 500.0 * col * np.array(o.colour), rtol=1e-12)
