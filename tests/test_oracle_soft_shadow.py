@@ -57,3 +57,21 @@ def test_penumbra_darkens_with_the_cone(bulb):
     assert all(0.0 <= v <= 1.0 for v in vals)
     assert all(a >= b for a, b in zip(vals, vals[1:])), vals
     assert vals[-1] < vals[0]
+
+
+def test_penetrating_main_light_keeps_the_path_fraction(bulb):
+    """With penetrating lights MainShadow marches to factor = delta /
+    resolution and a hit after a length i keeps 1 - (factor - i)/factor
+    (src/shaders.cpp:487-488, :548-553).  Beyond the bulb along the light, a
+    sphere across the path at i = 2.2: threshold 0.5, quality 1 and a
+    16-pixel width give factor 0.5 x 16 = 8, so the shadow is 2.2/8."""
+    s, L = bulb
+    s = scene.copy(s)
+    s.shadow_cone_angle = 0.0
+    s.penetrating_lights = 1
+    p = 3.0 * L
+    s.sphere_enabled = 1
+    s.sphere_centre[:] = tuple(p + 2.5 * L)
+    s.sphere_radius = 0.3
+    assert s.image_width == 16 and s.quality == 1.0
+    assert _shadow(s, p, thr=0.5) == pytest.approx(2.2 / 8.0, rel=1e-9)
