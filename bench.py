@@ -146,11 +146,14 @@ def bench_adjoint(torch, r, scene, n_scenes, size):
     camera and light) at size x size; MSE against the image at power 7.3;
     dL/d(power, camera, light) from the adjoint kernel through the
     torch.autograd.Function, timed, and checked against central
-    differences (h = 1e-4) of the FP64 forward on every scene."""
+    differences (h = 1e-4) of the FP64 forward on every scene
+    (diff.check_gradient: per parameter over the pixels where the image is
+    smooth within +-h; fd_r and vjp_r with the h^2 term removed)."""
+    import numpy as np
     from mandelbulber_amd import diff
     fwd_ms, bwd_ms = [], []
-    steps = (1e-4, 1e-5)
-    errs = {h: {"rel": [], "abs": [], "left": []} for h in steps}
+    rel = {"fd_r": [], "vjp_r": [], "fd_h": []}
+    left = []
     for i in range(n_scenes):
         base = scene.differentiable(size, size, seed=100 + i)
         theta = torch.tensor(scene.get_params(base), dtype=torch.float64, requires_grad=True)
@@ -169,21 +172,19 @@ def bench_adjoint(torch, r, scene, n_scenes, size):
         t2 = time.perf_counter()
         fwd_ms.append((t1 - t0) * 1e3)
         bwd_ms.append((t2 - t1) * 1e3)
-        for h in steps:
-            adj, fd, out, region = diff.check_gradient(r, base, h=h)
-            scale = max(abs(fd).max(), 1e-300)
-            err = abs(adj - fd)
-            big = abs(fd) > 0.05 * scale
-            errs[h]["abs"].append(float((err / scale).max()))
-            errs[h]["rel"].append(float((err[big] / abs(fd[big])).max()) if big.any() else 0.0)
-            errs[h]["left"].append(out / max(region, 1))
-    import statistics
-    fd_check = {f"h={h:g}": {"median_rel_err_large_components": statistics.median(errs[h]["rel"]),
-                              "max_rel_err_large_components": max(errs[h]["rel"]),
-                              "median_err_over_largest_component": statistics.median(errs[h]["abs"]),
-                              "max_err_over_largest_component": max(errs[h]["abs"]),
-                              "mean_hit_pixels_left_out": sum(errs[h]["left"]) / n_scenes}
-                for h in steps}
+        chk = diff.check_gradient(r, base, h=1e-4)
+        rel["fd_r"].append(diff.compare(chk.adj, chk.fd_r)[0])
+        rel["vjp_r"].append(diff.compare(chk.adj, chk.vjp_r)[0])
+        rel["fd_h"].append(diff.compare(chk.adj, chk.fd)[0])
+        left.append(chk.left / max(chk.region_px, 1))
+    left = np.array(left)
+    fd_check = {"h": 1e-4, "params": list(scene.DIFF_PARAMS),
+                "hit_pixels_left_out_mean_per_param": [round(float(v), 4) for v in left.mean(axis=0)],
+                "hit_pixels_left_out_mean": float(left.mean())}
+    for key, v in rel.items():
+        v = np.array(v)
+        fd_check[key] = {"max_rel_err": float(v.max()), "median_rel_err": float(np.median(v)),
+                         "components_above_1e-3": int((v > 1e-3).sum()), "components": int(v.size)}
     rays = size * size
     return {
         "workload": f"{n_scenes} Mandelbulb scenes {size}x{size}, power U[6,10], seeded camera and light, "

@@ -449,8 +449,10 @@ int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
  * primitives, limits, interior or constant-threshold mode (their
  * contributions are steps in theta); the surface colour index counts as a
  * constant.  Hit pixels whose ray meets the surface within about 3 degrees
- * of grazing (where the depth derivative diverges), or whose colour tangent
- * exceeds 1e5 per unit parameter (a degenerate normal), contribute nothing; when
+ * of grazing (where the depth derivative diverges), whose refined hit is not
+ * a root of the band-centre equation (the distance estimate jumps across
+ * the band there), or whose colour tangent exceeds 1e5 per unit parameter
+ * (a degenerate normal), contribute nothing; when
  * `skipped` (a DEVICE uint8 plane of the image's size, may be NULL) is
  * given, it is set to 1 on those pixels and 0 on the others the call
  * visits (pixels whose adjoint is zero are not visited).  Rays are marched
@@ -458,6 +460,17 @@ int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
 int mbr_adjoint(mbr_ctx *ctx, const mbr_render_opts *opts,
                 const float *const grad_rgb[3], double grad[9], uint8_t *skipped,
                 mbr_stats *stats);
+
+/* The tangents behind mbr_adjoint, pixel by pixel (no counterpart in the
+ * reference): for every pixel of the current scene's FP64 image, jac (a
+ * DEVICE float array of 27 * width * height) receives dC_ch/dtheta_i at
+ * index (i * 3 + ch) * width * height + pixel, theta as for mbr_adjoint and
+ * ch = R, G, B.  Pixels that mbr_adjoint leaves out get zeros (and 1 in
+ * `skipped` when given).  mbr_adjoint's result equals the sum over pixels
+ * and channels of grad_rgb times these tangents; this form serves
+ * pixel-level gradient checks and Jacobian-vector products. */
+int mbr_adjoint_jacobian(mbr_ctx *ctx, const mbr_render_opts *opts, float *jac,
+                         uint8_t *skipped, mbr_stats *stats);
 
 /* Batched rendering of many frames (animation frames, or unrelated scenes of
  * mixed formulas and sizes) as ONE launch of a HIP graph: the frames' band

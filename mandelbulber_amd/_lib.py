@@ -17,7 +17,7 @@ EXPORTS = (
     "mbr_destroy", "mbr_last_error", "mbr_set_scene", "mbr_render", "mbr_render_device",
     "mbr_scene_layout", "mbr_jitter_draw", "mbr_render_batch", "mbr_adjoint", "mbr_compile_image",
     "mbr_write_png", "mbr_post_ssao", "mbr_post_dof", "mbr_compile_image_device", "mbr_keyframe_scene",
-    "mbr_place_lights", "mbr_set_lights", "mbr_get_lights",
+    "mbr_place_lights", "mbr_set_lights", "mbr_get_lights", "mbr_adjoint_jacobian",
 )
 
 _lib = None
@@ -49,6 +49,7 @@ def _declare(lib):
     lib.mbr_jitter_draw.restype = C.c_uint32
     lib.mbr_adjoint.argtypes = [C.c_void_p, P(_abi.RenderOpts), C.c_void_p * 3, C.c_double * 9, C.c_void_p,
                                 P(_abi.Stats)]
+    lib.mbr_adjoint_jacobian.argtypes = [C.c_void_p, P(_abi.RenderOpts), C.c_void_p, C.c_void_p, P(_abi.Stats)]
     lib.mbr_compile_image.argtypes = [P(_abi.Scene), C.c_int, C.c_int, C.c_void_p * 3, C.c_void_p]
     lib.mbr_write_png.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
     lib.mbr_post_ssao.argtypes = [C.c_void_p, P(_abi.Scene), P(_abi.PostPlanes), C.c_int, C.c_void_p]

@@ -40,6 +40,9 @@ namespace mbr {
 constexpr int kAdjBlock = 256;
 constexpr int kParams = 9;
 constexpr double kTangentMax = 1e5;
+// residual of a refined hit, relative to its threshold, below which it
+// counts as a root of F (the refinement stops at 1e-13)
+constexpr double kRootTol = 1e-7;
 
 // Value and gradient in (p.x, p.y, p.z, power).
 struct G4 {
@@ -178,7 +181,7 @@ template <class RT>
 __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                        int band_rays, AdjParams P, const float *__restrict__ gR,
                                                        const float *__restrict__ gG, const float *__restrict__ gB,
-                                                       double *grad, uint8_t *skipped) {
+                                                       double *grad, uint8_t *skipped, float *__restrict__ jac) {
   const DevScene &sc = *scp;
   const ShadeK<double> &k = sc.sd;
   const FractalK<double> &f = sc.fd;
@@ -209,8 +212,11 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
     const int r = (int)rr;
     const int p = band_first + r;
     const int x = p % sc.width, y = p / sc.width;
-    const double aR = gR[p], aG = gG[p], aB = gB[p];
-    if (aR == 0.0 && aG == 0.0 && aB == 0.0) continue;
+    const size_t npix = (size_t)sc.width * sc.height;
+    const double aR = jac ? 0.0 : gR[p], aG = jac ? 0.0 : gG[p], aB = jac ? 0.0 : gB[p];
+    if (!jac && aR == 0.0 && aG == 0.0 && aB == 0.0) continue;
+    if (jac)
+      for (int i = 0; i < 3 * kParams; i++) jac[(size_t)i * npix + p] = 0.0f;
     const double x2 = ((double)x / sc.width - 0.5) * sc.aspect;
     const double z2 = (double)y / sc.height - 0.5;
     const V3<double> v0 = v3<double>(x2 * sc.fov, 1.0, z2 * sc.fov);
@@ -254,7 +260,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
       // tangency Ft -> 0 and dt/dtheta diverges: the refined hit then sits
       // next to a fold of t(theta), where the forward jumps for a small
       // change of theta; such pixels are left out (and flagged).
-      if (!(Ft < -0.05 * length(gD) * length(d))) {
+      // The implicit-function derivative holds at a root of F.  Where the
+      // distance estimate jumps across the binary search's band (its
+      // spherical coordinates are singular on the z axis and its preimages)
+      // the refinement stops short of a root, and the depth then follows
+      // the Newton iterates instead: such hits are left out (and flagged).
+      const bool at_root = fabs(D.v - c0 * thr) <= kRootTol * thr;
+      if (!(Ft < -0.05 * length(gD) * length(d)) || !at_root) {
         if (skipped) skipped[p] = 1;
         continue;
       }
@@ -324,6 +336,11 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
       if (skipped) skipped[p] = 1;
       continue;
     }
+    if (jac) {
+      for (int i = 0; i < kParams; i++)
+        for (int ch = 0; ch < 3; ch++) jac[(size_t)(i * 3 + ch) * npix + p] = (float)dC[ch][i];
+      continue;
+    }
     for (int i = 0; i < kParams; i++) acc[i] += aR * dC[0][i] + aG * dC[1][i] + aB * dC[2][i];
   }
   // block reduction, one atomic per parameter and workgroup
@@ -335,7 +352,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
     if (lane == 0) red[i][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x < kParams) {
+  if (grad && threadIdx.x < kParams) {
     double v = 0;
     for (int w = 0; w < kAdjBlock / 64; w++) v += red[threadIdx.x][w];
     atomicAdd(&grad[threadIdx.x], v);
@@ -344,14 +361,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(const DevScene *__restric
 
 hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
                           const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
-                          int blocks, hipStream_t s) {
+                          float *jac, int blocks, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
   if (b.rec_bytes == 8)
     hipLaunchKernelGGL((k_adjoint<double>), dim3(std::max(1, blocks)), dim3(kAdjBlock), 0, s, sc, b, band_first,
-                       band_rays, p, gR, gG, gB, grad, skipped);
+                       band_rays, p, gR, gG, gB, grad, skipped, jac);
   else
     hipLaunchKernelGGL((k_adjoint<float>), dim3(std::max(1, blocks)), dim3(kAdjBlock), 0, s, sc, b, band_first,
-                       band_rays, p, gR, gG, gB, grad, skipped);
+                       band_rays, p, gR, gG, gB, grad, skipped, jac);
   return hipGetLastError();
 }
 

@@ -1297,7 +1297,7 @@ int mbr_compile_image_device(mbr_ctx *c, const mbr_scene *s, int w, int h, const
   CompileParams p;
   p.n = w * h;
   p.hdr = s->hdr;
-  p.brightness = (float)s->brightness;
+  p.brightness = s->brightness;
   p.contrast = s->contrast;
   HIPCHECK(c, launch_compile(p, rgb[0], rgb[1], rgb[2], c->d_gamma, image16, st));
   return MBR_OK;
@@ -1766,11 +1766,13 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
 
 }  // extern "C"
 
-extern "C" {
+namespace {
 
-int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad_rgb[3], double grad[9],
-                uint8_t *skipped, mbr_stats *stats) {
-  if (!c || !grad_rgb || !grad || !grad_rgb[0] || !grad_rgb[1] || !grad_rgb[2]) return MBR_ERR_INVALID_ARG;
+// The adjoint pass of mbr_adjoint (grad_rgb, grad) and mbr_adjoint_jacobian
+// (jac): the FP64 march of the current scene with refined hits, then the
+// adjoint kernel per band.
+int run_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad_rgb[3], double grad[9],
+                uint8_t *skipped, float *jac, mbr_stats *stats) {
   if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
   const mbr_scene &s = c->scene;
   const DevScene &h = c->host_sc;
@@ -1801,6 +1803,8 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
   p.zoom_persp = s.zoom / s.persp;
   const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
   const long long last = (long long)s.image_width * s.image_height;
+  const float *gR = grad_rgb ? grad_rgb[0] : nullptr, *gG = grad_rgb ? grad_rgb[1] : nullptr,
+              *gB = grad_rgb ? grad_rgb[2] : nullptr;
   unsigned long long v[ST_COUNT];
   for (int attempt = 0;; attempt++) {
     HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, st));
@@ -1812,8 +1816,8 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
                                c->host_sc.plain, persist, c->d_stats, st));
       HIPCHECK(c, launch_march_fix(MBR_PREC_FP64, c->d_sc, sl.band, (int)band_first, s.formula, persist / 2,
                                    c->d_stats, st));
-      HIPCHECK(c, launch_adjoint(c->d_sc, sl.band, (int)band_first, band_rays, p, grad_rgb[0], grad_rgb[1],
-                                 grad_rgb[2], c->d_grad, skipped, persist, st));
+      HIPCHECK(c, launch_adjoint(c->d_sc, sl.band, (int)band_first, band_rays, p, gR, gG, gB,
+                                 jac ? nullptr : c->d_grad, skipped, jac, persist, st));
     }
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
@@ -1822,9 +1826,24 @@ int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
     rc = grow_pool(c, sl, v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);
     if (rc != MBR_OK) return rc;
   }
-  HIPCHECK(c, hipMemcpy(grad, c->d_grad, 9 * sizeof(double), hipMemcpyDeviceToHost));
+  if (grad) HIPCHECK(c, hipMemcpy(grad, c->d_grad, 9 * sizeof(double), hipMemcpyDeviceToHost));
   if (stats) fill_stats(stats, v, last, (int)((last + sl.band.cap - 1) / sl.band.cap));
   return MBR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbr_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad_rgb[3], double grad[9],
+                uint8_t *skipped, mbr_stats *stats) {
+  if (!c || !grad_rgb || !grad || !grad_rgb[0] || !grad_rgb[1] || !grad_rgb[2]) return MBR_ERR_INVALID_ARG;
+  return run_adjoint(c, opts, grad_rgb, grad, skipped, nullptr, stats);
+}
+
+int mbr_adjoint_jacobian(mbr_ctx *c, const mbr_render_opts *opts, float *jac, uint8_t *skipped, mbr_stats *stats) {
+  if (!c || !jac) return MBR_ERR_INVALID_ARG;
+  return run_adjoint(c, opts, nullptr, nullptr, skipped, jac, stats);
 }
 
 }  // extern "C"

@@ -53,7 +53,9 @@ int mbr_compile_image(const mbr_scene *s, int width, int height, const float *co
   for (size_t i = 0; i < n; i++) {
     float c[3];
     for (int k = 0; k < 3; k++) {
-      float v = rgb[k][i] * (float)s->brightness;
+      // float channel times the double brightness, rounded back to float
+      // (sImageAdjustments::brightness is a double, src/cimage.hpp:54)
+      float v = (float)(rgb[k][i] * s->brightness);
       v = (float)((v - 0.5) * s->contrast + 0.5);
       if (v < 0.0f) v = 0.0f;
       if (s->hdr) v = std::tanh(v);

@@ -127,10 +127,12 @@ struct AdjParams {
   double zoom_persp;  // the ray start sits zoom/persp behind vp
 };
 // dL/dtheta of the band's pixels added into grad[9] (device doubles); g* are
-// the image adjoint planes dL/dC (full image, row-major).
+// the image adjoint planes dL/dC (full image, row-major).  With jac (27
+// planes of the full image, plane i*3+ch = dC_ch/dtheta_i) every pixel is
+// visited and its tangents written; g* and grad may then be null.
 hipError_t launch_adjoint(const DevScene *sc, const BandBufs &b, int band_first, int band_rays, AdjParams p,
                           const float *gR, const float *gG, const float *gB, double *grad, uint8_t *skipped,
-                          int blocks, hipStream_t s);
+                          float *jac, int blocks, hipStream_t s);
 // Background, VolumetricShader and framebuffer planes.  fx: the scene has
 // volumetric lights or iteration fog, whose shadow marches (and AO rays)
 // run in a grid-stride kernel of persist_blocks workgroups templated on the
@@ -166,8 +168,7 @@ hipError_t launch_ssao(const SsaoParams &p, const float *z, const uint16_t *opac
 // CompileImage on the device (mbr_post.hip): gamma = 65536-entry table.
 struct CompileParams {
   int n, hdr;
-  float brightness;
-  double contrast;
+  double brightness, contrast;
 };
 hipError_t launch_compile(const CompileParams &p, const float *r, const float *g, const float *b,
                           const uint16_t *gamma, uint16_t *image16, hipStream_t s);

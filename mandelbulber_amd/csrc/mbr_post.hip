@@ -311,7 +311,8 @@ hipError_t dof_fold(const DofParams &p, const float *z, const unsigned long long
 }
 
 // CompileImage (src/cimage.cpp:132-213), one lane per pixel, in the order of
-// mbr_compile_image: FP32 brightness, FP64 contrast, clamp, tanh, clamp,
+// mbr_compile_image: FP64 brightness and contrast (each rounded to FP32),
+// clamp, tanh, clamp,
 // 16-bit index into the gamma table.
 __global__ void __launch_bounds__(256) k_compile(CompileParams p, const float *__restrict__ r,
                                                  const float *__restrict__ g, const float *__restrict__ b,
@@ -321,7 +322,7 @@ __global__ void __launch_bounds__(256) k_compile(CompileParams p, const float *_
   if (i >= p.n) return;
   const float in[3] = {r[i], g[i], b[i]};
   for (int k = 0; k < 3; k++) {
-    float v = in[k] * p.brightness;
+    float v = (float)(in[k] * p.brightness);
     v = (float)((v - 0.5) * p.contrast + 0.5);
     if (v < 0.0f) v = 0.0f;
     if (p.hdr) v = tanhf(v);

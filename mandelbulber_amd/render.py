@@ -150,6 +150,22 @@ class Renderer:
                                 C.byref(st)), self._ctx)
         return list(out), st
 
+    def adjoint_jacobian(self, jac, stream=None, skipped=None):
+        """Per-pixel tangents dC/dtheta of the current scene into jac, a
+        contiguous (9, 3, H, W) float32 device tensor (theta as for
+        adjoint); skipped as for adjoint."""
+        s = self.scene
+        assert tuple(jac.shape) == (9, 3, s.image_height, s.image_width) and jac.is_contiguous()
+        assert str(jac.dtype) == "torch.float32"
+        st = _abi.Stats()
+        sk = None
+        if skipped is not None:
+            assert tuple(skipped.shape) == (s.image_height, s.image_width) and skipped.is_contiguous()
+            sk = skipped.data_ptr()
+        check(lib().mbr_adjoint_jacobian(self._ctx, C.byref(self._opts(_abi.PREC_FP64, None, stream)),
+                                         jac.data_ptr(), sk, C.byref(st)), self._ctx)
+        return jac, st
+
     def post_ssao(self, scene, z, opacity, colour, image16, stream=None):
         """PostRendering_SSAO on one frame.  numpy planes: host call, image16
         (H, W, 3) uint16 is updated in place and returned; torch CUDA tensors:

@@ -16,10 +16,11 @@ CLI = os.path.join(os.path.dirname(HERE), "mandelbulber_amd", "mbr_nogui")
 
 
 def reference_compile(s, rgb):
-    """CalculatePixel restated in numpy: float32 channel arithmetic with the
-    double constants of the source, 16-bit truncation, gamma table."""
+    """CalculatePixel restated in numpy: float32 channels times the double
+    brightness and contrast of sImageAdjustments (each product in float64,
+    stored back to float32), 16-bit truncation, gamma table."""
     gamma = (np.power(np.arange(65536) / 65536.0, 1.0 / s.image_gamma) * 65535).astype(np.int64).astype(np.uint16)
-    v = (rgb.astype(np.float32) * np.float32(s.brightness)).astype(np.float32)
+    v = (rgb.astype(np.float32).astype(np.float64) * float(s.brightness)).astype(np.float32)
     v = ((v.astype(np.float64) - 0.5) * s.contrast + 0.5).astype(np.float32)
     v = np.maximum(v, np.float32(0))
     if s.hdr:
@@ -40,6 +41,35 @@ def test_compile_image_matches_restatement(adj):
     got = compile_image(s, rgb)
     ref = reference_compile(s, rgb)
     assert np.array_equal(got, ref)
+
+
+def _float_product_compile(s, rgb):
+    """The same tone map with the brightness product taken in float32 (the
+    slip the test below rules out)."""
+    s2 = scene.copy(s)
+    v = (rgb.astype(np.float32) * np.float32(s.brightness)).astype(np.float32)
+    s2.brightness = 1.0
+    return reference_compile(s2, v)
+
+
+@pytest.mark.parametrize("brightness", [0.6, 1.7])
+def test_brightness_product_is_taken_in_double(brightness):
+    """sImageAdjustments::brightness is a double (src/cimage.hpp:54), so
+    `float R = pixel.R * adj.brightness` (src/cimage.cpp:135) multiplies in
+    double and rounds once to float.  On inputs where that rounding and the
+    float32 product's land on different 16-bit codes, the library must give
+    the double product's codes."""
+    s = scene.defaults()
+    scene.set(s, "brightness", brightness)
+    rng = np.random.default_rng(5)
+    rgb = rng.uniform(0.0, 1.0 / brightness, size=(3, 200, 500)).astype(np.float32)
+    want = reference_compile(s, rgb)
+    slip = _float_product_compile(s, rgb)
+    differ = np.any(want != slip, axis=-1)
+    assert differ.sum() >= 10  # the two roundings do part ways here
+    got = compile_image(s, rgb)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got[differ], slip[differ])
 
 
 def test_compile_image_identity_adjustments_and_hdr():
