@@ -631,6 +631,10 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     sb.col[1 * n + tid] = colour.G;
     sb.col[2 * n + tid] = colour.B;
     if (sc.ao && sc.fast_ao) sb.fast_ao[tid] = fast_ao2<T, F, Deep>(sc, in, cnt);
+    // AmbientOcclusion estimates with the detail size calcParam holds when
+    // it runs: MainShadow's last threshold, float-rounded (src/shaders.cpp:
+    // 510-522), when there are shadows; the slow AO pass reads it from here
+    if (sc.ao && !sc.fast_ao) sb.fast_ao[tid] = in.detail;
     if (sc.n_lights > 0) {
       // k_aux shades the aux lights from the normal and the detail size
       // MainShadow left (the aux shadows inherit it)
@@ -804,7 +808,7 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
             if constexpr (Deep) deep = (double)in.detail < 2.0 * kMixedSwitch * length(P64);
             start_dist = in.delta;
             end_dist = in.delta / k.resolution;
-            detail = in.detail;
+            detail = sb.fast_ao[h];  // the detail size k_surface left (after MainShadow)
             thr_f = (T)(float)in.dist_thresh;  // float dist_thresh (src/shaders.cpp:642)
           }
           item = (unsigned long long)h * count + d;

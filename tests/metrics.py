@@ -27,8 +27,10 @@ def compare(a, b):
         rel_same = rel[same]
         depth_max = float(rel_same.max()) if rel_same.size else 0.0
         depth_p999 = float(np.quantile(rel_same, 0.999)) if rel_same.size else 0.0
+        depth_bad = int(np.count_nonzero(rel_same > 1e-4))
     else:
         depth_max = depth_p999 = 0.0
+        depth_bad = 0
     ra, rb = a.rgb.astype(np.float64), b.rgb.astype(np.float64)
     d = ra - rb
     # NaN colours (the reference's NaN orbits, e.g. mandelbulb2 on an axis):
@@ -42,11 +44,43 @@ def compare(a, b):
     d[:, nan_px] = 0.0
     mse = float(np.mean(d * d))
     psnr = float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
-    return {
+    out = {
         "pixels": n, "hits": int(hit_b.sum()), "flips": flips, "flip_frac": flips / n,
-        "depth_rel_max": depth_max, "depth_rel_p999": depth_p999, "psnr_db": psnr,
+        "depth_rel_max": depth_max, "depth_rel_p999": depth_p999, "depth_bad_px": depth_bad, "psnr_db": psnr,
         "rgb_max_abs": float(np.abs(d).max()), "nan_mismatch_px": int(np.count_nonzero(nan_px)),
     }
+    out.update(integer_planes(a, b))
+    return out
+
+
+def integer_planes(a, b):
+    """The integer planes of the framebuffer (colourBuffer u8 x 3, alphaBuffer
+    and opacityBuffer u16, src/cimage.hpp:157-162): pixels whose values
+    differ and the largest difference, per plane."""
+    out = {}
+    for name in ("colour", "alpha", "opacity"):
+        pa, pb = getattr(a, name, None), getattr(b, name, None)
+        if pa is None or pb is None:
+            continue
+        da = np.abs(pa.astype(np.int64) - pb.astype(np.int64))
+        if da.ndim == 3:
+            da = da.max(axis=0)
+        out[f"{name}_diff_px"] = int(np.count_nonzero(da))
+        out[f"{name}_max_diff"] = int(da.max()) if da.size else 0
+    return out
+
+
+def assert_planes_exact(m, label=""):
+    """The FP64 path's integer planes equal the oracle's bit for bit."""
+    for name in ("colour", "alpha", "opacity"):
+        assert m.get(f"{name}_diff_px", 0) == 0, f"{label}: {name} plane differs {m}"
+
+
+def assert_planes_close(m, frac=1e-3, label=""):
+    """FP32 modes: the integer planes differ on at most `frac` of the
+    pixels (a colour index or an opacity sum crossing a rounding boundary)."""
+    for name in ("colour", "alpha", "opacity"):
+        assert m.get(f"{name}_diff_px", 0) <= frac * m["pixels"], f"{label}: {name} plane differs {m}"
 
 
 def assert_parity(m, flip_frac=1e-3, depth_rel=1e-4, psnr_db=45.0, label=""):

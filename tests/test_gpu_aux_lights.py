@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +59,7 @@ def test_aux_lights_match_oracle(renderer, kind):
         if prec == _abi.PREC_FP64:
             # the FP64 frame: the same hits and depths as the oracle's
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+            assert_planes_exact(m)
     # the lights change the picture
     dark = scene.copy(sc)
     dark.aux_light_number = 0
@@ -104,6 +105,7 @@ def test_volumetric_lights_match_oracle(renderer, kind):
         assert_parity(m, label=f"volumetric {kind} {name}")
         if prec == _abi.PREC_FP64:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+            assert_planes_exact(m)
 
 
 def _fog_scene(kind):
@@ -143,6 +145,7 @@ def test_iteration_fog_matches_oracle(renderer, kind):
         assert_parity(m, label=f"iteration fog {kind} {name}")
         if prec == _abi.PREC_FP64:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+            assert_planes_exact(m)
     # the fog shows
     clear = scene.copy(sc)
     clear.iter_fog_opacity = 0.0

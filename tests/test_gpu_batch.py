@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -89,10 +89,12 @@ def test_batch_fp64_matches_oracle(renderer):
         f = Frame(s.image_width, s.image_height)
         h = _host(planes[i])
         f.rgb[:], f.z[:] = h["rgb"], h["z"]
+        f.colour[:], f.alpha[:], f.opacity[:] = h["colour"], h["alpha"].view(np.uint16), h["opacity"].view(np.uint16)
         m = compare(f, ref)
         print(f"  frame {i} formula {s.formula}: {m}", flush=True)
         assert_parity(m, label=f"batch frame {i}")
         assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12
+        assert_planes_exact(m)
 
 
 @pytest.mark.skip(reason="segfaulted once on the host inside mbr_render_batch while timing the launch shapes "

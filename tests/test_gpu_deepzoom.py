@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -42,3 +42,4 @@ def test_deep_zoom_mandelbox(renderer, seed):
         assert m["flips"] == 0 and m["depth_rel_max"] <= 3e-7
         if prec == _abi.PREC_FP64:
             assert m["psnr_db"] >= 120
+            assert_planes_exact(m)

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,7 @@ def test_generic_formula_matches_oracle(renderer, formula, power, name, shaded):
     if formula in (1, 17):
         # trig_DE and smoothMandelbox are analytic: no finite differences, the FP64 path reproduces the oracle
         assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+        assert_planes_exact(m)
     for prec, pname in [(_abi.PREC_FP32_FIXUP, "fp32"), (_abi.PREC_FP32_MARCH, "fp32-march")]:
         print(f"[formulas] {name} shaded={shaded} {pname} (rendered in FP64)", flush=True)
         f, _ = renderer.render(precision=prec)

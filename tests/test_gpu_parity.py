@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,7 @@ def _check(renderer, oracle_mod, sc, label, fp64_exact=True):
         assert_parity(m, label=f"{label} {name}")
         if prec == _abi.PREC_FP64 and fp64_exact:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12 and m["psnr_db"] >= 120, m
+            assert_planes_exact(m)
         if prec == _abi.PREC_FP32_FIXUP:
             # the march is the FP64 march: same hits, depths from the same
             # arithmetic (stored as float)

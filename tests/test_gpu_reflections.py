@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +83,7 @@ def test_reflections_match_oracle(renderer, kind):
         if prec == _abi.PREC_FP64:
             if kind != "hybrid":
                 assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+                assert_planes_exact(m)
             f64 = f
         else:
             assert np.array_equal(f.rgb, f64.rgb) and np.array_equal(f.z, f64.z)

@@ -4,7 +4,7 @@ and penetrating lights."""
 import pytest
 
 from mandelbulber_amd import _abi, scene
-from metrics import assert_parity, compare
+from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
 
@@ -38,3 +38,4 @@ def test_soft_shadow_matches_oracle(renderer, penetrating):
         assert_parity(m, label=f"soft shadow {name}")
         if prec == _abi.PREC_FP64:
             assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+            assert_planes_exact(m)
