@@ -439,6 +439,25 @@ int mbr_render(mbr_ctx *ctx, const mbr_render_opts *opts,
 int mbr_render_device(mbr_ctx *ctx, const mbr_render_opts *opts,
                       mbr_framebuffer *fb, mbr_stats *stats);
 
+/* mbr_render_device without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture or a serving loop that forbids hipMalloc.  Every
+ * working buffer (counters, step records, hit lists, shading planes, AO
+ * intensities, overflow pool) is carved from one static 256 MiB device
+ * array per GPU, whose address mbr_init takes once per device; the call
+ * makes no hipMalloc or hipFree.  The caller passes the scene (mbr_set_scene,
+ * which may allocate) and the DEVICE planes, as for mbr_render_device; the
+ * results are the same.  The frame is rendered in bands as large as the
+ * array holds for this scene; a scene needing more than 256 MiB / 16384
+ * bytes per ray (very high AO quality in FP64), raytraced reflections, or a
+ * frame whose long rays outgrow the array's overflow pool is rejected
+ * (MBR_ERR_OUT_OF_MEMORY / MBR_ERR_UNSUPPORTED, with mbr_last_error saying
+ * which).  All calls on one GPU share the array: calls that overlap on one
+ * GPU (from any context or thread) race and must be serialised by the
+ * caller.  No kernel timings are taken (the ms fields of stats are 0). */
+int mbr_render_device_noalloc(mbr_ctx *ctx, const mbr_render_opts *opts,
+                              mbr_framebuffer *fb, mbr_stats *stats);
+
 /* Gradient of a pixel loss (no counterpart in the reference, which renders
  * only forward): given the image adjoint dL/dC as three DEVICE float planes
  * of the scene's size (dL/dR, dL/dG, dL/dB of imageFloat), writes dL/dtheta

@@ -18,6 +18,7 @@ EXPORTS = (
     "mbr_scene_layout", "mbr_jitter_draw", "mbr_render_batch", "mbr_adjoint", "mbr_compile_image",
     "mbr_write_png", "mbr_post_ssao", "mbr_post_dof", "mbr_compile_image_device", "mbr_keyframe_scene",
     "mbr_place_lights", "mbr_set_lights", "mbr_get_lights", "mbr_adjoint_jacobian",
+    "mbr_render_device_noalloc",
 )
 
 _lib = None
@@ -43,6 +44,7 @@ def _declare(lib):
     lib.mbr_set_scene.argtypes = [C.c_void_p, P(_abi.Scene), P(_abi.Texture)]
     lib.mbr_render.argtypes = [C.c_void_p, P(_abi.RenderOpts), P(_abi.Framebuffer), P(_abi.Stats)]
     lib.mbr_render_device.argtypes = [C.c_void_p, P(_abi.RenderOpts), P(_abi.Framebuffer), P(_abi.Stats)]
+    lib.mbr_render_device_noalloc.argtypes = [C.c_void_p, P(_abi.RenderOpts), P(_abi.Framebuffer), P(_abi.Stats)]
     lib.mbr_scene_layout.argtypes = [C.c_size_t * 8]
     lib.mbr_scene_layout.restype = C.c_size_t
     lib.mbr_jitter_draw.argtypes = [C.c_uint32] * 4

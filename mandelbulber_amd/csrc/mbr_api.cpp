@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,18 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kBandRays = 1 << 21;   // rays per band (one 1080p frame)
 constexpr int kSlab = 96;            // step records per ray kept in the slab
 constexpr size_t kAOBudget = size_t(8) << 30;  // bytes of per-(hit, direction) AO intensities per band
+
+// Working memory of mbr_render_device_noalloc: one file-scope device array,
+// which every GPU the library runs on holds (its own copy) until the process
+// ends.  Host code never names it after set-up: mbr_init takes its device
+// address once per device (scratch_ptr) and everything else uses that.
+constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+// fewest rays per band the scratch must hold for a scene
+constexpr int kNoallocMinBand = 16384;
 
 // ---------------------------------------------------- rotation matrices
 // CRotationMatrix (src/algebra.cpp:80-218): row-major, post-multiplied
@@ -598,6 +611,20 @@ int mbr_init(const mbr_device_opts *opts, mbr_ctx **out) {
   }
   if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
     c->cu_count = 0;
+  if (c->device < kMaxDevices) {
+    // the device address of this GPU's copy of `scratch` (the device is
+    // current), once per device
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+      void *p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        mbr_destroy(c);
+        return MBR_ERR_HIP;
+      }
+      g_scratch_ptr[c->device] = p;
+    }
+  }
   *out = c;
   return MBR_OK;
 }
@@ -1009,8 +1036,9 @@ namespace {
 // batch passes none.
 int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, const DevScene &host, int formula,
                   AODev ao, long long first, long long last, const FrameOut &out, int persist, hipStream_t st,
-                  hipEvent_t *ev, volatile const int *cancel) {
+                  hipEvent_t *ev, volatile const int *cancel, unsigned long long *stats = nullptr) {
   BandBufs &b = sl.band;
+  unsigned long long *const d_stats = stats ? stats : c->d_stats;
   // no reflection levels: the shade keeps specular and object parts together
   b.sp_r = b.sp_g = b.sp_b = nullptr;
   b.ray_o = b.ray_d = b.ray_t0 = nullptr;
@@ -1036,15 +1064,15 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     HIPCHECK(c, launch_band_reset(b, st));
     if (e) HIPCHECK(c, hipEventRecord(e[0], st));
     HIPCHECK(c,
-             launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, host.plain, persist, c->d_stats, st));
-    HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, c->d_stats, st));
+             launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, host.plain, persist, d_stats, st));
+    HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, d_stats, st));
     if (e) HIPCHECK(c, hipEventRecord(e[1], st));
-    HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, c->d_stats, st,
+    HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, d_stats, st,
                              e ? e[4] : nullptr, e ? e[5] : nullptr));
     if (e) HIPCHECK(c, hipEventRecord(e[2], st));
     HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula,
                               host.vol_any || host.iter_fog, host.n_lights > 0 && host.sd.aux_visibility > 0,
-                              persist, c->d_stats, ao));
+                              persist, d_stats, ao));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
   return MBR_OK;
@@ -1274,6 +1302,152 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->ms_finish = ms_finish;
     stats->ms_ao = ms_ao;
     stats->ms_total = ms_march + ms_shade + ms_finish;
+  }
+  return MBR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Band buffers of one slot carved out of `base` (kScratchBytes): the counters
+// of the render, the per-ray planes of alloc_slot for `cap` rays, and an
+// overflow pool of one directory and one chunk per 64 rays.  Returns the
+// bytes used; with base == nullptr only sizes the layout.
+size_t carve_scratch(unsigned char *base, int precision, int cap, int ao_count, bool iters, Slot &sl,
+                     unsigned long long **stats) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    off = (off + 255) & ~size_t(255);
+    void *p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  const size_t n = (size_t)cap;
+  const size_t rec = precision == MBR_PREC_FP64 ? 8 : 4;
+  sl = Slot{};
+  BandBufs &b = sl.band;
+  b.cap = cap;
+  b.slab = kSlab;
+  b.rec_bytes = (int)rec;
+  *stats = (unsigned long long *)take(ST_COUNT * sizeof(unsigned long long));
+  b.counters = (unsigned long long *)take(CNT_COUNT * sizeof(unsigned long long));
+  b.found = (uint8_t *)take(n);
+  b.object = (uint8_t *)take(n);
+  b.count = (int *)take(n * sizeof(int));
+  b.depth = (double *)take(n * sizeof(double));
+  b.thr = (double *)take(n * sizeof(double));
+  b.last = (double *)take(n * sizeof(double));
+  b.px = (double *)take(n * sizeof(double));
+  b.py = (double *)take(n * sizeof(double));
+  b.pz = (double *)take(n * sizeof(double));
+  b.rec_step = take(n * kSlab * rec);
+  b.rec_dist = take(n * kSlab * rec);
+  b.rec_thr = take(n * kSlab * rec);
+  b.ovf_off = (long long *)take(n * sizeof(long long));
+  b.hit_list = (int *)take(n * sizeof(int));
+  b.redo_list = (int *)take(n * sizeof(int));
+  b.hand_ray = (int *)take(n * sizeof(int));
+  b.hand_i = (int *)take(n * sizeof(int));
+  b.hand_scan = (double *)take(n * sizeof(double));
+  b.hand_step = (double *)take(n * sizeof(double));
+  b.hand_drift = (double *)take(n * sizeof(double));
+  b.hand_dist = (double *)take(n * sizeof(double));
+  b.sh_r = take(n * rec);
+  b.sh_g = take(n * rec);
+  b.sh_b = take(n * rec);
+  b.col_r = (uint8_t *)take(n);
+  b.col_g = (uint8_t *)take(n);
+  b.col_b = (uint8_t *)take(n);
+  sl.sh_base = take(3 * n * rec);
+  sl.sh_spec = take(3 * n * rec);
+  sl.sh_col = take(3 * n * rec);
+  sl.sh_fast = take(n * rec);
+  sl.sh_aux = take(6 * n * rec);
+  if (ao_count > 0) {
+    sl.sh_ao_bytes = n * (size_t)ao_count * rec;
+    sl.sh_ao = take(sl.sh_ao_bytes);
+  }
+  const long long pool = std::max<long long>(256, (long long)n / 64);
+  b.ovf_dirs = pool;
+  b.ovf_chunks = pool;
+  b.ovf_dir = (int *)take((size_t)pool * kDirSlots * sizeof(int));
+  b.ovf_step = take((size_t)pool * kChunk * rec);
+  b.ovf_dist = take((size_t)pool * kChunk * rec);
+  b.ovf_thr = take((size_t)pool * kChunk * rec);
+  if (iters) {
+    b.rec_it = (int *)take(n * kSlab * sizeof(int));
+    b.ovf_it = (int *)take((size_t)pool * kChunk * sizeof(int));
+  }
+  sl.prec = precision;
+  return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *fb, mbr_stats *stats) {
+  if (!c || !fb) return MBR_ERR_INVALID_ARG;
+  if (!c->has_scene) return c->fail(MBR_ERR_NO_SCENE, "mbr_set_scene was not called");
+  const mbr_scene &s = c->scene;
+  if (fb->width != s.image_width || fb->height != s.image_height)
+    return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
+  int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
+  int rc = check_precision(c, precision);
+  if (rc != MBR_OK) return rc;
+  if (c->host_sc.refl_max > 0)
+    return c->fail(MBR_ERR_UNSUPPORTED, "mbr_render_device_noalloc: raytraced reflections need a band slot per "
+                                        "level; render them with mbr_render_device");
+  if (formula_generic(s.formula)) precision = MBR_PREC_FP64;
+  unsigned char *base = c->device < kMaxDevices ? (unsigned char *)g_scratch_ptr[c->device] : nullptr;
+  if (!base) return c->fail(MBR_ERR_HIP, "mbr_render_device_noalloc: no scratch address for this device");
+  int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
+  if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
+  if (row0 < 0) row0 = 0;
+  if (row0 >= row1) return c->fail(MBR_ERR_INVALID_ARG, "empty row range");
+  const bool slow_ao = c->host_sc.ao && !c->host_sc.fast_ao;
+  const int ao_count = slow_ao ? c->host_sc.ao_count : 0;
+  const bool iters = c->host_sc.iter_fog != 0;
+  // the largest band whose buffers fit the scratch (multiple of 256 rays)
+  Slot sl;
+  unsigned long long *d_stats = nullptr;
+  const size_t per_256 = carve_scratch(nullptr, precision, 512, ao_count, iters, sl, &d_stats) -
+                         carve_scratch(nullptr, precision, 256, ao_count, iters, sl, &d_stats);
+  const size_t fixed = carve_scratch(nullptr, precision, 256, ao_count, iters, sl, &d_stats) - per_256;
+  long long cap = kScratchBytes > fixed ? (long long)((kScratchBytes - fixed) / per_256) * 256 : 0;
+  cap = std::min<long long>(cap, kBandRays);
+  while (cap >= 256 && carve_scratch(nullptr, precision, (int)cap, ao_count, iters, sl, &d_stats) > kScratchBytes)
+    cap -= 256;
+  if (cap < kNoallocMinBand)
+    return c->fail(MBR_ERR_OUT_OF_MEMORY,
+                   "mbr_render_device_noalloc: this scene needs " + std::to_string(per_256 / 256) +
+                       " bytes per ray (AO directions x record size), so the " + std::to_string(kScratchBytes >> 20) +
+                       " MiB scratch holds fewer than " + std::to_string(kNoallocMinBand) +
+                       " rays per band; render it with mbr_render_device");
+  carve_scratch(base, precision, (int)cap, ao_count, iters, sl, &d_stats);
+  hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
+  HIPCHECK(c, hipSetDevice(c->device));
+  const FrameOut out = frame_out(fb);
+  const AODev ao{c->d_ao_f, c->d_ao_d, c->d_ao_rgb, c->host_sc.ao_count};
+  const long long first = (long long)row0 * s.image_width;
+  const long long last = (long long)row1 * s.image_width;
+  const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
+  HIPCHECK(c, launch_zero(d_stats, ST_COUNT, st));
+  rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st, nullptr,
+                     opts ? opts->cancel : nullptr, d_stats);
+  if (rc != MBR_OK) return rc;
+  unsigned long long v[ST_COUNT];
+  HIPCHECK(c, hipMemcpyAsync(v, d_stats, sizeof v, hipMemcpyDeviceToHost, st));
+  HIPCHECK(c, hipStreamSynchronize(st));
+  if (v[ST_POOL_MISS] != 0)
+    return c->fail(MBR_ERR_OUT_OF_MEMORY, "mbr_render_device_noalloc: rays of more than " + std::to_string(kSlab) +
+                                              " steps outgrew the scratch's overflow pool (" +
+                                              std::to_string(sl.band.ovf_chunks) +
+                                              " chunks per band); render this frame with mbr_render_device");
+  if (stats) {
+    fill_stats(stats, v, last - first, (int)((last - first + cap - 1) / cap));
+    stats->ms_march = stats->ms_shade = stats->ms_finish = stats->ms_ao = stats->ms_total = 0.0;
   }
   return MBR_OK;
 }

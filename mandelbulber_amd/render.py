@@ -219,6 +219,16 @@ class Renderer:
                                       C.byref(st)), self._ctx)
         return st
 
+    def render_device_noalloc(self, planes, precision=_abi.PREC_FP32_FIXUP, rows=None, stream=None):
+        """render_device through mbr_render_device_noalloc (no device
+        allocation: the library's static scratch array); returns Stats."""
+        s = self.scene
+        fb = device_framebuffer(planes, s.image_width, s.image_height)
+        st = _abi.Stats()
+        check(lib().mbr_render_device_noalloc(self._ctx, C.byref(self._opts(precision, rows, stream)), C.byref(fb),
+                                              C.byref(st)), self._ctx)
+        return st
+
 
 def post_planes(w, h, z, opacity, colour, image16, alpha=None):
     """mbr_post_planes over numpy arrays (host) or torch tensors (device):
