@@ -68,32 +68,62 @@ def timed(torch, r, planes, precision, steps, warmup):
     return time.perf_counter() - t0, stats
 
 
+def host_cores():
+    """(cores this process may use, what nproc reports, the cgroup CPU quota
+    or None): the CPUs in the affinity mask, capped by the cgroup quota."""
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    cores = min(avail, quota) if quota else avail
+    return cores, nproc, quota
+
+
 def cpu_baseline(sc, seconds, threads):
-    """The oracle (-O3 -ffast-math, the reference's release flags) on a band
-    of rows through the middle of the frame, sized to about `seconds`."""
+    """The oracle (-O3 -ffast-math, the reference's release flags) on a
+    uniform sample of the frame: bands of rows spread evenly from the top to
+    the bottom, so that sky, silhouette and surface rows count in
+    proportion.  A first short sample of the same shape sets the number of
+    rows so that the timed sample takes about `seconds`."""
     from oracle import oracle
-    h = sc.image_height
-    rows = 2
-    while True:
-        r0 = h // 2 - rows // 2
+    w, h = sc.image_width, sc.image_height
+
+    def bands(total):
+        # bands of at least two rows per thread (the oracle hands out whole
+        # rows), at most 12 of them, spread evenly
+        band = min(h, total, max(2 * threads, -(-total // 12)))
+        n = max(1, total // band)
+        if n == 1:
+            return [((h - band) // 2, (h - band) // 2 + band)]
+        return [(int(round(i * (h - band) / (n - 1))), int(round(i * (h - band) / (n - 1))) + band) for i in range(n)]
+
+    def run(total):
         t0 = time.perf_counter()
-        _, st = oracle.render(sc, rows=(r0, r0 + rows), threads=threads, fast=True)
-        dt = time.perf_counter() - t0
-        if dt >= seconds / 4 or rows >= h:
-            break
-        rows = min(h, max(rows * 2, int(rows * seconds / 4 / max(dt, 1e-3))))
-    # final timed sample
-    if dt < seconds / 2 and rows < h:
-        rows = min(h, int(rows * seconds / max(dt, 1e-3)))
-        r0 = h // 2 - rows // 2
-        t0 = time.perf_counter()
-        _, st = oracle.render(sc, rows=(r0, r0 + rows), threads=threads, fast=True)
-        dt = time.perf_counter() - t0
-    rays = rows * sc.image_width
+        done = 0
+        for r0, r1 in bands(total):
+            oracle.render(sc, rows=(r0, r1), threads=threads, fast=True)
+            done += r1 - r0
+        return time.perf_counter() - t0, done
+
+    dt, done = run(min(h, 4 * 2 * threads))
+    total = max(1, min(h, int(done * seconds / max(dt, 1e-3))))
+    dt, done = run(total)
+    rays = done * w
+    sample = bands(total)
     return {
         "value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": f"rows {r0}..{r0 + rows} of the {sc.image_width}x{h} frame ({rays} rays, "
-                  f"{dt:.1f} s), oracle/liboracle_fast.so (-O3 -ffast-math), {threads} threads",
+        "sample": f"{len(sample)} bands of {sample[0][1] - sample[0][0]} rows spread evenly over the {w}x{h} frame "
+                  f"({done} rows, {rays} rays, {dt:.1f} s), oracle/liboracle_fast.so (-O3 -ffast-math), "
+                  f"{threads} threads",
     }
 
 
@@ -359,8 +389,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        threads, nproc, quota = host_cores()
         cpu = cpu_baseline(sc, args.cpu_seconds, threads)
+        cpu["nproc"] = nproc
+        cpu["cgroup_cpu_quota"] = quota
 
     s0 = st32[-1]
     line = {

@@ -1,5 +1,6 @@
-"""Build the render-core library mandelbulber_amd/libmbr.so with hipcc for gfx950
-and the -nogui driver mandelbulber_amd/mbr_nogui.
+"""Build the render-core library mandelbulber_amd/libmbr.so with hipcc for gfx950,
+the -nogui driver mandelbulber_amd/mbr_nogui and the counter-calibration
+program mandelbulber_amd/mbr_calib.
 
 Usage: python -m mandelbulber_amd.build   (or __graft_entry__.build())
 """
@@ -13,6 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmbr.so")
 CLI = os.path.join(HERE, "mbr_nogui")
+CALIB = os.path.join(HERE, "mbr_calib")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["mbr_api.cpp", "mbr_kernels.hip", "mbr_adjoint.hip", "mbr_post.hip", "mbr_post_order.cpp",
@@ -33,9 +35,13 @@ FLAGS = [
 ]
 
 
+# per-source flags
+EXTRA = {}
+
+
 def _compile(src):
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, "-x", "hip", *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -60,6 +66,13 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"mbr_nogui build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, CLI)
+    # kernels of known instruction counts, for reading the profiler's counters
+    tmp = CALIB + ".tmp"
+    cmd = [HIPCC, "-x", "hip", *FLAGS, os.path.join(CSRC, "mbr_calib.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mbr_calib build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALIB)
     if verbose:
         print(f"built {OUT} and {CLI}")
     return OUT

@@ -1056,6 +1056,7 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
   sbuf.plain = host.plain;
   sbuf.formula = formula;
   sbuf.lights = host.n_lights > 0;
+  sbuf.ip8 = formula == 2 && host.fd.ipow == 8;
   int band = 0;
   for (long long band_first = first; band_first < last; band_first += b.cap, band++) {
     if (cancel && *cancel) return c->fail(MBR_ERR_CANCELLED, "render cancelled");
@@ -1063,9 +1064,10 @@ int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, con
     hipEvent_t *e = ev ? ev + 6 * band : nullptr;
     HIPCHECK(c, launch_band_reset(b, st));
     if (e) HIPCHECK(c, hipEventRecord(e[0], st));
-    HIPCHECK(c,
-             launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, host.plain, persist, d_stats, st));
-    HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, d_stats, st));
+    HIPCHECK(c, launch_march(precision, d_sc, b, (int)band_first, band_rays, formula, host.plain, persist, d_stats, st,
+                             sbuf.ip8 != 0));
+    HIPCHECK(c, launch_march_fix(precision, d_sc, b, (int)band_first, formula, persist / 2, d_stats, st,
+                                 sbuf.ip8 != 0));
     if (e) HIPCHECK(c, hipEventRecord(e[1], st));
     HIPCHECK(c, launch_shade(precision, d_sc, b, (int)band_first, ao, sbuf, persist, d_stats, st,
                              e ? e[4] : nullptr, e ? e[5] : nullptr));
@@ -1092,6 +1094,7 @@ ShadeBufs shade_bufs(Slot &sl, const DevScene &host, int formula) {
   sbuf.plain = host.plain;
   sbuf.formula = formula;
   sbuf.lights = host.n_lights > 0;
+  sbuf.ip8 = formula == 2 && host.fd.ipow == 8;
   return sbuf;
 }
 

@@ -620,7 +620,9 @@ struct DEIter {
   int L;
 };
 
-template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
+// IP: the Mandelbulb's integer power when the caller knows it at compile
+// time (8, the standard bulb), else 0 (read from f.ipow).
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8), int IP = 0>
 MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.z = p;
   s.c = (!P && f.julia_mode) ? ld3(f.julia) : p * f.constant_factor;
@@ -629,7 +631,7 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
   s.L = 0;
   s.inv_r = 0;
   if constexpr (F == 2 && A) {
-    if (f.ipow) {
+    if (IP || f.ipow) {
       const T r2 = p.x * p.x + p.y * p.y + p.z * p.z;
       s.inv_r = fast_rsq(r2);
       s.r = r2 * s.inv_r;
@@ -640,7 +642,7 @@ MBR_HD void de_begin(const FractalK<T> &f, V3<T> p, DEIter<T> &s) {
 // One loop pass.  Returns true when the call has finished; `dist` is then
 // the return value of Compute<normal> (0 when the orbit stayed bounded for
 // N iterations) and s.L the reference's iteration counter.
-template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8), int IP = 0>
 MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
   using M = Math<T>;
   V3<T> z = s.z;
@@ -663,7 +665,7 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
   }
   bool esc;
   if constexpr (F == 2 && A) {
-    if (f.ipow) {
+    if (IP || f.ipow) {
       // Integer power p (A: the FP32 kernels, and FP64 marching of the mixed
       // precision modes): the same map without inverse trig.  With
       // rho = |(x,y)|, (cos th, sin th) = (rho, z)/r and (cos ph, sin ph) =
@@ -672,23 +674,19 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
       // (de Moivre), formed by binary exponentiation.  1/r is carried from
       // the previous iteration; in FP32 reciprocals and square roots use the
       // hardware rsq/rcp.
-      const int p = f.ipow;
+      const int p = IP ? IP : f.ipow;
       T inv_r = s.inv_r;
       if ((!P && f.tglad_fold) || (!P && f.sph_fold)) inv_r = fast_rsq(z.x * z.x + z.y * z.y + z.z * z.z);
       const T rho2 = z.x * z.x + z.y * z.y;
-      T ure, uim = z.z * inv_r;  // e^{i th}
-      T wre, wim;                // e^{i ph}
-      if (rho2 > T(0)) {
-        const T inv_rho = fast_rsq(rho2);
-        ure = rho2 * inv_rho * inv_r;
-        wre = z.x * inv_rho;
-        wim = z.y * inv_rho;
-      } else {
-        // atan2(+-0, x) is +-0 for x = +0 and +-pi for x = -0
-        ure = T(0);
-        wre = Math<T>::copysign(T(1), z.x);
-        wim = T(0);
-      }
+      // e^{i th} = (rho + i z) / r and e^{i ph} = (x + i y) / rho; on the z
+      // axis (rho = 0) cos th = 0 and atan2(+-0, x) is +-0 for x = +0 and
+      // +-pi for x = -0 (selects, no branch)
+      const bool off_axis = rho2 > T(0);
+      const T inv_rho = fast_rsq(off_axis ? rho2 : T(1));
+      T ure = off_axis ? rho2 * inv_rho * inv_r : T(0);
+      T uim = z.z * inv_r;
+      T wre = off_axis ? z.x * inv_rho : Math<T>::copysign(T(1), z.x);
+      T wim = off_axis ? z.y * inv_rho : T(0);
       T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1;
       if (p == 8) {
         // the standard Mandelbulb: three squarings, r^7 = r^4 r^2 r
@@ -950,7 +948,7 @@ __attribute__((noinline)) __host__ __device__ T calc_distance_generic(const Frac
 }
 
 // Start one CalculateDistance; true when it finished without iterating.
-template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8), int IP = 0>
 MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                           DECall<T> &call, T &dist, DEInfo &info) {
   if constexpr (F == 0) {
@@ -977,7 +975,7 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
     dist = de_finish<T, P>(f, point, T(10), 0, false, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
   }
-  de_begin<T, F, P, A>(f, point, call.it);
+  de_begin<T, F, P, A, IP>(f, point, call.it);
   if (N <= 0) {
     dist = de_finish<T, P>(f, point, T(0), 0, true, N, minN, normalMode, detailSize, call.limitBox, info);
     return true;
@@ -986,22 +984,22 @@ MBR_HD bool de_call_begin(const FractalK<T> &f, V3<T> point, T detailSize, int N
 }
 
 // Advance one iteration; true when the call finished (dist, info valid).
-template <class T, int F, bool P = false, bool A = (sizeof(T) < 8)>
+template <class T, int F, bool P = false, bool A = (sizeof(T) < 8), int IP = 0>
 MBR_HD bool de_call_step(const FractalK<T> &f, DECall<T> &call, T detailSize, int N, int minN, bool normalMode,
                          T &dist, DEInfo &info) {
   T raw = 0;
-  if (!de_iter<T, F, P, A>(f, call.it, N, raw)) return false;
+  if (!de_iter<T, F, P, A, IP>(f, call.it, N, raw)) return false;
   dist = de_finish<T, P>(f, call.p, raw, call.it.L, true, N, minN, normalMode, detailSize, call.limitBox, info);
   return true;
 }
 
-template <class T, int F, bool A = (sizeof(T) < 8)>
+template <class T, int F, bool A = (sizeof(T) < 8), int IP = 0>
 MBR_HD T calc_distance_f(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                          DEInfo &info) {
   DECall<T> call;
   T dist = 0;
-  if (de_call_begin<T, F, false, A>(f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
-  while (!de_call_step<T, F, false, A>(f, call, detailSize, N, minN, normalMode, dist, info)) {
+  if (de_call_begin<T, F, false, A, IP>(f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
+  while (!de_call_step<T, F, false, A, IP>(f, call, detailSize, N, minN, normalMode, dist, info)) {
   }
   return dist;
 }

@@ -190,7 +190,7 @@ __device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, d
 // The reference loop from step s.i on, in FP64, then the binary search
 // (src/Render3D.cpp:145-183).  Sets `ambiguous` when a decision lies within
 // the drift margin of its threshold (see the hand-over rule above).
-template <class RT, int F, bool Refl = false>
+template <class RT, int F, bool Refl = false, int IP = 0>
 __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
                                    const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
                                    bool &ambiguous) {
@@ -209,7 +209,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
     march_before_de<Refl>(sc, start, dir, s);
     detail = s.thr;
     DEInfo info;
-    dist = calc_distance_f<double, F, kAlg>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
+    dist = calc_distance_f<double, F, kAlg, IP>(sc.fd, s.point, detail, sc.N, sc.minN, false, info);
     cnt.add(info);
     s.object = info.object;
     const double dist_raw = dist;
@@ -247,7 +247,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       else if (dist < distThresh * search_limit) { scan -= step; point = start + dir * scan; }
       distThresh = sc.constant_thresh ? k.quality : scan * k.resolution * k.persp / k.quality + (Refl ? s.thr0 : 0.0);
       DEInfo bi;
-      dist = calc_distance_f<double, F, kAlg>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
+      dist = calc_distance_f<double, F, kAlg, IP>(sc.fd, point, detail, sc.N, sc.minN, false, bi);
       cnt.add(bi);
       object = bi.object;
       step *= 0.5;
@@ -267,15 +267,15 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
       for (int it = 0; it < 8 && fabs(f) > 1e-13 * kthr * t; it++) {
         const double eps = 1e-3 * kthr * t;
         DEInfo bi;
-        const double dp = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * (t + eps), detail, sc.N, sc.minN, false, bi);
-        const double dm = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * (t - eps), detail, sc.N, sc.minN, false, bi);
+        const double dp = calc_distance_f<double, F, kAlg, IP>(sc.fd, start + dir * (t + eps), detail, sc.N, sc.minN, false, bi);
+        const double dm = calc_distance_f<double, F, kAlg, IP>(sc.fd, start + dir * (t - eps), detail, sc.N, sc.minN, false, bi);
         const double g = (dp - dm) / (2.0 * eps) - c0 * kthr;
         if (!(fabs(g) > 0.0)) break;
         double dt = -f / g;
         const double lim = kthr * t;
         dt = dt > lim ? lim : (dt < -lim ? -lim : dt);
         t += dt;
-        dist = calc_distance_f<double, F, kAlg>(sc.fd, start + dir * t, detail, sc.N, sc.minN, false, bi);
+        dist = calc_distance_f<double, F, kAlg, IP>(sc.fd, start + dir * t, detail, sc.N, sc.minN, false, bi);
         cnt.add(bi);
         object = bi.object;
         f = dist - c0 * kthr * t;
@@ -354,14 +354,19 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
 // pass, lane states and amortised bookkeeping as in k_ao.
 enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
 
-template <class T, class RT, int F, bool Plain>
-__global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
-                                                  int band_rays, unsigned long long *stats) {
+// IP: the integer power of the Mandelbulb when fixed at compile time (8).
+template <class T, class RT, int F, bool Plain, int IP>
+__device__ __forceinline__ void march_body(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                           int band_rays, unsigned long long *stats) {
   // integer-power Mandelbulb iterations without inverse trig wherever the
   // records are FP32; the FP64 mode keeps the trigonometric form
   constexpr bool kAlg = sizeof(RT) < 8;
   const DevScene &sc = *scp;
-  const FractalK<T> &f = sc.frac<T>();
+  // the power-8 variants read a few formula constants in every iteration: a
+  // local copy keeps them in registers (the pass's stores and atomics keep
+  // the compiler from hoisting loads of the scene block out of the loop)
+  const FractalK<T> f_local = (Plain && IP) ? sc.frac<T>() : FractalK<T>{};
+  const FractalK<T> &f = (Plain && IP) ? f_local : sc.frac<T>();
   const int N = sc.N, minN = sc.minN;
   const int lane = __lane_id();
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -382,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
     march_before_de(sc, start, dir, s);
     DEInfo info;
     state = MS_ITER;
-    if (de_call_begin<T, F, Plain, kAlg>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
+    if (de_call_begin<T, F, Plain, kAlg, IP>(f, cvt<T>(s.point), (T)s.thr, N, minN, false, call, dist, info)) {
       state = MS_DONE;
       object = info.object | (info.iters << 3);
       cnt.de_calls += 1;
@@ -427,7 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
 
     if (state == MS_ITER) {
       DEInfo info;
-      if (de_call_step<T, F, Plain, kAlg>(f, call, (T)s.thr, N, minN, false, dist, info)) {
+      if (de_call_step<T, F, Plain, kAlg, IP>(f, call, (T)s.thr, N, minN, false, dist, info)) {
         state = MS_DONE;
         object = info.object | (info.iters << 3);
         cnt.de_calls += 1;
@@ -473,10 +478,17 @@ __global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ s
   (void)missed;
 }
 
+template <class T, class RT, int F, bool Plain, int IP = 0>
+__global__ void __launch_bounds__(kBlock) k_march(const DevScene *__restrict__ scp, BandBufs b, int band_first,
+                                                  int band_rays, unsigned long long *stats) {
+  march_body<T, RT, F, Plain, IP>(scp, b, band_first, band_rays, stats);
+}
+
+
 // FP64 continuation of the handed-over rays: one lane per ray, a grid-stride
 // loop over the hand-over list whose length the march left in the band
 // counters.
-template <class RT, int F>
+template <class RT, int F, int IP = 0>
 __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict__ scp, BandBufs b, int band_first,
                                                       unsigned long long *stats) {
   const DevScene &sc = *scp;
@@ -500,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_march_fix(const DevScene *__restrict
     const RecPlanes<RT> rec = ray_records<RT>(b, r, stats);
     MarchOut o;
     bool ambiguous = false;
-    march_continue_f64<RT, F>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
+    march_continue_f64<RT, F, false, IP>(sc, start, dir, pixel, rec, s, o, cnt, ambiguous);
     if (ambiguous) {
       // re-marched from the start by k_march_redo
       b.redo_list[atomicAdd(&b.counters[CNT_REDO], 1ull)] = r;
@@ -711,13 +723,15 @@ __global__ void __launch_bounds__(kBlock) k_aux(const DevScene *__restrict__ scp
 // lanes instead of being paid on almost every pass.
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
 
-template <class T, int F, bool Deep, bool Plain>
+template <class T, int F, bool Deep, bool Plain, int IP = 0>
 __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
                                         unsigned long long *stats) {
   const DevScene &sc = *scp;
   const long long n_hits = (long long)b.counters[CNT_HITS];
   unsigned long long *queue = &b.counters[CNT_AOQ];
-  const FractalK<T> &f = sc.frac<T>();
+  // power-8 variants: the formula constants in registers (as in k_march)
+  const FractalK<T> f_local = (Plain && IP) ? sc.frac<T>() : FractalK<T>{};
+  const FractalK<T> &f = (Plain && IP) ? f_local : sc.frac<T>();
   const ShadeK<T> &k = sc.shade<T>();
   const T *dirs;
   if constexpr (sizeof(T) == 8) dirs = ao.dir_d; else dirs = ao.dir_f;
@@ -752,9 +766,10 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
         done = de_call_begin_p64<F>(sc.fd, f, P64 + cvt<double>(dir) * (double)r, detail, N, minN, false, call, dist,
                                     info);
       else
-        done = de_call_begin<T, F, Plain>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+        done = de_call_begin<T, F, Plain, (sizeof(T) < 8), IP>(f, P + dir * r, detail, N, minN, false, call, dist,
+                                                               info);
     } else {
-      done = de_call_begin<T, F, Plain>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+      done = de_call_begin<T, F, Plain, (sizeof(T) < 8), IP>(f, P + dir * r, detail, N, minN, false, call, dist, info);
     }
     if (done) {
       state = AO_DONE;
@@ -823,8 +838,8 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
 
     if (state == AO_ITER) {
       DEInfo info;
-      bool fin = de_call_step<T, F, Plain>(f, call, detail, N, minN, false, dist, info);
-      if (!fin) fin = de_call_step<T, F, Plain>(f, call, detail, N, minN, false, dist, info);
+      bool fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
+      if (!fin) fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
       if (fin) {
         state = AO_DONE;
         mi = info.max_iter;
@@ -872,11 +887,11 @@ __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp,
 }
 
 // The FP32 plain variant (the benchmarked AO pass) held to 6 waves per SIMD.
-template <class T, int F>
+template <class T, int F, int IP = 0>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
     k_ao_f32_plain(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
                    unsigned long long *stats) {
-  ao_body<T, F, false, true>(scp, b, ao, sb, stats);
+  ao_body<T, F, false, true, IP>(scp, b, ao, sb, stats);
 }
 
 template <class T, class RT>
@@ -1421,7 +1436,8 @@ hipError_t launch_refl_finish(const DevScene *sc, const BandBufs *levels, int n_
 }
 
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s) {
+                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s,
+                        bool ip8) {
   if (band_rays <= 0) return hipSuccess;
   if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max(1, persist_blocks));
@@ -1430,6 +1446,13 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
     using RT = decltype(rtag);
     auto go = [&](auto plain_tag) {
       constexpr bool Pl = decltype(plain_tag)::value;
+      if constexpr (Pl && sizeof(RT) < 8) {
+        if (formula == 2 && ip8) {
+          hipLaunchKernelGGL((k_march<T, RT, 2, true, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays,
+                             stats);
+          return;
+        }
+      }
       if (formula == 7)
         hipLaunchKernelGGL((k_march<T, RT, 7, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
       else if (formula == 8)
@@ -1454,7 +1477,7 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
 }
 
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
-                            int persist_blocks, unsigned long long *stats, hipStream_t s) {
+                            int persist_blocks, unsigned long long *stats, hipStream_t s, bool ip8) {
   const dim3 grid((unsigned)std::max(1, persist_blocks));
   if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
   if (formula_generic(formula)) {
@@ -1467,8 +1490,16 @@ hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b
       hipLaunchKernelGGL((k_march_fix<RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
     else if (formula == 8)
       hipLaunchKernelGGL((k_march_fix<RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
-    else
-      hipLaunchKernelGGL((k_march_fix<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    else {
+      bool done = false;
+      if constexpr (sizeof(RT) < 8) {
+        if (ip8) {
+          hipLaunchKernelGGL((k_march_fix<RT, 2, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+          done = true;
+        }
+      }
+      if (!done) hipLaunchKernelGGL((k_march_fix<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    }
     if (precision == 2) {
       if (formula == 7)
         hipLaunchKernelGGL((k_march_redo<RT, 7>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
@@ -1542,6 +1573,8 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
           hipLaunchKernelGGL((k_ao_f32_plain<T, 7>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
         else if (formula == 8)
           hipLaunchKernelGGL((k_ao_f32_plain<T, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+        else if (sbuf.ip8)
+          hipLaunchKernelGGL((k_ao_f32_plain<T, 2, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
         else
           hipLaunchKernelGGL((k_ao_f32_plain<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
         return;

@@ -100,10 +100,13 @@ struct AODev {
 // persist_blocks: workgroups of the persistent launches (a few per CU).
 hipError_t launch_band_reset(const BandBufs &b, hipStream_t s);
 hipError_t launch_zero(unsigned long long *p, int n, hipStream_t s);
+// ip8: the scene is a Mandelbulb of power 8 (kernels with the power fixed
+// at compile time for the mixed-precision modes)
 hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, int band_first, int band_rays,
-                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s);
+                        int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s,
+                        bool ip8 = false);
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
-                            int persist_blocks, unsigned long long *stats, hipStream_t s);
+                            int persist_blocks, unsigned long long *stats, hipStream_t s, bool ip8 = false);
 // Buffers of the shading launches (element type T = the path's precision).
 struct ShadeBufs {
   void *base, *spec, *col, *fast_ao, *ao;
@@ -114,6 +117,7 @@ struct ShadeBufs {
   int plain;                    // 1: no primitives, limits, interior, iteration threshold, pre-folds, Julia
   int formula;
   int lights;                   // 1: aux lights take part (k_aux runs after k_surface)
+  int ip8;                      // 1: Mandelbulb of power 8 (AO kernel with the power fixed)
 };
 // Shading in three launches: surface (normals, shadow, specular, colour,
 // fast AO), AO rays, composition; ev_ao (when not null) is recorded around

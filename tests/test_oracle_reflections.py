@@ -66,3 +66,101 @@ def test_mirror_floor_shows_the_bulb_below_it():
     d = np.abs(m.rgb.astype(np.float64) - p.rgb).max(axis=0)
     floor_rows = np.where(((p.z < 1e19) & (d > 1e-3)).any(axis=1))[0]
     assert floor_rows.size > 5
+
+
+def _mirror_sphere_scene(w=64, h=48):
+    """A mirror plane z = -10 seen from (0, -4, -6) looking along (0, 1, -1),
+    a sphere of radius 1.5 at (0, 4, -6) out of the camera's view but in the
+    mirror's, the bulb far outside both; no specular, glow, fog, AO or
+    shadow, so a mirror pixel is the sphere's object shade at the point its
+    mirrored ray reaches."""
+    s = scene.mandelbulb_plain(w, h)
+    scene.apply(s, {
+        "N": 20, "view_point_x": 0, "view_point_y": -4, "view_point_z": -6, "angle_alfa": 0, "angle_beta": -45,
+        "angle_gamma": 0, "specular": 0, "glow_intensity": 0, "shadows_enabled": 0, "fractal_color": 0,
+        "raytraced_reflections": 1, "reflections_max": 1,
+        "primitive_plane_enabled": 1, "primitive_plane_centre_X": 0, "primitive_plane_centre_Y": 0,
+        "primitive_plane_centre_Z": -10, "primitive_plane_normal_X": 0, "primitive_plane_normal_Y": 0,
+        "primitive_plane_normal_Z": 1, "primitive_plane_reflect": 1.0,
+        "primitive_sphere_enabled": 1, "primitive_sphere_centre_X": 0, "primitive_sphere_centre_Y": 4,
+        "primitive_sphere_centre_Z": -6, "primitive_sphere_radius": 1.5,
+    })
+    return s
+
+
+def _band_root(f, t_lo, t_hi, iters=200):
+    """t in [t_lo, t_hi] with f(t) = 0, f increasing through it (bisection)."""
+    for _ in range(iters):
+        t = 0.5 * (t_lo + t_hi)
+        if f(t) > 0:
+            t_hi = t
+        else:
+            t_lo = t
+    return 0.5 * (t_lo + t_hi)
+
+
+def test_mirror_plane_reflects_a_sphere_in_closed_form():
+    """The reflected ray in closed form (src/Render3D.cpp:440-499): the
+    primary ray ends in the binary search's band 0.99 thr <= D <= thr above
+    the plane (taken at its centre); the plane's normal is exact (a linear
+    distance has exact differences); the mirrored direction is d - 2 (d.n) n,
+    starting one threshold along it from the hit and carrying the
+    threshold on (distThreshInit); its hit lies in the band above the
+    sphere.  The pixel is then the sphere's ObjectShader at that point seen
+    along the mirrored direction (reflect 1: nothing of the plane's own
+    shade, no specular), which must lie between the shades of the band's
+    four corners (each ray ending at 0.99 thr or at thr).  A slip in the
+    mirrored direction, its start or its threshold moves the sphere point
+    and its normal out of that range."""
+    s = _mirror_sphere_scene()
+    frame, _ = oracle.render(s)
+    R, start, light = oracle.camera(s)
+    res = 1.0 / s.image_width
+    k = res * s.persp / s.quality
+    n = np.array([0.0, 0.0, 1.0])
+    c_plane = np.array([0.0, 0.0, -10.0])
+    c_sph, rad = np.array([0.0, 4.0, -6.0]), 1.5
+    lo = 1.0 - 0.01 * s.quality
+    checked = 0
+
+    def shade(d, m1, m2):
+        """The sphere's shade for primary band fraction m1 and mirrored band
+        fraction m2 (the distance at the hit over its threshold); None where
+        the mirrored ray misses the sphere below the bulb."""
+        dn = d @ n
+        # primary hit: (o + d t - c).n = m1 k t
+        t1 = ((start - c_plane) @ n) / (m1 * k - dn)
+        p1 = start + d * t1
+        thr1 = k * t1
+        d2 = d - 2.0 * dn * n
+        o2 = p1 + d2 * thr1
+        # mirrored ray: |o2 + d2 t - c| - rad = m2 (k t + thr1)
+        g = lambda t: np.linalg.norm(o2 + d2 * t - c_sph) - rad - m2 * (k * t + thr1)
+        ts = np.linspace(0.0, 50.0, 5001)
+        vals = np.array([g(t) for t in ts])
+        below = np.nonzero(vals < 0)[0]
+        if below.size == 0 or below[0] == 0:
+            return None
+        i = below[0]
+        t2 = _band_root(lambda t: -g(t), ts[i - 1], ts[i])
+        p2 = o2 + d2 * t2
+        if p2[2] > -3.0:
+            return None
+        return np.array(oracle.object_shader(s, p2, d2, k * t2 + thr1, t2, 0.0, 3).rgb[:])  # MBR_OBJ_SPHERE
+
+    for y in range(0, s.image_height, 3):
+        for x in range(0, s.image_width, 3):
+            d, _ = oracle.view_vector(s, x, y)
+            if d @ n >= -1e-3:
+                continue
+            corners = [shade(d, m1, m2) for m1 in (lo, 1.0) for m2 in (lo, 1.0)]
+            if any(c is None for c in corners):
+                continue
+            corners = np.array(corners)
+            got = frame.rgb[:, y, x].astype(np.float64)
+            # float32 planes: their rounding widens the range slightly
+            assert np.all(got >= corners.min(axis=0) - 1e-6) and np.all(got <= corners.max(axis=0) + 1e-6), \
+                (x, y, got, corners)
+            checked += 1
+    assert checked >= 15
+

@@ -296,3 +296,45 @@ def test_small_frame_against_brute_force_occupancy():
             if inside and not hit:
                 disagree += 1
     assert disagree == 0
+
+
+@pytest.mark.parametrize("merge", [1, 3])
+def test_volumetric_constant_steps_closed_form(merge):
+    """VolumetricShader (src/shaders.cpp:185-384) over n identical step
+    records: records n-1 .. 1 are visited (index 0 never), merged into
+    groups of `merge` records (each record's step is 1/merge of thr *
+    quality, and 0.4 of it for merge 3: groups close at 1.2 thr quality;
+    the records left over at the near end form no group), and every group
+    applies the same glow blend (opacity g = glow step / depth) and the same
+    volumetric-fog blend (density fd), so the pixel follows the affine
+    recurrence x <- A + B x m times: x_m = A (1 - B^m) / (1 - B) + B^m x_0
+    with B = (1 - fd)(1 - g), A = fd F + (1 - fd) g G for the colour,
+    B = 1 - fd, A = fd + (1 - fd) g for alpha, and the fog opacity is
+    1 - (1 - fd)^m.  The fog colours are all F, so the fog colour does not
+    depend on the distance."""
+    s = vol_scene(glow=0.3, fog=0.8)
+    for i, c in enumerate((s.fog_colour1, s.fog_colour2, s.fog_colour3)):
+        c.R, c.G, c.B = 50000, 20000, 9000
+    s.glow_color1.R, s.glow_color1.G, s.glow_color1.B = 60000, 30000, 1000
+    s.glow_color2.R, s.glow_color2.G, s.glow_color2.B = 4000, 40000, 65000
+    n, depth, thr, dist = 41, 4.0, 0.02, 1.7
+    step = thr * s.quality * (0.4 if merge == 3 else 1.0)
+    px = np.array([0.3, 0.5, 0.7])
+    out, op = O.volumetric(s, depth, [[step, dist, thr]] * n, (px[0], px[1], px[2], 0.25))
+    m = (n - 1) // merge
+    gstep = step * merge
+    glow = n * s.glow_intensity / 512.0 * s.DE_factor
+    G = (np.array([s.glow_color1.R, s.glow_color1.G, s.glow_color1.B], float) * (1 - glow)
+         + np.array([s.glow_color2.R, s.glow_color2.G, s.glow_color2.B], float) * glow) / 65536.0
+    g = glow * gstep / depth
+    dens = gstep * s.fog_distance_factor / (dist * dist + s.fog_distance_factor ** 2)
+    fd = 0.3 * s.fog_density * dens / (1 + s.fog_density * dens)
+    F = np.array([50000, 20000, 9000], float) / 65536.0
+    B = (1 - fd) * (1 - g)
+    A = fd * F + (1 - fd) * g * G
+    want = A * (1 - B ** m) / (1 - B) + B ** m * px
+    assert m == (40 if merge == 1 else 13)
+    np.testing.assert_allclose(out[:3], want, rtol=1e-12)
+    Ba, Aa = 1 - fd, fd + (1 - fd) * g
+    assert out[3] == pytest.approx(Aa * (1 - Ba ** m) / (1 - Ba) + Ba ** m * 0.25, rel=1e-12)
+    assert op == pytest.approx(1 - (1 - fd) ** m, rel=1e-5)  # totalOpacity is a float
