@@ -373,7 +373,9 @@ typedef struct mbr_render_opts {
  * DE steps and missed-DE %, src/Render3D.cpp:638-652). */
 typedef struct mbr_stats {
   double ms_total;                 /* GPU time of the render, HIP events */
-  double ms_march, ms_shade, ms_finish;
+  double ms_march, ms_shade, ms_finish;  /* summed over bands: frames of 2^18 or more
+                                            rays run two bands at a time, so these may
+                                            add up to more than ms_total */
   uint64_t rays, hits;
   uint64_t march_de_calls;         /* DE calls of primary march + bisection */
   uint64_t shade_de_calls;         /* DE calls of normals, shadow, AO, colour */

@@ -683,21 +683,40 @@ MBR_HD bool de_iter(const FractalK<T> &f, DEIter<T> &s, int N, T &dist) {
       // +-pi for x = -0 (selects, no branch)
       const bool off_axis = rho2 > T(0);
       const T inv_rho = fast_rsq(off_axis ? rho2 : T(1));
-      T ure = off_axis ? rho2 * inv_rho * inv_r : T(0);
-      T uim = z.z * inv_r;
       T wre = off_axis ? z.x * inv_rho : Math<T>::copysign(T(1), z.x);
       T wim = off_axis ? z.y * inv_rho : T(0);
-      T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1;
       if (p == 8) {
-        // the standard Mandelbulb: three squarings, r^7 = r^4 r^2 r
-        for (int k = 0; k < 3; k++) {
-          T t = ure * ure - uim * uim; uim = T(2) * ure * uim; ure = t;
-          t = wre * wre - wim * wim; wim = T(2) * wre * wim; wre = t;
+        // the standard Mandelbulb.  U = rho + i z is left unnormalised:
+        // U^8 = r^8 e^{8 i th} carries the factor r^8 of the new point, so
+        // z' = (Re U^8 cos 8ph, Re U^8 sin 8ph, Im U^8) + c.  U^2 is formed
+        // from rho^2 directly; rho = 0 on the axis (inv_rho = 1 there).
+        const T rho = rho2 * inv_rho;
+        T ure = rho2 - z.z * z.z, uim = T(2) * rho * z.z;
+        for (int k = 0; k < 2; k++) {
+          const T t = ure * ure - uim * uim; uim = T(2) * ure * uim; ure = t;
         }
-        pure = ure; puim = uim; pwre = wre; pwim = wim;
+        for (int k = 0; k < 3; k++) {
+          const T t = wre * wre - wim * wim; wim = T(2) * wre * wim; wre = t;
+        }
         const T r2 = r * r, r4 = r2 * r2;
-        rp1 = r4 * r2 * r;
-      } else {
+        s.aux = r4 * r4 * inv_r * s.aux * T(8) + T(1);  // 8 r^7 aux + 1
+        z = V3<T>{ure * wre, ure * wim, uim} + s.c;
+        const T zr2 = z.x * z.x + z.y * z.y + z.z * z.z;
+        s.inv_r = fast_rsq(zr2);
+        r = zr2 * s.inv_r;
+        esc = !(r <= f.bailout);
+        if (esc) dist = T(0.5) * r * fast_ln(r) * fast_rcp(s.aux);
+        s.z = z;
+        s.r = r;
+        if (esc) return true;
+        s.L++;
+        if (s.L >= N) { dist = 0; return true; }
+        return false;
+      }
+      T ure = off_axis ? rho2 * inv_rho * inv_r : T(0);
+      T uim = z.z * inv_r;
+      T pure = 1, puim = 0, pwre = 1, pwim = 0, rp1 = 1;
+      {
         T rb = r;
         int e = p - 1;  // r^(p-1)
         while (e) {

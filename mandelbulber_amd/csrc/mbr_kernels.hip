@@ -350,9 +350,13 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
   return rec;
 }
 
-// Persistent march: one primary ray per lane, one formula iteration per
-// pass, lane states and amortised bookkeeping as in k_ao.
+// Persistent march: one primary ray per lane, up to kMarchIters formula
+// iterations per pass, lane states and amortised bookkeeping as in k_ao.
 enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
+#ifndef MBR_MARCH_ITERS
+#define MBR_MARCH_ITERS 6
+#endif
+constexpr int kMarchIters = MBR_MARCH_ITERS;
 
 // IP: the integer power of the Mandelbulb when fixed at compile time (8).
 template <class T, class RT, int F, bool Plain, int IP>
@@ -432,7 +436,11 @@ __device__ __forceinline__ void march_body(const DevScene *__restrict__ scp, Ban
 
     if (state == MS_ITER) {
       DEInfo info;
-      if (de_call_step<T, F, Plain, kAlg, IP>(f, call, (T)s.thr, N, minN, false, dist, info)) {
+      bool fin = false;
+#pragma unroll
+      for (int k = 0; k < kMarchIters && !fin; k++)
+        fin = de_call_step<T, F, Plain, kAlg, IP>(f, call, (T)s.thr, N, minN, false, dist, info);
+      if (fin) {
         state = MS_DONE;
         object = info.object | (info.iters << 3);
         cnt.de_calls += 1;
@@ -715,13 +723,42 @@ __global__ void __launch_bounds__(kBlock) k_aux(const DevScene *__restrict__ scp
 //
 // A lane is in one of four states: NEED (wants an item), ITER (advancing a
 // distance estimate), DONE (estimate complete, AO step pending) or EXHAUSTED.
-// Every pass advances the ITER lanes by up to two formula iterations (an AO
-// distance estimate takes about 2.3 on average, so the pass overhead is paid
-// about once per estimate).  The AO step logic of DONE lanes runs once a
-// third of the wave waits for it, the item hand-out of NEED lanes once a
-// quarter does (or nothing else can run), so their cost is shared by many
-// lanes instead of being paid on almost every pass.
+// Every pass advances the ITER lanes by up to AOPolicy::iters formula
+// iterations (an AO distance estimate of the 1080p power-8 bulb takes 3.6 on
+// average).  The AO step logic of DONE lanes runs once 1/done_div of the
+// wave waits for it, the item hand-out of NEED lanes once 1/need_div does
+// (or nothing else can run), so their cost is shared by many lanes instead
+// of being paid on almost every pass.  FP32: 2 iterations, 1/3, 1/8; FP64
+// (twice the registers, 2 waves per SIMD): 1 iteration, 1/6, 1/8 (measured
+// best of the sweeps on MI355X).
 enum : int { AO_NEED = 0, AO_ITER = 1, AO_DONE = 2, AO_EXHAUSTED = 3 };
+// Per element type: formula iterations per pass, and the share of the wave
+// (1/div) that must wait in DONE or NEED before their block runs.
+#ifndef MBR_AO_ITERS
+#define MBR_AO_ITERS 2
+#endif
+#ifndef MBR_AO_DONE_DIV
+#define MBR_AO_DONE_DIV 3
+#endif
+#ifndef MBR_AO_NEED_DIV
+#define MBR_AO_NEED_DIV 8
+#endif
+#ifndef MBR_AO64_ITERS
+#define MBR_AO64_ITERS 1
+#endif
+#ifndef MBR_AO64_DONE_DIV
+#define MBR_AO64_DONE_DIV 6
+#endif
+#ifndef MBR_AO64_NEED_DIV
+#define MBR_AO64_NEED_DIV 8
+#endif
+template <class T>
+struct AOPolicy {
+  static constexpr bool f64 = sizeof(T) == 8;
+  static constexpr int iters = f64 ? MBR_AO64_ITERS : MBR_AO_ITERS;
+  static constexpr int done_div = f64 ? MBR_AO64_DONE_DIV : MBR_AO_DONE_DIV;
+  static constexpr int need_div = f64 ? MBR_AO64_NEED_DIV : MBR_AO_NEED_DIV;
+};
 
 template <class T, int F, bool Deep, bool Plain, int IP = 0>
 __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
@@ -785,7 +822,7 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
     if (!(m_need | m_iter | m_done)) break;
     const int n_need = __popcll(m_need), n_iter = __popcll(m_iter), n_done = __popcll(m_done);
 
-    if (n_need && (4 * n_need >= n_need + n_iter + n_done || n_iter + n_done == 0)) {
+    if (n_need && (AOPolicy<T>::need_div * n_need >= n_need + n_iter + n_done || n_iter + n_done == 0)) {
       // hand out the next items to the lanes that need one, in lane order
       const unsigned n = (unsigned)n_need;
       const unsigned off = (unsigned)__popcll(m_need & lt_mask);
@@ -838,8 +875,10 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
 
     if (state == AO_ITER) {
       DEInfo info;
-      bool fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
-      if (!fin) fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
+      bool fin = false;
+#pragma unroll
+      for (int k = 0; k < AOPolicy<T>::iters && !fin; k++)
+        fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
       if (fin) {
         state = AO_DONE;
         mi = info.max_iter;
@@ -851,7 +890,7 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
     const unsigned long long m_done2 = __ballot(state == AO_DONE);
     const unsigned long long m_iter2 = __ballot(state == AO_ITER);
     const int n_done2 = __popcll(m_done2), n_iter2 = __popcll(m_iter2);
-    if (n_done2 && (3 * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
+    if (n_done2 && (AOPolicy<T>::done_div * n_done2 >= n_done2 + n_iter2 || n_iter2 == 0) && state == AO_DONE) {
       // one step of the AO ray (src/shaders.cpp:625-662)
       ao_steps++;
       if (dist < thr_f || mi) {
