@@ -359,6 +359,12 @@ def main():
     dtm, stm = timed(torch, r, planes, _abi.PREC_FP32_MARCH, 1, 1)
     fm = to_frame(torch, planes)
     diffm = compare(fm, f64)
+    # FP64 with the FP32 modes' trig-free power-8 iteration (march,
+    # continuation, AO): splits the FP32/FP64 ratio into the iteration form
+    # (fp64 / fp64_demoivre) and the arithmetic (fp64_demoivre / fp32)
+    dtd, std_ = timed(torch, r, planes, _abi.PREC_FP64_DEMOIVRE, args.fp64_steps, 0)
+    fd = to_frame(torch, planes)
+    diffd = compare(fd, f64)
 
     value = rays * args.steps / dt32 / 1e6
     value64 = rays * args.fp64_steps / dt64 / 1e6
@@ -413,6 +419,13 @@ def main():
                  "ao_achieved_tflops": ao64_flops / (ao64_ms * 1e-3) / 1e12,
                  "ao_frac_of_f64_peak": ao64_flops / (ao64_ms * 1e-3) / 1e12 / PEAK_TFLOPS["f64"]},
         "speedup_vs_fp64": value / value64,
+        "fp64_demoivre": {"value": rays * args.fp64_steps / dtd / 1e6, "unit": "Mrays/s",
+                          "ms_per_step": dtd / args.fp64_steps * 1e3,
+                          "kernel_ms": {"march": std_[-1].ms_march, "shade": std_[-1].ms_shade,
+                                        "ao": std_[-1].ms_ao, "finish": std_[-1].ms_finish},
+                          "pixel_diff_vs_fp64": {k: diffd[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
+                          "speedup_from_iteration_form": dt64 / dtd,
+                          "speedup_from_fp32_arithmetic": (dtd / args.fp64_steps) / (dt32 / args.steps)},
         "fp32_march": {"value": rays / dtm / 1e6, "unit": "Mrays/s", "march_ms": stm[-1].ms_march,
                        "fp64_redo_rays": stm[-1].fp64_redo_rays,
                        "pixel_diff_vs_fp64": {k: diffm[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")}},

@@ -108,11 +108,20 @@ enum mbr_jitter {
  *   FP32_MARCH  as FP32_FIXUP, but the march iterates in FP32 until FP32
  *               error could change a hit decision, continues in FP64 from
  *               there, and re-marches in FP64 the rays whose result the FP32
- *               part could have changed. */
+ *               part could have changed.
+ *   FP64_DEMOIVRE  FP64 everywhere, but the march, its FP64 continuation
+ *               and the AO rays of the power-8 Mandelbulb iterate with the
+ *               trig-free integer-power form of the FP32 modes (de Moivre,
+ *               hardware rsq) instead of the reference's asin/atan2/pow/
+ *               sin/cos; normals, shadow and colouring keep the trig form.
+ *               Not bit-exact with the reference; it separates what the
+ *               iteration form gains from what FP32 arithmetic gains.  Other
+ *               scenes render as FP64. */
 enum mbr_precision {
   MBR_PREC_FP64 = 0,
   MBR_PREC_FP32_FIXUP = 1,
-  MBR_PREC_FP32_MARCH = 2
+  MBR_PREC_FP32_MARCH = 2,
+  MBR_PREC_FP64_DEMOIVRE = 3
 };
 
 typedef struct mbr_rgb { int R, G, B; } mbr_rgb; /* sRGB (src/cimage.hpp:39-44) */

@@ -17,7 +17,7 @@ STATUS = {
 FORMULA_NONE, FORMULA_TRIG_OPTIM, FORMULA_MENGER_SPONGE, FORMULA_TGLAD = 0, 2, 7, 8
 PERSP_THREE_POINT, PERSP_FISH_EYE, PERSP_EQUIRECTANGULAR = 0, 1, 2
 JITTER_NONE, JITTER_HASH = 0, 1
-PREC_FP64, PREC_FP32_FIXUP, PREC_FP32_MARCH = 0, 1, 2
+PREC_FP64, PREC_FP32_FIXUP, PREC_FP32_MARCH, PREC_FP64_DEMOIVRE = 0, 1, 2, 3
 
 
 class RGB(C.Structure):

@@ -1034,7 +1034,9 @@ namespace {
 
 // The shading buffers of slot sl for a frame of the scene `host` (no
 // reflection levels: the shade keeps specular and object parts together).
-ShadeBufs frame_shade_bufs(Slot &sl, const DevScene &host, int formula) {
+// The power-8 Mandelbulb runs its trig-free kernels (ip8) in the FP32 modes
+// and in FP64_DEMOIVRE (trig_free); plain FP64 keeps the reference's trig.
+ShadeBufs frame_shade_bufs(Slot &sl, const DevScene &host, int formula, bool trig_free) {
   BandBufs &b = sl.band;
   b.sp_r = b.sp_g = b.sp_b = nullptr;
   b.ray_o = b.ray_d = b.ray_t0 = nullptr;
@@ -1052,7 +1054,7 @@ ShadeBufs frame_shade_bufs(Slot &sl, const DevScene &host, int formula) {
   sbuf.plain = host.plain;
   sbuf.formula = formula;
   sbuf.lights = host.n_lights > 0;
-  sbuf.ip8 = formula == 2 && host.fd.ipow == 8;
+  sbuf.ip8 = trig_free && formula == 2 && host.fd.ipow == 8;
   return sbuf;
 }
 
@@ -1087,9 +1089,10 @@ int enqueue_band(mbr_ctx *c, Slot &sl, const ShadeBufs &sbuf, int precision, con
 // batch passes none.
 int enqueue_frame(mbr_ctx *c, Slot &sl, int precision, const DevScene *d_sc, const DevScene &host, int formula,
                   AODev ao, long long first, long long last, const FrameOut &out, int persist, hipStream_t st,
-                  hipEvent_t *ev, volatile const int *cancel, unsigned long long *stats = nullptr) {
+                  hipEvent_t *ev, volatile const int *cancel, unsigned long long *stats = nullptr,
+                  bool demoivre = false) {
   unsigned long long *const d_stats = stats ? stats : c->d_stats;
-  const ShadeBufs sbuf = frame_shade_bufs(sl, host, formula);
+  const ShadeBufs sbuf = frame_shade_bufs(sl, host, formula, precision != MBR_PREC_FP64 || demoivre);
   const long long cap = sl.band.cap;
   int band = 0;
   for (long long band_first = first; band_first < last; band_first += cap, band++) {
@@ -1119,7 +1122,9 @@ constexpr int kSplitBands = 2;
 int enqueue_frame_split(mbr_ctx *c, Slot *sl[2], int precision, const DevScene *d_sc, const DevScene &host,
                         int formula, AODev ao, long long first, long long last, int n_bands, const FrameOut &out,
                         int persist, hipStream_t st, hipStream_t st1, hipEvent_t *ev, volatile const int *cancel) {
-  const ShadeBufs sbuf[2] = {frame_shade_bufs(*sl[0], host, formula), frame_shade_bufs(*sl[1], host, formula)};
+  // FP32 modes only: the trig-free power-8 kernels where they apply
+  const ShadeBufs sbuf[2] = {frame_shade_bufs(*sl[0], host, formula, true),
+                             frame_shade_bufs(*sl[1], host, formula, true)};
   const long long size = (last - first + n_bands - 1) / n_bands;
   if (size > sl[0]->band.cap || size > sl[1]->band.cap) return c->fail(MBR_ERR_INVALID_ARG, "split band above slot");
   HIPCHECK(c, hipEventRecord(c->split_fork, st));
@@ -1157,7 +1162,7 @@ ShadeBufs shade_bufs(Slot &sl, const DevScene &host, int formula) {
   sbuf.plain = host.plain;
   sbuf.formula = formula;
   sbuf.lights = host.n_lights > 0;
-  sbuf.ip8 = formula == 2 && host.fd.ipow == 8;
+  sbuf.ip8 = 0;  // reflections render in FP64 with the reference's trig
   return sbuf;
 }
 
@@ -1265,9 +1270,14 @@ void fill_stats(mbr_stats *stats, const unsigned long long *v, long long rays, i
   stats->ao_steps = v[ST_AO_STEPS];
 }
 
-int check_precision(mbr_ctx *c, int precision) {
-  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP && precision != MBR_PREC_FP32_MARCH)
+// Checks the requested precision; MBR_PREC_FP64_DEMOIVRE renders as FP64
+// with the trig-free power-8 iteration (demoivre set).
+int check_precision(mbr_ctx *c, int &precision, bool &demoivre) {
+  if (precision != MBR_PREC_FP64 && precision != MBR_PREC_FP32_FIXUP && precision != MBR_PREC_FP32_MARCH &&
+      precision != MBR_PREC_FP64_DEMOIVRE)
     return c->fail(MBR_ERR_INVALID_ARG, "unknown precision");
+  demoivre = precision == MBR_PREC_FP64_DEMOIVRE;
+  if (demoivre) precision = MBR_PREC_FP64;
   return MBR_OK;
 }
 
@@ -1282,7 +1292,8 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   if (fb->width != s.image_width || fb->height != s.image_height)
     return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
   int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
-  int rc = check_precision(c, precision);
+  bool demoivre = false;
+  int rc = check_precision(c, precision, demoivre);
   if (rc != MBR_OK) return rc;
   // formulas without kernels of their own (delta DE, trig_DE) and
   // reflections render in FP64
@@ -1350,7 +1361,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
                                persist, st, c->side[1], c->ev.data(), opts ? opts->cancel : nullptr);
     else
       rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
-                         c->ev.data(), opts ? opts->cancel : nullptr);
+                         c->ev.data(), opts ? opts->cancel : nullptr, nullptr, demoivre);
     if (rc != MBR_OK) return rc;
     HIPCHECK(c, hipEventRecord(frame_ev[1], st));
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
@@ -1483,7 +1494,8 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   if (fb->width != s.image_width || fb->height != s.image_height)
     return c->fail(MBR_ERR_INVALID_ARG, "framebuffer size differs from the scene's image size");
   int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
-  int rc = check_precision(c, precision);
+  bool demoivre = false;
+  int rc = check_precision(c, precision, demoivre);
   if (rc != MBR_OK) return rc;
   if (c->host_sc.refl_max > 0)
     return c->fail(MBR_ERR_UNSUPPORTED, "mbr_render_device_noalloc: raytraced reflections need a band slot per "
@@ -1524,7 +1536,7 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
   HIPCHECK(c, launch_zero(d_stats, ST_COUNT, st));
   rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st, nullptr,
-                     opts ? opts->cancel : nullptr, d_stats);
+                     opts ? opts->cancel : nullptr, d_stats, demoivre);
   if (rc != MBR_OK) return rc;
   unsigned long long v[ST_COUNT];
   HIPCHECK(c, hipMemcpyAsync(v, d_stats, sizeof v, hipMemcpyDeviceToHost, st));
@@ -1801,7 +1813,7 @@ struct BatchFrame {
 // `slots` chains greedily by ray count (largest first, to the least loaded
 // chain), chains fork from and join back to stream s0.
 int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_framebuffer *fbs, int precision,
-                      int slots, int bpc, hipStream_t s0, hipGraphExec_t *exec, int *nodes) {
+                      bool demoivre, int slots, int bpc, hipStream_t s0, hipGraphExec_t *exec, int *nodes) {
   const int persist = (c->cu_count > 0 ? c->cu_count : 256) * bpc;
   std::vector<int> order(frames.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
@@ -1819,10 +1831,14 @@ int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_fra
   hipGraph_t graph = nullptr;
   int rc = MBR_OK;
   HIPCHECK(c, hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+  int forked = 1;  // chains 1..forked-1 wait on the fork
   auto body = [&]() -> int {
     HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, s0));
     HIPCHECK(c, hipEventRecord(fork, s0));
-    for (int k = 1; k < slots; k++) HIPCHECK(c, hipStreamWaitEvent(c->side[k], fork, 0));
+    for (int k = 1; k < slots; k++) {
+      HIPCHECK(c, hipStreamWaitEvent(c->side[k], fork, 0));
+      forked = k + 1;
+    }
     for (int k = 0; k < slots; k++) {
       hipStream_t st = k == 0 ? s0 : c->side[k];
       for (int f : order) {
@@ -1831,7 +1847,7 @@ int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_fra
         const AODev ao{c->d_batch_ao_f + 4 * fr.ao_off, c->d_batch_ao_d + 4 * fr.ao_off,
                        c->d_batch_ao_rgb + 4 * fr.ao_off, fr.p.d.ao_count};
         int r = enqueue_frame(c, c->slots[k], precision, c->d_batch_sc + f, fr.p.d, fr.p.s.formula, ao, 0, fr.rays,
-                              frame_out(&fbs[f]), persist, st, nullptr, nullptr);
+                              frame_out(&fbs[f]), persist, st, nullptr, nullptr, nullptr, demoivre);
         if (r != MBR_OK) return r;
       }
     }
@@ -1842,6 +1858,15 @@ int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_fra
     return MBR_OK;
   };
   rc = body();
+  if (rc != MBR_OK) {
+    // a chain left forked would keep its stream in the (failed) capture:
+    // join every forked chain back before ending it (joining one twice is
+    // harmless)
+    for (int k = 1; k < forked; k++) {
+      (void)hipEventRecord(join[k], c->side[k]);
+      (void)hipStreamWaitEvent(s0, join[k], 0);
+    }
+  }
   hipError_t ec = hipStreamEndCapture(s0, &graph);
   if (rc == MBR_OK && ec != hipSuccess) rc = c->hip_fail(ec, "hipStreamEndCapture");
   if (rc == MBR_OK) {
@@ -1881,7 +1906,8 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
                      const mbr_batch_opts *opts, mbr_stats *stats, mbr_batch_result *result) {
   if (!c || !scenes || !fbs || n <= 0) return MBR_ERR_INVALID_ARG;
   int precision = opts ? opts->precision : MBR_PREC_FP32_FIXUP;
-  int rc = check_precision(c, precision);
+  bool demoivre = false;
+  int rc = check_precision(c, precision, demoivre);
   if (rc != MBR_OK) return rc;
   // a frame of a generic formula (FP64 only) makes the whole batch FP64
   for (int i = 0; i < n; i++)
@@ -1976,7 +2002,7 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
       const auto t0 = std::chrono::steady_clock::now();
       hipGraphExec_t ex = nullptr;
       int nodes = 0;
-      int r2 = build_batch_graph(c, frames, fbs, precision, sl_n, b_n, s0, &ex, &nodes);
+      int r2 = build_batch_graph(c, frames, fbs, precision, demoivre, sl_n, b_n, s0, &ex, &nodes);
       if (r2 != MBR_OK) return r2;
       res.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       res.graph_nodes = nodes;

@@ -196,7 +196,7 @@ __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<doub
                                    bool &ambiguous) {
   // trig-free integer-power iteration where the records are FP32, as in
   // k_march
-  constexpr bool kAlg = sizeof(RT) < 8;
+  constexpr bool kAlg = sizeof(RT) < 8 || IP != 0;
   const ShadeK<double> &k = sc.sd;
   const double search_limit = 1.0 - 0.01 * k.quality;
   const double margin = 4.0 * sqrt(s.drift);  // s.drift: variance of the FP32 scan deviation
@@ -364,7 +364,7 @@ __device__ __forceinline__ void march_body(const DevScene *__restrict__ scp, Ban
                                            int band_rays, unsigned long long *stats) {
   // integer-power Mandelbulb iterations without inverse trig wherever the
   // records are FP32; the FP64 mode keeps the trigonometric form
-  constexpr bool kAlg = sizeof(RT) < 8;
+  constexpr bool kAlg = sizeof(RT) < 8 || IP != 0;
   const DevScene &sc = *scp;
   // the power-8 variants read a few formula constants in every iteration: a
   // local copy keeps them in registers (the pass's stores and atomics keep
@@ -764,6 +764,8 @@ template <class T, int F, bool Deep, bool Plain, int IP = 0>
 __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBufs b, AODev ao, SurfBufs<T> sb,
                                         unsigned long long *stats) {
   const DevScene &sc = *scp;
+  // trig-free integer-power iteration in FP32, and in FP64 for the fixed power
+  constexpr bool kAlg = sizeof(T) < 8 || IP != 0;
   const long long n_hits = (long long)b.counters[CNT_HITS];
   unsigned long long *queue = &b.counters[CNT_AOQ];
   // power-8 variants: the formula constants in registers (as in k_march)
@@ -803,10 +805,10 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
         done = de_call_begin_p64<F>(sc.fd, f, P64 + cvt<double>(dir) * (double)r, detail, N, minN, false, call, dist,
                                     info);
       else
-        done = de_call_begin<T, F, Plain, (sizeof(T) < 8), IP>(f, P + dir * r, detail, N, minN, false, call, dist,
+        done = de_call_begin<T, F, Plain, kAlg, IP>(f, P + dir * r, detail, N, minN, false, call, dist,
                                                                info);
     } else {
-      done = de_call_begin<T, F, Plain, (sizeof(T) < 8), IP>(f, P + dir * r, detail, N, minN, false, call, dist, info);
+      done = de_call_begin<T, F, Plain, kAlg, IP>(f, P + dir * r, detail, N, minN, false, call, dist, info);
     }
     if (done) {
       state = AO_DONE;
@@ -878,7 +880,7 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
       bool fin = false;
 #pragma unroll
       for (int k = 0; k < AOPolicy<T>::iters && !fin; k++)
-        fin = de_call_step<T, F, Plain, (sizeof(T) < 8), IP>(f, call, detail, N, minN, false, dist, info);
+        fin = de_call_step<T, F, Plain, kAlg, IP>(f, call, detail, N, minN, false, dist, info);
       if (fin) {
         state = AO_DONE;
         mi = info.max_iter;
@@ -919,10 +921,10 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
   flush_stats(stats, v);
 }
 
-template <class T, int F, bool Deep, bool Plain>
+template <class T, int F, bool Deep, bool Plain, int IP = 0>
 __global__ void __launch_bounds__(kBlock) k_ao(const DevScene *__restrict__ scp, BandBufs b, AODev ao,
                                                SurfBufs<T> sb, unsigned long long *stats) {
-  ao_body<T, F, Deep, Plain>(scp, b, ao, sb, stats);
+  ao_body<T, F, Deep, Plain, IP>(scp, b, ao, sb, stats);
 }
 
 // The FP32 plain variant (the benchmarked AO pass) held to 6 waves per SIMD.
@@ -1485,7 +1487,8 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
     using RT = decltype(rtag);
     auto go = [&](auto plain_tag) {
       constexpr bool Pl = decltype(plain_tag)::value;
-      if constexpr (Pl && sizeof(RT) < 8) {
+      if constexpr (Pl) {
+        // the caller sets ip8 only where the trig-free iteration is wanted
         if (formula == 2 && ip8) {
           hipLaunchKernelGGL((k_march<T, RT, 2, true, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays,
                              stats);
@@ -1531,11 +1534,9 @@ hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b
       hipLaunchKernelGGL((k_march_fix<RT, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
     else {
       bool done = false;
-      if constexpr (sizeof(RT) < 8) {
-        if (ip8) {
-          hipLaunchKernelGGL((k_march_fix<RT, 2, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
-          done = true;
-        }
+      if (ip8) {
+        hipLaunchKernelGGL((k_march_fix<RT, 2, 8>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+        done = true;
       }
       if (!done) hipLaunchKernelGGL((k_march_fix<RT, 2>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
     }
@@ -1621,6 +1622,12 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
       if constexpr (!D && !Pl && std::is_same<T, double>::value) {
         if (generic) {
           hipLaunchKernelGGL((k_ao<T, 0, false, false>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
+          return;
+        }
+      }
+      if constexpr (!D && Pl && std::is_same<T, double>::value) {
+        if (formula == 2 && sbuf.ip8) {  // FP64_DEMOIVRE
+          hipLaunchKernelGGL((k_ao<T, 2, false, true, 8>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
           return;
         }
       }
