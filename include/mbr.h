@@ -374,7 +374,10 @@ typedef struct mbr_render_opts {
   int precision;                   /* enum mbr_precision */
   void *stream;                    /* hipStream_t; NULL = the context's stream */
   volatile const int *cancel;      /* cooperative stop flag (programClosed,
-                                      src/interface.h:924); may be NULL */
+                                      src/interface.h:924) read on the host
+                                      between bands; may be NULL.  To stop
+                                      a frame already on the GPU, call
+                                      mbr_cancel from another thread. */
   int row_begin, row_end;          /* render rows [row_begin,row_end); 0,0 = all */
 } mbr_render_opts;
 
@@ -431,6 +434,16 @@ int mbr_set_setting(mbr_scene *s, const char *key, const char *value);
 int mbr_init(const mbr_device_opts *opts, mbr_ctx **out);
 void mbr_destroy(mbr_ctx *ctx);
 const char *mbr_last_error(const mbr_ctx *ctx);
+
+/* Stops the context's render in progress, from any thread: the reference
+ * polls programClosed before every pixel (src/Render3D.cpp:369-378) and
+ * returns.  Here the flag lives in pinned host memory that the march and AO
+ * kernels read while handing out work, so a frame on the GPU stops within
+ * a few hundred microseconds: rays not yet marched are left empty and AO
+ * rays not yet started are skipped.  The render then returns
+ * MBR_ERR_CANCELLED with an incomplete frame, and the flag is cleared.  A
+ * cancel made while no render runs stops the next render at once. */
+int mbr_cancel(mbr_ctx *ctx);
 
 /* Replaces Params2Cl + CclSupport::SetParams (interface.cpp:3634,
  * cl_support.cpp:484).  Copies the scene; derives the rotation matrix, ray

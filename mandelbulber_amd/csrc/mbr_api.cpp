@@ -339,6 +339,10 @@ struct mbr_ctx {
   int cu_count = 0;
   std::vector<hipEvent_t> ev;  // 6 per band of the last frame
   hipEvent_t split_fork = nullptr, split_join = nullptr;  // the two-stream frame's fork and join
+  // mbr_cancel's flag: coherent pinned host memory (h_cancel) that the
+  // kernels read through d_cancel
+  int *h_cancel = nullptr;
+  int *d_cancel = nullptr;
   // scratch planes of mbr_render (host-buffer entry point)
   void *d_planes = nullptr;
   size_t d_planes_bytes = 0;
@@ -520,6 +524,7 @@ int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count, bool 
     HIPCHECK(c, hipMalloc((void **)&b.rec_it, (size_t)b.cap * (size_t)b.slab * sizeof(int)));
     HIPCHECK(c, hipMalloc((void **)&b.ovf_it, (size_t)b.ovf_chunks * kChunk * sizeof(int)));
   }
+  b.cancel = c->d_cancel;
   return MBR_OK;
 }
 
@@ -612,6 +617,12 @@ int mbr_init(const mbr_device_opts *opts, mbr_ctx **out) {
   }
   if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
     c->cu_count = 0;
+  if (hipHostMalloc((void **)&c->h_cancel, sizeof(int), hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->d_cancel, c->h_cancel, 0) != hipSuccess) {
+    mbr_destroy(c);
+    return MBR_ERR_HIP;
+  }
+  __atomic_store_n(c->h_cancel, 0, __ATOMIC_SEQ_CST);
   if (c->device < kMaxDevices) {
     // the device address of this GPU's copy of `scratch` (the device is
     // current), once per device
@@ -648,6 +659,7 @@ void mbr_destroy(mbr_ctx *c) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->h_cancel) (void)hipHostFree(c->h_cancel);
   if (c->split_fork) (void)hipEventDestroy(c->split_fork);
   if (c->split_join) (void)hipEventDestroy(c->split_join);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -655,6 +667,12 @@ void mbr_destroy(mbr_ctx *c) {
 }
 
 const char *mbr_last_error(const mbr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mbr_cancel(mbr_ctx *c) {
+  if (!c || !c->h_cancel) return MBR_ERR_INVALID_ARG;
+  __atomic_store_n(c->h_cancel, 1, __ATOMIC_SEQ_CST);
+  return MBR_OK;
+}
 
 }  // extern "C"
 
@@ -1270,6 +1288,15 @@ void fill_stats(mbr_stats *stats, const unsigned long long *v, long long rays, i
   stats->ao_steps = v[ST_AO_STEPS];
 }
 
+// After a render has finished on the device: MBR_ERR_CANCELLED when
+// mbr_cancel was called since the last cancelled render (the flag is then
+// cleared; the frame is incomplete), else MBR_OK.
+int consume_cancel(mbr_ctx *c) {
+  if (c->h_cancel && __atomic_exchange_n(c->h_cancel, 0, __ATOMIC_SEQ_CST) != 0)
+    return c->fail(MBR_ERR_CANCELLED, "render cancelled (mbr_cancel)");
+  return MBR_OK;
+}
+
 // Checks the requested precision; MBR_PREC_FP64_DEMOIVRE renders as FP64
 // with the trig-free power-8 iteration (demoivre set).
 int check_precision(mbr_ctx *c, int &precision, bool &demoivre) {
@@ -1366,6 +1393,8 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     HIPCHECK(c, hipEventRecord(frame_ev[1], st));
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
+    rc = consume_cancel(c);
+    if (rc != MBR_OK) return rc;
     if (v[ST_POOL_MISS] == 0) break;
     // some ray found the overflow pool full: grow it to hold every
     // overflowing ray of the frame and render again
@@ -1527,6 +1556,7 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
                        " MiB scratch holds fewer than " + std::to_string(kNoallocMinBand) +
                        " rays per band; render it with mbr_render_device");
   carve_scratch(base, precision, (int)cap, ao_count, iters, sl, &d_stats);
+  sl.band.cancel = c->d_cancel;
   hipStream_t st = (opts && opts->stream) ? (hipStream_t)opts->stream : c->stream;
   HIPCHECK(c, hipSetDevice(c->device));
   const FrameOut out = frame_out(fb);
@@ -1541,6 +1571,8 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   unsigned long long v[ST_COUNT];
   HIPCHECK(c, hipMemcpyAsync(v, d_stats, sizeof v, hipMemcpyDeviceToHost, st));
   HIPCHECK(c, hipStreamSynchronize(st));
+  rc = consume_cancel(c);
+  if (rc != MBR_OK) return rc;
   if (v[ST_POOL_MISS] != 0)
     return c->fail(MBR_ERR_OUT_OF_MEMORY, "mbr_render_device_noalloc: rays of more than " + std::to_string(kSlab) +
                                               " steps outgrew the scratch's overflow pool (" +
@@ -2012,6 +2044,8 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
       (void)hipGraphExecDestroy(ex);
       if (r2 != MBR_OK) return r2;
       HIPCHECK(c, hipMemcpy(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost));
+      r2 = consume_cancel(c);
+      if (r2 != MBR_OK) return r2;
       if (v[ST_POOL_MISS] == 0) return MBR_OK;
       if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
       for (int k = 0; k < kMaxSlots; k++) {
@@ -2110,6 +2144,8 @@ int run_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
     }
     HIPCHECK(c, hipMemcpyAsync(v, c->d_stats, sizeof v, hipMemcpyDeviceToHost, st));
     HIPCHECK(c, hipStreamSynchronize(st));
+    rc = consume_cancel(c);
+    if (rc != MBR_OK) return rc;
     if (v[ST_POOL_MISS] == 0) break;
     if (attempt + 1 >= kPoolAttempts) return c->fail(MBR_ERR_HIP, "overflow pool still full after growing it");
     rc = grow_pool(c, sl, v[ST_OVF_RAYS], v[ST_OVF_CHUNKS]);

@@ -350,6 +350,29 @@ __device__ __forceinline__ RecPlanes<RT> ray_records(const BandBufs &b, int r, u
   return rec;
 }
 
+// Cancel polling (mbr_cancel, the reference's programClosed test per pixel,
+// src/Render3D.cpp:369-378).  take_work(queue, n) is the hand-out of a band
+// work queue (lane 0 only): it returns the first of n items and whether the
+// band was stopped.  A hand-out whose range crosses a multiple of
+// 2^poll_shift reads the host flag (a read over the bus, so not on every
+// hand-out); when set it adds kStopBit to the queue, so every later
+// hand-out sees the stop in the value its own atomic returns.
+constexpr unsigned long long kStopBit = 1ull << 48;
+
+__device__ __forceinline__ unsigned long long take_work(const BandBufs &b, unsigned long long *queue,
+                                                        unsigned long long n, int poll_shift, int &stopped) {
+  unsigned long long v = atomicAdd(queue, n);
+  if (b.cancel && !(v & kStopBit)) {
+    const unsigned long long first = v & (kStopBit - 1);
+    if (((first + n) >> poll_shift) != (first >> poll_shift) && *b.cancel != 0) {
+      atomicAdd(queue, kStopBit);
+      v |= kStopBit;
+    }
+  }
+  stopped = (v & kStopBit) != 0;
+  return v & (kStopBit - 1);
+}
+
 // Persistent march: one primary ray per lane, up to kMarchIters formula
 // iterations per pass, lane states and amortised bookkeeping as in k_ao.
 enum : int { MS_NEED = 0, MS_ITER = 1, MS_DONE = 2, MS_EXHAUSTED = 3 };
@@ -407,8 +430,10 @@ __device__ __forceinline__ void march_body(const DevScene *__restrict__ scp, Ban
 
     if (n_need && (4 * n_need >= n_need + n_iter + n_done || n_iter + n_done == 0)) {
       unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&b.counters[CNT_RAYQ], (unsigned long long)n_need);
+      int stop = 0;
+      if (lane == 0) base = take_work(b, &b.counters[CNT_RAYQ], (unsigned long long)n_need, 12, stop);
       base = __shfl(base, 0);
+      stop = __shfl(stop, 0);
       if (state == MS_NEED) {
         const unsigned long long mine = base + (unsigned long long)__popcll(m_need & lt_mask);
         if (mine >= (unsigned long long)band_rays) {
@@ -418,7 +443,9 @@ __device__ __forceinline__ void march_body(const DevScene *__restrict__ scp, Ban
           b.ovf_off[r] = -1;
           const int p = band_first + r;
           const int x = p % sc.width, y = p / sc.width;
-          const bool cut = view_vector(sc, x, y, dir);
+          // a cancelled band's remaining rays take the empty record of a
+          // cut ray, so later kernels read defined (if meaningless) values
+          const bool cut = stop || view_vector(sc, x, y, dir);
           pixel = sc.pixel_base + (uint32_t)p;
           s = ray_start_state(sc);
           if (cut) {
@@ -836,7 +863,12 @@ __device__ __forceinline__ void ao_body(const DevScene *__restrict__ scp, BandBu
         const unsigned extra = n - avail;
         const unsigned khits = (extra + count - 1) / count;
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(queue, (unsigned long long)khits);
+        if (lane == 0) {
+          int stop = 0;
+          base = take_work(b, queue, (unsigned long long)khits, 8, stop);
+          // cancelled: no further hits (their AO intensities stay unwritten)
+          if (stop) base = (unsigned long long)n_hits;
+        }
         base = __shfl(base, 0);
         if (off >= avail) {
           const unsigned j = off - avail;

@@ -20,8 +20,8 @@ enum StatIdx {
 
 // Band counters (BandBufs::counters), zeroed on the device at band start:
 // hits appended, rays handed to the FP64 continuation, primary-ray queue,
-// overflow runs handed out, AO work queue,
-// rays listed for the FP64 redo.
+// overflow runs handed out, AO work queue, rays listed for the FP64 redo,
+// overflow chunks.
 enum CounterIdx {
   CNT_HITS = 0, CNT_HAND = 1, CNT_RAYQ = 2, CNT_DIRS = 3, CNT_AOQ = 4, CNT_REDO = 5, CNT_CHUNKS = 6, CNT_COUNT = 8
 };
@@ -77,6 +77,9 @@ struct BandBufs {
   void *sp_r, *sp_g, *sp_b;
   double *ray_o, *ray_d, *ray_t0;
   uint8_t *active;
+  // the context's cancel flag (mbr_cancel): pinned host memory the march
+  // and AO kernels poll while handing out work; null: never polled
+  const volatile int *cancel;
 };
 
 struct FrameOut {

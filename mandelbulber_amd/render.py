@@ -97,6 +97,11 @@ class Renderer:
         o.row_begin, o.row_end = rows if rows is not None else (0, 0)
         return o
 
+    def cancel(self):
+        """mbr_cancel: stop the render in progress on this context (from
+        another thread); it then raises MbrError(MBR_ERR_CANCELLED)."""
+        check(lib().mbr_cancel(self._ctx), self._ctx)
+
     def render(self, precision=_abi.PREC_FP32_FIXUP, rows=None):
         """Render into host planes; returns (Frame, Stats)."""
         s = self.scene

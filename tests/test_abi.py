@@ -44,3 +44,4 @@ def test_null_arguments_rejected():
     assert lib().mbr_scene_defaults(None) == -1
     assert lib().mbr_set_scene(None, None, None) == -1
     assert lib().mbr_render(None, None, None, None) == -1
+    assert lib().mbr_cancel(None) == -1
