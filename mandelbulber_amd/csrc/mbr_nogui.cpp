@@ -22,7 +22,9 @@
 // The image goes to file_destination + frame number (%05d) + ".png", as
 // IndexFilename (src/files.cpp:30-35) names it; with the settings' tiles > 1
 // the frame is rendered tile by tile and joined into file_destination +
-// "_fromTiles.png" (16-bit).  There is no JPEG encoder in
+// "_fromTiles.png" (16-bit).  A frame or tile whose output file exists is
+// skipped, as in the reference's -nogui mode, so a second run resumes an
+// interrupted animation or tiled frame.  There is no JPEG encoder in
 // this build: -format jpg writes an 8-bit PNG.  Network rendering (-ip,
 // -port) is not part of the render core and is rejected.
 #include <sys/stat.h>
@@ -174,13 +176,31 @@ int render_image(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::str
 // done the tiles are joined into file_destination + "_fromTiles.png", a
 // 16-bit PNG of (width x tiles) x (height x tiles), and removed
 // (SaveFromTilesPNG16, src/files.cpp:269-373).
+//
+// Resume: -nogui always saves, and the reference then skips every frame or
+// tile whose output file already exists (src/Render3D.cpp:2335-2350), so an
+// interrupted animation or tiled render continues where it stopped when run
+// again; tiles left by the interrupted run are joined with the new ones.
+
 int render_frame(mbr_ctx *ctx, const mbr_scene &s, int precision, const std::string &format, std::string out,
                  int index) {
   const int tiles = s.tiles < 1 ? 1 : (s.tiles > 100 ? 100 : s.tiles);
-  if (tiles == 1) return render_image(ctx, s, precision, format, out, index, nullptr);
+  if (tiles == 1) {
+    const std::string path = out.empty() ? index_filename(s.file_destination, "png", index) : out;
+    if (file_exists(path)) {
+      std::printf("Output file (%s) exists. Skip to next file index\n", path.c_str());
+      return 0;
+    }
+    return render_image(ctx, s, precision, format, out, index, nullptr);
+  }
   const int w = s.image_width, h = s.image_height;
   for (int tile = 0; tile < tiles * tiles; tile++) {
     const int index2 = index * tiles * tiles + tile;
+    if (file_exists(index_filename(s.file_destination, "tile", index2))) {
+      std::printf("Output file (%s) exists. Skip to next file index\n",
+                  index_filename(s.file_destination, "tile", index2).c_str());
+      continue;
+    }
     std::printf("---------- Tile: %d -------------\n", index2);
     mbr_scene ts = s;
     ts.tiles = tiles;

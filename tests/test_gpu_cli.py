@@ -161,3 +161,59 @@ def test_cli_tiles_join_into_the_full_frame(tmp_path):
     assert s.jitter_mode == 0
     expect = _library_png(s, precision=_abi.PREC_FP64)
     assert np.array_equal(px.astype(np.uint16), expect)
+
+
+def test_cli_animation_resumes_skipping_existing_frames(tmp_path):
+    """-nogui skips a frame whose output file exists (src/Render3D.cpp:
+    2335-2350): after an interrupted flight (frame 1 missing), a second run
+    renders only frame 1 and leaves the other files untouched."""
+    path = tmp_path / "path.txt"
+    cams = [(0.0, -3.5, 0.0, 0.0, 0.0, 0.0), (0.1, -3.4, 0.05, 0.05, -0.02, 0.01),
+            (0.2, -3.3, 0.1, 0.1, -0.04, 0.02)]
+    path.write_text("".join("%.11f %.11f %.11f %f %f %f\n" % c for c in cams))
+    main = tmp_path / "main.fract"
+    main.write_text(SETTINGS + f"file_animation_path {path};\nfile_destination {tmp_path}/fl/frame;\n")
+    cmd = [CLI, "-flight", "-res", "48x36", "-format", "png16", "-end", "10", str(main)]
+    print("[cli] resume: first run", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    files = [tmp_path / "fl" / f"frame{i:05d}.png" for i in range(3)]
+    full = [f.read_bytes() for f in files]
+    files[1].unlink()
+    # mark the kept frames: a resumed run must not rewrite them
+    files[0].write_bytes(b"kept")
+    print("[cli] resume: second run", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("exists. Skip to next file index") == 2
+    assert r.stdout.count("image saved") == 1
+    assert files[0].read_bytes() == b"kept"
+    assert files[1].read_bytes() == full[1] and files[2].read_bytes() == full[2]
+
+
+def test_cli_tiles_resume_joins_tiles_left_by_an_interrupted_run(tmp_path):
+    """A tile file already present is not rendered again: it is joined as
+    it is (here a tile of zeros planted in place of tile 3), the others are
+    rendered and joined around it."""
+    from test_image import _read_png
+    f = tmp_path / "scene.fract"
+    dest = tmp_path / "out" / "tiled"
+    f.write_text(SETTINGS + f"file_destination {dest};\npost_SSAO_enabled 0;\ntiles 2;\njitter_mode 0;\n")
+    (tmp_path / "out").mkdir()
+    planted = tmp_path / "out" / "tiled00003.tile"
+    planted.write_bytes(np.zeros(3 * 48 * 36, np.uint16).tobytes())
+    print("[cli] tiles resume", flush=True)
+    r = subprocess.run([CLI, "-res", "48x36", "-format", "png16", "-precision", "fp64", str(f)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("exists. Skip to next file index") == 1
+    assert r.stdout.count("---------- Tile:") == 3
+    px, _ = _read_png(str(tmp_path / "out" / "tiled_fromTiles.png"))
+    s = scene.load(str(f), ["image_width=96", "image_height=72", "tiles=1"])
+    expect = _library_png(s, precision=_abi.PREC_FP64)
+    # tile 3 is the bottom-right quarter
+    assert not px[36:, 48:].any()
+    assert np.array_equal(px[:36].astype(np.uint16), expect[:36])
+    assert np.array_equal(px[36:, :48].astype(np.uint16), expect[36:, :48])
