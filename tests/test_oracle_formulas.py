@@ -8,7 +8,17 @@ mandelbulb2/3/4 and xenodreambuie, and the delta DE of CalculateDistance
 - on the positive x axis trig_DE's derivative vector stays on the axis, so its
   distance estimate equals trig_optim's (already pinned) there;
 - the delta DE's linear and logarithmic forms differ by the factor ln r, and a
-  point that never escapes has distance 0."""
+  point that never escapes has distance 0;
+- foldingIntPow2 (:794-836): its map in complex form, (x + iy)^2 / rho^2
+  Re(B), Im(B), B = (rho - iz)^2, with the z factor applied after c; the
+  squared norm; the box fold as a mirror at +-F (same orbit in Julia mode);
+  the sphere fold as z *= 4 and as the inversion p / r^2;
+- aexion (:916-947): the tetrahedral start and the 4-D map restated in the
+  sum/difference coordinates in which it decouples, one and two iterations;
+- benesi (:948-970): the planar squaring -2|x| (y + iz)^2 / rho, both
+  branches of the x rule, and the squared norm.
+Each of these fails when a sign, a term, the fold offset, the sphere-fold
+radius, the z-factor order or the branch rule is changed in the oracle."""
 import cmath
 import math
 
@@ -237,3 +247,143 @@ def test_hybrid_sequence_by_hand(cyclic, want):
     for x in (0.3, -0.7, 1.9):
         r, it = oracle.compute(s, (x, 0.0, 0.0), mode=MODE_DELTA1)
         assert it == 4 and r == pytest.approx(abs(want(x)), rel=1e-15)
+
+
+# --- foldingIntPow2 (src/fractal.cpp:794-836) ---------------------------------
+# Box fold at +-F, sphere fold (min radius^2 0.25, fixed 1), doubling, then the
+# power-2 map (x, y, z) -> ((x^2 - y^2) t, 2xy t, -2 z rho), t = 1 - z^2/rho^2,
+# rho = |(x, y)|, plus c, and finally z *= Zf.  In complex terms the map is
+# (x + iy)^2 / rho^2 * Re(B) and Im(B) with B = (rho - iz)^2: numpy's complex
+# arithmetic states it independently of the oracle's scalar form.
+
+def _fip_scene(F=10.0, zf=1.0, julia=J):
+    s = _scene(16, 2.0, julia=julia)
+    s.folding_int_pow_factor = F
+    s.folding_int_pow_z_factor = zf
+    return s
+
+
+def _fip_map(w, c, zf):
+    x, y, z = w
+    rho = math.hypot(x, y)
+    a = complex(x, y) ** 2 / rho ** 2
+    b = complex(rho, -z) ** 2
+    xy = a * b.real
+    return np.array([xy.real + c[0], xy.imag + c[1], (b.imag + c[2]) * zf])
+
+
+@pytest.mark.parametrize("p", [(0.7, -0.5, 0.6), (-0.6, 0.3, -0.9), (0.3, 0.95, 0.2), (-0.8, -0.7, -0.2)])
+@pytest.mark.parametrize("zf", [1.0, 5.0])
+def test_folding_int_pow2_map_outside_the_folds(p, zf):
+    """|p| >= 1 and every |p_i| <= F: no fold applies, z1 = map(2p) (with
+    z *= Zf after adding c, so a Zf applied before the constant fails)."""
+    assert math.sqrt(sum(v * v for v in p)) >= 1.0
+    z1 = _fip_map(2.0 * np.array(p), J, zf)
+    assert _r1(_fip_scene(zf=zf), p) == pytest.approx(np.linalg.norm(z1), rel=1e-12)
+
+
+def test_folding_int_pow2_map_preserves_the_squared_norm():
+    """With c = 0 the map squares the radius: |z1| = |2p|^2 (Zf = 1)."""
+    for p in [(0.7, -0.5, 0.6), (-0.6, 0.3, -0.9), (1.1, 0.2, -0.3)]:
+        assert sum(v * v for v in p) >= 1.0
+        r = 2.0 * math.sqrt(sum(v * v for v in p))
+        assert _r1(_fip_scene(julia=(0.0, 0.0, 0.0)), p) == pytest.approx(r * r, rel=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_folding_int_pow2_box_fold_mirrors_at_plus_minus_F(axis, sign):
+    """Past +-F a component becomes +-2F - z_i: in Julia mode (fixed c) the
+    point and its mirror image inside the box have the same orbit."""
+    F = 1.2
+    p = np.array([0.5, -0.6, 0.7])
+    p[axis] = sign * 1.5
+    q = p.copy()
+    q[axis] = sign * 2.0 * F - p[axis]
+    s = _fip_scene(F=F)
+    assert _r1(s, tuple(p)) == pytest.approx(_r1(s, tuple(q)), rel=1e-14)
+    # two iterations as well (the fold acts on every iteration's input)
+    a, _ = oracle.compute(s, tuple(p), mode=MODE_DELTA1, n_iter=2)
+    b, _ = oracle.compute(s, tuple(q), mode=MODE_DELTA1, n_iter=2)
+    assert a == pytest.approx(b, rel=1e-13)
+
+
+def test_folding_int_pow2_sphere_fold():
+    """r^2 < 0.25: z *= 4 (then no fold applies to 4p when 1 <= |4p| < F), so
+    p and 4p have the same orbit; 0.25 <= r^2 < 1: z *= 1/r^2, the inversion
+    p / r^2."""
+    s = _fip_scene()
+    small = (0.2, -0.25, 0.15)           # |p| ~ 0.35
+    assert _r1(s, small) == _r1(s, tuple(4.0 * np.array(small)))
+    mid = np.array([0.5, 0.4, -0.45])    # |p| ~ 0.78
+    r2 = float(mid @ mid)
+    assert _r1(s, tuple(mid)) == pytest.approx(_r1(s, tuple(mid / r2)), rel=1e-13)
+
+
+# --- aexion (src/fractal.cpp:916-947) ---------------------------------------
+# The first iteration maps z and c to the four tetrahedral projections
+# |(+x+y+z, -x-y+z, -x+y-z, +x-y-z)| + c_add; every iteration is then the 4-D
+# map whose sum and difference coordinates s1 = x + y, d1 = x - y,
+# s2 = z + w, d2 = z - w decouple:
+#   s1' = s2^2 - d2^2 + (cx + cy),  d1' = 2 s1 d1 + (cx - cy),
+#   s2' = s1^2 - d1^2 + (cz + cw),  d2' = 2 s2 d2 + (cz - cw),
+# and |(x, y, z, w)|^2 = (s1^2 + d1^2 + s2^2 + d2^2) / 2.
+
+_TETRA = np.array([[1, 1, 1], [-1, -1, 1], [-1, 1, -1], [1, -1, -1]], dtype=float)
+
+
+def _aexion_orbit(p, c, cadd, n):
+    z = np.abs(_TETRA @ np.array(p)) + cadd
+    k = np.abs(_TETRA @ np.array(c)) + cadd
+    s1, d1, s2, d2 = z[0] + z[1], z[0] - z[1], z[2] + z[3], z[2] - z[3]
+    for _ in range(n):
+        s1, d1, s2, d2 = (s2 * s2 - d2 * d2 + k[0] + k[1], 2 * s1 * d1 + k[0] - k[1],
+                          s1 * s1 - d1 * d1 + k[2] + k[3], 2 * s2 * d2 + k[2] - k[3])
+    return math.sqrt((s1 * s1 + d1 * d1 + s2 * s2 + d2 * d2) / 2.0)
+
+
+@pytest.mark.parametrize("p", POINTS)
+@pytest.mark.parametrize("cadd", [0.0, -0.35])
+def test_aexion_in_sum_and_difference_coordinates(p, cadd):
+    s = _scene(19, 2.0, julia=J)
+    s.c_add = cadd
+    for n in (1, 2):
+        got, _ = oracle.compute(s, p, mode=MODE_DELTA1, n_iter=n)
+        assert got == pytest.approx(_aexion_orbit(p, J, cadd, n), rel=1e-12), n
+
+
+def test_aexion_constant_is_the_point_outside_julia_mode():
+    """Without Julia mode c is the point itself, transformed like z."""
+    p = (0.31, -0.22, 0.18)
+    s = _scene(19, 2.0)
+    s.c_add = -0.2
+    got, _ = oracle.compute(s, p, mode=MODE_DELTA1, n_iter=1)
+    assert got == pytest.approx(_aexion_orbit(p, p, -0.2, 1), rel=1e-12)
+
+
+# --- benesi (src/fractal.cpp:948-970) ---------------------------------------
+# With rho = |(y, z)|: x' = x^2 - rho^2 when c.x < 0 or x < rho, else
+# rho^2 - x^2; (y' + i z') = -2|x| (y + iz)^2 / rho; plus c.  The map keeps
+# |z1 - c| = |p|^2.
+
+def _benesi(p, c):
+    x, y, z = p
+    rho = math.hypot(y, z)
+    nx = x * x - rho * rho if (c[0] < 0 or x < rho) else rho * rho - x * x
+    w = -2.0 * abs(x) * complex(y, z) ** 2 / rho
+    return np.array([nx + c[0], w.real + c[1], w.imag + c[2]])
+
+
+@pytest.mark.parametrize("p", [(0.3, -0.7, 0.45), (-1.1, 0.2, -0.6), (0.9, 0.3, -0.2), (1.4, -0.5, 0.6)])
+@pytest.mark.parametrize("c", [(0.13, -0.27, 0.31), (-0.2, 0.1, -0.05)])
+def test_benesi_planar_squaring_and_sign_rule(p, c):
+    """Both branches of the x rule: c.x < 0 always takes x^2 - rho^2; with
+    c.x >= 0 the points with x >= rho (0.9, 1.4 here) take rho^2 - x^2."""
+    s = _scene(20, 2.0, julia=c)
+    assert _r1(s, p) == pytest.approx(np.linalg.norm(_benesi(p, c)), rel=1e-12)
+
+
+def test_benesi_squares_the_radius():
+    s = _scene(20, 2.0, julia=(0.0, 0.0, 0.0))
+    for p in [(0.3, -0.7, 0.45), (1.4, -0.5, 0.6), (-0.2, 0.05, 0.9)]:
+        assert _r1(s, p) == pytest.approx(sum(v * v for v in p), rel=1e-12)
