@@ -28,6 +28,9 @@ METRIC = "Mrays/s at 1080p Mandelbulb: FP32+FP64-fixup vs plain FP64, ≤0.1% px
 # FP32 issue, which this build leaves disabled; the unpacked FP32 rate equals
 # the FP64 one.
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+# pixel differences reported between two renders (tests/metrics.py): differing
+# pixels = hit/miss flips + sheet jumps (+ NaN-only colours)
+DIFF_KEYS = ("flip_frac", "hitmiss_px", "sheet_px", "depth_rel_max", "depth_rel_max_all", "psnr_db")
 
 # HBM bytes per k_ao launch on this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1,
 # the factors that make a 1 GiB device copy read 1 GiB and write 1 GiB),
@@ -245,7 +248,7 @@ def bench_deep_zoom(torch, r, _abi, scene, compare, width, height):
         frames[name] = to_frame(torch, planes)
         out[name] = {"value": width * height / dt / 1e6, "unit": "Mrays/s", "ms": dt * 1e3, "hits": st.hits}
     d = compare(frames["fp32_fixup"], frames["fp64"])
-    out["fp32_vs_fp64"] = {k: d[k] for k in ("flip_frac", "depth_rel_max", "psnr_db", "rgb_max_abs")}
+    out["fp32_vs_fp64"] = {k: d[k] for k in ("flip_frac", "hitmiss_px", "sheet_px", "depth_rel_max", "depth_rel_max_all", "psnr_db", "rgb_max_abs")}
     out["speedup_vs_fp64"] = out["fp32_fixup"]["value"] / out["fp64"]["value"]
     return out
 
@@ -423,13 +426,13 @@ def main():
                           "ms_per_step": dtd / args.fp64_steps * 1e3,
                           "kernel_ms": {"march": std_[-1].ms_march, "shade": std_[-1].ms_shade,
                                         "ao": std_[-1].ms_ao, "finish": std_[-1].ms_finish},
-                          "pixel_diff_vs_fp64": {k: diffd[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
+                          "pixel_diff_vs_fp64": {k: diffd[k] for k in DIFF_KEYS},
                           "speedup_from_iteration_form": dt64 / dtd,
                           "speedup_from_fp32_arithmetic": (dtd / args.fp64_steps) / (dt32 / args.steps)},
         "fp32_march": {"value": rays / dtm / 1e6, "unit": "Mrays/s", "march_ms": stm[-1].ms_march,
                        "fp64_redo_rays": stm[-1].fp64_redo_rays,
-                       "pixel_diff_vs_fp64": {k: diffm[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")}},
-        "pixel_diff_vs_fp64": {k: diff[k] for k in ("flip_frac", "depth_rel_max", "psnr_db")},
+                       "pixel_diff_vs_fp64": {k: diffm[k] for k in DIFF_KEYS}},
+        "pixel_diff_vs_fp64": {k: diff[k] for k in DIFF_KEYS},
         "kernel_ms": {"march": s0.ms_march, "shade": s0.ms_shade, "ao": s0.ms_ao, "finish": s0.ms_finish},
         "march_counters": {"fp64_redo_rays": s0.fp64_redo_rays, "step_overflow_rays": s0.step_overflow_rays,
                            "march_de_calls": s0.march_de_calls, "shade_de_calls": s0.shade_de_calls,

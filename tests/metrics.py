@@ -1,8 +1,16 @@
 """Image comparison metrics of the parity tests.
 
-The bars (per config): at most 0.1% of pixels may flip hit/miss, the hit
-depth (zBuffer) relative error on pixels both renders hit must be <= 1e-4,
-and the RGB PSNR of imageFloat must be >= 45 dB (peak 1.0).
+The bars (per config): at most 0.1% of pixels may differ, the hit depth
+(zBuffer) relative error on agreeing pixels must be <= 1e-4, and the RGB
+PSNR of imageFloat must be >= 45 dB (peak 1.0).
+
+A differing pixel ("flips", "flip_frac") is one of: a hit/miss flip
+("hitmiss_px"); a sheet jump ("sheet_px": both renders hit, but at depths
+1% or more apart, i.e. on another sheet of the surface, which happens at
+grazing silhouettes when the last bits of a DE differ); a NaN colour in one
+render only.  Agreeing pixels are the hits on the same sheet; "depth_rel_max"
+is taken over them, and "depth_rel_max_all" over every pixel both renders
+hit (sheet jumps included), so neither count hides the other.
 """
 import numpy as np
 
@@ -14,16 +22,21 @@ def compare(a, b):
     hit_a = a.z < MISS_Z
     hit_b = b.z < MISS_Z
     n = hit_a.size
-    flips = int(np.count_nonzero(hit_a != hit_b))
+    hitmiss = int(np.count_nonzero(hit_a != hit_b))
+    flips = hitmiss
+    sheet = 0
+    depth_max_all = 0.0
     both = hit_a & hit_b
     if both.any():
         za = a.z[both].astype(np.float64)
         zb = b.z[both].astype(np.float64)
         rel = np.abs(za - zb) / np.maximum(np.abs(zb), 1e-30)
-        # agreeing pixels: hit the same surface (depths within 1%); a pixel
-        # whose ray lands on another sheet of the surface is a flip
+        depth_max_all = float(rel.max())
+        # agreeing pixels: hit the same sheet (depths within 1%); a ray that
+        # lands on another sheet is a differing pixel
         same = rel < 1e-2
-        flips += int(np.count_nonzero(~same))
+        sheet = int(np.count_nonzero(~same))
+        flips += sheet
         rel_same = rel[same]
         depth_max = float(rel_same.max()) if rel_same.size else 0.0
         depth_p999 = float(np.quantile(rel_same, 0.999)) if rel_same.size else 0.0
@@ -46,6 +59,7 @@ def compare(a, b):
     psnr = float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
     out = {
         "pixels": n, "hits": int(hit_b.sum()), "flips": flips, "flip_frac": flips / n,
+        "hitmiss_px": hitmiss, "sheet_px": sheet, "depth_rel_max_all": depth_max_all,
         "depth_rel_max": depth_max, "depth_rel_p999": depth_p999, "depth_bad_px": depth_bad, "psnr_db": psnr,
         "rgb_max_abs": float(np.abs(d).max()), "nan_mismatch_px": int(np.count_nonzero(nan_px)),
     }
