@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MBR_ABI_VERSION 1
+#define MBR_ABI_VERSION 2
 
 /* largest reflections_max the render core accepts (mirrored rays per pixel) */
 #define MBR_MAX_REFLECTIONS 16
@@ -405,6 +405,8 @@ typedef struct mbr_stats {
   uint64_t ao_de_calls;            /* DE calls of the AO rays (in shade_de_calls) */
   uint64_t ao_iterations;          /* formula iterations of those calls */
   uint64_t ao_steps;               /* AO ray steps taken */
+  double ms_ao_span;               /* from the first band's AO launch start to the
+                                      last band's AO end (equals ms_ao on one stream) */
 } mbr_stats;
 
 typedef struct mbr_device_opts {

@@ -5,7 +5,7 @@ when the library is loaded (see ``mandelbulber_amd._lib``).
 """
 import ctypes as C
 
-MBR_ABI_VERSION = 1
+MBR_ABI_VERSION = 2
 
 MBR_OK = 0
 STATUS = {
@@ -179,7 +179,7 @@ class Stats(C.Structure):
         ("missed_de", C.c_uint64),
         ("step_overflow_rays", C.c_uint64), ("fp64_redo_rays", C.c_uint64),
         ("ms_ao", C.c_double), ("ao_de_calls", C.c_uint64), ("ao_iterations", C.c_uint64),
-        ("ao_steps", C.c_uint64),
+        ("ao_steps", C.c_uint64), ("ms_ao_span", C.c_double),
     ]
 
 

@@ -1422,6 +1422,18 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   }
   float ms_frame = 0;
   HIPCHECK(c, hipEventElapsedTime(&ms_frame, frame_ev[0], frame_ev[1]));
+  // the AO phase of the frame: first AO start to last AO end, over bands
+  // that may have run on two streams (event times relative to the frame's
+  // first event, recorded on st before any band)
+  double ao_first = 1e300, ao_last = -1e300;
+  for (int band = 0; band < n_bands; band++) {
+    const hipEvent_t *e = &c->ev[6 * band];
+    float a = 0, b = 0;
+    HIPCHECK(c, hipEventElapsedTime(&a, frame_ev[0], e[4]));
+    HIPCHECK(c, hipEventElapsedTime(&b, frame_ev[0], e[5]));
+    ao_first = std::min(ao_first, (double)a);
+    ao_last = std::max(ao_last, (double)b);
+  }
   if (stats) {
     fill_stats(stats, v, last - first, n_bands);
     // the parts are sums over bands; with two streams they overlap, and the
@@ -1430,6 +1442,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     stats->ms_shade = ms_shade;
     stats->ms_finish = ms_finish;
     stats->ms_ao = ms_ao;
+    stats->ms_ao_span = n_bands > 0 ? ao_last - ao_first : 0.0;
     stats->ms_total = ms_frame;
   }
   return MBR_OK;
