@@ -63,6 +63,7 @@ enum mbr_formula {
   MBR_FORMULA_MANDELBULB4 = 15,
   MBR_FORMULA_FOLDING_INT_POW2 = 16,
   MBR_FORMULA_SMOOTH_MANDELBOX = 17,
+  MBR_FORMULA_MANDELBOX_VARY_SCALE_4D = 18,
   MBR_FORMULA_AEXION = 19,
   MBR_FORMULA_BENESI = 20,
   MBR_FORMULA_BRISTORBROT = 21,
@@ -71,10 +72,11 @@ enum mbr_formula {
   MBR_FORMULA_SPHERICAL_FOLD = 26, MBR_FORMULA_POW_XYZ = 27,
   MBR_FORMULA_SCALE_X = 28, MBR_FORMULA_SCALE_Y = 29, MBR_FORMULA_SCALE_Z = 30,
   MBR_FORMULA_OFFSET_X = 31, MBR_FORMULA_OFFSET_Y = 32, MBR_FORMULA_OFFSET_Z = 33,
-  MBR_FORMULA_ANGLE_MULTIPLY_X = 34, MBR_FORMULA_ANGLE_MULTIPLY_Y = 35, MBR_FORMULA_ANGLE_MULTIPLY_Z = 36
+  MBR_FORMULA_ANGLE_MULTIPLY_X = 34, MBR_FORMULA_ANGLE_MULTIPLY_Y = 35, MBR_FORMULA_ANGLE_MULTIPLY_Z = 36,
+  MBR_FORMULA_GENERALIZED_FOLD_BOX = 37
   /* all but 2, 7 and 8 render through the generic FP64 path (analytic DE for
-   * 1 and 17, delta DE for the rest, as the reference's analitycDE): every
-   * precision mode renders them in FP64 */
+   * 1, 17, 18 and 37, delta DE for the rest, as the reference's analitycDE):
+   * every precision mode renders them in FP64 */
 };
 
 /* enumPerspectiveType (src/common_math.h) */
@@ -176,7 +178,7 @@ typedef struct mbr_scene {
   int spherical_folding_mode;      /* spherical_folding_mode (pre-fold) */
   double spherical_folding_fixed;  /* spherical_folding_fixed */
   double spherical_folding_min;    /* spherical_folding_min */
-  int ifs_folding_mode;            /* IFS_folding_mode (not supported: must be 0) */
+  int ifs_folding_mode;            /* IFS_folding_mode: the IFS vectors fold z before every iteration */
 
   /* Mandelbox (sFractalMandelbox, src/fractal.h:172-179) */
   double mb_scale;                 /* mandelbox_scale */
@@ -349,6 +351,21 @@ typedef struct mbr_scene {
   double sphere_reflect;                /* primitive_sphere_reflect */
   double inv_sphere_reflect;            /* primitive_invertedSphere_reflect */
   double water_reflect;                 /* primitive_water_reflect */
+
+  /* generalizedFoldBox (formula 37, src/fractal.cpp:531-749) and
+   * mandelboxVaryScale4D (formula 18, :887-915); both also use the Mandelbox
+   * sphere fold, scale, offset, rotation and colour factors above */
+  int mb_fold_mode;                     /* mandelbox_fold_mode: fold polyhedron, 0 tetrahedron,
+                                           1 cube, 2 octahedron, 3 dodecahedron, 4 octahedron +
+                                           cube, 5 icosahedron, 6 hexagonal prism, 7 pentagonal
+                                           prism (enumGeneralizedFoldBoxType, src/fractal.h:72-82) */
+  double mb_solid;                      /* mandelbox_solid: distance of the fold planes */
+  double mb_melt;                       /* mandelbox_melt */
+  double mb_vary_scale_vary;            /* mandelbox_vary_scale_vary */
+  double mb_vary_fold;                  /* mandelbox_vary_fold */
+  double mb_vary_minr;                  /* mandelbox_vary_minr */
+  double mb_vary_rpower;                /* mandelbox_vary_rpower */
+  double mb_vary_wadd;                  /* mandelbox_vary_wadd */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and

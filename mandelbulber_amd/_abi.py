@@ -109,6 +109,9 @@ class Scene(C.Structure):
         ("iter_fog_opacity_trim", C.c_double),
         ("plane_reflect", C.c_double), ("box_reflect", C.c_double), ("inv_box_reflect", C.c_double),
         ("sphere_reflect", C.c_double), ("inv_sphere_reflect", C.c_double), ("water_reflect", C.c_double),
+        ("mb_fold_mode", C.c_int), ("mb_solid", C.c_double), ("mb_melt", C.c_double),
+        ("mb_vary_scale_vary", C.c_double), ("mb_vary_fold", C.c_double), ("mb_vary_minr", C.c_double),
+        ("mb_vary_rpower", C.c_double), ("mb_vary_wadd", C.c_double),
     ]
 
 

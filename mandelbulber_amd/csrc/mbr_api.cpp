@@ -92,6 +92,77 @@ Rot set_rotation3(const double a[3]) {
   return r;
 }
 
+// Face normals of generalizedFoldBox's fold polyhedra in the order
+// RecalculateIFSParams lists them (src/callbacks.cpp:1557-1664), selected by
+// genFoldBox.type (src/fractal.cpp:538-576; enumGeneralizedFoldBoxType):
+// 0 tetrahedron, 1 cube, 2 octahedron, 3 dodecahedron, 4 octahedron + cube,
+// 5 icosahedron, 6 hexagonal prism, 7 pentagonal prism; any other type keeps
+// the tetrahedron.  Returns the number of faces.
+int fold_polyhedron(int type, double nv[20][3]) {
+  int n = 0;
+  auto put = [&](double x, double y, double z) {
+    nv[n][0] = x;
+    nv[n][1] = y;
+    nv[n][2] = z;
+    n++;
+  };
+  const double t = 1.0 / std::sqrt(3.0);
+  // the tetrahedron's normals carry an odd number of minus signs, the four
+  // the octahedron adds an even number
+  static const int odd[4][3] = {{1, 1, -1}, {1, -1, 1}, {-1, 1, 1}, {-1, -1, -1}};
+  static const int even[4][3] = {{1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}};
+  auto diagonals = [&](const int (*sg)[3]) {
+    for (int i = 0; i < 4; i++) put(sg[i][0] * t, sg[i][1] * t, sg[i][2] * t);
+  };
+  auto axes = [&]() {
+    for (int a = 0; a < 3; a++)
+      for (int sgn = 1; sgn >= -1; sgn -= 2) put(a == 0 ? sgn : 0, a == 1 ? sgn : 0, a == 2 ? sgn : 0);
+  };
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  switch (type) {
+    case 1:
+      axes();
+      break;
+    case 2:
+    case 4:
+      diagonals(odd);
+      diagonals(even);
+      if (type == 4) axes();
+      break;
+    case 3: {
+      const double b = 1.0 / std::sqrt(phi * phi + 1.0), ab = phi * b;
+      for (int k = 0; k < 4; k++) put(0, (k < 2 ? b : -b), (k % 2 ? -ab : ab));
+      for (int k = 0; k < 4; k++) put((k < 2 ? b : -b), (k % 2 ? -ab : ab), 0);
+      for (int k = 0; k < 4; k++) put((k % 2 ? -ab : ab), 0, (k < 2 ? b : -b));
+      break;
+    }
+    case 5: {
+      const double f = std::sqrt(phi * phi + 1.0 / (phi * phi));
+      const double cc = phi / f, dd = 1.0 / phi / f;
+      diagonals(odd);
+      diagonals(even);
+      for (int k = 0; k < 4; k++) put(0, (k < 2 ? dd : -dd), (k % 2 ? -cc : cc));
+      for (int k = 0; k < 4; k++) put((k < 2 ? cc : -cc), 0, (k % 2 ? -dd : dd));
+      for (int k = 0; k < 4; k++) put((k < 2 ? dd : -dd), (k % 2 ? -cc : cc), 0);
+      break;
+    }
+    case 6:
+    case 7: {
+      // the prisms' angle step is the reference's 3.14159 * 2 / k
+      const int k = type == 6 ? 6 : 5;
+      const double step = 3.14159 * 2.0 / k;
+      put(0, 0, 1);
+      put(0, 0, -1);
+      for (int i = 0; i < k; i++) put(std::cos(step * i), std::sin(step * i), 0);
+      break;
+    }
+    default:
+      diagonals(odd);
+      break;
+  }
+  return n;
+}
+
 template <class T>
 void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   std::memset(&f, 0, sizeof f);
@@ -150,6 +221,7 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
       f.ifs_edge[c] = (T)s.ifs_edge[c];
     }
     f.ifs_menger = s.ifs_menger_sponge_mode;
+    f.ifs_fold = s.ifs_folding_mode;
     int period = 0;
     for (int k = 0; k < 5; k++) {
       f.hyb_f[k] = s.hybrid_formula[k];
@@ -187,6 +259,19 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
   f.mb_col_Z = (T)s.mb_color_Z;
   f.mb_col_Sp1 = (T)s.mb_color_Sp1;
   f.mb_col_Sp2 = (T)s.mb_color_Sp2;
+  {
+    double nv[20][3];
+    f.gf_sides = fold_polyhedron(s.mb_fold_mode, nv);
+    for (int i = 0; i < f.gf_sides; i++)
+      for (int c = 0; c < 3; c++) f.gf_nv[i][c] = (T)nv[i][c];
+  }
+  f.gf_solid = (T)s.mb_solid;
+  f.gf_melt = (T)s.mb_melt;
+  f.v4_scale_vary = (T)s.mb_vary_scale_vary;
+  f.v4_fold = (T)s.mb_vary_fold;
+  f.v4_minr = (T)s.mb_vary_minr;
+  f.v4_rpower = (T)s.mb_vary_rpower;
+  f.v4_wadd = (T)s.mb_vary_wadd;
   // RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555)
   Rot main = set_rotation3(s.mb_rotation_main);
   f.mb_main_rot = !main.identity;
@@ -536,6 +621,8 @@ int band_cap(int precision, int ao_count) {
   return (int)std::min<size_t>((size_t)kBandRays, std::max<size_t>(4096, kAOBudget / per_ray));
 }
 
+int kernel_family(const mbr_scene &s) { return s.ifs_folding_mode ? kGenericFamily : s.formula; }
+
 int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
   if (s.formula == MBR_FORMULA_HYBRID) {
@@ -553,8 +640,7 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   }
   if (!formula_supported(s.formula))
     return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
-                                            " is not in the render core (supported: 1-8, 11, 13-17, 19-21)");
-  if (s.ifs_folding_mode) return c->fail(MBR_ERR_UNSUPPORTED, "IFS folding mode is not in the render core");
+                                            " is not in the render core (supported: 1-8, 10-37)");
   if (s.reflect > 0.0 && !s.raytraced_reflections)
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
   if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
@@ -819,7 +905,7 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   // live scene constants) of the optional distance-estimate terms.
   d.plain = d.fd.limits == 0 && d.fd.only_plane == 0 && d.fd.interior == 0 && d.fd.iter_thresh == 0 &&
             !d.fd.prim_plane && !d.fd.prim_box && !d.fd.prim_ibox && !d.fd.prim_sphere && !d.fd.prim_isphere &&
-            !d.fd.prim_water && !d.fd.tglad_fold && !d.fd.sph_fold && !d.fd.julia_mode;
+            !d.fd.prim_water && !d.fd.tglad_fold && !d.fd.sph_fold && !d.fd.julia_mode && !d.fd.ifs_fold;
   return MBR_OK;
 }
 
@@ -1324,7 +1410,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   if (rc != MBR_OK) return rc;
   // formulas without kernels of their own (delta DE, trig_DE) and
   // reflections render in FP64
-  if (formula_generic(s.formula) || c->host_sc.refl_max > 0) precision = MBR_PREC_FP64;
+  if (formula_generic(kernel_family(s)) || c->host_sc.refl_max > 0) precision = MBR_PREC_FP64;
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
   if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
   if (row0 < 0) row0 = 0;
@@ -1381,13 +1467,13 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     HIPCHECK(c, launch_zero(c->d_stats, ST_COUNT, st));
     HIPCHECK(c, hipEventRecord(frame_ev[0], st));
     if (refl > 0)
-      rc = enqueue_frame_refl(c, levels, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
+      rc = enqueue_frame_refl(c, levels, c->d_sc, c->host_sc, kernel_family(s), ao, first, last, out, persist, st,
                               c->ev.data(), opts ? opts->cancel : nullptr);
     else if (split)
-      rc = enqueue_frame_split(c, pair, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, n_bands, out,
+      rc = enqueue_frame_split(c, pair, precision, c->d_sc, c->host_sc, kernel_family(s), ao, first, last, n_bands, out,
                                persist, st, c->side[1], c->ev.data(), opts ? opts->cancel : nullptr);
     else
-      rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st,
+      rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, kernel_family(s), ao, first, last, out, persist, st,
                          c->ev.data(), opts ? opts->cancel : nullptr, nullptr, demoivre);
     if (rc != MBR_OK) return rc;
     HIPCHECK(c, hipEventRecord(frame_ev[1], st));
@@ -1542,7 +1628,7 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   if (c->host_sc.refl_max > 0)
     return c->fail(MBR_ERR_UNSUPPORTED, "mbr_render_device_noalloc: raytraced reflections need a band slot per "
                                         "level; render them with mbr_render_device");
-  if (formula_generic(s.formula)) precision = MBR_PREC_FP64;
+  if (formula_generic(kernel_family(s))) precision = MBR_PREC_FP64;
   unsigned char *base = c->device < kMaxDevices ? (unsigned char *)g_scratch_ptr[c->device] : nullptr;
   if (!base) return c->fail(MBR_ERR_HIP, "mbr_render_device_noalloc: no scratch address for this device");
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
@@ -1578,7 +1664,7 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   const long long last = (long long)row1 * s.image_width;
   const int persist = c->cu_count > 0 ? c->cu_count * 8 : 2048;
   HIPCHECK(c, launch_zero(d_stats, ST_COUNT, st));
-  rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, s.formula, ao, first, last, out, persist, st, nullptr,
+  rc = enqueue_frame(c, sl, precision, c->d_sc, c->host_sc, kernel_family(s), ao, first, last, out, persist, st, nullptr,
                      opts ? opts->cancel : nullptr, d_stats, demoivre);
   if (rc != MBR_OK) return rc;
   unsigned long long v[ST_COUNT];
@@ -1891,7 +1977,7 @@ int build_batch_graph(mbr_ctx *c, std::vector<BatchFrame> &frames, const mbr_fra
         const BatchFrame &fr = frames[f];
         const AODev ao{c->d_batch_ao_f + 4 * fr.ao_off, c->d_batch_ao_d + 4 * fr.ao_off,
                        c->d_batch_ao_rgb + 4 * fr.ao_off, fr.p.d.ao_count};
-        int r = enqueue_frame(c, c->slots[k], precision, c->d_batch_sc + f, fr.p.d, fr.p.s.formula, ao, 0, fr.rays,
+        int r = enqueue_frame(c, c->slots[k], precision, c->d_batch_sc + f, fr.p.d, kernel_family(fr.p.s), ao, 0, fr.rays,
                               frame_out(&fbs[f]), persist, st, nullptr, nullptr, nullptr, demoivre);
         if (r != MBR_OK) return r;
       }
@@ -1956,7 +2042,7 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   if (rc != MBR_OK) return rc;
   // a frame of a generic formula (FP64 only) makes the whole batch FP64
   for (int i = 0; i < n; i++)
-    if (formula_generic(scenes[i].formula)) precision = MBR_PREC_FP64;
+    if (formula_generic(kernel_family(scenes[i]))) precision = MBR_PREC_FP64;
   HIPCHECK(c, hipSetDevice(c->device));
   // derive every scene on the host
   std::vector<BatchFrame> frames(n);
@@ -2118,7 +2204,7 @@ int run_adjoint(mbr_ctx *c, const mbr_render_opts *opts, const float *const grad
   if (h.shadow || h.ao || h.shadow_fx || h.refl_max > 0 || s.glow_intensity != 0.0 || s.fog_density != 0.0 || h.post_fog ||
       s.perspective_type != MBR_PERSP_THREE_POINT || s.tiles != 1 || s.constant_DE_threshold || s.interior_mode ||
       s.limits_enabled || s.iter_thresh || h.fd.prim_plane || h.fd.prim_box || h.fd.prim_ibox ||
-      h.fd.prim_sphere || h.fd.prim_isphere || h.fd.prim_water || h.fd.tglad_fold || h.fd.sph_fold)
+      h.fd.prim_sphere || h.fd.prim_isphere || h.fd.prim_water || h.fd.tglad_fold || h.fd.sph_fold || h.fd.ifs_fold)
     return c->fail(MBR_ERR_UNSUPPORTED, "mbr_adjoint: shadows, AO, volumetric lights, glow, fog, post fog, fish-eye, tiles, "
                                         "constant threshold, interior mode, limits, iteration threshold, "
                                         "primitives and pre-folds have no gradient here");

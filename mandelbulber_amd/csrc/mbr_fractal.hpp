@@ -23,12 +23,15 @@ enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
 // CalculateDistance per DE call, analytic for trig_DE, by finite
 // differences of the orbit radius (delta DE) for the rest.
 MBR_HD bool formula_analytic(int formula) {
-  return formula == 1 || formula == 2 || formula == 7 || formula == 8 || formula == 10 || formula == 17;
+  return formula == 1 || formula == 2 || formula == 7 || formula == 8 || formula == 10 || formula == 17 ||
+         formula == 18 || formula == 37;
 }
 // termination groups of src/fractal.cpp:1133-1149: Menger-like (r > 1000,
 // (r - 2) / DE) and Mandelbox-like (r > 1024, r / |DE|)
 MBR_HD bool formula_menger_group(int formula) { return formula == 7 || formula == 10; }
-MBR_HD bool formula_box_group(int formula) { return formula == 8 || formula == 17; }
+MBR_HD bool formula_box_group(int formula) {
+  return formula == 8 || formula == 17 || formula == 18 || formula == 37;
+}
 
 // SmoothConditionAGreaterB / ALessB (src/common_math.h:87-88)
 template <class T>
@@ -40,6 +43,20 @@ enum { OBJ_FRACTAL = 0, OBJ_PLANE = 1, OBJ_WATER = 2, OBJ_SPHERE = 3, OBJ_SPHERE
 
 template <class T>
 MBR_HD V3<T> ld3(const T *p) { return V3<T>{p[0], p[1], p[2]}; }
+
+// a.b and m*v with every product and sum rounded on its own (CVector3::Dot
+// and CRotationMatrix::RotateVector as they compile on x86-64)
+template <class T>
+MBR_HD T dot_nc(V3<T> a, V3<T> b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class T>
+MBR_HD V3<T> mat_apply_nc(const T *m, V3<T> v) {
+#pragma clang fp contract(off)
+  return V3<T>{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+               m[6] * v.x + m[7] * v.y + m[8] * v.z};
+}
 
 // z -> m*z with a full matrix product (rotinv matrices are never identity-
 // flagged in the reference, src/algebra.cpp:202-217).
@@ -123,6 +140,28 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
         p = k >= 0 ? f.hyb_p[k] : T(0);
         mscale = p;
         mabs = M::fabs(p);
+      }
+      // IFS folding mode (src/fractal.cpp:101-127): the kaleidoscopic
+      // vectors fold z without their intensities or edges; the scenes that
+      // turn it on render in FP64 only
+      if (f.ifs_fold) {
+        if (f.ifs_abs[0]) z.x = M::fabs(z.x);
+        if (f.ifs_abs[1]) z.y = M::fabs(z.y);
+        if (f.ifs_abs[2]) z.z = M::fabs(z.z);
+        for (int i = 0; i < 9; i++) {
+          if (!f.ifs_en[i]) continue;
+          if (f.ifs_rot_on[i]) z = mat_apply_nc(f.ifs_rot[i], z);
+          const V3<T> dirv = ld3(f.ifs_dir[i]);
+          const T len = dot_nc(z, dirv);
+          if (len < f.ifs_dist[i]) z = z - dirv * (T(2) * (len - f.ifs_dist[i]));
+        }
+        const V3<T> off = ld3(f.ifs_offset);
+        V3<T> zo = z - off;
+        if (f.ifs_main_on) zo = mat_apply_nc(f.ifs_main, zo);
+        z = zo + off;
+        z = z * f.ifs_scale;
+        z = z - off * (f.ifs_scale - T(1));
+        r = length_nc(z);
       }
     }
     if (f.tglad_fold) {
@@ -495,6 +534,104 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z.y = newy + constant.y;
       z.z = newz + constant.z;
       r = length_nc(z);
+    } else if (af == 37) {  // generalizedFoldBox (src/fractal.cpp:531-749)
+      // Walk the segment 0 -> z through the fold polyhedron X.Nv = solid:
+      // the first face it crosses (mirror), the face the reflected segment
+      // crosses next (rotation line), a third (inversion point); z folds
+      // towards the last feature found unless it lies within the melt layer.
+      const int sides = f.gf_sides;
+      const T solid = f.gf_solid, melt = f.gf_melt;
+      T a = 1;
+      int side = -1;
+      for (int i = 0; i < sides; i++) {
+        const T c = dot_nc(z, ld3(f.gf_nv[i]));
+        if (c > T(0) && a * c > solid) { side = i; a = solid / c; }
+      }
+      if (side != -1) {
+        const int side_m = side;
+        const V3<T> nm = ld3(f.gf_nv[side_m]);
+        const V3<T> x_m = z - nm * (dot_nc(z, nm) - solid);
+        V3<T> lv = x_m - nm;
+        V3<T> y = nm;
+        a = 1;
+        side = -1;
+        for (int i = 0; i < sides; i++) {
+          if (i == side_m) continue;
+          const V3<T> ni = ld3(f.gf_nv[i]);
+          const T b = solid - dot_nc(y, ni);
+          const T c = dot_nc(lv, ni);
+          if (c > T(0) && a * c > b) { side = i; a = b / c; }
+        }
+        V3<T> target = x_m;
+        T colour = f.mb_col_X;
+        if (side != -1) {
+          const V3<T> xi_mr = y + lv * a;
+          const int side_r = side;
+          const V3<T> nr = ld3(f.gf_nv[side_r]);
+          // CVector3::Cross (src/algebra.hpp:93-100)
+          V3<T> l_r{nm.y * nr.z - nm.z * nr.y, -nm.x * nr.z + nm.z * nr.x, nm.x * nr.y - nm.y * nr.x};
+          T a_rmin = (dot_nc(z, l_r) - dot_nc(xi_mr, l_r)) / dot_nc(l_r, l_r);
+          if (a_rmin < T(0)) { a_rmin = -a_rmin; l_r = l_r * T(-1); }
+          const V3<T> x_r = xi_mr + l_r * a_rmin;
+          lv = x_r - xi_mr;
+          y = xi_mr;
+          a = 1;
+          side = -1;
+          for (int i = 0; i < sides; i++) {
+            if (i == side_m || i == side_r) continue;
+            const V3<T> ni = ld3(f.gf_nv[i]);
+            const T b = solid - dot_nc(y, ni);
+            const T c = dot_nc(lv, ni);
+            if (c > T(0) && a * c > b) { side = i; a = b / c; }
+          }
+          if (side != -1) {
+            target = y + lv * a;
+            colour = f.mb_col_Z;
+          } else {
+            target = x_r;
+            colour = f.mb_col_Y;
+          }
+        }
+        const V3<T> z2x = target - z;
+        if (dot_nc(z2x, z2x) > melt * melt) {
+          const T mag = length_nc(z2x);
+          z = z + z2x * (T(2) * (mag - melt) / (mag + T(.00000001)));
+          tgladColor += colour;
+        }
+      }
+      r = length_nc(z);
+      const T r2 = r * r;
+      z = z + ld3(f.mb_offset);
+      if (r2 < f.mb_mR2) { z = z * f.mb_factor1; tgladDE *= f.mb_factor1; tgladColor += f.mb_col_Sp1; }
+      else if (r2 < f.mb_fR2) { T tf2 = f.mb_fR2 / r2; z = z * tf2; tgladDE *= tf2; tgladColor += f.mb_col_Sp2; }
+      z = z - ld3(f.mb_offset);
+      if (f.mb_main_rot) z = mat_apply(f.mb_main, z);
+      z = z * mscale + constant;
+      tgladDE = tgladDE * mabs + T(1);
+      r = length_nc(z);
+    } else if (af == 18) {  // mandelboxVaryScale4D (src/fractal.cpp:887-915)
+      // the scale drifts each iteration; w is the fourth coordinate
+      mscale = mscale + f.v4_scale_vary * (M::fabs(mscale) - T(1));
+      const T fold = f.v4_fold, minr = f.v4_minr;
+      const V3<T> old = z;
+      z.x = M::fabs(z.x + fold) - M::fabs(z.x - fold) - z.x;
+      z.y = M::fabs(z.y + fold) - M::fabs(z.y - fold) - z.y;
+      z.z = M::fabs(z.z + fold) - M::fabs(z.z - fold) - z.z;
+      w = M::fabs(w + fold) - M::fabs(w - fold) - w;
+      if (z.x != old.x) tgladColor += f.mb_col_X;
+      if (z.y != old.y) tgladColor += f.mb_col_Y;
+      if (z.z != old.z) tgladColor += f.mb_col_Z;
+      // pow(q, 1) is q in glibc's correctly rounded pow; the device pow can
+      // be an ulp off, which the colouring orbit (10 N iterations) magnifies
+      const T q = z.x * z.x + z.y * z.y + z.z * z.z + w * w;
+      const T rr = f.v4_rpower == T(1) ? q : M::pow(q, f.v4_rpower);
+      T m = mscale;
+      if (rr < minr * minr) { m = mscale / (minr * minr); tgladColor += f.mb_col_Sp1; }
+      else if (rr < T(1)) { m = mscale / rr; tgladColor += f.mb_col_Sp2; }
+      z = z * m + constant;
+      w = w * m + f.v4_wadd;
+      tgladDE = tgladDE * M::fabs(m) + T(1);
+      r = M::sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
     }
 
     // delta-DE runs (src/fractal.cpp:1110-1119) stop on their own tests
@@ -1088,9 +1225,11 @@ MBR_HD T calc_distance_p64(const FractalK<double> &fd, const FractalK<T> &f, V3<
 template <class T>
 MBR_HD T calc_distance(const FractalK<T> &f, V3<T> point, T detailSize, int N, int minN, bool normalMode,
                        DEInfo &info) {
-  if (f.formula == 7) return calc_distance_f<T, 7>(f, point, detailSize, N, minN, normalMode, info);
-  if (f.formula == 8) return calc_distance_f<T, 8>(f, point, detailSize, N, minN, normalMode, info);
-  if (f.formula == 2) return calc_distance_f<T, 2>(f, point, detailSize, N, minN, normalMode, info);
+  if (!f.ifs_fold) {  // the IFS folding pre-fold runs on the generic path only
+    if (f.formula == 7) return calc_distance_f<T, 7>(f, point, detailSize, N, minN, normalMode, info);
+    if (f.formula == 8) return calc_distance_f<T, 8>(f, point, detailSize, N, minN, normalMode, info);
+    if (f.formula == 2) return calc_distance_f<T, 2>(f, point, detailSize, N, minN, normalMode, info);
+  }
   return calc_distance_generic<T>(f, point, detailSize, N, minN, normalMode, info);
 }
 

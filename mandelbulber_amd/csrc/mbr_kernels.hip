@@ -152,6 +152,7 @@ enum : int { SO_CONTINUE = 0, SO_MISS = 1, SO_HANDOFF = 2 };
 // (distThreshInit, src/Render3D.cpp:92, :102-109); primary rays start from 0.
 template <bool Refl = false>
 __device__ __forceinline__ void march_before_de(const DevScene &sc, V3<double> start, V3<double> dir, RayState &s) {
+#pragma clang fp contract(off)
   const ShadeK<double> &k = sc.sd;
   s.point = start + dir * s.scan;
   s.thr = sc.constant_thresh ? k.quality : s.scan * k.resolution * k.persp / k.quality + (Refl ? s.thr0 : 0.0);
@@ -163,6 +164,7 @@ __device__ __forceinline__ void march_before_de(const DevScene &sc, V3<double> s
 template <class T, class RT>
 __device__ __forceinline__ int march_after_de(const DevScene &sc, RayState &s, double d, int obj_it, uint32_t pixel,
                                               const RecPlanes<RT> &rec) {
+#pragma clang fp contract(off)
   const ShadeK<double> &k = sc.sd;
   s.object = obj_it & 7;
   const double dist = d > 3.0 ? 3.0 : d;
@@ -194,6 +196,7 @@ template <class RT, int F, bool Refl = false, int IP = 0>
 __device__ void march_continue_f64(const DevScene &sc, V3<double> start, V3<double> dir, uint32_t pixel,
                                    const RecPlanes<RT> &rec, RayState s, MarchOut &o, LaneCount &cnt,
                                    bool &ambiguous) {
+#pragma clang fp contract(off)
   // trig-free integer-power iteration where the records are FP32, as in
   // k_march
   constexpr bool kAlg = sizeof(RT) < 8 || IP != 0;

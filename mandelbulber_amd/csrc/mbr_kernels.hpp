@@ -28,10 +28,13 @@ enum CounterIdx {
 
 // Formulas of the render core; the generic ones (no kernels of their own)
 // run in FP64 through the F = 0 kernels.
-inline bool formula_supported(int f) {
-  return (f >= 1 && f <= 8) || (f >= 10 && f <= 17) || (f >= 19 && f <= 36);
+inline bool formula_supported(int f) { return (f >= 1 && f <= 8) || (f >= 10 && f <= 37); }
+// Kernel family of a scene: its formula, or kGenericFamily when the scene
+// turns on the IFS folding pre-fold, which only the generic kernels run.
+constexpr int kGenericFamily = 1000;
+inline bool formula_generic(int f) {
+  return f == kGenericFamily || (formula_supported(f) && f != 2 && f != 7 && f != 8);
 }
-inline bool formula_generic(int f) { return formula_supported(f) && f != 2 && f != 7 && f != 8; }
 
 // Maximum step records of one ray (the loop bound of src/Render3D.cpp:95).
 constexpr int kMaxSteps = 10000;

@@ -29,6 +29,13 @@ struct FractalK {
   int ifs_abs[3], ifs_menger, ifs_en[9], ifs_rot_on[9], ifs_main_on;
   T ifs_scale, ifs_offset[3], ifs_edge[3];
   T ifs_rot[9][9], ifs_dir[9][3], ifs_dist[9], ifs_int[9], ifs_main[9];
+  int ifs_fold;  // IFS folding mode: the same vectors as a pre-fold of every iteration
+  // generalizedFoldBox: face normals of the fold polyhedron (genFoldBox.type),
+  // the fold plane offset (solid) and the melt layer
+  int gf_sides;
+  T gf_nv[20][3], gf_solid, gf_melt;
+  // mandelboxVaryScale4D: scale variation, fold, minimum radius, radius power, w addend
+  T v4_scale_vary, v4_fold, v4_minr, v4_rpower, v4_wadd;
   // hybrid sequence: formulas, iterations and powers of the five slots,
   // cyclic flag and the length of one cycle
   int hyb_f[5], hyb_it[5], hyb_cyclic, hyb_period;

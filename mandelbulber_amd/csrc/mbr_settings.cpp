@@ -153,7 +153,7 @@ std::vector<KeyDef> build_table() {
   D("mandelbox_color_Y", OFF(mb_color_Y)); D("mandelbox_color_Z", OFF(mb_color_Z));
   D("mandelbox_color_Sp1", OFF(mb_color_Sp1)); D("mandelbox_color_Sp2", OFF(mb_color_Sp2));
   I("mandelbox_rotation_enabled", OFF(mb_rotations_enabled));
-  G("mandelbox_fold_mode"); G("mandelbox_solid"); G("mandelbox_melt");
+  I("mandelbox_fold_mode", OFF(mb_fold_mode)); D("mandelbox_solid", OFF(mb_solid)); D("mandelbox_melt", OFF(mb_melt));
   const char *comp[3] = {"alfa", "beta", "gamma"};
   const char *axis[3] = {"X", "Y", "Z"};
   for (int c = 0; c < 3; c++)
@@ -169,8 +169,9 @@ std::vector<KeyDef> build_table() {
   D("FoldingIntPow_folding_factor", OFF(folding_int_pow_factor)); D("FoldingIntPow_z_factor", OFF(folding_int_pow_z_factor)); G("dynamic_DE_correction");
   I("penetrating_lights", OFF(penetrating_lights));
   I("raytraced_reflections", OFF(raytraced_reflections)); I("reflections_max", OFF(reflections_max));
-  G("mandelbox_vary_scale_vary"); G("mandelbox_vary_fold"); G("mandelbox_vary_minr");
-  G("mandelbox_vary_rpower"); G("mandelbox_vary_wadd"); D("c_add", OFF(c_add));
+  D("mandelbox_vary_scale_vary", OFF(mb_vary_scale_vary)); D("mandelbox_vary_fold", OFF(mb_vary_fold));
+  D("mandelbox_vary_minr", OFF(mb_vary_minr)); D("mandelbox_vary_rpower", OFF(mb_vary_rpower));
+  D("mandelbox_vary_wadd", OFF(mb_vary_wadd)); D("c_add", OFF(c_add));
   D("volumetric_fog_density", OFF(fog_density));
   D("volumetric_fog_colour_1_distance", OFF(fog_colour1_distance));
   D("volumetric_fog_colour_2_distance", OFF(fog_colour2_distance));
@@ -419,6 +420,15 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->mb_sharpness = 3.0;
   s->folding_int_pow_factor = 2.0;
   s->folding_int_pow_z_factor = 5.0;
+  // share/defaults:300-302, 326-330
+  s->mb_fold_mode = 1;
+  s->mb_solid = 1.0;
+  s->mb_melt = 0.0;
+  s->mb_vary_scale_vary = 0.1;
+  s->mb_vary_fold = 1.0;
+  s->mb_vary_minr = 0.5;
+  s->mb_vary_rpower = 1.0;
+  s->mb_vary_wadd = 0.0;
   s->c_add = -1.3;
   s->ssao_enabled = 1;               // share/defaults: post_SSAO_enabled 1, quality 20
   s->ssao_quality = 20;

@@ -582,6 +582,7 @@ MBR_HD VolOut<T> volumetric(const DevScene &sc, T depth, int stepCount, RGBT<T> 
 // per-pixel view vector (src/Render3D.cpp:389-438); true when the fish-eye
 // cut removes the pixel
 __device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3<double> &out) {
+#pragma clang fp contract(off)
   const double tiles = sc.tiles;
   const double x2 = ((double)x / sc.width / tiles - 0.5 + sc.tile_x / tiles) * sc.aspect;
   const double z2 = ((double)y / sc.height / tiles - 0.5 + sc.tile_y / tiles);
@@ -602,7 +603,7 @@ __device__ __forceinline__ bool view_vector(const DevScene &sc, int x, int y, V3
   } else {
     v = v3<double>(x2 * fov, 1.0, z2 * fov);
   }
-  out = sc.R_identity ? v : mat_apply(sc.R, v);
+  out = sc.R_identity ? v : mat_apply_nc(sc.R, v);
   return cut;
 }
 

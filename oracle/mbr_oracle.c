@@ -49,6 +49,8 @@ static inline double v3dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.
 /* CVector3::Normalize (src/algebra.hpp:104-111): multiply by 1/length */
 static inline v3 v3norm(v3 a) { double n = 1.0 / v3len(a); return v3m(a.x * n, a.y * n, a.z * n); }
 static inline v3 v3from(const double *p) { return v3m(p[0], p[1], p[2]); }
+/* CVector3::Cross (src/algebra.hpp:93-100), its term order */
+static inline v3 v3cross(v3 a, v3 b) { return v3m(a.y * b.z - a.z * b.y, -a.x * b.z + a.z * b.x, a.x * b.y - a.y * b.x); }
 
 /* ------------------------------------------------------- rotation matrix */
 /* CRotationMatrix (src/algebra.cpp:80-218): row-major m[0..8] = m11..m33,
@@ -112,7 +114,61 @@ typedef struct {
   v3 ifs_dir[9];
   int *seq;        /* hybrid formula sequence, N entries (formulaSequence) */
   double *seq_pow; /* hybridPowerSequence */
+  v3 fold_nv[20];  /* generalizedFoldBox: the fold polyhedron's face normals */
+  int fold_sides;
 } fractal_ctx;
+
+/* The face normals of generalizedFoldBox's polyhedra, as RecalculateIFSParams
+ * precomputes them (src/callbacks.cpp:1557-1664); type = genFoldBox.type. */
+static int fold_normals(int type, v3 *nv) {
+  const double t = 1.0 / sqrt(3.0);
+  const v3 tet[4] = {{t, t, -t}, {t, -t, t}, {-t, t, t}, {-t, -t, -t}};
+  const v3 oct4[4] = {{t, t, t}, {-t, -t, t}, {-t, t, -t}, {t, -t, -t}};
+  const v3 cube[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  int n = 0;
+  switch (type) {
+    case 1:  /* cube */
+      for (int i = 0; i < 6; i++) nv[n++] = cube[i];
+      return n;
+    case 2:  /* octahedron */
+    case 4:  /* octahedron + cube */
+      for (int i = 0; i < 4; i++) nv[n++] = tet[i];
+      for (int i = 0; i < 4; i++) nv[n++] = oct4[i];
+      if (type == 4)
+        for (int i = 0; i < 6; i++) nv[n++] = cube[i];
+      return n;
+    case 3: {  /* dodecahedron */
+      const double aa = (1.0 + sqrt(5.0)) / 2.0, bb = 1.0 / sqrt(aa * aa + 1.0);
+      const v3 d[12] = {{0, bb, aa * bb},  {0, bb, -aa * bb},  {0, -bb, aa * bb},  {0, -bb, -aa * bb},
+                        {bb, aa * bb, 0},  {bb, -aa * bb, 0},  {-bb, aa * bb, 0},  {-bb, -aa * bb, 0},
+                        {aa * bb, 0, bb},  {-aa * bb, 0, bb},  {aa * bb, 0, -bb},  {-aa * bb, 0, -bb}};
+      for (int i = 0; i < 12; i++) nv[n++] = d[i];
+      return n;
+    }
+    case 5: {  /* icosahedron */
+      const double aa = (1.0 + sqrt(5.0)) / 2.0, ff = sqrt(aa * aa + 1.0 / (aa * aa));
+      const double cc = aa / ff, dd = 1.0 / aa / ff;
+      for (int i = 0; i < 4; i++) nv[n++] = tet[i];
+      for (int i = 0; i < 4; i++) nv[n++] = oct4[i];
+      const v3 ic[12] = {{0, dd, cc}, {0, dd, -cc}, {0, -dd, cc}, {0, -dd, -cc}, {cc, 0, dd}, {cc, 0, -dd},
+                         {-cc, 0, dd}, {-cc, 0, -dd}, {dd, cc, 0}, {dd, -cc, 0}, {-dd, cc, 0}, {-dd, -cc, 0}};
+      for (int i = 0; i < 12; i++) nv[n++] = ic[i];
+      return n;
+    }
+    case 6:  /* hexagonal prism: the reference's angle step 3.14159 * 2 / 6 */
+    case 7: {  /* pentagonal prism */
+      const int k = type == 6 ? 6 : 5;
+      const double th = 3.14159 * 2.0 / k;
+      nv[n++] = v3m(0, 0, 1);
+      nv[n++] = v3m(0, 0, -1);
+      for (int i = 0; i < k; i++) nv[n++] = v3m(cos(th * i), sin(th * i), 0);
+      return n;
+    }
+    default:  /* 0, tetrahedron */
+      for (int i = 0; i < 4; i++) nv[n++] = tet[i];
+      return n;
+  }
+}
 
 static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
   f->s = s;
@@ -153,6 +209,7 @@ static void fractal_ctx_init(fractal_ctx *f, const mbr_scene *s) {
     }
   }
   /* RecalculateIFSParams, Mandelbox part (src/callbacks.cpp:1548-1555) */
+  f->fold_sides = fold_normals(s->mb_fold_mode, f->fold_nv);
   rot_set3(&f->mb_main, s->mb_rotation_main);
   for (int fold = 0; fold < 2; fold++)
     for (int axis = 0; axis < 3; axis++) {
@@ -228,6 +285,23 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
       formula = f->seq[tempL];
       p = f->seq_pow[tempL];
       scale = p;
+    }
+    if (s->ifs_folding_mode) {  /* src/fractal.cpp:100-126 */
+      if (s->ifs_abs[0]) z.x = fabs(z.x);
+      if (s->ifs_abs[1]) z.y = fabs(z.y);
+      if (s->ifs_abs[2]) z.z = fabs(z.z);
+      for (int i = 0; i < 9; i++) {
+        if (!s->ifs_enabled[i]) continue;
+        z = rot_apply(&f->ifs_rot[i], z);
+        double length = v3dot(z, f->ifs_dir[i]);
+        if (length < s->ifs_distance[i])
+          z = v3sub(z, v3mul(f->ifs_dir[i], 2.0 * (length - s->ifs_distance[i])));
+      }
+      v3 off = v3from(s->ifs_offset);
+      z = v3add(rot_apply(&f->ifs_main, v3sub(z, off)), off);
+      z = v3mul(z, s->ifs_scale);
+      z = v3sub(z, v3mul(off, s->ifs_scale - 1.0));
+      r = v3len(z);
     }
     if (s->tglad_folding_mode) {
       double lim = s->folding_limit, val = s->folding_value;
@@ -434,6 +508,111 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
         r = v3len(z);
         break;
       }
+      case MBR_FORMULA_GENERALIZED_FOLD_BOX: {  /* :531-749 */
+        const v3 *Nv = f->fold_nv;
+        const int sides = f->fold_sides;
+        const double melt = s->mb_melt, solid = s->mb_solid;
+        /* the closest fold plane X.Nv = solid cutting the segment origin..z */
+        v3 Lv = z, Y = v3m(0, 0, 0);
+        double a = 1, b, c;
+        int side = -1;
+        for (int i = 0; i < sides; i++) {
+          b = solid;
+          c = v3dot(Lv, Nv[i]);
+          if ((c > 0) && ((a * c) > b)) { side = i; a = b / c; }
+        }
+        if (side != -1) {  /* mirror check */
+          const int side_m = side;
+          const v3 Nv_m = Nv[side_m];
+          const v3 X_m = v3sub(z, v3mul(Nv_m, v3dot(z, Nv_m) - solid));
+          Lv = v3sub(X_m, Nv_m);
+          Y = Nv_m;
+          a = 1;
+          side = -1;
+          for (int i = 0; i < sides; i++) {
+            if (i != side_m) {
+              b = solid - v3dot(Y, Nv[i]);
+              c = v3dot(Lv, Nv[i]);
+              if ((c > 0) && ((a * c) > b)) { side = i; a = b / c; }
+            }
+          }
+          if (side != -1) {  /* rotation check */
+            const v3 Xmr_intersect = v3add(Y, v3mul(Lv, a));
+            const int side_r = side;
+            const v3 Nv_r = Nv[side_r];
+            v3 L_r = v3cross(Nv_m, Nv_r);
+            double a_rmin = (v3dot(z, L_r) - v3dot(Xmr_intersect, L_r)) / v3dot(L_r, L_r);
+            if (a_rmin < 0) { a_rmin = -a_rmin; L_r = v3mul(L_r, -1.0); }
+            const v3 X_r = v3add(Xmr_intersect, v3mul(L_r, a_rmin));
+            Lv = v3sub(X_r, Xmr_intersect);
+            Y = Xmr_intersect;
+            a = 1;
+            side = -1;
+            for (int i = 0; i < sides; i++) {
+              if ((i != side_m) && (i != side_r)) {
+                b = solid - v3dot(Y, Nv[i]);
+                c = v3dot(Lv, Nv[i]);
+                if ((c > 0) && ((a * c) > b)) { side = i; a = b / c; }
+              }
+            }
+            if (side != -1) {  /* inversion point */
+              const v3 X_i = v3add(Y, v3mul(Lv, a));
+              const v3 z2X = v3sub(X_i, z);
+              if (v3dot(z2X, z2X) > (melt * melt)) {
+                double z2X_mag = v3len(z2X);
+                z = v3add(z, v3mul(z2X, 2 * (z2X_mag - melt) / (z2X_mag + .00000001)));
+                tgladColor += s->mb_color_Z;
+              }
+            } else {  /* rotation line */
+              const v3 z2X = v3sub(X_r, z);
+              if (v3dot(z2X, z2X) > (melt * melt)) {
+                double z2X_mag = v3len(z2X);
+                z = v3add(z, v3mul(z2X, 2 * (z2X_mag - melt) / (z2X_mag + .00000001)));
+                tgladColor += s->mb_color_Y;
+              }
+            }
+          } else {  /* mirror plane */
+            const v3 z2X = v3sub(X_m, z);
+            if (v3dot(z2X, z2X) > (melt * melt)) {
+              double z2X_mag = v3len(z2X);
+              z = v3add(z, v3mul(z2X, 2 * (z2X_mag - melt) / (z2X_mag + .00000001)));
+              tgladColor += s->mb_color_X;
+            }
+          }
+        }
+        r = v3len(z);
+        double r2 = r * r;
+        z = v3add(z, v3from(s->mb_offset));
+        if (r2 < mR2) { z = v3mul(z, tglad_factor1); tgladDE *= tglad_factor1; tgladColor += s->mb_color_Sp1; }
+        else if (r2 < fR2) { double tf2 = fR2 / r2; z = v3mul(z, tf2); tgladDE *= tf2; tgladColor += s->mb_color_Sp2; }
+        z = v3sub(z, v3from(s->mb_offset));
+        z = rot_apply(&f->mb_main, z);
+        z = v3add(v3mul(z, scale), constant);
+        tgladDE = tgladDE * fabs(scale) + 1.0;
+        r = v3len(z);
+        break;
+      }
+      case MBR_FORMULA_MANDELBOX_VARY_SCALE_4D: {  /* :887-915 */
+        const double fold = s->mb_vary_fold, minR = s->mb_vary_minr;
+        scale = scale + s->mb_vary_scale_vary * (fabs(scale) - 1.0);
+        v3 oldz = z;
+        z.x = fabs(z.x + fold) - fabs(z.x - fold) - z.x;
+        z.y = fabs(z.y + fold) - fabs(z.y - fold) - z.y;
+        z.z = fabs(z.z + fold) - fabs(z.z - fold) - z.z;
+        w = fabs(w + fold) - fabs(w - fold) - w;
+        if (z.x != oldz.x) tgladColor += s->mb_color_X;
+        if (z.y != oldz.y) tgladColor += s->mb_color_Y;
+        if (z.z != oldz.z) tgladColor += s->mb_color_Z;
+        double rr = pow(z.x * z.x + z.y * z.y + z.z * z.z + w * w, s->mb_vary_rpower);
+        double m = scale;
+        if (rr < minR * minR) { m = scale / (minR * minR); tgladColor += s->mb_color_Sp1; }
+        else if (rr < 1.0) { m = scale / rr; tgladColor += s->mb_color_Sp2; }
+        z = v3add(v3mul(z, m), constant);
+        w = w * m + s->mb_vary_wadd;
+        tgladDE = tgladDE * fabs(m) + 1.0;
+        r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z + w * w);
+        break;
+      }
       case MBR_FORMULA_SMOOTH_MANDELBOX: {  /* :751-793 */
         double sm = s->mb_sharpness, lim = s->mb_folding_limit, val = s->mb_folding_value;
 #define SGT(a, b) (1.0 / (1.0 + exp(sm * ((b) - (a)))))
@@ -615,7 +794,8 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
 
     if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) {
       if (r > 1000) { distance = (r - 2.0) / tgladDE; break; }
-    } else if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX) {
+    } else if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX ||
+               formula == MBR_FORMULA_MANDELBOX_VARY_SCALE_4D || formula == MBR_FORMULA_GENERALIZED_FOLD_BOX) {
       if (r > 1024) { distance = r / fabs(tgladDE); break; }
     } else {
       if (mode == MODE_NORMAL) {
@@ -645,7 +825,8 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     if (tgladColor > 1000) tgladColor = 1000;
     return distance * 5000.0 + tgladColor * 100.0 + min * 1000.0;
   }
-  if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX)
+  if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX ||
+      formula == MBR_FORMULA_MANDELBOX_VARY_SCALE_4D || formula == MBR_FORMULA_GENERALIZED_FOLD_BOX)
     return tgladColor * 100.0 + v3len(z) * s->mb_color_R;
   if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) return min * 1000.0;
   return distance * 5000.0;
@@ -725,7 +906,9 @@ static double calc_distance(const fractal_ctx *f, v3 point, double detailSize, i
 
   const int analytic = s->formula == MBR_FORMULA_TRIG_DE || s->formula == MBR_FORMULA_TRIG_OPTIM ||
                        s->formula == MBR_FORMULA_MENGER_SPONGE || s->formula == MBR_FORMULA_TGLAD ||
-                       s->formula == MBR_FORMULA_SMOOTH_MANDELBOX || s->formula == MBR_FORMULA_KALEIDOSCOPIC;
+                       s->formula == MBR_FORMULA_SMOOTH_MANDELBOX || s->formula == MBR_FORMULA_KALEIDOSCOPIC ||
+                       s->formula == MBR_FORMULA_MANDELBOX_VARY_SCALE_4D ||
+                       s->formula == MBR_FORMULA_GENERALIZED_FOLD_BOX;
   if (!s->primitive_only_plane && !analytic) {
     /* delta DE (src/fractal.cpp:1307-1342): orbit radius at the point and
      * at +1e-10 along x, y, z, the point updated in place as there */

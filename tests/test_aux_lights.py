@@ -17,6 +17,21 @@ def _lights_scene(number, seed=1234, pre=(), formula=2):
     if formula == 7:
         s.formula = 7
         s.N = 20
+    elif formula == 37:  # generalizedFoldBox, dodecahedron
+        s = scene.mandelbox_view(64, 48, shaded=False)
+        s.formula, s.mb_fold_mode, s.mb_solid, s.mb_melt = 37, 3, 1.0, 0.05
+    elif formula == 18:  # mandelboxVaryScale4D
+        s = scene.mandelbox_view(64, 48, shaded=False)
+        s.formula = 18
+        s.mb_vary_scale_vary, s.mb_vary_fold, s.mb_vary_minr = 0.1, 1.0, 0.5
+        s.mb_vary_rpower, s.mb_vary_wadd = 1.0, 0.05
+    elif formula == "ifs":  # the IFS folding pre-fold on the Mandelbulb
+        s.ifs_folding_mode = 1
+        s.ifs_abs[0] = 1
+        s.ifs_enabled[0] = 1
+        s.ifs_direction[0][0], s.ifs_direction[0][1], s.ifs_direction[0][2] = 1.0, -1.0, 0.0
+        s.ifs_angles[0][2] = 0.2
+        s.ifs_rotation[0] = 0.15
     s.aux_light_number = number
     s.aux_light_random_seed = seed
     for i in pre:
@@ -30,6 +45,7 @@ def _as_tuples(tab):
 
 @pytest.mark.parametrize("number,seed,pre,formula", [
     (20, 1234, (), 2), (7, 5, (1, 3), 2), (3, 99, (0, 1, 2, 3), 2), (0, 1234, (2,), 2), (12, 3000000000 - 2**32, (), 7),
+    (10, 37, (), 37), (10, 18, (), 18), (10, 77, (), "ifs"),
 ])
 def test_placement_matches_oracle(number, seed, pre, formula):
     sc = _lights_scene(number, seed, pre, formula)

@@ -5,7 +5,10 @@ and with shadow + AO + fog.  The delta-DE formulas difference orbit radii
 (pow, atan2, acos, sin, cos) against glibc, so FP64 is held to the parity
 bars rather than to bit equality (the depth bar of the two formulas with
 transcendentals in every iteration on the 99.9th percentile); the other
-precision modes render the same FP64 path and must equal the FP64 frame."""
+precision modes render the same FP64 path and must equal the FP64 frame.
+generalizedFoldBox (all eight fold polyhedra), mandelboxVaryScale4D and the
+IFS folding pre-fold are analytic and, but for pow and the Mandelbulb's
+trig, free of transcendentals: there the FP64 frame equals the oracle's."""
 import numpy as np
 import pytest
 
@@ -160,3 +163,63 @@ def test_hybrid_matches_oracle(renderer, name, shaded):
     print(f"  depth outliers (> 1e-4): {outliers}", flush=True)
     assert m["flips"] + outliers <= 1e-3 * m["pixels"], m
     assert m["psnr_db"] >= 45.0, m
+
+
+def _fold_scene(kind, shaded):
+    """generalizedFoldBox (gf<type>), mandelboxVaryScale4D (vary, vary-rpow)
+    and the IFS folding pre-fold on the Mandelbox (ifs-box) and on the
+    Mandelbulb (ifs-bulb)."""
+    if kind == "ifs-bulb":
+        s = scene.mandelbulb_full(192, 144, seed=1) if shaded else scene.mandelbulb_plain(256, 192)
+    else:
+        s = scene.mandelbox_view(192, 144, shaded=shaded)
+    if kind.startswith("gf"):
+        s.formula = 37
+        s.mb_fold_mode = int(kind[2:])
+        s.mb_solid, s.mb_melt = 1.0, 0.05 if s.mb_fold_mode % 2 else 0.0
+    elif kind.startswith("vary"):
+        s.formula = 18
+        s.mb_vary_scale_vary, s.mb_vary_fold, s.mb_vary_minr = 0.1, 1.0, 0.5
+        s.mb_vary_rpower, s.mb_vary_wadd = (1.1 if kind == "vary-rpow" else 1.0), 0.05
+    else:
+        s.ifs_folding_mode = 1
+        s.ifs_enabled[0] = s.ifs_enabled[1] = 1
+        s.ifs_abs[0 if kind == "ifs-bulb" else 2] = 1
+        for c, v in enumerate((1.0, -1.0, 0.0)):
+            s.ifs_direction[0][c] = v
+        for c, v in enumerate((0.0, 1.0, -1.0)):
+            s.ifs_direction[1][c] = v
+        s.ifs_distance[1] = 0.1
+        s.ifs_angles[1][2] = 0.2
+        s.ifs_rotation[0] = 0.15
+        s.ifs_offset[0] = 0.2
+        s.ifs_scale = 1.0
+    if shaded:
+        s.ao_quality = 2
+    return s
+
+
+# analytic and free of transcendentals (pow(x, 1) = x): the FP64 path must
+# reproduce the oracle bit for bit
+FOLD_EXACT = ("gf0", "gf1", "gf2", "gf3", "gf4", "gf5", "gf6", "gf7", "vary", "ifs-box")
+
+
+@pytest.mark.parametrize("shaded", [False, True])
+@pytest.mark.parametrize("kind", FOLD_EXACT + ("vary-rpow", "ifs-bulb"))
+def test_fold_formulas_match_oracle(renderer, kind, shaded):
+    from oracle import oracle
+    sc = _fold_scene(kind, shaded)
+    ref, _ = oracle.render(sc)
+    renderer.set_scene(sc)
+    print(f"[formulas] {kind} shaded={shaded} fp64", flush=True)
+    f64, _ = renderer.render(precision=_abi.PREC_FP64)
+    m = compare(f64, ref)
+    print(f"  {m}", flush=True)
+    assert m["hits"] > 200
+    assert_parity(m, label=f"{kind} shaded={shaded}")
+    if kind in FOLD_EXACT:
+        assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+        assert_planes_exact(m)
+    print(f"[formulas] {kind} shaded={shaded} fp32 (rendered in FP64)", flush=True)
+    f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
+    assert np.array_equal(f32.z, f64.z) and np.array_equal(f32.rgb, f64.rgb, equal_nan=True)

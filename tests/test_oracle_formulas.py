@@ -387,3 +387,172 @@ def test_benesi_squares_the_radius():
     s = _scene(20, 2.0, julia=(0.0, 0.0, 0.0))
     for p in [(0.3, -0.7, 0.45), (1.4, -0.5, 0.6), (-0.2, 0.05, 0.9)]:
         assert _r1(s, p) == pytest.approx(sum(v * v for v in p), rel=1e-12)
+
+
+# --- generalizedFoldBox (src/fractal.cpp:531-749) ----------------------------
+# With the cube's six face normals the fold is the Mandelbox box fold: a point
+# beyond one face is mirrored in it, beyond an edge rotated half a turn about
+# the edge (= mirrored in both faces), beyond a corner inverted through it (=
+# mirrored in all three), so with solid = the fold limit and melt 0 the
+# formula is tglad with folding value 2 x limit, up to the reference's
+# 2 |d| / (|d| + 1e-8) factor.
+
+def _box_pair(limit=1.0, offset=(0.0, 0.0, 0.0)):
+    t = S.mandelbox_view(64, 48, shaded=False)
+    t.mb_folding_limit, t.mb_folding_value = limit, 2.0 * limit
+    for i in range(3):
+        t.mb_offset[i] = offset[i]
+    g = S.copy(t)
+    g.formula = 37
+    g.mb_fold_mode = 1
+    g.mb_solid, g.mb_melt = limit, 0.0
+    return t, g
+
+
+@pytest.mark.parametrize("p", [(1.6, 0.3, -0.2), (1.5, 1.4, 0.2), (1.3, -1.6, 1.2), (0.4, 0.2, 0.1), (2.5, 2.2, -3.0)])
+def test_generalized_fold_box_cube_is_the_mandelbox_box_fold(p):
+    t, g = _box_pair(offset=(0.15, -0.2, 0.05))   # the sphere fold about the offset as well
+    for n in (1, 2):
+        rt, _ = oracle.compute(t, p, mode=MODE_DELTA1, n_iter=n)
+        rg, _ = oracle.compute(g, p, mode=MODE_DELTA1, n_iter=n)
+        assert rg == pytest.approx(rt, rel=1e-6), (p, n)
+
+
+def test_generalized_fold_box_cube_distance_equals_the_mandelbox():
+    t, g = _box_pair()
+    rng = np.random.default_rng(37)
+    checked = 0
+    for _ in range(200):
+        p = tuple(rng.uniform(-40, 40, 3))
+        dt, mt, it, _ = oracle.calculate_distance(t, p)
+        if mt or it > 10:
+            continue  # a chaotic orbit near the set magnifies the 1e-8 factor
+        dg, mg, ig, _ = oracle.calculate_distance(g, p)
+        assert not mg and ig == it, p
+        assert dg == pytest.approx(dt, rel=1e-5), p
+        checked += 1
+    assert checked > 40
+
+
+def test_generalized_fold_box_inside_the_solid_is_only_the_sphere_fold():
+    """A point inside every fold plane is not folded: with scale 1 and no
+    sphere fold (radius >= fixed radius) z1 = z + c."""
+    _, g = _box_pair(limit=3.0)
+    g.mb_scale = 1.0
+    p = (1.1, -1.2, 0.9)
+    for mode in range(8):
+        g.mb_fold_mode = mode
+        assert _r1(g, p) == pytest.approx(2.0 * math.sqrt(sum(v * v for v in p)), rel=1e-14), mode
+
+
+@pytest.mark.parametrize("mode,sides", [(0, 4), (1, 6), (2, 8), (3, 12), (4, 14), (5, 20), (6, 8), (7, 7)])
+def test_generalized_fold_box_mirror_in_each_face(mode, sides):
+    """Just beyond the middle of a face only that face cuts the segment from
+    the origin: the point is mirrored in the face plane x.n = solid, so
+    |z1 - c| = |p - 2 (p.n - solid) n| (scale 1, melt 0, the 1e-8 factor
+    aside).  The faces are taken from the reference's polyhedra: every
+    normal is a unit vector."""
+    _, g = _box_pair(limit=1.0)
+    g.mb_fold_mode = mode
+    g.mb_scale = 1.0
+    g.mb_fixed_radius, g.mb_min_radius = 0.5, 0.25   # sphere fold only below 0.5
+    if mode == 3:
+        aa = (1 + 5 ** 0.5) / 2
+        bb = 1 / math.sqrt(aa * aa + 1)
+        n = np.array([0.0, bb, aa * bb])
+    elif mode in (0, 2, 4, 5):
+        n = np.array([1.0, 1.0, -1.0]) / math.sqrt(3.0)
+    else:
+        n = np.array([0.0, 0.0, 1.0])
+    assert abs(np.linalg.norm(n) - 1) < 1e-15
+    p = n * 1.4
+    want = p - 2 * (p @ n - 1.0) * n
+    z1, _ = oracle.compute(g, tuple(p), mode=MODE_DELTA1, n_iter=1)
+    # constant c = p (no Julia): |z1| = |want + p|
+    assert z1 == pytest.approx(np.linalg.norm(want + p), rel=1e-7), (mode, sides)
+
+
+# --- mandelboxVaryScale4D (src/fractal.cpp:887-915) ---------------------------
+# |z + f| - |z - f| - z is the box fold at limit f (value 2f) on each of x, y,
+# z and w; with scale_vary 0 the scale stays fixed, with rpower 1 the radius
+# test is on r^2, with w_add 0 w stays 0: the formula is the Mandelbox with
+# fold limit f, min radius minR and fixed radius 1.
+
+def _vary_pair():
+    t = S.mandelbox_view(64, 48, shaded=False)
+    t.mb_fixed_radius = 1.0
+    v = S.copy(t)
+    v.formula = 18
+    v.mb_vary_scale_vary, v.mb_vary_fold, v.mb_vary_minr = 0.0, t.mb_folding_limit, t.mb_min_radius
+    v.mb_vary_rpower, v.mb_vary_wadd = 1.0, 0.0
+    t.mb_folding_value = 2.0 * t.mb_folding_limit
+    return t, v
+
+
+def test_vary_scale_4d_reduces_to_the_mandelbox():
+    t, v = _vary_pair()
+    rng = np.random.default_rng(18)
+    checked = 0
+    for _ in range(120):
+        p = tuple(rng.uniform(-40, 40, 3) if _ % 2 else rng.uniform(-5, 5, 3))
+        for n in (1, 3):
+            a, _ = oracle.compute(t, p, mode=MODE_DELTA1, n_iter=n)
+            b, _ = oracle.compute(v, p, mode=MODE_DELTA1, n_iter=n)
+            assert b == pytest.approx(a, rel=1e-12), (p, n)
+        dt, mt, it, _ = oracle.calculate_distance(t, p)
+        if mt or it > 12:
+            continue  # chaotic orbits near the set magnify last-bit differences
+        dv, mv, iv, _ = oracle.calculate_distance(v, p)
+        assert not mv and iv == it, p
+        assert dv == pytest.approx(dt, rel=1e-10)
+        checked += 1
+    assert checked > 20
+
+
+def test_vary_scale_4d_scale_and_w():
+    """scale_vary s: the scale becomes k + s (|k| - 1) before the first
+    iteration; w_add a: w1 = a (w starts at 0, folds to 0), so
+    |z1|^2 = |k' p + c|^2 + a^2 for a point inside the fold box whose radius
+    needs no sphere fold (1 <= r^2)."""
+    _, v = _vary_pair()
+    v.mb_scale = 2.5
+    v.mb_vary_scale_vary = 0.2
+    v.mb_vary_wadd = 0.3
+    v.mb_vary_fold = 2.0
+    p = np.array([0.7, -0.6, 0.5])
+    k = 2.5 + 0.2 * (2.5 - 1.0)
+    z1 = k * p + p
+    want = math.sqrt(z1 @ z1 + 0.3 ** 2)
+    got, _ = oracle.compute(v, tuple(p), mode=MODE_DELTA1, n_iter=1)
+    assert got == pytest.approx(want, rel=1e-14)
+
+
+# --- IFS folding mode (src/fractal.cpp:100-126) --------------------------------
+# The pre-fold is the kaleidoscopic IFS step (intensity 1, no edge fold, no
+# Menger mode) run before the formula; followed by the identity formula
+# (offset block with p = 0) the orbit is the kaleidoscopic formula's.
+
+def test_ifs_folding_mode_is_the_kaleidoscopic_step():
+    k = S.menger_view(64, 48, shaded=False)
+    k.formula = 10
+    k.ifs_menger_sponge_mode = 0
+    k.ifs_scale = 2.2
+    for i, d in enumerate([(1, -1, 0), (1, 0, -1), (0, 1, -1)]):
+        for c in range(3):
+            k.ifs_direction[i][c] = d[c]
+        k.ifs_enabled[i] = 1
+        k.ifs_intensity[i] = 1.0
+    k.ifs_abs[0] = k.ifs_abs[1] = k.ifs_abs[2] = 1
+    k.ifs_rotation[0], k.ifs_rotation[1] = 0.3, -0.2
+    k.ifs_angles[1][2] = 0.25
+    for c in range(3):
+        k.ifs_offset[c] = 1.0
+    f = S.copy(k)
+    f.formula = 31        # offsetX
+    f.power = 0.0         # + 0: the identity
+    f.ifs_folding_mode = 1
+    for p in [(0.3, -0.7, 0.45), (-1.1, 0.2, -0.6), (0.9, 0.3, -0.2), (1.4, -0.5, 0.6)]:
+        for n in (1, 2, 3):
+            a, _ = oracle.compute(k, p, mode=MODE_DELTA1, n_iter=n)
+            b, _ = oracle.compute(f, p, mode=MODE_DELTA1, n_iter=n)
+            assert b == a, (p, n)
