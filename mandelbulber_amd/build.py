@@ -60,8 +60,13 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     # the -nogui driver: host C++ over the C ABI, found next to the library
     tmp = CLI + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "mbr_nogui.cpp"),
-           "-o", tmp, "-L" + HERE, "-lmbr", "-Wl,-rpath,$ORIGIN"]
+    # it allocates the batched animation's device framebuffers with the HIP
+    # runtime API (host code only)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(rocm, "include"), os.path.join(CSRC, "mbr_nogui.cpp"),
+           "-o", tmp, "-L" + HERE, "-lmbr", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"mbr_nogui build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -138,6 +138,30 @@ def test_cli_flight_path(tmp_path):
     assert np.array_equal(px.astype(np.uint16), _library_png(fs))
 
 
+def test_cli_batched_flight_equals_frame_by_frame(tmp_path):
+    """Animations render in batches (mbr_render_batch: one graph launch for
+    up to -batch frames into device framebuffers); every frame's file equals,
+    byte for byte, the file of the frame rendered on its own (-batch 1)."""
+    path = tmp_path / "path.txt"
+    cams = [(0.02 * i, -3.5 + 0.05 * i, 0.01 * i, 0.03 * i, -0.01 * i, 0.005 * i) for i in range(5)]
+    path.write_text("".join("%.11f %.11f %.11f %f %f %f\n" % c for c in cams))
+    out = {}
+    for batch in (2, 1):
+        main = tmp_path / f"main{batch}.fract"
+        main.write_text(SETTINGS + f"file_animation_path {path};\nfile_destination {tmp_path}/b{batch}/frame;\n")
+        print(f"[cli] flight -batch {batch}", flush=True)
+        r = subprocess.run([CLI, "-flight", "-batch", str(batch), "-res", "48x36", "-format", "png16", "-end", "10",
+                            str(main)], capture_output=True, text=True, timeout=300)
+        print(r.stdout[-1200:], flush=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+        if batch == 2:
+            assert r.stdout.count("batch of 2 frames") == 2   # frames 0-1, 2-3; frame 4 alone
+        else:
+            assert "batch of" not in r.stdout
+        out[batch] = [(tmp_path / f"b{batch}" / f"frame{i:05d}.png").read_bytes() for i in range(5)]
+    assert out[2] == out[1]
+
+
 def test_cli_tiles_join_into_the_full_frame(tmp_path):
     """tiles 2: four 48x36 tiles rendered one by one, stored, joined into
     file_destination_fromTiles.png (96x72, 16-bit) and removed; without SSAO
