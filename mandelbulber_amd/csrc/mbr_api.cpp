@@ -645,13 +645,6 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
     return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
   if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
     return c->fail(MBR_ERR_INVALID_ARG, "reflections_max above " + std::to_string(MBR_MAX_REFLECTIONS));
-  if (s.raytraced_reflections && s.reflections_max > 0) {
-    bool fx = s.iter_fog_enabled;
-    for (int i = 0; i < 5; i++) fx = fx || s.volumetric_light_enabled[i];
-    if (fx)
-      return c->fail(MBR_ERR_UNSUPPORTED, "raytraced reflections with volumetric lights or iteration fog are not "
-                                          "in the render core");
-  }
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
   if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
   if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
@@ -1336,7 +1329,8 @@ int enqueue_frame_refl(mbr_ctx *c, std::vector<Slot *> &lv, const DevScene *d_sc
     }
     if (e) HIPCHECK(c, hipEventRecord(e[2], st));
     HIPCHECK(c, launch_refl_finish(d_sc, c->d_levels, n_levels, (int)band_first, band_rays, out,
-                                   host.n_lights > 0 && host.sd.aux_visibility > 0, st));
+                                   host.n_lights > 0 && host.sd.aux_visibility > 0, host.shadow_fx, formula, ao,
+                                   c->d_stats, st));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
   return MBR_OK;
@@ -1432,7 +1426,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
     if (rc != MBR_OK) return rc;
     for (int k = 0; k < refl; k++) {
       Slot &lk = c->refl_slots[(size_t)k];
-      rc = alloc_slot(c, lk, precision, sl.band.cap, ao_count, false);
+      rc = alloc_slot(c, lk, precision, sl.band.cap, ao_count, c->host_sc.iter_fog != 0);
       if (rc == MBR_OK) rc = alloc_refl(c, lk, true);
       if (rc != MBR_OK) return rc;
       levels.push_back(&lk);

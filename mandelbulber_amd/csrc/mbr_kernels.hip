@@ -1051,6 +1051,123 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
   }
 }
 
+// The stages of VolumetricShader that run distance estimates, at a step
+// group's point: the volumetric lights (src/shaders.cpp:314-341; stage 0)
+// and the iteration fog (:385-465; stage 1), whose record's iteration count
+// sets the opacity and the fog colour and whose group is lit by the main
+// light and the enabled aux slots through their shadows, plus AO.  `pin` is
+// the shader input of the ray's hit, moved to each group's point and
+// threshold; b and r name the ray's step records.
+template <class T, int F>
+struct FxStages {
+  const DevScene &sc;
+  const BandBufs &b;
+  int r;
+  ShaderIn<T> &pin;
+  LaneCount &cnt;
+  const AODev &ao;
+
+  __device__ void iter_fog(int index, T step, VolOut<T> &o, float &totalOpacity) const {
+    int L = 0;
+    if (index < b.slab) {
+      L = b.rec_it[(long long)index * b.cap + r];
+    } else {
+      const long long dj = b.ovf_off[r];
+      const int ch = dj >= 0 ? b.ovf_dir[dj * kDirSlots + (index - b.slab) / kChunk] : -1;
+      if (ch >= 0) L = b.ovf_it[(long long)ch * kChunk + (index - b.slab) % kChunk];
+    }
+    T opacity = iter_opacity<T>(step, T(L), T(sc.N), (T)sc.iter_fog_trim, (T)sc.iter_fog_opacity);
+    if (!(opacity > T(0))) return;
+    const T trim = (T)sc.iter_fog_trim;
+    const T iterFactor = T(2) * (T(L) - trim) / (T(sc.N) - trim);
+    T kk = iterFactor;
+    if (kk > T(1)) kk = 1;
+    if (kk < T(0)) kk = 0;
+    T kn = T(1) - kk;
+    T fR = T(sc.fog1[0]) * kn + T(sc.fog2[0]) * kk;
+    T fG = T(sc.fog1[1]) * kn + T(sc.fog2[1]) * kk;
+    T fB = T(sc.fog1[2]) * kn + T(sc.fog2[2]) * kk;
+    T k2 = iterFactor - T(1);
+    if (k2 < T(0)) k2 = 0;
+    if (k2 > T(1)) k2 = 1;
+    kn = T(1) - k2;
+    fR = fR * kn + T(sc.fog3[0]) * k2;
+    fG = fG * kn + T(sc.fog3[1]) * k2;
+    fB = fB * kn + T(sc.fog3[2]) * k2;
+    const ShadeK<T> &k = sc.shade<T>();
+    RGBT<T> nc = {0, 0, 0};
+    const T mi = (T)sc.main_light_intensity, dl = k.direct_light;
+    if (mi * dl > T(0)) {
+      const T sh = main_shadow<T, F, false>(sc, pin, cnt);
+      nc.R += sh * T(sc.main_light_colour[0]) / T(65536) * mi * dl;
+      nc.G += sh * T(sc.main_light_colour[1]) / T(65536) * mi * dl;
+      nc.B += sh * T(sc.main_light_colour[2]) / T(65536) * mi * dl;
+    }
+    for (int s4 = 0; s4 < 4; s4++) {
+      if (!sc.aux_slot_on[s4]) continue;
+      const double *Lt = sc.vol_aux[s4];
+      const V3<T> d = v3<T>((T)Lt[0], (T)Lt[1], (T)Lt[2]) - pin.point;
+      const T dist = length(d);
+      const T d2 = dist * dist;
+      const V3<T> lv = normalize(d);
+      const T light = aux_shadow<T, F, false>(sc, pin, dist, lv, cnt);
+      const T intensity = (T)Lt[3] * T(100);
+      nc.R += light * T(Lt[4]) / T(65536) / d2 * intensity;
+      nc.G += light * T(Lt[5]) / T(65536) / d2 * intensity;
+      nc.B += light * T(Lt[6]) / T(65536) / d2 * intensity;
+    }
+    if (sc.ao && !sc.fast_ao) {
+      AOTable<T> tab;
+      if constexpr (sizeof(T) == 8) tab.dir = ao.dir_d; else tab.dir = ao.dir_f;
+      tab.rgb = ao.rgb;
+      tab.count = ao.count;
+      const RGBT<T> a = ambient_occlusion<T, F, false>(sc, pin, tab, cnt);
+      nc.R += a.R * k.global_ilum;
+      nc.G += a.G * k.global_ilum;
+      nc.B += a.B * k.global_ilum;
+    }
+    if (opacity > T(1)) opacity = 1;
+    o.rgb.R = o.rgb.R * (T(1) - opacity) + nc.R * opacity * fR / T(65536);
+    o.rgb.G = o.rgb.G * (T(1) - opacity) + nc.G * opacity * fG / T(65536);
+    o.rgb.B = o.rgb.B * (T(1) - opacity) + nc.B * opacity * fB / T(65536);
+    totalOpacity = (float)(opacity + (T(1) - opacity) * T(totalOpacity));
+    o.alpha = opacity + (T(1) - opacity) * o.alpha;
+  }
+
+  __device__ void operator()(int stage, int index, V3<double> p64, T thr, T step, VolOut<T> &o,
+                             float &totalOpacity) const {
+    pin.point64 = p64;
+    pin.point = cvt<T>(p64);
+    pin.dist_thresh = thr;
+    if (stage == 1) {
+      if (sc.iter_fog) iter_fog(index, step, o, totalOpacity);
+      return;
+    }
+    if (sc.vol_main) {
+      const T sh = main_shadow<T, F, false>(sc, pin, cnt);
+      const T I0 = (T)sc.vol_intensity[0];
+      o.rgb.R += sh * step * I0 * T(sc.main_light_colour[0]) / T(65536);
+      o.rgb.G += sh * step * I0 * T(sc.main_light_colour[1]) / T(65536);
+      o.rgb.B += sh * step * I0 * T(sc.main_light_colour[2]) / T(65536);
+      o.alpha += (sh + sh + sh) / T(3) * step * I0;
+    }
+    for (int k = 0; k < 4; k++) {
+      if (!sc.vol_aux_on[k]) continue;
+      const double *L = sc.vol_aux[k];
+      const V3<T> d = v3<T>((T)L[0], (T)L[1], (T)L[2]) - pin.point;
+      const T dl = length(d);
+      const T d2 = dl * dl;
+      const V3<T> lv = normalize(d);
+      const T light = aux_shadow<T, F, false>(sc, pin, dl, lv, cnt);
+      const T Ik = (T)sc.vol_intensity[k + 1];
+      o.rgb.R += light * T(L[4]) / T(65536) * Ik * step / d2;
+      o.rgb.G += light * T(L[5]) / T(65536) * Ik * step / d2;
+      o.rgb.B += light * T(L[6]) / T(65536) * Ik * step / d2;
+      o.alpha += light * Ik * step / d2;
+    }
+  }
+};
+
 // ---------------------------------------------------------------- finish
 // One pixel: background or shaded surface, then VolumetricShader over the
 // ray's step records.  FX: the volumetric lights' shadow marches run here
@@ -1099,111 +1216,9 @@ __device__ __forceinline__ void finish_body(const DevScene &sc, const BandBufs &
     load_hit(sc, b, r, pin);
     pin.view = cvt<T>(vray.dir);
   }
-  // iteration fog of a step group (src/shaders.cpp:385-465), at pin's point:
-  // the record's iteration count sets the opacity and the fog colour, the
-  // group is lit by the main light and the enabled aux slots through their
-  // shadows, plus AO
-  auto iter_fog_stage = [&](int index, T step, VolOut<T> &o, float &totalOpacity) {
-    if constexpr (FX) {
-      int L = 0;
-      if (index < b.slab) {
-        L = b.rec_it[(long long)index * b.cap + r];
-      } else {
-        const long long dj = b.ovf_off[r];
-        const int ch = dj >= 0 ? b.ovf_dir[dj * kDirSlots + (index - b.slab) / kChunk] : -1;
-        if (ch >= 0) L = b.ovf_it[(long long)ch * kChunk + (index - b.slab) % kChunk];
-      }
-      T opacity = iter_opacity<T>(step, T(L), T(sc.N), (T)sc.iter_fog_trim, (T)sc.iter_fog_opacity);
-      if (!(opacity > T(0))) return;
-      const T trim = (T)sc.iter_fog_trim;
-      const T iterFactor = T(2) * (T(L) - trim) / (T(sc.N) - trim);
-      T kk = iterFactor;
-      if (kk > T(1)) kk = 1;
-      if (kk < T(0)) kk = 0;
-      T kn = T(1) - kk;
-      T fR = T(sc.fog1[0]) * kn + T(sc.fog2[0]) * kk;
-      T fG = T(sc.fog1[1]) * kn + T(sc.fog2[1]) * kk;
-      T fB = T(sc.fog1[2]) * kn + T(sc.fog2[2]) * kk;
-      T k2 = iterFactor - T(1);
-      if (k2 < T(0)) k2 = 0;
-      if (k2 > T(1)) k2 = 1;
-      kn = T(1) - k2;
-      fR = fR * kn + T(sc.fog3[0]) * k2;
-      fG = fG * kn + T(sc.fog3[1]) * k2;
-      fB = fB * kn + T(sc.fog3[2]) * k2;
-      const ShadeK<T> &k = sc.shade<T>();
-      RGBT<T> nc = {0, 0, 0};
-      const T mi = (T)sc.main_light_intensity, dl = k.direct_light;
-      if (mi * dl > T(0)) {
-        const T sh = main_shadow<T, F, false>(sc, pin, cnt);
-        nc.R += sh * T(sc.main_light_colour[0]) / T(65536) * mi * dl;
-        nc.G += sh * T(sc.main_light_colour[1]) / T(65536) * mi * dl;
-        nc.B += sh * T(sc.main_light_colour[2]) / T(65536) * mi * dl;
-      }
-      for (int s4 = 0; s4 < 4; s4++) {
-        if (!sc.aux_slot_on[s4]) continue;
-        const double *Lt = sc.vol_aux[s4];
-        const V3<T> d = v3<T>((T)Lt[0], (T)Lt[1], (T)Lt[2]) - pin.point;
-        const T dist = length(d);
-        const T d2 = dist * dist;
-        const V3<T> lv = normalize(d);
-        const T light = aux_shadow<T, F, false>(sc, pin, dist, lv, cnt);
-        const T intensity = (T)Lt[3] * T(100);
-        nc.R += light * T(Lt[4]) / T(65536) / d2 * intensity;
-        nc.G += light * T(Lt[5]) / T(65536) / d2 * intensity;
-        nc.B += light * T(Lt[6]) / T(65536) / d2 * intensity;
-      }
-      if (sc.ao && !sc.fast_ao) {
-        AOTable<T> tab;
-        if constexpr (sizeof(T) == 8) tab.dir = ao.dir_d; else tab.dir = ao.dir_f;
-        tab.rgb = ao.rgb;
-        tab.count = ao.count;
-        const RGBT<T> a = ambient_occlusion<T, F, false>(sc, pin, tab, cnt);
-        nc.R += a.R * k.global_ilum;
-        nc.G += a.G * k.global_ilum;
-        nc.B += a.B * k.global_ilum;
-      }
-      if (opacity > T(1)) opacity = 1;
-      o.rgb.R = o.rgb.R * (T(1) - opacity) + nc.R * opacity * fR / T(65536);
-      o.rgb.G = o.rgb.G * (T(1) - opacity) + nc.G * opacity * fG / T(65536);
-      o.rgb.B = o.rgb.B * (T(1) - opacity) + nc.B * opacity * fB / T(65536);
-      totalOpacity = (float)(opacity + (T(1) - opacity) * T(totalOpacity));
-      o.alpha = opacity + (T(1) - opacity) * o.alpha;
-    } else {
-      (void)index; (void)step; (void)o; (void)totalOpacity;
-    }
-  };
   auto fx = [&](int stage, int index, V3<double> p64, T thr, T step, VolOut<T> &o, float &totalOpacity) {
     if constexpr (FX) {
-      pin.point64 = p64;
-      pin.point = cvt<T>(p64);
-      pin.dist_thresh = thr;
-      if (stage == 1) {
-        if (sc.iter_fog) iter_fog_stage(index, step, o, totalOpacity);
-        return;
-      }
-      if (sc.vol_main) {
-        const T sh = main_shadow<T, F, false>(sc, pin, cnt);
-        const T I0 = (T)sc.vol_intensity[0];
-        o.rgb.R += sh * step * I0 * T(sc.main_light_colour[0]) / T(65536);
-        o.rgb.G += sh * step * I0 * T(sc.main_light_colour[1]) / T(65536);
-        o.rgb.B += sh * step * I0 * T(sc.main_light_colour[2]) / T(65536);
-        o.alpha += (sh + sh + sh) / T(3) * step * I0;
-      }
-      for (int k = 0; k < 4; k++) {
-        if (!sc.vol_aux_on[k]) continue;
-        const double *L = sc.vol_aux[k];
-        const V3<T> d = v3<T>((T)L[0], (T)L[1], (T)L[2]) - pin.point;
-        const T dl = length(d);
-        const T d2 = dl * dl;
-        const V3<T> lv = normalize(d);
-        const T light = aux_shadow<T, F, false>(sc, pin, dl, lv, cnt);
-        const T Ik = (T)sc.vol_intensity[k + 1];
-        o.rgb.R += light * T(L[4]) / T(65536) * Ik * step / d2;
-        o.rgb.G += light * T(L[5]) / T(65536) * Ik * step / d2;
-        o.rgb.B += light * T(L[6]) / T(65536) * Ik * step / d2;
-        o.alpha += light * Ik * step / d2;
-      }
+      FxStages<T, F>{sc, b, r, pin, cnt, ao}(stage, index, p64, thr, step, o, totalOpacity);
     } else {
       (void)stage; (void)index; (void)p64; (void)thr; (void)step; (void)o; (void)totalOpacity;
     }
@@ -1364,14 +1379,18 @@ __global__ void __launch_bounds__(kBlock) k_refl_march(const DevScene *__restric
 // Per pixel: the levels from the last one reached down to the primary ray
 // (src/Render3D.cpp:501-571): each shaded surface (or background) blended
 // with the next level's result by its reflection coefficient, then that
-// level's VolumetricShader over its own step records.
-template <bool VIS>
+// level's VolumetricShader over its own step records.  FX: with volumetric
+// lights or iteration fog, each level's VolumetricShader runs their shadow
+// marches (formula F) from that level's hit, as the primary ray's does.
+template <int F, bool VIS, bool FX>
 __global__ void __launch_bounds__(kBlock) k_refl_finish(const DevScene *__restrict__ scp,
                                                         const BandBufs *__restrict__ levels, int n_levels,
-                                                        int band_first, int band_rays, FrameOut out) {
+                                                        int band_first, int band_rays, FrameOut out, AODev ao,
+                                                        unsigned long long *stats) {
   const DevScene &sc = *scp;
+  LaneCount cnt = {0, 0};
   const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= band_rays) return;
+  if (r < band_rays) {
   const int p = band_first + r;
   int rayEnd = 0;
   for (int k = 1; k < n_levels; k++) {
@@ -1428,7 +1447,7 @@ __global__ void __launch_bounds__(kBlock) k_refl_finish(const DevScene *__restri
     VisRay vray;
     const VisRay *vr = nullptr;
     const bool vis = VIS && sc.n_lights > 0 && sc.sd.aux_visibility > 0;
-    if (vis) {
+    if (vis || FX) {
       vray.dir = dir;
       vray.start = ray == 0 ? v3<double>(sc.start[0], sc.start[1], sc.start[2])
                             : v3<double>(b.ray_o[3 * (long long)r], b.ray_o[3 * (long long)r + 1],
@@ -1442,7 +1461,20 @@ __global__ void __launch_bounds__(kBlock) k_refl_finish(const DevScene *__restri
       vray.scan_end = sum;
       vr = &vray;
     }
-    const VolOut<double> v = volumetric<double>(sc, depth, n, px, alpha, rec, vr, vis);
+    ShaderIn<double> pin;
+    if constexpr (FX) {
+      load_hit(sc, b, r, pin);
+      pin.view = dir;
+    }
+    auto fx = [&](int stage, int index, V3<double> p64, double thr, double step, VolOut<double> &o,
+                  float &totalOpacity) {
+      if constexpr (FX) {
+        FxStages<double, F>{sc, b, r, pin, cnt, ao}(stage, index, p64, thr, step, o, totalOpacity);
+      } else {
+        (void)stage; (void)index; (void)p64; (void)thr; (void)step; (void)o; (void)totalOpacity;
+      }
+    };
+    const VolOut<double> v = volumetric<double>(sc, depth, n, px, alpha, rec, vr, vis, fx);
     result = v.rgb;
     if (ray == 0) {
       opacity0 = v.opacity;
@@ -1460,6 +1492,13 @@ __global__ void __launch_bounds__(kBlock) k_refl_finish(const DevScene *__restri
   if (out.colour[2]) out.colour[2][p] = found0 ? b0.col_b[r] : 0;
   if (out.opacity) out.opacity[p] = x86_d2u16(opacity0 * 65535.0);
   if (out.alpha) out.alpha[p] = x86_d2u16(alpha0 * 65535.0);
+  }
+  if constexpr (FX) {
+    unsigned long long v[ST_COUNT] = {0};
+    v[ST_SHADE_DE] = cnt.de_calls;
+    v[ST_SHADE_ITERS] = cnt.iterations;
+    flush_stats(stats, v);
+  }
 }
 
 // ---------------------------------------------------------------- launch
@@ -1500,14 +1539,26 @@ hipError_t launch_refl_march(const DevScene *sc, const BandBufs &next, int band_
 }
 
 hipError_t launch_refl_finish(const DevScene *sc, const BandBufs *levels, int n_levels, int band_first,
-                              int band_rays, FrameOut out, int vis, hipStream_t s) {
+                              int band_rays, FrameOut out, int vis, int fx, int formula, AODev ao,
+                              unsigned long long *stats, hipStream_t s) {
   if (band_rays <= 0) return hipSuccess;
-  if (vis)
-    hipLaunchKernelGGL((k_refl_finish<true>), grid_for(band_rays), dim3(kBlock), 0, s, sc, levels, n_levels,
-                       band_first, band_rays, out);
-  else
-    hipLaunchKernelGGL((k_refl_finish<false>), grid_for(band_rays), dim3(kBlock), 0, s, sc, levels, n_levels,
-                       band_first, band_rays, out);
+  const dim3 grid = grid_for(band_rays);
+  auto run = [&](auto ftag) {
+    constexpr int F = decltype(ftag)::value;
+    if (fx)
+      hipLaunchKernelGGL((k_refl_finish<F, true, true>), grid, dim3(kBlock), 0, s, sc, levels, n_levels, band_first,
+                         band_rays, out, ao, stats);
+    else if (vis)
+      hipLaunchKernelGGL((k_refl_finish<0, true, false>), grid, dim3(kBlock), 0, s, sc, levels, n_levels,
+                         band_first, band_rays, out, ao, stats);
+    else
+      hipLaunchKernelGGL((k_refl_finish<0, false, false>), grid, dim3(kBlock), 0, s, sc, levels, n_levels,
+                         band_first, band_rays, out, ao, stats);
+  };
+  if (!fx || formula_generic(formula)) run(std::integral_constant<int, 0>{});
+  else if (formula == 7) run(std::integral_constant<int, 7>{});
+  else if (formula == 8) run(std::integral_constant<int, 8>{});
+  else run(std::integral_constant<int, 2>{});
   return hipGetLastError();
 }
 

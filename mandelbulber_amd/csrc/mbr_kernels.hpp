@@ -163,7 +163,8 @@ hipError_t launch_refl_setup(const DevScene *sc, const BandBufs &prev, const Ban
 hipError_t launch_refl_march(const DevScene *sc, const BandBufs &next, int band_first, int formula, int blocks,
                              unsigned long long *stats, hipStream_t s);
 hipError_t launch_refl_finish(const DevScene *sc, const BandBufs *levels, int n_levels, int band_first,
-                              int band_rays, FrameOut out, int vis, hipStream_t s);
+                              int band_rays, FrameOut out, int vis, int fx, int formula, AODev ao,
+                              unsigned long long *stats, hipStream_t s);
 
 // Screen-space ambient occlusion of one compiled frame (mbr_post.hip).
 // cs/sn: device tables of cos/sin(2 pi i / quality), i < quality.

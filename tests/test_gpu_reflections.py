@@ -1,6 +1,8 @@
 """Raytraced reflections on the GPU against the oracle (MainThread's
 reflection loop, src/Render3D.cpp:440-589): a reflecting Mandelbox, a
-mirror floor under the bulb with aux-light glow, and a reflecting hybrid.
+mirror floor under the bulb with aux-light glow, volumetric lights or
+iteration fog (each level's VolumetricShader runs their shadow marches from
+that level's hit), and a reflecting hybrid.
 Reflection scenes render in FP64 whatever precision is asked for; the
 frame must equal the oracle's in hits and depths."""
 import numpy as np
@@ -43,6 +45,22 @@ def _scene(kind):
         s.plane_reflect = 0.8
         s.reflect = 0.3
         s.reflections_max = 3
+        if kind == "floor_vol":
+            # the main light's and an aux light's volumetric light on every level
+            s.aux_light_pre_enabled[1] = 1
+            s.aux_light_pre[1][:] = (0.8, -1.5, 0.5)
+            s.aux_light_pre_intensity[1] = 0.02
+            s.aux_light_intensity = 0.5
+            s.volumetric_light_enabled[0] = 1
+            s.volumetric_light_intensity[0] = 0.05
+            s.volumetric_light_enabled[2] = 1
+            s.volumetric_light_intensity[2] = 0.5
+        if kind == "floor_fog":
+            s.iter_fog_enabled = 1
+            s.iter_fog_opacity = 300.0
+            s.iter_fog_opacity_trim = 0.0
+            s.ao_quality = 2
+            s.global_illumination = 1
         if kind == "floor_glow":
             s.aux_light_pre_enabled[0] = 1
             s.aux_light_pre[0][:] = (0.8, -1.5, 0.5)
@@ -59,7 +77,7 @@ def _scene(kind):
     return s
 
 
-@pytest.mark.parametrize("kind", ["box", "floor_glow", "hybrid"])
+@pytest.mark.parametrize("kind", ["box", "floor_glow", "hybrid", "floor_vol", "floor_fog"])
 def test_reflections_match_oracle(renderer, kind):
     from oracle import oracle
     sc = _scene(kind)
