@@ -623,6 +623,21 @@ int band_cap(int precision, int ao_count) {
 
 int kernel_family(const mbr_scene &s) { return s.ifs_folding_mode ? kGenericFamily : s.formula; }
 
+// The precision a scene renders in.  The generic formulas run in FP64,
+// except the analytic ones (trig_DE, kaleidoscopic IFS, smoothMandelbox,
+// mandelboxVaryScale4D, generalizedFoldBox) without the IFS folding
+// pre-fold, aux lights, volumetric lights or iteration fog: their FP32 modes
+// march in FP64 and shade in FP32 (FP32_MARCH runs as FP32_FIXUP for them).
+// h: the scene's derived block, when made (batches reject those features).
+int scene_precision(const mbr_scene &s, const DevScene *h, int precision) {
+  if (precision == MBR_PREC_FP64) return precision;
+  const int fam = kernel_family(s);
+  if (!formula_generic(fam)) return precision;
+  const bool analytic = fam == 1 || fam == 10 || fam == 17 || fam == 18 || fam == 37;
+  if (!analytic || (h && (h->n_lights > 0 || h->shadow_fx))) return MBR_PREC_FP64;
+  return MBR_PREC_FP32_FIXUP;
+}
+
 int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.abi_version != MBR_ABI_VERSION) return c->fail(MBR_ERR_INVALID_ARG, "scene abi_version mismatch");
   if (s.formula == MBR_FORMULA_HYBRID) {
@@ -1404,7 +1419,7 @@ int mbr_render_device(mbr_ctx *c, const mbr_render_opts *opts, mbr_framebuffer *
   if (rc != MBR_OK) return rc;
   // formulas without kernels of their own (delta DE, trig_DE) and
   // reflections render in FP64
-  if (formula_generic(kernel_family(s)) || c->host_sc.refl_max > 0) precision = MBR_PREC_FP64;
+  precision = c->host_sc.refl_max > 0 ? MBR_PREC_FP64 : scene_precision(s, &c->host_sc, precision);
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
   if (row1 <= 0 || row1 > s.image_height) row1 = s.image_height;
   if (row0 < 0) row0 = 0;
@@ -1622,7 +1637,7 @@ int mbr_render_device_noalloc(mbr_ctx *c, const mbr_render_opts *opts, mbr_frame
   if (c->host_sc.refl_max > 0)
     return c->fail(MBR_ERR_UNSUPPORTED, "mbr_render_device_noalloc: raytraced reflections need a band slot per "
                                         "level; render them with mbr_render_device");
-  if (formula_generic(kernel_family(s))) precision = MBR_PREC_FP64;
+  precision = scene_precision(s, &c->host_sc, precision);
   unsigned char *base = c->device < kMaxDevices ? (unsigned char *)g_scratch_ptr[c->device] : nullptr;
   if (!base) return c->fail(MBR_ERR_HIP, "mbr_render_device_noalloc: no scratch address for this device");
   int row0 = opts ? opts->row_begin : 0, row1 = opts ? opts->row_end : 0;
@@ -2034,9 +2049,12 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   bool demoivre = false;
   int rc = check_precision(c, precision, demoivre);
   if (rc != MBR_OK) return rc;
-  // a frame of a generic formula (FP64 only) makes the whole batch FP64
-  for (int i = 0; i < n; i++)
-    if (formula_generic(kernel_family(scenes[i]))) precision = MBR_PREC_FP64;
+  // the batch renders in the precision every frame can take (a generic
+  // formula's frame that renders in FP64 only makes the whole batch FP64)
+  for (int i = 0; i < n; i++) {
+    const int p = scene_precision(scenes[i], nullptr, precision);
+    if (p == MBR_PREC_FP64 || (p == MBR_PREC_FP32_FIXUP && precision == MBR_PREC_FP32_MARCH)) precision = p;
+  }
   HIPCHECK(c, hipSetDevice(c->device));
   // derive every scene on the host
   std::vector<BatchFrame> frames(n);

@@ -109,7 +109,10 @@ MBR_HD void rotate_right(T *m, bool &ident, int axis, T angle) {
   ident = false;
 }
 
-template <class T, int Mode>
+// G: the formulas of the generic path are compiled in (always in FP64; in
+// FP32 for the FP32 shading of the analytic generic formulas, whose kernels
+// ask for them, so that the FP32 kernels of formulas 2, 7 and 8 stay small).
+template <class T, int Mode, bool G = (sizeof(T) == 8)>
 MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
 #pragma clang fp contract(off)
   using M = Math<T>;
@@ -246,8 +249,8 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
       z = z * mscale + constant;
       tgladDE = tgladDE * mabs + T(1);
       r = length_nc(z);
-    } else if constexpr (sizeof(T) < 8) {
-      // the generic formulas run in FP64 only
+    } else if constexpr (!G) {
+      // the generic formulas: in the kernels that ask for them only
     } else if (af == 1) {  // trig_DE (src/fractal.cpp:189-209)
       const T r1 = M::pow(r, p - T(1));
       const T r2 = r1 * r;
@@ -1047,7 +1050,7 @@ MBR_HD T de_finish(const FractalK<T> &f, V3<T> point, T distance, int L, bool fr
 // One orbit run of the generic path, kept out of line (one copy per mode).
 template <class T, int Mode>
 __attribute__((noinline)) __host__ __device__ T compute_generic(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
-  return compute<T, Mode>(f, z, N, iter_count);
+  return compute<T, Mode, true>(f, z, N, iter_count);
 }
 
 // Not inlined: the body (five orbit runs over every generic formula) would

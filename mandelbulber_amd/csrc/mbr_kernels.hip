@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     shade2 = Math<T>::pow(shade2, T(30));
     if (shade2 > T(15)) shade2 = 15;
     const T specular = shade2 * k.specular;
-    RGBT<T> colour = surface_colour<T, Deep>(sc, in, cnt);
+    RGBT<T> colour = surface_colour<T, Deep, sizeof(T) == 8 || F == 0>(sc, in, cnt);
     const long long n = sb.n;
     for (int c = 0; c < 3; c++) {
       sb.base[c * n + tid] = k.main_light[c] * shade * shadow;
@@ -1566,7 +1566,8 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
                         int formula, int plain, int persist_blocks, unsigned long long *stats, hipStream_t s,
                         bool ip8) {
   if (band_rays <= 0) return hipSuccess;
-  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
+  // generic formulas: FP64, or the FP64 march of FP32_FIXUP (FP32 records)
+  if (formula_generic(formula) && precision > 1) return hipErrorInvalidValue;
   const dim3 grid((unsigned)std::max(1, persist_blocks));
   auto run = [&](auto tag, auto rtag) {
     using T = decltype(tag);
@@ -1589,10 +1590,9 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
         hipLaunchKernelGGL((k_march<T, RT, 2, Pl>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays, stats);
     };
     if (formula_generic(formula)) {
-      // generic formulas: FP64 only (the host promotes their renders)
-      if constexpr (std::is_same<T, double>::value && std::is_same<RT, double>::value)
-        hipLaunchKernelGGL((k_march<double, double, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first,
-                           band_rays, stats);
+      if constexpr (std::is_same<T, double>::value)
+        hipLaunchKernelGGL((k_march<double, RT, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first, band_rays,
+                           stats);
       return;
     }
     if (plain) go(std::true_type{});
@@ -1607,9 +1607,12 @@ hipError_t launch_march(int precision, const DevScene *sc, const BandBufs &b, in
 hipError_t launch_march_fix(int precision, const DevScene *sc, const BandBufs &b, int band_first, int formula,
                             int persist_blocks, unsigned long long *stats, hipStream_t s, bool ip8) {
   const dim3 grid((unsigned)std::max(1, persist_blocks));
-  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
+  if (formula_generic(formula) && precision > 1) return hipErrorInvalidValue;
   if (formula_generic(formula)) {
-    hipLaunchKernelGGL((k_march_fix<double, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    if (precision == 0)
+      hipLaunchKernelGGL((k_march_fix<double, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
+    else
+      hipLaunchKernelGGL((k_march_fix<float, 0>), grid, dim3(kBlock), 0, s, sc, b, band_first, stats);
     return hipGetLastError();
   }
   auto run = [&](auto rtag) {
@@ -1678,7 +1681,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     const bool generic = formula_generic(formula);
     auto surf = [&](auto deep_tag) {
       constexpr bool D = decltype(deep_tag)::value;
-      if constexpr (!D && std::is_same<T, double>::value) {
+      if constexpr (!D) {
         if (generic) {
           hipLaunchKernelGGL((k_surface<T, 0, false>), grid, dim3(kBlock), 0, s, sc, b, band_first, sb, stats);
           return;
@@ -1705,7 +1708,7 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
           hipLaunchKernelGGL((k_ao_f32_plain<T, 2>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
         return;
       }
-      if constexpr (!D && !Pl && std::is_same<T, double>::value) {
+      if constexpr (!D && !Pl) {
         if (generic) {
           hipLaunchKernelGGL((k_ao<T, 0, false, false>), grid, dim3(kBlock), 0, s, sc, b, ao, sb, stats);
           return;
@@ -1765,7 +1768,9 @@ hipError_t launch_shade(int precision, const DevScene *sc, const BandBufs &b, in
     hipLaunchKernelGGL((k_compose<T, T>), grid, dim3(kBlock), 0, s, sc, b, ao, sb);
     return hipGetLastError();
   };
-  if (formula_generic(formula) && precision != 0) return hipErrorInvalidValue;
+  // generic formulas in FP32: FP32_FIXUP without aux lights (the host keeps
+  // other generic scenes in FP64)
+  if (formula_generic(formula) && (precision > 1 || (precision == 1 && sbuf.lights))) return hipErrorInvalidValue;
   if (precision == 0) return run(double{});
   return run(float{});
 }

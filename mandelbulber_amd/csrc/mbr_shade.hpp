@@ -87,6 +87,27 @@ MBR_HD V3<T> normals(const DevScene &sc, ShaderIn<T> &in, LaneCount &cnt) {
   V3<T> n = v3<T>(0, 0, 0);
   DEInfo info;
   const int N5 = sc.N * 5;
+  if constexpr (F == 0 && sizeof(T) < 8) {
+    // FP32 shading of the generic formulas: the four estimates in FP64 at
+    // the FP64 hit point (differences of FP32 estimates a stencil step
+    // apart keep few digits)
+    if (!sc.slow_shading) {
+      T delta = in.delta * k.smoothness;
+      if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
+      const double dd = (double)delta, det = (double)in.detail;
+      auto de64 = [&](double ox, double oy, double oz) {
+        const double d = calc_distance_generic<double>(sc.fd, in.point64 + v3<double>(ox, oy, oz), det, N5, 0, true,
+                                                       info);
+        cnt.add(info);
+        return d;
+      };
+      const double s1 = de64(0, 0, 0);
+      n = v3<T>((T)(de64(dd, 0, 0) - s1), (T)(de64(0, dd, 0) - s1), (T)(de64(0, 0, dd) - s1));
+      if (n.x == 0 && n.y == 0 && n.z == 0) n.x = 1;
+      else n = normalize(n);
+      return n;
+    }
+  }
   if (!sc.slow_shading) {
     T delta = in.delta * k.smoothness;
     if (sc.interior) delta = in.dist_thresh * k.quality * T(0.2) * k.smoothness;
@@ -326,7 +347,8 @@ MBR_HD void index_to_colour(int index, const int (*palette)[3], int out[3]) {
   }
 }
 
-template <class T, bool Deep = false>
+// G: the colouring orbit may be a generic formula's (compute's G)
+template <class T, bool Deep = false, bool G = (sizeof(T) == 8)>
 MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCount &cnt) {
   RGBT<T> out;
   const int *pc;
@@ -339,7 +361,7 @@ MBR_HD RGBT<T> surface_colour(const DevScene &sc, const ShaderIn<T> &in, LaneCou
         // deep zoom: the FP32 point would merge neighbouring pixels' orbits
         cval = (T)compute<double, MODE_COLOURING>(sc.fd, in.point64, N10, &L);
       else
-        cval = compute<T, MODE_COLOURING>(sc.frac<T>(), in.point, N10, &L);
+        cval = compute<T, MODE_COLOURING, G>(sc.frac<T>(), in.point, N10, &L);
       cnt.de_calls += 1;
       cnt.iterations += (uint32_t)((L < N10) ? L + 1 : N10);
       int nrCol = x86_d2i(floor((double)cval));
@@ -384,7 +406,7 @@ MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<
   shade2 = Math<T>::pow(shade2, T(30));
   if (shade2 > T(15)) shade2 = 15;
   const T specular = shade2 * k.specular;
-  RGBT<T> colour = surface_colour<T, Deep>(sc, in, cnt);
+  RGBT<T> colour = surface_colour<T, Deep, sizeof(T) == 8 || F == 0>(sc, in, cnt);
   colour_out = colour;
   RGBT<T> ambient = {0, 0, 0};
   if (sc.ao) {

@@ -5,10 +5,14 @@ and with shadow + AO + fog.  The delta-DE formulas difference orbit radii
 (pow, atan2, acos, sin, cos) against glibc, so FP64 is held to the parity
 bars rather than to bit equality (the depth bar of the two formulas with
 transcendentals in every iteration on the 99.9th percentile); the other
-precision modes render the same FP64 path and must equal the FP64 frame.
-generalizedFoldBox (all eight fold polyhedra), mandelboxVaryScale4D and the
-IFS folding pre-fold are analytic and, but for pow and the Mandelbulb's
-trig, free of transcendentals: there the FP64 frame equals the oracle's."""
+precision modes render the same FP64 path and must equal the FP64 frame,
+except for the analytic generic formulas (trig_DE, kaleidoscopic IFS,
+smoothMandelbox, mandelboxVaryScale4D, generalizedFoldBox): their FP32 modes
+march in FP64 (the FP64 frame's depths and hits) and shade in FP32, held to
+the parity bars.  generalizedFoldBox (all eight fold polyhedra),
+mandelboxVaryScale4D and the IFS folding pre-fold are analytic and, but for
+pow and the Mandelbulb's trig, free of transcendentals: there the FP64 frame
+equals the oracle's."""
 import numpy as np
 import pytest
 
@@ -16,6 +20,9 @@ from mandelbulber_amd import _abi, scene
 from metrics import assert_parity, compare, assert_planes_exact
 
 pytestmark = pytest.mark.gpu
+
+# formulas whose FP32 modes shade in FP32 (FP32_MARCH runs as FP32_FIXUP)
+FP32_SHADED = (1, 10, 17, 18, 37)
 
 FORMULAS = [(1, 8.0, "trig_DE"), (3, 2.0, "fast_trig"), (4, 2.0, "hypercomplex"), (5, 2.0, "quaternion"),
             (6, 2.0, "minus_fast_trig"), (11, 5.0, "xenodreambuie"), (13, 2.0, "mandelbulb2"),
@@ -75,10 +82,30 @@ def test_generic_formula_matches_oracle(renderer, formula, power, name, shaded):
         # trig_DE and smoothMandelbox are analytic: no finite differences, the FP64 path reproduces the oracle
         assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
         assert_planes_exact(m)
+    _check_fp32(renderer, formula, f64, ref, f"{name} shaded={shaded}")
+
+
+def _check_fp32(renderer, formula, f64, ref, label, ifs_fold=False):
+    """The FP32 modes: the FP64 frame itself where the scene renders in FP64
+    only; for the analytic generic formulas the FP64 march's depths and hits
+    with FP32 shading, at the parity bars, and FP32_MARCH equal to
+    FP32_FIXUP."""
+    fp32 = formula in FP32_SHADED and not ifs_fold
+    frames = []
     for prec, pname in [(_abi.PREC_FP32_FIXUP, "fp32"), (_abi.PREC_FP32_MARCH, "fp32-march")]:
-        print(f"[formulas] {name} shaded={shaded} {pname} (rendered in FP64)", flush=True)
+        print(f"[formulas] {label} {pname}" + ("" if fp32 else " (rendered in FP64)"), flush=True)
         f, _ = renderer.render(precision=prec)
-        assert np.array_equal(f.z, f64.z) and np.array_equal(f.rgb, f64.rgb, equal_nan=True)
+        frames.append(f)
+        if not fp32:
+            assert np.array_equal(f.z, f64.z) and np.array_equal(f.rgb, f64.rgb, equal_nan=True)
+    if fp32:
+        f = frames[0]
+        assert np.array_equal(f.z, f64.z)
+        assert not np.array_equal(f.rgb, f64.rgb)   # the shading ran in FP32
+        m = compare(f, ref)
+        print(f"  fp32 vs oracle: {m}", flush=True)
+        assert_parity(m, label=f"{label} fp32")
+        assert np.array_equal(frames[1].z, f.z) and np.array_equal(frames[1].rgb, f.rgb, equal_nan=True)
 
 
 def _ifs_scene(variant, shaded):
@@ -116,8 +143,7 @@ def test_kaleidoscopic_ifs_matches_oracle(renderer, variant, shaded):
     print(f"  {m}", flush=True)
     assert m["hits"] > 200
     assert_parity(m, label=f"ifs {variant} shaded={shaded}")
-    f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
-    assert np.array_equal(f32.z, f64.z)
+    _check_fp32(renderer, 10, f64, ref, f"kaleidoscopic IFS {variant} shaded={shaded}")
 
 
 HYBRIDS = {
@@ -220,6 +246,4 @@ def test_fold_formulas_match_oracle(renderer, kind, shaded):
     if kind in FOLD_EXACT:
         assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
         assert_planes_exact(m)
-    print(f"[formulas] {kind} shaded={shaded} fp32 (rendered in FP64)", flush=True)
-    f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
-    assert np.array_equal(f32.z, f64.z) and np.array_equal(f32.rgb, f64.rgb, equal_nan=True)
+    _check_fp32(renderer, sc.formula, f64, ref, f"{kind} shaded={shaded}", ifs_fold=bool(sc.ifs_folding_mode))
