@@ -625,15 +625,18 @@ int kernel_family(const mbr_scene &s) { return s.ifs_folding_mode ? kGenericFami
 
 // The precision a scene renders in.  The generic formulas run in FP64,
 // except the analytic ones (trig_DE, kaleidoscopic IFS, smoothMandelbox,
-// mandelboxVaryScale4D, generalizedFoldBox) without the IFS folding
-// pre-fold, aux lights, volumetric lights or iteration fog: their FP32 modes
-// march in FP64 and shade in FP32 (FP32_MARCH runs as FP32_FIXUP for them).
+// mandelboxVaryScale4D, generalizedFoldBox, and any analytic formula under
+// the IFS folding pre-fold) without aux lights, volumetric lights or
+// iteration fog: their FP32 modes march in FP64 and shade in FP32
+// (FP32_MARCH runs as FP32_FIXUP for them).
 // h: the scene's derived block, when made (batches reject those features).
 int scene_precision(const mbr_scene &s, const DevScene *h, int precision) {
   if (precision == MBR_PREC_FP64) return precision;
   const int fam = kernel_family(s);
   if (!formula_generic(fam)) return precision;
-  const bool analytic = fam == 1 || fam == 10 || fam == 17 || fam == 18 || fam == 37;
+  // the IFS folding pre-fold keeps the FP32 mode of an analytic formula
+  const int f = fam == kGenericFamily ? s.formula : fam;
+  const bool analytic = f == 1 || f == 2 || f == 7 || f == 8 || f == 10 || f == 17 || f == 18 || f == 37;
   if (!analytic || (h && (h->n_lights > 0 || h->shadow_fx))) return MBR_PREC_FP64;
   return MBR_PREC_FP32_FIXUP;
 }

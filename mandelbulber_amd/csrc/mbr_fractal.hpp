@@ -144,9 +144,11 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
         mscale = p;
         mabs = M::fabs(p);
       }
+    }
+    if constexpr (G) {
       // IFS folding mode (src/fractal.cpp:101-127): the kaleidoscopic
       // vectors fold z without their intensities or edges; the scenes that
-      // turn it on render in FP64 only
+      // turn it on run the generic kernels
       if (f.ifs_fold) {
         if (f.ifs_abs[0]) z.x = M::fabs(z.x);
         if (f.ifs_abs[1]) z.y = M::fabs(z.y);

@@ -7,9 +7,9 @@ bars rather than to bit equality (the depth bar of the two formulas with
 transcendentals in every iteration on the 99.9th percentile); the other
 precision modes render the same FP64 path and must equal the FP64 frame,
 except for the analytic generic formulas (trig_DE, kaleidoscopic IFS,
-smoothMandelbox, mandelboxVaryScale4D, generalizedFoldBox): their FP32 modes
-march in FP64 (the FP64 frame's depths and hits) and shade in FP32, held to
-the parity bars.  generalizedFoldBox (all eight fold polyhedra),
+smoothMandelbox, mandelboxVaryScale4D, generalizedFoldBox, and formulas 2, 7
+and 8 under the IFS folding pre-fold): their FP32 modes march in FP64 (the
+FP64 frame's depths and hits) and shade in FP32, held to the parity bars.  generalizedFoldBox (all eight fold polyhedra),
 mandelboxVaryScale4D and the IFS folding pre-fold are analytic and, but for
 pow and the Mandelbulb's trig, free of transcendentals: there the FP64 frame
 equals the oracle's."""
@@ -90,7 +90,8 @@ def _check_fp32(renderer, formula, f64, ref, label, ifs_fold=False):
     only; for the analytic generic formulas the FP64 march's depths and hits
     with FP32 shading, at the parity bars, and FP32_MARCH equal to
     FP32_FIXUP."""
-    fp32 = formula in FP32_SHADED and not ifs_fold
+    # under the IFS folding pre-fold formulas 2, 7 and 8 run the generic kernels too
+    fp32 = formula in FP32_SHADED or (ifs_fold and formula in (2, 7, 8))
     frames = []
     for prec, pname in [(_abi.PREC_FP32_FIXUP, "fp32"), (_abi.PREC_FP32_MARCH, "fp32-march")]:
         print(f"[formulas] {label} {pname}" + ("" if fp32 else " (rendered in FP64)"), flush=True)
