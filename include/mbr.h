@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MBR_ABI_VERSION 2
+#define MBR_ABI_VERSION 3
 
 /* largest reflections_max the render core accepts (mirrored rays per pixel) */
 #define MBR_MAX_REFLECTIONS 16
@@ -366,6 +366,17 @@ typedef struct mbr_scene {
   double mb_vary_minr;                  /* mandelbox_vary_minr */
   double mb_vary_rpower;                /* mandelbox_vary_rpower */
   double mb_vary_wadd;                  /* mandelbox_vary_wadd */
+
+  /* fake lights (orbit-trap lights, FakeLights src/shaders.cpp:1225-1270, and
+   * their glow in VolumetricShader :302-312; fake_lights_enabled above):
+   * the orbit's inverse squared distances to the trap point, summed over the
+   * iterations min_iter..max_iter, light the surface and the air */
+  int fake_lights_min_iter;             /* fake_lights_min_iter */
+  int fake_lights_max_iter;             /* fake_lights_max_iter */
+  double fake_lights_intensity;         /* fake_lights_intensity */
+  double fake_lights_visibility;        /* fake_lights_visibility */
+  double fake_lights_visibility_size;   /* fake_lights_visibility_size */
+  double fake_lights_orbit_trap[3];     /* fake_lights_orbit_trap_X, _Y, _Z */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and

@@ -5,7 +5,7 @@ when the library is loaded (see ``mandelbulber_amd._lib``).
 """
 import ctypes as C
 
-MBR_ABI_VERSION = 2
+MBR_ABI_VERSION = 3
 
 MBR_OK = 0
 STATUS = {
@@ -112,6 +112,9 @@ class Scene(C.Structure):
         ("mb_fold_mode", C.c_int), ("mb_solid", C.c_double), ("mb_melt", C.c_double),
         ("mb_vary_scale_vary", C.c_double), ("mb_vary_fold", C.c_double), ("mb_vary_minr", C.c_double),
         ("mb_vary_rpower", C.c_double), ("mb_vary_wadd", C.c_double),
+        ("fake_lights_min_iter", C.c_int), ("fake_lights_max_iter", C.c_int),
+        ("fake_lights_intensity", C.c_double), ("fake_lights_visibility", C.c_double),
+        ("fake_lights_visibility_size", C.c_double), ("fake_lights_orbit_trap", C.c_double * 3),
     ]
 
 

@@ -222,6 +222,9 @@ void fill_fractal(FractalK<T> &f, const mbr_scene &s) {
     }
     f.ifs_menger = s.ifs_menger_sponge_mode;
     f.ifs_fold = s.ifs_folding_mode;
+    f.fl_min = s.fake_lights_min_iter;
+    f.fl_max = s.fake_lights_max_iter;
+    for (int c = 0; c < 3; c++) f.fl_trap[c] = (T)s.fake_lights_orbit_trap[c];
     int period = 0;
     for (int k = 0; k < 5; k++) {
       f.hyb_f[k] = s.hybrid_formula[k];
@@ -632,6 +635,7 @@ int kernel_family(const mbr_scene &s) { return s.ifs_folding_mode ? kGenericFami
 // h: the scene's derived block, when made (batches reject those features).
 int scene_precision(const mbr_scene &s, const DevScene *h, int precision) {
   if (precision == MBR_PREC_FP64) return precision;
+  if (s.fake_lights_enabled) return MBR_PREC_FP64;  // fake lights render in FP64
   const int fam = kernel_family(s);
   if (!formula_generic(fam)) return precision;
   // the IFS folding pre-fold keeps the FP32 mode of an analytic formula
@@ -664,7 +668,6 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
     return c->fail(MBR_ERR_INVALID_ARG, "reflections_max above " + std::to_string(MBR_MAX_REFLECTIONS));
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
-  if (s.fake_lights_enabled) return c->fail(MBR_ERR_UNSUPPORTED, "fake lights are not in the render core");
   if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
   if (s.image_width <= 0 || s.image_height <= 0 || s.N <= 0 || s.N > 100000)
     return c->fail(MBR_ERR_INVALID_ARG, "bad image size or N");
@@ -875,6 +878,10 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
     d.vol_intensity[i] = s.volumetric_light_intensity[i];
   }
   d.post_fog = s.post_fog_enabled;
+  d.fake_lights = s.fake_lights_enabled;
+  d.fl_intensity = s.fake_lights_intensity;
+  d.fl_visibility = s.fake_lights_visibility;
+  d.fl_visibility_size = s.fake_lights_visibility_size;
   d.N = s.N;
   d.minN = s.minN;
   rgb3(d.main_light_colour, s.main_light_colour);
@@ -1168,7 +1175,7 @@ ShadeBufs frame_shade_bufs(Slot &sl, const DevScene &host, int formula, bool tri
   sbuf.deep = host.deep;
   sbuf.plain = host.plain;
   sbuf.formula = formula;
-  sbuf.lights = host.n_lights > 0;
+  sbuf.lights = host.n_lights > 0 || host.fake_lights;
   sbuf.ip8 = trig_free && formula == 2 && host.fd.ipow == 8;
   return sbuf;
 }
@@ -1191,7 +1198,8 @@ int enqueue_band(mbr_ctx *c, Slot &sl, const ShadeBufs &sbuf, int precision, con
                            e ? e[4] : nullptr, e ? e[5] : nullptr));
   if (e) HIPCHECK(c, hipEventRecord(e[2], st));
   HIPCHECK(c, launch_finish(precision, d_sc, b, (int)band_first, band_rays, out, st, formula,
-                            host.vol_any || host.iter_fog, host.n_lights > 0 && host.sd.aux_visibility > 0, persist,
+                            host.vol_any || host.iter_fog || host.fake_lights,
+                            host.n_lights > 0 && host.sd.aux_visibility > 0, persist,
                             d_stats, ao));
   if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   return MBR_OK;
@@ -1276,7 +1284,7 @@ ShadeBufs shade_bufs(Slot &sl, const DevScene &host, int formula) {
   sbuf.deep = 0;
   sbuf.plain = host.plain;
   sbuf.formula = formula;
-  sbuf.lights = host.n_lights > 0;
+  sbuf.lights = host.n_lights > 0 || host.fake_lights;
   sbuf.ip8 = 0;  // reflections render in FP64 with the reference's trig
   return sbuf;
 }
@@ -1347,7 +1355,8 @@ int enqueue_frame_refl(mbr_ctx *c, std::vector<Slot *> &lv, const DevScene *d_sc
     }
     if (e) HIPCHECK(c, hipEventRecord(e[2], st));
     HIPCHECK(c, launch_refl_finish(d_sc, c->d_levels, n_levels, (int)band_first, band_rays, out,
-                                   host.n_lights > 0 && host.sd.aux_visibility > 0, host.shadow_fx, formula, ao,
+                                   host.n_lights > 0 && host.sd.aux_visibility > 0,
+                                   host.shadow_fx || host.fake_lights, formula, ao,
                                    c->d_stats, st));
     if (e) HIPCHECK(c, hipEventRecord(e[3], st));
   }
@@ -2066,9 +2075,10 @@ int mbr_render_batch(mbr_ctx *c, const mbr_scene *scenes, int n, const mbr_textu
   for (int i = 0; i < n; i++) {
     rc = prepare_scene(c, &scenes[i], lightmap, frames[i].p);
     if (rc != MBR_OK) return c->fail(rc, "scene " + std::to_string(i) + ": " + c->err);
-    if (scene_has_lights(frames[i].p.s) || frames[i].p.d.shadow_fx || frames[i].p.d.refl_max > 0)
+    if (scene_has_lights(frames[i].p.s) || frames[i].p.s.fake_lights_enabled || frames[i].p.d.shadow_fx ||
+        frames[i].p.d.refl_max > 0)
       return c->fail(MBR_ERR_UNSUPPORTED, "scene " + std::to_string(i) +
-                                              ": aux lights, volumetric lights, iteration fog and reflections "
+                                              ": aux lights, fake lights, volumetric lights, iteration fog and reflections "
                                               "render through mbr_set_scene, not in a batch");
     const mbr_scene &s = frames[i].p.s;
     if (fbs[i].width != s.image_width || fbs[i].height != s.image_height)

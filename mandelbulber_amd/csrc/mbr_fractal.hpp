@@ -16,7 +16,7 @@
 namespace mbr {
 
 // the values of enumCalculationMode (src/fractal.h)
-enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
+enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4, MODE_ORBIT_TRAP = 5 };
 
 // Formulas of the render core.  2, 7 and 8 have kernels of their own (F = 2,
 // 7, 8); the others run through the generic FP64 path (F = 0): one
@@ -127,6 +127,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
   T mscale = f.mb_scale, mabs = f.mb_abs_scale;  // the hybrid sets scale = p
   T tgladDE = 1;
   T tgladColor = 1;
+  T orbitTrapTotal = 0;  // the fake lights' orbit-trap sum
   V3<T> constant = f.julia_mode ? ld3(f.julia) : z * f.constant_factor;  // aexion rewrites it
   T r = length_nc(z);
   T min = Mode == MODE_COLOURING ? T(3.0e38) : T(0);  // min radius of the orbit (1e200 in FP64)
@@ -645,6 +646,16 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     } else if (Mode == MODE_DELTA2) {
       if (L == delta_limit) break;
     }
+    // the fake lights' orbit trap (src/fractal.cpp:1121-1130): inverse
+    // squared distances to the trap point over iterations min..max, the sum
+    // returned when the orbit leaves the trap's 1000 radius; otherwise the
+    // last distance, or (as in every mode) the Menger or Mandelbox group's
+    // escape value below
+    if (Mode == MODE_ORBIT_TRAP) {
+      distance = length_nc(z - ld3(f.fl_trap));
+      if (L >= f.fl_min && L <= f.fl_max) orbitTrapTotal += T(1) / (distance * distance);
+      if (distance > T(1000)) { distance = orbitTrapTotal; break; }
+    }
 
     // termination; "!(a <= b)" keeps an overflowed (NaN/inf) FP32 orbit
     // counted as escaped, and equals "a > b" for every finite value.  The
@@ -679,6 +690,7 @@ MBR_HD T compute(const FractalK<T> &f, V3<T> z, int N, int *iter_count) {
     return distance;
   }
   if (Mode == MODE_DELTA1 || Mode == MODE_DELTA2) return r;
+  if (Mode == MODE_ORBIT_TRAP) return distance;
   if (f.formula == 12) {  // hybrid colouring (src/fractal.cpp:1218-1229)
     if (min > T(100)) min = T(100);
     if (distance > T(20)) distance = T(20);

@@ -685,9 +685,9 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     // it runs: MainShadow's last threshold, float-rounded (src/shaders.cpp:
     // 510-522), when there are shadows; the slow AO pass reads it from here
     if (sc.ao && !sc.fast_ao) sb.fast_ao[tid] = in.detail;
-    if (sc.n_lights > 0) {
-      // k_aux shades the aux lights from the normal and the detail size
-      // MainShadow left (the aux shadows inherit it)
+    if (sc.n_lights > 0 || sc.fake_lights) {
+      // k_aux shades the aux (and fake) lights from the normal and the
+      // detail size MainShadow left (the aux shadows inherit it)
       sb.aux[0 * n + tid] = in.normal.x;
       sb.aux[1 * n + tid] = in.normal.y;
       sb.aux[2 * n + tid] = in.normal.z;
@@ -725,6 +725,16 @@ __global__ void __launch_bounds__(kBlock) k_aux(const DevScene *__restrict__ scp
     in.detail = sb.aux[3 * n + tid];
     RGBT<T> al, as;
     aux_lights<T, F, Deep>(sc, in, al, as, cnt);
+    if constexpr (sizeof(T) == 8) {
+      if (sc.fake_lights) {
+        // (auxLights + fakeLights) x colour and auxSpecular + fakeSpecular
+        // (ObjectShader, src/shaders.cpp:88-108)
+        RGBT<T> fl, fs;
+        fake_lights<T>(sc, in, fl, fs, cnt);
+        al = RGBT<T>{al.R + fl.R, al.G + fl.G, al.B + fl.B};
+        as = RGBT<T>{as.R + fs.R, as.G + fs.G, as.B + fs.B};
+      }
+    }
     sb.aux[0 * n + tid] = al.R;
     sb.aux[1 * n + tid] = al.G;
     sb.aux[2 * n + tid] = al.B;
@@ -1017,7 +1027,7 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
       T oG = (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG;
       T oB = (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB;
       T sR = sb.spec[0 * n + tid], sG = sb.spec[1 * n + tid], sB = sb.spec[2 * n + tid];
-      if (sc.n_lights > 0) {
+      if (sc.n_lights > 0 || sc.fake_lights) {
         oR += sb.aux[0 * n + tid] * cR;
         oG += sb.aux[1 * n + tid] * cG;
         oB += sb.aux[2 * n + tid] * cB;
@@ -1033,7 +1043,7 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
       ((RT *)b.sp_b)[r] = (RT)sB;
     } else if (lane == 0) {
       const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
-      if (sc.n_lights > 0) {
+      if (sc.n_lights > 0 || sc.fake_lights) {
         // (ambient + main light) x colour + aux x colour, then aux specular
         // + main specular (ObjectShader, src/shaders.cpp:96-108)
         ((RT *)b.sh_r)[r] = (RT)(((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.aux[0 * n + tid] * cR) +
@@ -1142,6 +1152,24 @@ struct FxStages {
     if (stage == 1) {
       if (sc.iter_fog) iter_fog(index, step, o, totalOpacity);
       return;
+    }
+    if constexpr (sizeof(T) == 8) {
+      if (sc.fake_lights) {
+#pragma clang fp contract(off)
+        // fake lights' glow (src/shaders.cpp:302-312); 1e-30f is the
+        // reference's float literal
+        int L = 0;
+        T rt = compute<T, MODE_ORBIT_TRAP>(sc.frac<T>(), pin.point, sc.N, &L);
+        cnt.de_calls += 1;
+        rt = Math<T>::sqrt(T(1) / (rt + T(1.0e-30f)));
+        const T e = T(10) / T(sc.fl_visibility_size);
+        const T glow = T(1) / (Math<T>::pow(rt, e) * Math<T>::pow(T(10), e) + T(1e-100));
+        const T v = T(sc.fl_visibility);
+        o.rgb.R += glow * step * v;
+        o.rgb.G += glow * step * v;
+        o.rgb.B += glow * step * v;
+        o.alpha += glow * step * v;
+      }
     }
     if (sc.vol_main) {
       const T sh = main_shadow<T, F, false>(sc, pin, cnt);

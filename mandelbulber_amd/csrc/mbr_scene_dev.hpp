@@ -30,6 +30,9 @@ struct FractalK {
   T ifs_scale, ifs_offset[3], ifs_edge[3];
   T ifs_rot[9][9], ifs_dir[9][3], ifs_dist[9], ifs_int[9], ifs_main[9];
   int ifs_fold;  // IFS folding mode: the same vectors as a pre-fold of every iteration
+  // fake lights' orbit trap: iterations summed and the trap point
+  int fl_min, fl_max;
+  T fl_trap[3];
   // generalizedFoldBox: face normals of the fold polyhedron (genFoldBox.type),
   // the fold plane offset (solid) and the melt layer
   int gf_sides;
@@ -97,6 +100,10 @@ struct DevScene {
   const double *lights_d;
   const float *lights_f;
   int n_lights, light_div;
+  // fake lights (FakeLights and their glow): on, intensity, visibility and
+  // visibility size
+  int fake_lights;
+  double fl_intensity, fl_visibility, fl_visibility_size;
 
   // volumetric lights (VolumetricShader, src/shaders.cpp:314-341): vol_main
   // for the main light; vol_aux_on[k] when aux light slot k is enabled and

@@ -225,10 +225,13 @@ std::vector<KeyDef> build_table() {
   D("primitive_invertedBox_reflect", OFF(inv_box_reflect)); D("primitive_sphere_reflect", OFF(sphere_reflect));
   D("primitive_invertedSphere_reflect", OFF(inv_sphere_reflect)); D("primitive_water_reflect", OFF(water_reflect));
   I("fake_lights_enabled", OFF(fake_lights_enabled));
-  for (const char *k : {"fake_lights_min_iter", "fake_lights_max_iter", "fake_lights_visibility",
-                        "fake_lights_visibility_size", "fake_lights_intensity", "fake_lights_orbit_trap_X",
-                        "fake_lights_orbit_trap_Y", "fake_lights_orbit_trap_Z"})
-    G(k);
+  I("fake_lights_min_iter", OFF(fake_lights_min_iter)); I("fake_lights_max_iter", OFF(fake_lights_max_iter));
+  D("fake_lights_visibility", OFF(fake_lights_visibility));
+  D("fake_lights_visibility_size", OFF(fake_lights_visibility_size));
+  D("fake_lights_intensity", OFF(fake_lights_intensity));
+  D("fake_lights_orbit_trap_X", OFF_A(fake_lights_orbit_trap, 0, double));
+  D("fake_lights_orbit_trap_Y", OFF_A(fake_lights_orbit_trap, 1, double));
+  D("fake_lights_orbit_trap_Z", OFF_A(fake_lights_orbit_trap, 2, double));
   G("ocl_custom_DE_mode"); G("ocl_custom_formula_name"); G("ocl_delta_DE_step"); G("ocl_DOF_method");
   for (int i = 0; i < 15; i++) G("ocl_custom_par_" + std::to_string(i));
   t.push_back({"file_destination", KT_STRING, OFF(file_destination), 1.0});
@@ -429,6 +432,13 @@ extern "C" int mbr_scene_defaults(mbr_scene *s) {
   s->mb_vary_minr = 0.5;
   s->mb_vary_rpower = 1.0;
   s->mb_vary_wadd = 0.0;
+  // fake lights (src/settings.cpp:520-528)
+  s->fake_lights_intensity = 1.0;
+  s->fake_lights_visibility = 1.0;
+  s->fake_lights_visibility_size = 5.0;
+  s->fake_lights_orbit_trap[0] = 2.0;
+  s->fake_lights_min_iter = 1;
+  s->fake_lights_max_iter = 2;
   s->c_add = -1.3;
   s->ssao_enabled = 1;               // share/defaults: post_SSAO_enabled 1, quality 20
   s->ssao_quality = 20;

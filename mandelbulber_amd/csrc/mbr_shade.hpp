@@ -329,6 +329,36 @@ MBR_HD void aux_lights(const DevScene &sc, ShaderIn<T> &in, RGBT<T> &shade, RGBT
   }
 }
 
+// FakeLights (src/shaders.cpp:1225-1270): the orbit-trap sum at the hit and
+// one step of dist_thresh x smoothness along each axis give the fake light's
+// direction; Lambert and a power-30 specular term of intensity / sum.
+// Fake-light scenes render in FP64.
+template <class T>
+MBR_HD void fake_lights(const DevScene &sc, const ShaderIn<T> &in, RGBT<T> &shade, RGBT<T> &spec, LaneCount &cnt) {
+  const FractalK<T> &f = sc.frac<T>();
+  const T delta = in.dist_thresh * sc.shade<T>().smoothness;
+  int L = 0;
+  const T rr = compute<T, MODE_ORBIT_TRAP>(f, in.point, sc.N, &L);
+  const T fakeLight = T(sc.fl_intensity) / rr;
+  const T r = T(1) / (rr + T(1e-30));
+  const T rx = T(1) / (compute<T, MODE_ORBIT_TRAP>(f, in.point + v3<T>(delta, 0, 0), sc.N, &L) + T(1e-30));
+  const T ry = T(1) / (compute<T, MODE_ORBIT_TRAP>(f, in.point + v3<T>(0, delta, 0), sc.N, &L) + T(1e-30));
+  const T rz = T(1) / (compute<T, MODE_ORBIT_TRAP>(f, in.point + v3<T>(0, 0, delta), sc.N, &L) + T(1e-30));
+  cnt.de_calls += 4;
+  V3<T> n = v3<T>(r - rx, r - ry, r - rz);
+  if (n.x == 0 && n.y == 0 && n.z == 0) n.x = 0;
+  else n = normalize(n);
+  T lam = fakeLight * dot(in.normal, n);
+  if (lam < 0) lam = 0;
+  shade = RGBT<T>{lam, lam, lam};
+  const V3<T> half = normalize(n - in.view);
+  T fs = dot(in.normal, half);
+  if (fs < T(0)) fs = 0;
+  fs = Math<T>::pow(fs, T(30)) * fakeLight;
+  if (fs > T(15)) fs = 15;
+  spec = RGBT<T>{fs, fs, fs};
+}
+
 MBR_HD void index_to_colour(int index, const int (*palette)[3], int out[3]) {
   if (index < 0) {
     out[0] = palette[255][0]; out[1] = palette[255][1]; out[2] = palette[255][2];

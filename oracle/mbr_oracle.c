@@ -244,7 +244,7 @@ static inline unsigned short x86_d2u16(double d) { return (unsigned short)(unsig
 static inline unsigned char x86_d2u8(double d) { return (unsigned char)(unsigned)x86_d2i(d); }
 
 /* --------------------------------------------------------------- Compute */
-enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4 };
+enum { MODE_NORMAL = 0, MODE_COLOURING = 1, MODE_DELTA1 = 3, MODE_DELTA2 = 4, MODE_ORBIT_TRAP = 5 };
 
 /* CVector3(alfa, beta) (src/algebra.hpp:33-38) */
 static v3 angle_vec(double alfa, double beta) { return v3m(cos(beta) * cos(alfa), cos(beta) * sin(alfa), sin(beta)); }
@@ -271,6 +271,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
   double mR2 = minRadius * minRadius;
   double tglad_factor1 = fR2 / mR2;
   double tgladDE = 1.0;
+  double orbitTrapTotal = 0.0;
   double scale = s->mb_scale;
   double tgladColor = 1.0;
   int formula = s->formula;
@@ -792,6 +793,17 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
       if (L == delta_limit) break;
     }
 
+    /* the fake lights' orbit trap (:1121-1130): inverse squared distances to
+     * the trap point over iterations min_iter..max_iter; the sum is returned
+     * when the orbit leaves the trap's 1000 radius, else the last distance
+     * (or, as in every mode, the escape value of the Menger and Mandelbox
+     * groups below) */
+    if (mode == MODE_ORBIT_TRAP) {
+      distance = v3len(v3sub(z, v3from(s->fake_lights_orbit_trap)));
+      if (L >= s->fake_lights_min_iter && L <= s->fake_lights_max_iter) orbitTrapTotal += (1.0 / (distance * distance));
+      if (distance > 1000) { distance = orbitTrapTotal; break; }
+    }
+
     if (formula == MBR_FORMULA_MENGER_SPONGE || formula == MBR_FORMULA_KALEIDOSCOPIC) {
       if (r > 1000) { distance = (r - 2.0) / tgladDE; break; }
     } else if (formula == MBR_FORMULA_TGLAD || formula == MBR_FORMULA_SMOOTH_MANDELBOX ||
@@ -819,6 +831,7 @@ static double compute(const fractal_ctx *f, int mode, v3 z, int N, int *iter_cou
     return distance;
   }
   if (mode == MODE_DELTA1 || mode == MODE_DELTA2) return r;
+  if (mode == MODE_ORBIT_TRAP) return distance;
   if (s->formula == MBR_FORMULA_HYBRID) {  /* src/fractal.cpp:1218-1229 */
     if (min > 100) min = 100;
     if (distance > 20) distance = 20;
@@ -1546,8 +1559,39 @@ static void light_set_init(light_set *L, const mbr_scene *s, const mbr_light *ta
     if (light_used(L, i)) L->any = 1;
 }
 
-/* ObjectShader (src/shaders.cpp:21-112) without env mapping and fake
- * lights (rejected scene features; their terms are zero). */
+/* FakeLights (src/shaders.cpp:1225-1270): the orbit-trap sum at the point
+ * and one threshold-sized step along each axis gives the fake light's
+ * direction; Lambert and a power-30 specular term, intensity / sum.
+ * Parity partly pinned: the orbit trap by closed forms, this shader and the
+ * glow only by linearity in intensity and visibility. */
+static shade_t fake_lights(shader_in *in, shade_t *fakeSpec) {
+  const mbr_scene *s = in->f->s;
+  shade_t out = {0, 0, 0, 0};
+  int it = 0;
+  double delta = in->dist_thresh * s->smoothness;
+  double rr = compute(in->f, MODE_ORBIT_TRAP, in->point, s->N, &it);
+  double fakeLight = s->fake_lights_intensity / rr;
+  double r = 1.0 / (rr + 1e-30);
+  double rx = 1.0 / (compute(in->f, MODE_ORBIT_TRAP, v3add(in->point, v3m(delta, 0.0, 0.0)), s->N, &it) + 1e-30);
+  double ry = 1.0 / (compute(in->f, MODE_ORBIT_TRAP, v3add(in->point, v3m(0.0, delta, 0.0)), s->N, &it) + 1e-30);
+  double rz = 1.0 / (compute(in->f, MODE_ORBIT_TRAP, v3add(in->point, v3m(0.0, 0.0, delta)), s->N, &it) + 1e-30);
+  v3 n = v3m(r - rx, r - ry, r - rz);
+  if (n.x == 0 && n.y == 0 && n.z == 0) n.x = 0.0;
+  else n = v3norm(n);
+  double fakeLight2 = fakeLight * v3dot(in->normal, n);
+  if (fakeLight2 < 0) fakeLight2 = 0;
+  out.R = out.G = out.B = fakeLight2;
+  v3 half = v3norm(v3sub(n, in->viewVector));
+  double fs = v3dot(in->normal, half);
+  if (fs < 0.0) fs = 0.0;
+  fs = pow(fs, 30.0) * fakeLight;
+  if (fs > 15.0) fs = 15.0;
+  fakeSpec->R = fakeSpec->G = fakeSpec->B = fs;
+  return out;
+}
+
+/* ObjectShader (src/shaders.cpp:21-112) without env mapping (a rejected
+ * scene feature; its term is zero). */
 static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *specularOut) {
   const mbr_scene *s = in->f->s;
   shade_t output;
@@ -1590,13 +1634,15 @@ static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *spe
   output.B = 0.0 * ambient.B + (ambient2.B + mainLight.B * shade * shadow) * colour.B;
   shade_t auxSpecular;
   shade_t auxLights = aux_lights_shader(in, &auxSpecular);
-  output.R += (auxLights.R + 0.0) * colour.R;
-  output.G += (auxLights.G + 0.0) * colour.G;
-  output.B += (auxLights.B + 0.0) * colour.B;
+  shade_t fakeLights = {0, 0, 0, 0}, fakeSpecular = {0, 0, 0, 0};
+  if (s->fake_lights_enabled) fakeLights = fake_lights(in, &fakeSpecular);
+  output.R += (auxLights.R + fakeLights.R) * colour.R;
+  output.G += (auxLights.G + fakeLights.G) * colour.G;
+  output.B += (auxLights.B + fakeLights.B) * colour.B;
   output.alpha = 1.0;
-  specularOut->R = auxSpecular.R + 0.0 + mainLight.R * specular * shadow;
-  specularOut->G = auxSpecular.G + 0.0 + mainLight.G * specular * shadow;
-  specularOut->B = auxSpecular.B + 0.0 + mainLight.B * specular * shadow;
+  specularOut->R = auxSpecular.R + fakeSpecular.R + mainLight.R * specular * shadow;
+  specularOut->G = auxSpecular.G + fakeSpecular.G + mainLight.G * specular * shadow;
+  specularOut->B = auxSpecular.B + fakeSpecular.B + mainLight.B * specular * shadow;
   specularOut->alpha = output.alpha;
   return output;
 }
@@ -1627,7 +1673,7 @@ static shade_t background_shader(const mbr_scene *s, v3 viewVector) {
 
 /* VolumetricShader (src/shaders.cpp:185-478): glow, basic fog and
  * volumetric fog (visible aux lights contribute nothing when no light is
- * enabled; fake lights, volumetric lights and iteration fog are rejected
+ * enabled; volumetric lights and iteration fog are rejected
  * scene features).  *opacity_written is 0 when no step group was emitted
  * (the reference then leaves opacityOut uninitialised; we report 0). */
 static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_rec *buf, int stepCount,
@@ -1714,6 +1760,20 @@ static shade_t volumetric_shader(const mbr_scene *s, double depth, const step_re
           output.alpha += lightDensity;
         }
       }
+    }
+
+    /* fake lights' glow (:302-312): the orbit-trap sum at the group's point
+     * (1e-30f is the reference's float literal) */
+    if (s->fake_lights_enabled && pin) {
+      int it = 0;
+      double rt = compute(in.f, MODE_ORBIT_TRAP, buf[index].point, s->N, &it);
+      rt = sqrt(1.0f / (rt + 1.0e-30f));
+      double fakeLight = 1.0 / (pow(rt, 10.0 / s->fake_lights_visibility_size) *
+                                    pow(10.0, 10.0 / s->fake_lights_visibility_size) + 1e-100);
+      output.R += fakeLight * step * s->fake_lights_visibility;
+      output.G += fakeLight * step * s->fake_lights_visibility;
+      output.B += fakeLight * step * s->fake_lights_visibility;
+      output.alpha += fakeLight * step * s->fake_lights_visibility;
     }
 
     /* volumetric lights with shadows in fog (:314-341): the main light

@@ -201,3 +201,44 @@ def test_batch_rejects_aux_lights(renderer):
     }
     with pytest.raises(RuntimeError, match="aux lights"):
         renderer.render_batch([sc], [planes])
+
+
+@pytest.mark.parametrize("kind", ["surface", "glow", "box"])
+def test_fake_lights_match_oracle(renderer, kind):
+    """Fake (orbit-trap) lights: FakeLights on the surface (src/shaders.cpp:
+    1225-1270) and their glow in VolumetricShader (:302-312), FP64 against
+    the oracle bit for bit; the FP32 modes render such scenes in FP64."""
+    from oracle import oracle
+    if kind == "box":
+        s = scene.mandelbox_view(96, 72, shaded=True)
+        s.N = 30
+        s.ao_quality = 2
+    else:
+        s = scene.mandelbulb_full(96, 72, seed=1) if kind == "glow" else scene.mandelbulb_plain(96, 72)
+        s.N = 30
+        s.ao_quality = 2
+    s.fake_lights_enabled = 1
+    for i, v in enumerate((0.0, 0.3, 0.2)):
+        s.fake_lights_orbit_trap[i] = v
+    s.fake_lights_min_iter, s.fake_lights_max_iter = 0, 3
+    s.fake_lights_intensity = 100.0
+    s.fake_lights_visibility = 20.0 if kind != "surface" else 0.0
+    s.fake_lights_visibility_size = 10.0
+    ref, _ = oracle.render(s)
+    renderer.set_scene(s)
+    print(f"[fake lights] {kind} fp64", flush=True)
+    f64, _ = renderer.render(precision=_abi.PREC_FP64)
+    m = compare(f64, ref)
+    print(f"  {m}", flush=True)
+    assert m["hits"] > 500
+    assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12, m
+    assert_parity(m, label=f"fake lights {kind}")
+    assert_planes_exact(m)
+    f32, _ = renderer.render(precision=_abi.PREC_FP32_FIXUP)
+    assert np.array_equal(f32.rgb, f64.rgb) and np.array_equal(f32.z, f64.z)
+    # the fake lights show
+    off = scene.copy(s)
+    off.fake_lights_enabled = 0
+    renderer.set_scene(off)
+    f0, _ = renderer.render(precision=_abi.PREC_FP64)
+    assert np.abs(f0.rgb - f64.rgb).max() > 1e-3
