@@ -377,6 +377,8 @@ typedef struct mbr_scene {
   double fake_lights_visibility;        /* fake_lights_visibility */
   double fake_lights_visibility_size;   /* fake_lights_visibility_size */
   double fake_lights_orbit_trap[3];     /* fake_lights_orbit_trap_X, _Y, _Z */
+  int background_as_fulldome;           /* background_as_fuldome: the background texture is a
+                                           fisheye dome (two hemispheres side by side) */
 } mbr_scene;
 
 /* Optional AO lightmap (file_lightmap).  The reference loads it as JPEG and
@@ -482,6 +484,15 @@ int mbr_cancel(mbr_ctx *ctx);
  * RecalculateIFSParams does (src/callbacks.cpp:1538-1555). */
 int mbr_set_scene(mbr_ctx *ctx, const mbr_scene *scene,
                   const mbr_texture *lightmap);
+
+/* The textures of textured_background (file_background) and of environment
+ * mapping (file_envmap), decoded by the caller into 8-bit RGB as the
+ * reference's cTexture holds them (src/texture.cpp).  The context keeps
+ * device copies until replaced; NULL clears one.  Call before
+ * mbr_set_scene: a scene with textured_background set and no background
+ * texture returns MBR_ERR_UNSUPPORTED.  Replaces the reference's
+ * backgroundTexture / envmapTexture loading (src/Render3D.cpp:1845-1862). */
+int mbr_set_textures(mbr_ctx *ctx, const mbr_texture *background, const mbr_texture *envmap);
 
 /* Replaces CclSupport::Render (cl_support.cpp:527-742) and the CPU
  * MainThread pool: renders the scene into caller-owned HOST planes. */

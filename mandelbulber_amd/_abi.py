@@ -115,6 +115,7 @@ class Scene(C.Structure):
         ("fake_lights_min_iter", C.c_int), ("fake_lights_max_iter", C.c_int),
         ("fake_lights_intensity", C.c_double), ("fake_lights_visibility", C.c_double),
         ("fake_lights_visibility_size", C.c_double), ("fake_lights_orbit_trap", C.c_double * 3),
+        ("background_as_fulldome", C.c_int),
     ]
 
 

@@ -18,7 +18,7 @@ EXPORTS = (
     "mbr_scene_layout", "mbr_jitter_draw", "mbr_render_batch", "mbr_adjoint", "mbr_compile_image",
     "mbr_write_png", "mbr_post_ssao", "mbr_post_dof", "mbr_compile_image_device", "mbr_keyframe_scene",
     "mbr_place_lights", "mbr_set_lights", "mbr_get_lights", "mbr_adjoint_jacobian",
-    "mbr_render_device_noalloc", "mbr_cancel",
+    "mbr_render_device_noalloc", "mbr_cancel", "mbr_set_textures",
 )
 
 _lib = None
@@ -63,6 +63,7 @@ def _declare(lib):
     lib.mbr_get_lights.argtypes = [C.c_void_p, P(_abi.Light), C.c_int, P(C.c_int)]
     lib.mbr_post_dof.argtypes = [C.c_void_p, P(_abi.Scene), P(_abi.PostPlanes), C.c_int, C.c_void_p]
     lib.mbr_cancel.argtypes = [C.c_void_p]
+    lib.mbr_set_textures.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.mbr_render_batch.argtypes = [C.c_void_p, P(_abi.Scene), C.c_int, P(_abi.Texture), P(_abi.Framebuffer),
                                      P(_abi.BatchOpts), P(_abi.Stats), P(_abi.BatchResult)]
 

@@ -422,6 +422,9 @@ struct mbr_ctx {
   double *d_ao_d = nullptr;
   int *d_ao_rgb = nullptr;
   int ao_count = 0;
+  // textures of mbr_set_textures (device copies, 8-bit RGB rows)
+  uint8_t *d_bg = nullptr, *d_env = nullptr;
+  int bg_w = 0, bg_h = 0, env_w = 0, env_h = 0;
   Slot slots[kMaxSlots];
   unsigned long long *d_stats = nullptr;
   int cu_count = 0;
@@ -668,7 +671,8 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
     return c->fail(MBR_ERR_INVALID_ARG, "reflections_max above " + std::to_string(MBR_MAX_REFLECTIONS));
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
-  if (s.textured_background) return c->fail(MBR_ERR_UNSUPPORTED, "textured background is not in the render core");
+  if (s.textured_background && !c->d_bg)
+    return c->fail(MBR_ERR_UNSUPPORTED, "textured background: no background texture (mbr_set_textures)");
   if (s.image_width <= 0 || s.image_height <= 0 || s.N <= 0 || s.N > 100000)
     return c->fail(MBR_ERR_INVALID_ARG, "bad image size or N");
   return MBR_OK;
@@ -749,7 +753,7 @@ void mbr_destroy(mbr_ctx *c) {
   for (hipStream_t st : c->side)
     if (st) (void)hipStreamDestroy(st);
   void *bp[] = {c->d_batch_sc, c->d_batch_ao_f, c->d_batch_ao_d, c->d_batch_ao_rgb, c->d_grad, c->d_ssao_tab,
-                c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma, c->d_lights_d, c->d_lights_f};
+                c->d_post, c->d_dof, c->d_dof_pairs, c->d_gamma, c->d_lights_d, c->d_lights_f, c->d_bg, c->d_env};
   for (void *p : bp)
     if (p) (void)hipFree(p);
   for (Slot &sl : c->refl_slots) free_band(sl);
@@ -878,6 +882,11 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
     d.vol_intensity[i] = s.volumetric_light_intensity[i];
   }
   d.post_fog = s.post_fog_enabled;
+  d.textured_bg = s.textured_background;
+  d.fulldome = s.background_as_fulldome;
+  d.bg_tex = c->d_bg;
+  d.bg_w = c->bg_w;
+  d.bg_h = c->bg_h;
   d.fake_lights = s.fake_lights_enabled;
   d.fl_intensity = s.fake_lights_intensity;
   d.fl_visibility = s.fake_lights_visibility;
@@ -1112,6 +1121,31 @@ int mbr_get_lights(mbr_ctx *c, mbr_light *lights, int capacity, int *count) {
   if (capacity < *count) return c->fail(MBR_ERR_INVALID_ARG, "capacity below the light count");
   std::memcpy(lights, c->lights.data(), c->lights.size() * sizeof(mbr_light));
   return MBR_OK;
+}
+
+int mbr_set_textures(mbr_ctx *c, const mbr_texture *background, const mbr_texture *envmap) {
+  if (!c) return MBR_ERR_INVALID_ARG;
+  for (const mbr_texture *t : {background, envmap})
+    if (t && (!t->rgb || t->width <= 0 || t->height <= 0 || (long long)t->width * t->height > (1ll << 28)))
+      return c->fail(MBR_ERR_INVALID_ARG, "bad texture");
+  HIPCHECK(c, hipSetDevice(c->device));
+  // the previous frame may still read the old copies
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  auto put = [&](const mbr_texture *t, uint8_t *&d, int &w, int &h) -> int {
+    if (d) (void)hipFree(d);
+    d = nullptr;
+    w = h = 0;
+    if (!t) return MBR_OK;
+    const size_t bytes = (size_t)t->width * t->height * 3;
+    HIPCHECK(c, hipMalloc((void **)&d, bytes));
+    HIPCHECK(c, hipMemcpy(d, t->rgb, bytes, hipMemcpyHostToDevice));
+    w = t->width;
+    h = t->height;
+    return MBR_OK;
+  };
+  int rc = put(background, c->d_bg, c->bg_w, c->bg_h);
+  if (rc == MBR_OK) rc = put(envmap, c->d_env, c->env_w, c->env_h);
+  return rc;
 }
 
 int mbr_set_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *lightmap) {

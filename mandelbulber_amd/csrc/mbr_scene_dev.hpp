@@ -104,6 +104,10 @@ struct DevScene {
   // visibility size
   int fake_lights;
   double fl_intensity, fl_visibility, fl_visibility_size;
+  // textured background (BackgroundShader's texture branches): the
+  // context's background texture (8-bit RGB rows), dome layout
+  int textured_bg, fulldome, bg_w, bg_h;
+  const uint8_t *bg_tex;
 
   // volumetric lights (VolumetricShader, src/shaders.cpp:314-341): vol_main
   // for the main light; vol_aux_on[k] when aux light slot k is enabled and

@@ -84,7 +84,7 @@ std::vector<KeyDef> build_table() {
   RGB("fog_colour_1_", OFF_RGB(fog_colour1, R), OFF_RGB(fog_colour1, G), OFF_RGB(fog_colour1, B));
   RGB("fog_colour_2_", OFF_RGB(fog_colour2, R), OFF_RGB(fog_colour2, G), OFF_RGB(fog_colour2, B));
   RGB("fog_colour_3_", OFF_RGB(fog_colour3, R), OFF_RGB(fog_colour3, G), OFF_RGB(fog_colour3, B));
-  I("textured_background", OFF(textured_background)); G("background_as_fuldome");
+  I("textured_background", OFF(textured_background)); I("background_as_fuldome", OFF(background_as_fulldome));
   I("shadows_enabled", OFF(shadow)); I("ambient_occlusion_enabled", OFF(global_illumination));
   I("fast_ambient_occlusion_mode", OFF(fast_global_illumination));
   I("fractal_color", OFF(coloring_enabled));

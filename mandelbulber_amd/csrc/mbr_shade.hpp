@@ -454,8 +454,69 @@ MBR_HD RGBT<T> object_shader(const DevScene &sc, ShaderIn<T> &in, const AOTable<
   return out;
 }
 
+// cTexture::Pixel / Interpolation (src/texture.cpp:83-111) on 8-bit RGB
+// rows: bilinear over the texels (ix, iy), (ix + 1, iy), (ix, iy + 1),
+// (ix + 1, iy + 1), each channel's sum truncated to 8 bits; black outside
+// [0, w) x [0, h - 1).  ix + 1 at the last column reads the next row's
+// first texel, as the reference's flat index does; the one texel past the
+// bitmap's end that it then reads (last column, second-last row) reads
+// black here.
+MBR_HD inline void tex_pixel(const uint8_t *bm, int w, int h, double x, double y, int out[3]) {
+#pragma clang fp contract(off)
+  out[0] = out[1] = out[2] = 0;
+  if (!(x >= 0 && x < w && y >= 0 && y < h - 1.0)) return;
+  const int ix = (int)x, iy = (int)y;
+  const double rx = x - (int)x, ry = y - (int)y;
+  const long long n = (long long)w * h;
+  const long long i1 = (long long)iy * w + ix, i2 = i1 + 1;
+  const long long i3 = (long long)(iy + 1) * w + ix, i4 = i3 + 1;
+  for (int c = 0; c < 3; c++) {
+    const double k1 = bm[3 * i1 + c], k2 = bm[3 * i2 + c], k3 = bm[3 * i3 + c];
+    const double k4 = i4 < n ? bm[3 * i4 + c] : 0;
+    out[c] = (int)(uint8_t)(int)(k1 * (1.0 - rx) * (1.0 - ry) + k2 * rx * (1.0 - ry) + k3 * (1.0 - rx) * ry +
+                                 k4 * (rx * ry));
+  }
+}
+
+// BackgroundShader's texture branches (src/shaders.cpp:118-157): the view
+// vector's angles index the background texture, as a fisheye dome of two
+// hemispheres side by side or as an equirectangular map.
+MBR_HD inline void background_texture(const DevScene &sc, V3<double> v, double out[3]) {
+#pragma clang fp contract(off)
+  const double pi = 3.14159265358979323846;
+  double alfa = atan2(v.y, v.x);
+  double beta = atan2(v.z, sqrt(v.x * v.x + v.y * v.y));
+  double texX, texY;
+  if (sc.fulldome) {
+    const int texWidth = (int)(sc.bg_w * 0.5);
+    const int texHeight = sc.bg_h;
+    int offset = 0;
+    if (beta < 0) {
+      beta = -beta;
+      alfa += pi;
+      offset = texWidth;
+    }
+    texX = 0.5 * texWidth + cos(alfa) * (1.0 - beta / (0.5 * pi)) * texWidth * 0.5 + offset;
+    texY = 0.5 * texHeight + sin(alfa) * (1.0 - beta / (0.5 * pi)) * texHeight * 0.5;
+  } else {
+    alfa = fmod(alfa + 2.5 * pi, 2 * pi);
+    if (beta > 0.5 * pi) beta = 0.5 * pi - beta;
+    if (beta < -0.5 * pi) beta = -0.5 * pi + beta;
+    texX = alfa / (2.0 * pi) * sc.bg_w;
+    texY = (beta / (pi) + 0.5) * sc.bg_h;
+  }
+  int px[3];
+  tex_pixel(sc.bg_tex, sc.bg_w, sc.bg_h, texX, texY, px);
+  for (int c = 0; c < 3; c++) out[c] = px[c] / 256.0;
+}
+
 template <class T>
 MBR_HD RGBT<T> background(const DevScene &sc, V3<T> view) {
+  if (sc.textured_bg) {
+    double t[3];
+    background_texture(sc, v3<double>((double)view.x, (double)view.y, (double)view.z), t);
+    return RGBT<T>{T(t[0]), T(t[1]), T(t[2])};
+  }
   T grad = -view.z + T(1);  // viewVector . (0,0,-1) + 1
   const int *c_lo, *c_hi;
   if (grad < T(1)) {

@@ -97,6 +97,20 @@ class Renderer:
         o.row_begin, o.row_end = rows if rows is not None else (0, 0)
         return o
 
+    def set_textures(self, background=None, envmap=None):
+        """mbr_set_textures: the background and environment-map textures,
+        each an (h, w, 3) uint8 array or None (before set_scene)."""
+        keep, ptrs = [], []
+        for t in (background, envmap):
+            if t is None:
+                ptrs.append(None)
+                continue
+            a = np.ascontiguousarray(t, dtype=np.uint8)
+            tex = _abi.Texture(a.shape[1], a.shape[0], a.ctypes.data_as(C.POINTER(C.c_uint8)))
+            keep.append((a, tex))
+            ptrs.append(C.byref(tex))
+        check(lib().mbr_set_textures(self._ctx, ptrs[0], ptrs[1]), self._ctx)
+
     def cancel(self):
         """mbr_cancel: stop the render in progress on this context (from
         another thread); it then raises MbrError(MBR_ERR_CANCELLED)."""

@@ -1647,10 +1647,71 @@ static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *spe
   return output;
 }
 
-/* BackgroundShader, gradient branch (src/shaders.cpp:159-181); the
- * gradient is stored in sRGB16 before scaling. */
+/* The textures of the textured background and environment mapping (set
+ * by oracle_set_textures; NULL = none). */
+static const mbr_texture *g_bg_tex = NULL, *g_env_tex = NULL;
+
+void oracle_set_textures(const mbr_texture *bg, const mbr_texture *env) {
+  g_bg_tex = bg;
+  g_env_tex = env;
+}
+
+/* cTexture::Pixel / Interpolation (src/texture.cpp:83-111): bilinear over
+ * four texels of the flat bitmap (ix + 1 at the last column is the next
+ * row's first texel), truncated to 8 bits, black outside [0, w) x [0, h-1).
+ * The texel one past the bitmap's end, which the reference reads at the
+ * last column of the second-last row, reads black here. */
+static void texture_pixel(const mbr_texture *t, double x, double y, int out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (!(x >= 0 && x < t->width && y >= 0 && y < t->height - 1.0)) return;
+  int ix = (int)x, iy = (int)y;
+  double rx = (x - (int)x), ry = (y - (int)y);
+  long long n = (long long)t->width * t->height;
+  long long i1 = (long long)iy * t->width + ix, i2 = i1 + 1;
+  long long i3 = (long long)(iy + 1) * t->width + ix, i4 = i3 + 1;
+  for (int c = 0; c < 3; c++) {
+    int k1 = t->rgb[3 * i1 + c], k2 = t->rgb[3 * i2 + c], k3 = t->rgb[3 * i3 + c];
+    int k4 = i4 < n ? t->rgb[3 * i4 + c] : 0;
+    out[c] = (unsigned char)(k1 * (1.0 - rx) * (1.0 - ry) + k2 * (rx) * (1.0 - ry) + k3 * (1.0 - rx) * ry + k4 * (rx * ry));
+  }
+}
+
+/* BackgroundShader (src/shaders.cpp:114-183): the texture branches (a
+ * fisheye dome or an equirectangular map indexed by the view vector's
+ * angles; the alpha the reference leaves unset is 0 here) and the gradient,
+ * stored in sRGB16 before scaling. */
 static shade_t background_shader(const mbr_scene *s, v3 viewVector) {
   shade_t px;
+  if (s->textured_background && g_bg_tex) {
+    double alfa = atan2(viewVector.y, viewVector.x);
+    double beta = atan2(viewVector.z, sqrt(viewVector.x * viewVector.x + viewVector.y * viewVector.y));
+    double texX, texY;
+    if (s->background_as_fulldome) {
+      int texWidth = g_bg_tex->width * 0.5;
+      int texHeight = g_bg_tex->height;
+      int offset = 0;
+      if (beta < 0) {
+        beta = -beta;
+        alfa += M_PI;
+        offset = texWidth;
+      }
+      texX = 0.5 * texWidth + cos(alfa) * (1.0 - beta / (0.5 * M_PI)) * texWidth * 0.5 + offset;
+      texY = 0.5 * texHeight + sin(alfa) * (1.0 - beta / (0.5 * M_PI)) * texHeight * 0.5;
+    } else {
+      alfa = fmod(alfa + 2.5 * M_PI, 2 * M_PI);
+      if (beta > 0.5 * M_PI) beta = 0.5 * M_PI - beta;
+      if (beta < -0.5 * M_PI) beta = -0.5 * M_PI + beta;
+      texX = alfa / (2.0 * M_PI) * g_bg_tex->width;
+      texY = (beta / (M_PI) + 0.5) * g_bg_tex->height;
+    }
+    int p[3];
+    texture_pixel(g_bg_tex, texX, texY, p);
+    px.R = p[0] / 256.0;
+    px.G = p[1] / 256.0;
+    px.B = p[2] / 256.0;
+    px.alpha = 0.0;
+    return px;
+  }
   v3 vec = v3norm(v3m(0.0, 0.0, -1.0));
   double grad = (v3dot(viewVector, vec) + 1.0);
   unsigned short R, G, B;

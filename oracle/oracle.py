@@ -48,6 +48,7 @@ def _load(name):
     lib.oracle_ao_table.restype = C.c_int
     lib.oracle_march.argtypes = [S, D3, D3, C.c_uint32, P(MarchOut)]
     lib.oracle_march.restype = C.c_int
+    lib.oracle_set_textures.argtypes = [P(_abi.Texture), P(_abi.Texture)]
     lib.oracle_render.argtypes = [S, P(_abi.Texture), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
                                   P(_abi.Framebuffer), P(OracleStats)]
     lib.oracle_render.restype = C.c_int
@@ -190,8 +191,16 @@ class HostFrame:
         return f
 
 
-def render(scene, rows=None, threads=0, jitter=None, glibc_seed=1234, fast=False, lightmap=None):
-    """Render with the oracle; returns (HostFrame, OracleStats)."""
+def _texture(arr):
+    a = np.ascontiguousarray(arr, dtype=np.uint8)
+    return a, _abi.Texture(a.shape[1], a.shape[0], a.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+
+def render(scene, rows=None, threads=0, jitter=None, glibc_seed=1234, fast=False, lightmap=None, background=None,
+           envmap=None):
+    """Render with the oracle; returns (HostFrame, OracleStats).  background,
+    envmap: (h, w, 3) uint8 textures of the textured background and
+    environment mapping."""
     w, h = scene.image_width, scene.image_height
     frame = HostFrame(w, h)
     fb = frame.fb()
@@ -203,8 +212,13 @@ def render(scene, rows=None, threads=0, jitter=None, glibc_seed=1234, fast=False
     if lightmap is not None:
         arr = np.ascontiguousarray(lightmap, dtype=np.uint8)
         tex = _abi.Texture(arr.shape[1], arr.shape[0], arr.ctypes.data_as(C.POINTER(C.c_uint8)))
-    rc = lib(fast).oracle_render(C.byref(scene), C.byref(tex) if tex is not None else None, r0, r1, threads,
-                                 jitter, glibc_seed, C.byref(fb), C.byref(st))
+    texs = [_texture(t) if t is not None else None for t in (background, envmap)]
+    lib(fast).oracle_set_textures(*[C.byref(t[1]) if t is not None else None for t in texs])
+    try:
+        rc = lib(fast).oracle_render(C.byref(scene), C.byref(tex) if tex is not None else None, r0, r1, threads,
+                                     jitter, glibc_seed, C.byref(fb), C.byref(st))
+    finally:
+        lib(fast).oracle_set_textures(None, None)
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
     return frame, st

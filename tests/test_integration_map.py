@@ -44,7 +44,7 @@ def _field_range(name):
 
 def test_params2mbr_matches_key_table():
     lines = _params2mbr_lines()
-    assert len(lines) == 280
+    assert len(lines) == 281
     seen = set()
     for field, key in lines:
         s0 = _abi.Scene()
