@@ -594,7 +594,9 @@ int alloc_slot(mbr_ctx *c, Slot &sl, int precision, int cap, int ao_count, bool 
     HIPCHECK(c, hipMalloc(&sl.sh_spec, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_col, 3 * n * b.rec_bytes));
     HIPCHECK(c, hipMalloc(&sl.sh_fast, n * b.rec_bytes));
-    HIPCHECK(c, hipMalloc(&sl.sh_aux, 6 * n * b.rec_bytes));
+    // aux lights: normal and detail, then diffuse and specular sums (6);
+    // environment mapping: its reflect-scaled colour (3)
+    HIPCHECK(c, hipMalloc(&sl.sh_aux, 9 * n * b.rec_bytes));
     // one directory and one chunk per 64 rays to begin with; grown when a
     // frame needs more
     int rc = alloc_pool(c, sl, std::max<long long>(256, (long long)n / 64), std::max<long long>(256, (long long)n / 64));
@@ -638,7 +640,8 @@ int kernel_family(const mbr_scene &s) { return s.ifs_folding_mode ? kGenericFami
 // h: the scene's derived block, when made (batches reject those features).
 int scene_precision(const mbr_scene &s, const DevScene *h, int precision) {
   if (precision == MBR_PREC_FP64) return precision;
-  if (s.fake_lights_enabled) return MBR_PREC_FP64;  // fake lights render in FP64
+  // fake lights and environment mapping render in FP64
+  if (s.fake_lights_enabled || (s.reflect > 0.0 && !s.raytraced_reflections)) return MBR_PREC_FP64;
   const int fam = kernel_family(s);
   if (!formula_generic(fam)) return precision;
   // the IFS folding pre-fold keeps the FP32 mode of an analytic formula
@@ -666,8 +669,8 @@ int check_supported(mbr_ctx *c, const mbr_scene &s) {
   if (!formula_supported(s.formula))
     return c->fail(MBR_ERR_UNSUPPORTED, "formula " + std::to_string(s.formula) +
                                             " is not in the render core (supported: 1-8, 10-37)");
-  if (s.reflect > 0.0 && !s.raytraced_reflections)
-    return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0) is not in the render core");
+  if (s.reflect > 0.0 && !s.raytraced_reflections && !c->d_env)
+    return c->fail(MBR_ERR_UNSUPPORTED, "environment mapping (reflect > 0): no envmap texture (mbr_set_textures)");
   if (s.raytraced_reflections && s.reflections_max > MBR_MAX_REFLECTIONS)
     return c->fail(MBR_ERR_INVALID_ARG, "reflections_max above " + std::to_string(MBR_MAX_REFLECTIONS));
   if (s.aux_light_number < 0) return c->fail(MBR_ERR_INVALID_ARG, "negative aux_light_number");
@@ -887,6 +890,11 @@ int prepare_scene(mbr_ctx *c, const mbr_scene *scene_in, const mbr_texture *ligh
   d.bg_tex = c->d_bg;
   d.bg_w = c->bg_w;
   d.bg_h = c->bg_h;
+  d.env_map = s.reflect > 0.0 && !s.raytraced_reflections && c->d_env;
+  d.env_tex = c->d_env;
+  d.env_w = c->env_w;
+  d.env_h = c->env_h;
+  d.reflect = s.reflect;
   d.fake_lights = s.fake_lights_enabled;
   d.fl_intensity = s.fake_lights_intensity;
   d.fl_visibility = s.fake_lights_visibility;
@@ -1646,7 +1654,7 @@ size_t carve_scratch(unsigned char *base, int precision, int cap, int ao_count, 
   sl.sh_spec = take(3 * n * rec);
   sl.sh_col = take(3 * n * rec);
   sl.sh_fast = take(n * rec);
-  sl.sh_aux = take(6 * n * rec);
+  sl.sh_aux = take(9 * n * rec);
   if (ao_count > 0) {
     sl.sh_ao_bytes = n * (size_t)ao_count * rec;
     sl.sh_ao = take(sl.sh_ao_bytes);

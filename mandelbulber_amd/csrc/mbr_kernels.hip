@@ -685,6 +685,15 @@ __global__ void __launch_bounds__(kBlock) k_surface(const DevScene *__restrict__
     // it runs: MainShadow's last threshold, float-rounded (src/shaders.cpp:
     // 510-522), when there are shadows; the slow AO pass reads it from here
     if (sc.ao && !sc.fast_ao) sb.fast_ao[tid] = in.detail;
+    if constexpr (sizeof(T) == 8) {
+      if (sc.env_map) {
+        // environment mapping x reflect (ObjectShader, src/shaders.cpp:73-80);
+        // k_compose multiplies it by the AO
+        double e[3];
+        env_mapping(sc, cvt<double>(in.normal), cvt<double>(in.view), e);
+        for (int c = 0; c < 3; c++) sb.aux[(6 + c) * n + tid] = (T)(e[c] * sc.reflect);
+      }
+    }
     if (sc.n_lights > 0 || sc.fake_lights) {
       // k_aux shades the aux (and fake) lights from the normal and the
       // detail size MainShadow left (the aux shadows inherit it)
@@ -1020,12 +1029,15 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
         amb.B /= den;
       }
     }
+    // environment mapping x reflect, times the AO (0 without it)
+    RGBT<T> env = {0, 0, 0};
+    if (sc.env_map) env = RGBT<T>{sb.aux[6 * n + tid], sb.aux[7 * n + tid], sb.aux[8 * n + tid]};
     if (lane == 0 && b.sp_r) {
       // reflections: the object part and the specular part apart
       const T cR = sb.col[0 * n + tid], cG = sb.col[1 * n + tid], cB = sb.col[2 * n + tid];
-      T oR = (amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR;
-      T oG = (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG;
-      T oB = (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB;
+      T oR = env.R * amb.R + (amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR;
+      T oG = env.G * amb.G + (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG;
+      T oB = env.B * amb.B + (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB;
       T sR = sb.spec[0 * n + tid], sG = sb.spec[1 * n + tid], sB = sb.spec[2 * n + tid];
       if (sc.n_lights > 0 || sc.fake_lights) {
         oR += sb.aux[0 * n + tid] * cR;
@@ -1046,16 +1058,22 @@ __global__ void __launch_bounds__(kBlock) k_compose(const DevScene *__restrict__
       if (sc.n_lights > 0 || sc.fake_lights) {
         // (ambient + main light) x colour + aux x colour, then aux specular
         // + main specular (ObjectShader, src/shaders.cpp:96-108)
-        ((RT *)b.sh_r)[r] = (RT)(((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.aux[0 * n + tid] * cR) +
+        ((RT *)b.sh_r)[r] = (RT)(((env.R * amb.R + (amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR) +
+                                  sb.aux[0 * n + tid] * cR) +
                                  (sb.aux[3 * n + tid] + sb.spec[0 * n + tid]));
-        ((RT *)b.sh_g)[r] = (RT)(((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.aux[1 * n + tid] * cG) +
+        ((RT *)b.sh_g)[r] = (RT)(((env.G * amb.G + (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG) +
+                                  sb.aux[1 * n + tid] * cG) +
                                  (sb.aux[4 * n + tid] + sb.spec[1 * n + tid]));
-        ((RT *)b.sh_b)[r] = (RT)(((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.aux[2 * n + tid] * cB) +
+        ((RT *)b.sh_b)[r] = (RT)(((env.B * amb.B + (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB) +
+                                  sb.aux[2 * n + tid] * cB) +
                                  (sb.aux[5 * n + tid] + sb.spec[2 * n + tid]));
       } else {
-        ((RT *)b.sh_r)[r] = (RT)((amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR + sb.spec[0 * n + tid]);
-        ((RT *)b.sh_g)[r] = (RT)((amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG + sb.spec[1 * n + tid]);
-        ((RT *)b.sh_b)[r] = (RT)((amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB + sb.spec[2 * n + tid]);
+        ((RT *)b.sh_r)[r] =
+            (RT)((env.R * amb.R + (amb.R * k.global_ilum + sb.base[0 * n + tid]) * cR) + sb.spec[0 * n + tid]);
+        ((RT *)b.sh_g)[r] =
+            (RT)((env.G * amb.G + (amb.G * k.global_ilum + sb.base[1 * n + tid]) * cG) + sb.spec[1 * n + tid]);
+        ((RT *)b.sh_b)[r] =
+            (RT)((env.B * amb.B + (amb.B * k.global_ilum + sb.base[2 * n + tid]) * cB) + sb.spec[2 * n + tid]);
       }
     }
   }

@@ -108,6 +108,11 @@ struct DevScene {
   // context's background texture (8-bit RGB rows), dome layout
   int textured_bg, fulldome, bg_w, bg_h;
   const uint8_t *bg_tex;
+  // environment mapping (EnvMapping, reflect > 0 without raytraced
+  // reflections): the context's envmap texture and the reflect factor
+  int env_map, env_w, env_h;
+  const uint8_t *env_tex;
+  double reflect;
 
   // volumetric lights (VolumetricShader, src/shaders.cpp:314-341): vol_main
   // for the main light; vol_aux_on[k] when aux light slot k is enabled and

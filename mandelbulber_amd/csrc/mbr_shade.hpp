@@ -510,6 +510,30 @@ MBR_HD inline void background_texture(const DevScene &sc, V3<double> v, double o
   for (int c = 0; c < 3; c++) out[c] = px[c] / 256.0;
 }
 
+// EnvMapping (src/shaders.cpp:785-812): the view vector mirrored about the
+// normal indexes the environment texture by its angles (wrapped with the
+// reference's 8.25 and 8 texture offsets).
+MBR_HD inline void env_mapping(const DevScene &sc, V3<double> n, V3<double> view, double out[3]) {
+#pragma clang fp contract(off)
+  const double pi = 3.14159265358979323846;
+  const double d = -(view.x * n.x + view.y * n.y + view.z * n.z);
+  const double rx = n.x * 2.0 * d + view.x, ry = n.y * 2.0 * d + view.y, rz = n.z * 2.0 * d + view.z;
+  const double alfa = atan2(ry, rx) + pi;
+  double beta = atan2(rz, sqrt(rx * rx + ry * ry));
+  const double w = sc.env_w, h = sc.env_h;
+  if (beta > 0.5 * pi) beta = 0.5 * pi - beta;
+  if (beta < -0.5 * pi) beta = -0.5 * pi + beta;
+  double dtx = (alfa / (2.0 * pi)) * w + w * 8.25;
+  double dty = (beta / (pi) + 0.5) * h + h * 8.0;
+  dtx = fmod(dtx, w);
+  dty = fmod(dty, h);
+  if (dtx < 0) dtx = 0;
+  if (dty < 0) dty = 0;
+  int px[3];
+  tex_pixel(sc.env_tex, sc.env_w, sc.env_h, dtx, dty, px);
+  for (int c = 0; c < 3; c++) out[c] = px[c] / 256.0;
+}
+
 template <class T>
 MBR_HD RGBT<T> background(const DevScene &sc, V3<T> view) {
   if (sc.textured_bg) {

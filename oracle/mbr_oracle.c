@@ -1590,8 +1590,68 @@ static shade_t fake_lights(shader_in *in, shade_t *fakeSpec) {
   return out;
 }
 
-/* ObjectShader (src/shaders.cpp:21-112) without env mapping (a rejected
- * scene feature; its term is zero). */
+/* The textures of the textured background and environment mapping (set
+ * by oracle_set_textures; NULL = none). */
+static const mbr_texture *g_bg_tex = NULL, *g_env_tex = NULL;
+
+void oracle_set_textures(const mbr_texture *bg, const mbr_texture *env) {
+  g_bg_tex = bg;
+  g_env_tex = env;
+}
+
+/* cTexture::Pixel / Interpolation (src/texture.cpp:83-111): bilinear over
+ * four texels of the flat bitmap (ix + 1 at the last column is the next
+ * row's first texel), truncated to 8 bits, black outside [0, w) x [0, h-1).
+ * The texel one past the bitmap's end, which the reference reads at the
+ * last column of the second-last row, reads black here. */
+static void texture_pixel(const mbr_texture *t, double x, double y, int out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (!(x >= 0 && x < t->width && y >= 0 && y < t->height - 1.0)) return;
+  int ix = (int)x, iy = (int)y;
+  double rx = (x - (int)x), ry = (y - (int)y);
+  long long n = (long long)t->width * t->height;
+  long long i1 = (long long)iy * t->width + ix, i2 = i1 + 1;
+  long long i3 = (long long)(iy + 1) * t->width + ix, i4 = i3 + 1;
+  for (int c = 0; c < 3; c++) {
+    int k1 = t->rgb[3 * i1 + c], k2 = t->rgb[3 * i2 + c], k3 = t->rgb[3 * i3 + c];
+    int k4 = i4 < n ? t->rgb[3 * i4 + c] : 0;
+    out[c] = (unsigned char)(k1 * (1.0 - rx) * (1.0 - ry) + k2 * (rx) * (1.0 - ry) + k3 * (1.0 - rx) * ry + k4 * (rx * ry));
+  }
+}
+
+/* EnvMapping (src/shaders.cpp:785-812): the view vector mirrored about the
+ * normal indexes the environment texture by its angles, wrapped with the
+ * reference's 8.25 and 8 texture offsets. */
+static shade_t env_mapping(v3 normal, v3 viewVector) {
+  shade_t out = {0, 0, 0, 0};
+  if (!g_env_tex) return out;
+  double dot = -v3dot(viewVector, normal);
+  v3 reflect = v3add(v3mul(v3mul(normal, 2.0), dot), viewVector);
+  double alfaTexture = atan2(reflect.y, reflect.x) + M_PI;
+  double betaTexture = atan2(reflect.z, sqrt(reflect.x * reflect.x + reflect.y * reflect.y));
+  double texWidth = g_env_tex->width, texHeight = g_env_tex->height;
+  if (betaTexture > 0.5 * M_PI) betaTexture = 0.5 * M_PI - betaTexture;
+  if (betaTexture < -0.5 * M_PI) betaTexture = -0.5 * M_PI + betaTexture;
+  double dtx = (alfaTexture / (2.0 * M_PI)) * texWidth + texWidth * 8.25;
+  double dty = (betaTexture / (M_PI) + 0.5) * texHeight + texHeight * 8.0;
+  dtx = fmod(dtx, texWidth);
+  dty = fmod(dty, texHeight);
+  if (dtx < 0) dtx = 0;
+  if (dty < 0) dty = 0;
+  int p[3];
+  texture_pixel(g_env_tex, dtx, dty, p);
+  out.R = p[0] / 256.0;
+  out.G = p[1] / 256.0;
+  out.B = p[2] / 256.0;
+  return out;
+}
+
+void oracle_env_mapping(const double normal[3], const double view[3], double rgb[3]) {
+  shade_t e = env_mapping(v3m(normal[0], normal[1], normal[2]), v3m(view[0], view[1], view[2]));
+  rgb[0] = e.R; rgb[1] = e.G; rgb[2] = e.B;
+}
+
+/* ObjectShader (src/shaders.cpp:21-112). */
 static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *specularOut) {
   const mbr_scene *s = in->f->s;
   shade_t output;
@@ -1629,9 +1689,14 @@ static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *spe
   ambient2.R = ambient.R * s->global_ilum;
   ambient2.G = ambient.G * s->global_ilum;
   ambient2.B = ambient.B * s->global_ilum;
-  output.R = 0.0 * ambient.R + (ambient2.R + mainLight.R * shade * shadow) * colour.R;
-  output.G = 0.0 * ambient.G + (ambient2.G + mainLight.G * shade * shadow) * colour.G;
-  output.B = 0.0 * ambient.B + (ambient2.B + mainLight.B * shade * shadow) * colour.B;
+  shade_t envMapping = {0, 0, 0, 0};
+  if (!s->raytraced_reflections && s->reflect > 0) envMapping = env_mapping(in->normal, in->viewVector);
+  envMapping.R *= s->reflect;
+  envMapping.G *= s->reflect;
+  envMapping.B *= s->reflect;
+  output.R = envMapping.R * ambient.R + (ambient2.R + mainLight.R * shade * shadow) * colour.R;
+  output.G = envMapping.G * ambient.G + (ambient2.G + mainLight.G * shade * shadow) * colour.G;
+  output.B = envMapping.B * ambient.B + (ambient2.B + mainLight.B * shade * shadow) * colour.B;
   shade_t auxSpecular;
   shade_t auxLights = aux_lights_shader(in, &auxSpecular);
   shade_t fakeLights = {0, 0, 0, 0}, fakeSpecular = {0, 0, 0, 0};
@@ -1645,35 +1710,6 @@ static shade_t object_shader(shader_in *in, shade_t *surfaceColour, shade_t *spe
   specularOut->B = auxSpecular.B + fakeSpecular.B + mainLight.B * specular * shadow;
   specularOut->alpha = output.alpha;
   return output;
-}
-
-/* The textures of the textured background and environment mapping (set
- * by oracle_set_textures; NULL = none). */
-static const mbr_texture *g_bg_tex = NULL, *g_env_tex = NULL;
-
-void oracle_set_textures(const mbr_texture *bg, const mbr_texture *env) {
-  g_bg_tex = bg;
-  g_env_tex = env;
-}
-
-/* cTexture::Pixel / Interpolation (src/texture.cpp:83-111): bilinear over
- * four texels of the flat bitmap (ix + 1 at the last column is the next
- * row's first texel), truncated to 8 bits, black outside [0, w) x [0, h-1).
- * The texel one past the bitmap's end, which the reference reads at the
- * last column of the second-last row, reads black here. */
-static void texture_pixel(const mbr_texture *t, double x, double y, int out[3]) {
-  out[0] = out[1] = out[2] = 0;
-  if (!(x >= 0 && x < t->width && y >= 0 && y < t->height - 1.0)) return;
-  int ix = (int)x, iy = (int)y;
-  double rx = (x - (int)x), ry = (y - (int)y);
-  long long n = (long long)t->width * t->height;
-  long long i1 = (long long)iy * t->width + ix, i2 = i1 + 1;
-  long long i3 = (long long)(iy + 1) * t->width + ix, i4 = i3 + 1;
-  for (int c = 0; c < 3; c++) {
-    int k1 = t->rgb[3 * i1 + c], k2 = t->rgb[3 * i2 + c], k3 = t->rgb[3 * i3 + c];
-    int k4 = i4 < n ? t->rgb[3 * i4 + c] : 0;
-    out[c] = (unsigned char)(k1 * (1.0 - rx) * (1.0 - ry) + k2 * (rx) * (1.0 - ry) + k3 * (1.0 - rx) * ry + k4 * (rx * ry));
-  }
 }
 
 /* BackgroundShader (src/shaders.cpp:114-183): the texture branches (a

@@ -49,6 +49,7 @@ def _load(name):
     lib.oracle_march.argtypes = [S, D3, D3, C.c_uint32, P(MarchOut)]
     lib.oracle_march.restype = C.c_int
     lib.oracle_set_textures.argtypes = [P(_abi.Texture), P(_abi.Texture)]
+    lib.oracle_env_mapping.argtypes = [D3, D3, D3]
     lib.oracle_render.argtypes = [S, P(_abi.Texture), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
                                   P(_abi.Framebuffer), P(OracleStats)]
     lib.oracle_render.restype = C.c_int
@@ -194,6 +195,18 @@ class HostFrame:
 def _texture(arr):
     a = np.ascontiguousarray(arr, dtype=np.uint8)
     return a, _abi.Texture(a.shape[1], a.shape[0], a.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+
+def env_mapping(normal, view, envmap):
+    """EnvMapping's texture colour for a unit normal and view vector."""
+    a, t = _texture(envmap)
+    out = (C.c_double * 3)()
+    lib().oracle_set_textures(None, C.byref(t))
+    try:
+        lib().oracle_env_mapping((C.c_double * 3)(*normal), (C.c_double * 3)(*view), out)
+    finally:
+        lib().oracle_set_textures(None, None)
+    return np.array(out[:])
 
 
 def render(scene, rows=None, threads=0, jitter=None, glibc_seed=1234, fast=False, lightmap=None, background=None,

@@ -80,3 +80,24 @@ def test_without_a_texture_the_gradient():
     g = oracle.background(s, (0.0, 0.6, 0.8))
     s.textured_background = 1
     assert np.array_equal(oracle.background(s, (0.0, 0.6, 0.8)), g)   # no texture set: the gradient
+
+
+def test_environment_map_of_the_mirrored_view():
+    """EnvMapping (src/shaders.cpp:785-812): with the index texture the
+    sample gives the texel of the mirrored view r = v - 2 (v.n) n at
+    x = (atan2(r_y, r_x) + pi) / 2pi W (+ 8.25 W, wrapped), y = (beta / pi +
+    0.5) H (+ 8 H, wrapped)."""
+    tex = _index_texture()
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        r = v - 2.0 * np.dot(v, n) * n
+        a = math.atan2(r[1], r[0]) + math.pi
+        b = math.atan2(r[2], math.hypot(r[0], r[1]))
+        tx = math.fmod(a / (2 * math.pi) * W + W * 8.25, W)
+        ty = math.fmod((b / math.pi + 0.5) * H + H * 8.0, H)
+        rgb = oracle.env_mapping(tuple(n), tuple(v), tex) * 256.0
+        _check(rgb, tx, ty)
