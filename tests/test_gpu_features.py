@@ -9,8 +9,12 @@ scenes, and for the reference's own step jitter.
   src/primitives.cpp): each on a 96x72 Mandelbulb scene, GPU FP64 against the
   oracle (bit-exact depth, integer planes and >= 120 dB where the arithmetic
   is the same; the bars elsewhere), GPU FP32_FIXUP against the bars.
-- share/examples/*.fract (tests/golden/example_scenes.json, 64x48): every
-  scene mbr_set_scene accepts, GPU FP64 against the oracle.
+- share/examples/*.fract (tests/golden/example_scenes.json, 64x48): all 77
+  scenes, GPU FP64 against the oracle.  The scenes' JPEG background and
+  environment textures cannot be decoded here, so both sides get the same
+  synthetic 8-bit textures (mbr_set_textures, as a caller that decodes them
+  would); the nine scenes with a textured background or environment mapping
+  sample them.
 - Some scenes decide hits by orbits that run to N iterations (iteration
   threshold mode) or magnify the last bits of pow/atan2/sin/cos (the delta
   DE of the generic formulas): there the reference's own release build
@@ -132,8 +136,8 @@ def test_feature_matches_oracle(renderer, name):
 
 
 def test_reference_example_scenes(renderer):
-    """Every example scene the render core accepts, GPU FP64 against the
-    oracle at 64x48: the bars, or the fast-math envelope where the
+    """Every example scene, GPU FP64 against the oracle at 64x48 (textured
+    scenes on the synthetic textures): the bars, or the fast-math envelope where the
     reference's own builds differ by more; bit-exact (depth, integer planes,
     >= 120 dB) wherever the two reference builds agree bit for bit and the
     formula has kernels of its own (2, 7, 8), reflections aside (their own
@@ -141,32 +145,52 @@ def test_reference_example_scenes(renderer):
     from examples import load_examples
     from mandelbulber_amd._lib import MbrError
     from oracle import oracle
-    rendered, rejected, exact = [], [], []
-    for name, s in load_examples():
-        try:
-            renderer.set_scene(s)
-        except MbrError as e:
-            assert e.rc == -4, (name, e)  # MBR_ERR_UNSUPPORTED: environment map, textured background, fake lights
-            rejected.append(name)
-            continue
-        ref, _ = oracle.render(s)
-        env = compare(oracle.render(s, fast=True)[0], ref)
-        frame, _ = renderer.render(precision=_abi.PREC_FP64)
-        m = compare(frame, ref)
-        print(f"[examples] {name}: formula {s.formula} flips {m['flips']} depth {m['depth_rel_max']:.2e} "
-              f"(p99.9 {m['depth_rel_p999']:.2e}) psnr {m['psnr_db']:.1f} colour/alpha/opacity px "
-              f"{m['colour_diff_px']}/{m['alpha_diff_px']}/{m['opacity_diff_px']}; fast-math build: flips "
-              f"{env['flips']} depth {env['depth_rel_max']:.2e} psnr {env['psnr_db']:.1f}", flush=True)
-        rendered.append(name)
-        _within(m, env, name)
-        if (s.formula in (2, 7, 8) and not s.raytraced_reflections and env["flips"] == 0
-                and env["depth_rel_max"] == 0 and env["psnr_db"] >= 120):
-            exact.append(name)
-            assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12 and m["psnr_db"] >= 120, (name, m)
-            assert_planes_exact(m, name)
-    print(f"[examples] rendered {len(rendered)} ({len(exact)} held bit-exact), rejected {len(rejected)}: "
-          f"{rejected}", flush=True)
-    assert len(rendered) >= 68
+    bg, envmap = _texture(96, 48, 3), _texture(64, 32, 5)
+    tex = dict(background=bg, envmap=envmap)
+    renderer.set_textures(bg, envmap)
+    rendered, rejected, exact, textured = [], [], [], []
+    try:
+        for name, s in load_examples():
+            try:
+                renderer.set_scene(s)
+            except MbrError as e:
+                assert e.rc == -4, (name, e)  # MBR_ERR_UNSUPPORTED
+                rejected.append(name)
+                continue
+            if s.textured_background or (s.reflect > 0 and not s.raytraced_reflections):
+                textured.append(name)
+            _example_scene(renderer, oracle, name, s, tex, rendered, exact)
+    finally:
+        renderer.set_textures(None, None)
+    print(f"[examples] rendered {len(rendered)} ({len(exact)} held bit-exact, {len(textured)} textured), "
+          f"rejected {len(rejected)}: {rejected}", flush=True)
+    assert len(rendered) == 77 and len(textured) == 9
+
+
+def _texture(w, h, seed):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    t = np.stack([x * 255 // (w - 1), y * 255 // (h - 1), (x * y) % 256], axis=-1)
+    return ((t + rng.integers(0, 40, size=t.shape)) % 256).astype(np.uint8)
+
+
+def _example_scene(renderer, oracle, name, s, tex, rendered, exact):
+    """One example scene (already set on the renderer) against the oracle."""
+    ref, _ = oracle.render(s, **tex)
+    env = compare(oracle.render(s, fast=True, **tex)[0], ref)
+    frame, _ = renderer.render(precision=_abi.PREC_FP64)
+    m = compare(frame, ref)
+    print(f"[examples] {name}: formula {s.formula} flips {m['flips']} depth {m['depth_rel_max']:.2e} "
+          f"(p99.9 {m['depth_rel_p999']:.2e}) psnr {m['psnr_db']:.1f} colour/alpha/opacity px "
+          f"{m['colour_diff_px']}/{m['alpha_diff_px']}/{m['opacity_diff_px']}; fast-math build: flips "
+          f"{env['flips']} depth {env['depth_rel_max']:.2e} psnr {env['psnr_db']:.1f}", flush=True)
+    rendered.append(name)
+    _within(m, env, name)
+    if (s.formula in (2, 7, 8) and not s.raytraced_reflections and env["flips"] == 0
+            and env["depth_rel_max"] == 0 and env["psnr_db"] >= 120):
+        exact.append(name)
+        assert m["flips"] == 0 and m["depth_rel_max"] <= 1e-12 and m["psnr_db"] >= 120, (name, m)
+        assert_planes_exact(m, name)
 
 
 @pytest.mark.parametrize("case", ["plain_bailout100", "plain_bailout10", "shaded_seed1", "shaded_seed2"])
