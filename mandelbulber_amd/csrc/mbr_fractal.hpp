@@ -1222,13 +1222,23 @@ MBR_HD bool de_call_begin_p64(const FractalK<double> &fd, const FractalK<float> 
   return false;
 }
 
-// Whole CalculateDistance at an FP64 point in the caller's precision T.
+// Whole CalculateDistance at an FP64 point in the caller's precision T.  An
+// FP32 caller whose detail is below FP32 resolution at the point runs the
+// whole estimate in FP64: at deep zoom the switch to FP32 comes late in the
+// orbit and at a different iteration in every lane, so a wave would pay for
+// both paths (the Mandelbox surface kernel of a 1e-6-wide 4K frame took 1.45x
+// the FP64 kernel's time that way).
 template <class T, int F>
 MBR_HD T calc_distance_p64(const FractalK<double> &fd, const FractalK<T> &f, V3<double> point, T detailSize, int N,
                            int minN, bool normalMode, DEInfo &info) {
   if constexpr (sizeof(T) == 8) {
     return calc_distance_f<double, F>(fd, point, detailSize, N, minN, normalMode, info);
   } else {
+    const V3<float> pf = cvt<float>(point);
+    const float cmax = length(pf) * (f.julia_mode ? 0.0f : (f.constant_factor > 1.0f ? f.constant_factor : 1.0f));
+    const float zc = f.julia_mode ? length(ld3(f.julia)) : cmax;
+    if ((double)detailSize < kMixedSwitch * (double)(length(pf) > zc ? length(pf) : zc))
+      return (float)calc_distance_f<double, F>(fd, point, (double)detailSize, N, minN, normalMode, info);
     DECall<float> call;
     float dist = 0;
     if (de_call_begin_p64<F>(fd, f, point, detailSize, N, minN, normalMode, call, dist, info)) return dist;
